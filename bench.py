@@ -1,0 +1,192 @@
+"""Benchmark: 30-qubit ComplexF64 brick-wall random circuit (depth 20, U on
+every qubit + nearest-neighbour CZ layers), <Z_i> for all i and 1,000,000
+alias-method shots, on one MI355X.
+
+One step = reinit to |0...0> + fused evolve + one-pass <Z_i> + alias table
+build + 1e6 shots, with the state resident in HBM.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--qubits 30] [--depth 20]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+for p in (ROOT, ROOT / "braketsimulator.jl_amd", ROOT / "oracle"):
+    if str(p) not in sys.path:
+        sys.path.insert(0, str(p))
+
+METRIC = "30q SV HBM GB/s + gates/s; 24q adjoint grads/s vs param-shift at 1e-10"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(n: int, ins, budget_s: float = 15.0):
+    """The oracle (C restatement of the reference's per-gate loops, OpenMP
+    over every host core) on the same 30-qubit circuit, on a bounded prefix
+    of its gates."""
+    import numpy as np
+
+    import oracle as O
+    from bsim.gates import kernel_call
+
+    cores = os.cpu_count() or 1
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    psi = np.zeros(2 ** n, dtype=np.complex128)
+    psi[0] = 1
+    t0 = time.perf_counter()
+    done = 0
+    bytes_ = 0.0
+    for i in ins:
+        call = kernel_call(i.operator, i.target)
+        O.apply_call(psi, n, call)
+        done += 1
+        # reference traffic: dense gates stream the whole state, controlled
+        # gates (CZ) the control-set half (src/gate_kernels.jl:579-606)
+        bytes_ += (16.0 if call.controls else 32.0) * 2 ** n
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    del psi
+    return {
+        "value": bytes_ / dt / 1e9,
+        "unit": "GB/s",
+        "gates_per_s": done / dt,
+        "cores": int(os.environ.get("OMP_NUM_THREADS", cores)),
+        "kind": "port",
+        "sample": f"first {done} of {len(ins)} gates of the {n}-qubit brick-wall, reference per-gate loops "
+                  f"restated in C (oracle/refcpu.c), {dt:.1f} s; reference Julia unavailable",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--qubits", type=int, default=30)
+    ap.add_argument("--depth", type=int, default=20)
+    ap.add_argument("--shots", type=int, default=1000000)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import bsim
+    from bsim import _lib as L
+    from bsim import workloads as W
+    from bsim.simulators import StateVectorSimulator, lower_instructions
+
+    n, depth = a.qubits, a.depth
+    ins = W.brickwall(n, depth, seed=0)
+    n_gates = len(ins)
+    ol = lower_instructions(ins, L.BSIM_SV)
+    stats = (ctypes.c_int64 * 4)()
+    L.call("bsim_plan_stats", n, 0, ol.ptr, ol.n, stats)
+    passes = int(stats[0])
+    tile_bits, low_bits = ctypes.c_int(), ctypes.c_int()
+    L.call("bsim_get_tile_config", ctypes.byref(tile_bits), ctypes.byref(low_bits))
+    log(f"workload: {n}q brick-wall depth {depth}: {n_gates} gates -> {passes} fused passes "
+        f"(tile {tile_bits.value} bits, {low_bits.value} contiguous)")
+
+    sim = StateVectorSimulator(n, a.shots)
+    lib = L.load()
+    phases = {"evolve": 0.0, "expect_z": 0.0, "alias_build": 0.0, "shots": 0.0}
+
+    def step(timed):
+        t = time.perf_counter()
+        L.call("bsim_state_reset", sim.handle)
+        L.call("bsim_evolve", sim.handle, ol.ptr, ol.n, 0)
+        t1 = time.perf_counter()
+        z = sim.expect_z_all()
+        t2 = time.perf_counter()
+        L.call("bsim_alias_build", sim.handle, L.ALIAS_PARALLEL, 0.0)
+        t3 = time.perf_counter()
+        out = np.empty(a.shots, dtype=np.int64)
+        L.call("bsim_sample", sim.handle, a.shots, ctypes.c_uint64(7), L.i64ptr(out))
+        t4 = time.perf_counter()
+        if timed:
+            phases["evolve"] += t1 - t
+            phases["expect_z"] += t2 - t1
+            phases["alias_build"] += t3 - t2
+            phases["shots"] += t4 - t3
+        return z
+
+    for i in range(a.warmup):
+        log(f"warmup step {i}")
+        step(False)
+    L.call("bsim_synchronize", sim.handle)
+    lib.bsim_set_profiling(1)
+    lib.bsim_kernel_stats(1, None, None)
+    log(f"timing {a.steps} steps")
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        z = step(True)
+    L.call("bsim_synchronize", sim.handle)
+    dt = time.perf_counter() - t0
+    launches, kms = ctypes.c_int64(), ctypes.c_double()
+    lib.bsim_kernel_stats(1, ctypes.byref(launches), ctypes.byref(kms))
+    lib.bsim_set_profiling(0)
+
+    pass_bytes = 32.0 * 2 ** n
+    evolve_s = phases["evolve"] / a.steps
+    value = passes * pass_bytes / evolve_s / 1e9
+    avg_kernel_ms = kms.value / max(launches.value, 1)
+    achieved = pass_bytes / (avg_kernel_ms * 1e-3) / 1e9
+    ms_per_step = dt / a.steps * 1e3
+    rec = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "complex128 (f64)",
+        "data": "synthetic (SplitMix64 angles, seed 0)",
+        "config": {
+            "workload": f"brick-wall random circuit, {n} qubits, depth {depth}, ComplexF64 state "
+                        f"({2 ** n * 16 / 2 ** 30:.0f} GiB), <Z_i> all i + {a.shots} alias-method shots",
+            "qubits": n, "depth": depth, "gates": n_gates, "shots": a.shots,
+            "fused_passes": passes, "tile_bits": tile_bits.value, "low_bits": low_bits.value,
+        },
+        "gates_per_s": round(n_gates / evolve_s, 1),
+        "phase_ms": {k: round(v / a.steps * 1e3, 3) for k, v in phases.items()},
+        "shots_per_s": round(a.shots / (phases["shots"] / a.steps), 1),
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_tile (fused LDS-tiled gate pass)",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "avg_kernel_ms": round(avg_kernel_ms, 4),
+            "launches_timed": int(launches.value),
+        },
+        "z_checksum": float(np.sum(z)),
+    }
+    if not a.no_cpu:
+        log("cpu baseline (oracle) ...")
+        rec["cpu_baseline"] = cpu_baseline(n, ins)
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

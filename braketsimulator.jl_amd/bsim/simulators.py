@@ -1,0 +1,467 @@
+"""StateVectorSimulator / DensityMatrixSimulator backed by libbsim.
+
+Mirrors src/sv_simulator.jl and src/dm_simulator.jl: ``reinit``
+(reinit!), ``evolve`` (evolve!), ``apply_gate`` (apply_gate!),
+``apply_noise`` (apply_noise!), ``expectation``, ``probabilities``,
+``state_vector`` / ``density_matrix``, ``apply_observables`` /
+``state_with_observables`` and ``samples`` (src/result_types.jl:56-80).
+Julia's ``!`` suffix has no Python spelling; the names are otherwise the
+reference's.  All amplitudes live on the GPU; host arrays appear only when
+a result is read back.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _lib as L
+from .gates import Gate, GPhase, KernelCall, Ry, Unitary, H as HGate, kernel_call, derivative_matrices
+from .noises import Noise, PhaseFlip
+from .observables import H as HObs
+from .observables import HermitianObservable, I as IObs, Observable, TensorProduct, X as XObs, Y as YObs
+from .observables import Z as ZObs, is_pauli_word, pauli_string
+
+
+class Instruction:
+    """An operator with its target qubits (src/schemas.jl Instruction)."""
+
+    __slots__ = ("operator", "target")
+
+    def __init__(self, operator, target):
+        self.operator = operator
+        self.target = tuple(int(t) for t in (target if hasattr(target, "__iter__") else [target]))
+
+    def __repr__(self):
+        return f"Instruction({self.operator!r}, {list(self.target)})"
+
+
+class Measure:
+    name = "Measure"
+    n_qubits = 1
+
+
+class Barrier:
+    name = "Barrier"
+
+
+class Reset:
+    name = "Reset"
+
+
+class Delay:
+    name = "Delay"
+
+
+NO_OPS = (Measure, Barrier, Reset, Delay)
+
+
+# ---- lowering to bsim_op arrays ----------------------------------------
+
+class OpList:
+    """A ctypes array of bsim_op plus the numpy buffers it points into."""
+
+    def __init__(self, n: int):
+        self.arr = (L.Op * max(n, 1))()
+        self.n = 0
+        self._keep = []
+
+    def _mat(self, m) -> C.POINTER(C.c_double):
+        a = L.cmat(m) if np.ndim(m) == 2 else np.ascontiguousarray(m, dtype=np.complex128)
+        self._keep.append(a)
+        return a.ctypes.data_as(C.POINTER(C.c_double))
+
+    def add(self, typ, targets, matrix, controls=(), cvals=(), n_kraus=0, params=None, dmats=None):
+        op = self.arr[self.n]
+        op.type = typ
+        op.n_targets = len(targets)
+        op.n_controls = len(controls)
+        op.n_kraus = n_kraus
+        for i, t in enumerate(targets):
+            op.targets[i] = int(t)
+        for i, c in enumerate(controls):
+            op.controls[i] = int(c)
+            op.control_values[i] = int(cvals[i]) if i < len(cvals) else 1
+        op.matrix = self._mat(matrix)
+        op.n_params = 0
+        if params:
+            op.n_params = len(params)
+            for i, (p, dm) in enumerate(zip(params, dmats)):
+                op.param_index[i] = int(p)
+                op.dmatrix[i] = self._mat(dm)
+        self.n += 1
+
+    def add_call(self, call: KernelCall, params=None, dmats=None):
+        if call.diagonal:
+            d = np.asarray(call.matrix, dtype=np.complex128)
+            if dmats is not None:
+                dmats = [np.diag(np.asarray(x)) if np.ndim(x) == 1 else x for x in dmats]
+                self.add(L.OP_GATE, call.targets, np.diag(d), call.controls, call.cvals, params=params, dmats=dmats)
+            else:
+                self.add(L.OP_DIAG, call.targets, d, call.controls, call.cvals)
+        else:
+            self.add(L.OP_GATE, call.targets, call.matrix, call.controls, call.cvals, params=params, dmats=dmats)
+
+    @property
+    def ptr(self):
+        return C.cast(self.arr, C.POINTER(L.Op))
+
+
+def lower_instructions(instructions, kind: int, noise_mode: str = "reference", param_map=None) -> OpList:
+    """Instructions -> bsim_op array.  ``param_map`` (adjoint) maps
+    (instruction index, angle index) -> global parameter index."""
+    ol = OpList(len(instructions))
+    for i, ins in enumerate(instructions):
+        op = ins.operator
+        if isinstance(op, NO_OPS) or op is None:
+            continue
+        if isinstance(op, Gate):
+            call = kernel_call(op, ins.target)
+            params = dmats = None
+            if param_map is not None and op.parametrized:
+                idx = [(j, param_map[(i, j)]) for j in range(len(op.angles)) if (i, j) in param_map]
+                if idx:
+                    allm = derivative_matrices(op, ins.target)
+                    params = [p for _, p in idx]
+                    dmats = [allm[j] for j, _ in idx]
+            ol.add_call(call, params, dmats)
+        elif isinstance(op, Noise):
+            if kind != L.BSIM_DM:
+                raise TypeError("noise channels need a DensityMatrixSimulator")
+            if isinstance(op, PhaseFlip):
+                ol.add(L.OP_PHASEFLIP, ins.target, np.array([op.probability], dtype=np.complex128))
+            else:
+                ks = op.kraus_rep()
+                flat = np.concatenate([L.cmat(k).ravel(order="F") for k in ks])
+                typ = L.OP_KRAUS_PHYS if noise_mode == "physical" else L.OP_KRAUS_REF
+                ol.add(typ, ins.target, flat, n_kraus=len(ks))
+        else:
+            raise TypeError(f"unsupported operator {op!r}")
+    return ol
+
+
+def _kron_to_ascending(m: np.ndarray, targets) -> np.ndarray:
+    """Re-index a matrix in kron order of ``targets`` (first = MSB) to kron
+    order of the ascending targets."""
+    k = len(targets)
+    order = sorted(range(k), key=lambda i: targets[i])
+    t = m.reshape([2] * (2 * k))
+    perm = order + [k + o for o in order]
+    return t.transpose(perm).reshape(2 ** k, 2 ** k)
+
+
+def _seed() -> int:
+    return int.from_bytes(os.urandom(8), "little")
+
+
+class _Simulator:
+    kind = L.BSIM_SV
+
+    def __init__(self, qubit_count: int, shots: int = 0):
+        self._h = C.c_void_p()
+        self.qubit_count = int(qubit_count)
+        self.shots = int(shots)
+        L.call("bsim_state_create", self.qubit_count, self.kind, C.byref(self._h))
+        self._observed = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                L.load().bsim_state_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+        obs = getattr(self, "_observed", None)
+        if obs is not None:
+            obs.__del__()
+
+    # -- lifecycle ----------------------------------------------------------
+    def reinit(self, qubit_count: int, shots: int):
+        """reinit! (src/sv_simulator.jl:96-127, src/dm_simulator.jl:96-128)."""
+        if qubit_count != self.qubit_count:
+            L.load().bsim_state_destroy(self._h)
+            self._h = C.c_void_p()
+            self.qubit_count = int(qubit_count)
+            L.call("bsim_state_create", self.qubit_count, self.kind, C.byref(self._h))
+        else:
+            L.call("bsim_state_reset", self._h)
+        self.shots = int(shots)
+        self._observed = None
+        return self
+
+    def similar(self, shots=None):
+        return type(self)(self.qubit_count, self.shots if shots is None else shots)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def _len(self) -> int:
+        return 2 ** (self.qubit_count if self.kind == L.BSIM_SV else 2 * self.qubit_count)
+
+    def upload(self, data: np.ndarray):
+        a = np.ascontiguousarray(np.asarray(data, dtype=np.complex128).ravel(order="F"))
+        L.call("bsim_state_upload", self._h, a.ctypes.data_as(C.POINTER(C.c_double)), a.size)
+
+    def _download(self) -> np.ndarray:
+        out = np.empty(self._len(), dtype=np.complex128)
+        L.call("bsim_state_download", self._h, out.ctypes.data_as(C.POINTER(C.c_double)), out.size)
+        return out
+
+    # -- evolution -------------------------------------------------------------
+    def evolve(self, operations, mode: int = 0):
+        """evolve!: apply ``operations`` (Instructions) in order.  mode 0 fuses
+        gates into LDS-tiled HBM passes, mode 1 launches one kernel per op."""
+        ol = lower_instructions(list(operations), self.kind, getattr(self, "noise_mode", "reference"))
+        L.call("bsim_evolve", self._h, ol.ptr, ol.n, int(mode))
+        return self
+
+    def _apply_call(self, call: KernelCall, vq_targets, conj: int):
+        lib = L.load()
+        t = np.array(vq_targets, dtype=np.int32)
+        k = len(t)
+        if call.double_excitation is not None:
+            orig = np.array(list(reversed(list(vq_targets))), dtype=np.int32)
+            L.check(lib.bsim_apply_double_excitation(self._h, L.iptr(orig), call.double_excitation), "double_excitation")
+            return
+        if call.diagonal:
+            d = np.ascontiguousarray(call.matrix, dtype=np.complex128)
+            L.check(lib.bsim_apply_diagonal(self._h, k, L.iptr(t), L.dptr(d), conj), "apply_diagonal")
+            return
+        m = L.cmat(call.matrix)
+        if call.controls:
+            nq_shift = vq_targets[0] - call.targets[0]
+            c = np.array([x + nq_shift for x in call.controls], dtype=np.int32)
+            cv = np.array(call.cvals, dtype=np.int32)
+            L.check(lib.bsim_apply_controlled(self._h, len(c), L.iptr(c), L.iptr(cv), k, L.iptr(t), L.dptr(m), conj),
+                    "apply_controlled")
+        elif k == 1:
+            L.check(lib.bsim_apply_1q(self._h, int(t[0]), L.dptr(m), conj), "apply_1q")
+        elif k == 2:
+            L.check(lib.bsim_apply_2q(self._h, int(t[0]), int(t[1]), L.dptr(m), conj), "apply_2q")
+        else:
+            L.check(lib.bsim_apply_kq(self._h, k, L.iptr(t), L.dptr(m), conj), "apply_kq")
+
+    # -- results -----------------------------------------------------------------
+    def probabilities(self) -> np.ndarray:
+        out = np.empty(2 ** self.qubit_count, dtype=np.float64)
+        L.call("bsim_probabilities", self._h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def marginal_probabilities(self, targets) -> np.ndarray:
+        t = np.array(targets, dtype=np.int32)
+        out = np.empty(2 ** len(t), dtype=np.float64)
+        L.call("bsim_marginal_probabilities", self._h, len(t), L.iptr(t), out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def expect_pauli_words(self, words) -> np.ndarray:
+        words = list(words)
+        out = np.empty(len(words), dtype=np.float64)
+        if words:
+            buf = "".join(words).encode()
+            L.call("bsim_expect_pauli", self._h, len(words), buf, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def expect_z_all(self) -> np.ndarray:
+        out = np.empty(self.qubit_count, dtype=np.float64)
+        L.call("bsim_expect_z_all", self._h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def _expect_matrix(self, m: np.ndarray, targets) -> float:
+        t = np.array(targets, dtype=np.int32)
+        mm = L.cmat(m)
+        out = C.c_double()
+        L.call("bsim_expect_matrix", self._h, len(t), L.iptr(t), L.dptr(mm), C.byref(out))
+        return out.value
+
+    def _observable_matrix(self, obs: Observable, targets, squared=False):
+        """Matrix and targets in the convention of the reference kernel that
+        ``obs`` reaches (see expectation)."""
+        targets = list(targets)
+        if isinstance(obs, TensorProduct):
+            fs = [f.matrix() for f in obs.factors]
+            if squared:
+                fs = [np.eye(f.shape[0], dtype=np.complex128) if o.pauli is not None or isinstance(o, HObs) else f @ f
+                      for f, o in zip(fs, obs.factors)]
+            m = np.ones((1, 1), dtype=np.complex128)
+            for f in fs:
+                m = np.kron(m, f)
+            sv_as_is = self.kind == L.BSIM_SV and len(targets) <= 2
+            if not sv_as_is:
+                m = _kron_to_ascending(m, targets)
+            return m, targets
+        m = obs.matrix()
+        if squared:
+            m = m @ m
+        return m, targets
+
+    def expectation(self, observable: Observable, *targets) -> float:
+        """<O> on ``targets`` (src/sv_simulator.jl:240-248,
+        src/dm_simulator.jl:253-260)."""
+        targets = list(targets) if targets else list(range(self.qubit_count))
+        if is_pauli_word(observable):
+            if observable.pauli is not None and len(targets) != 1:
+                raise ValueError("a one-qubit observable takes one target")
+            return float(self.expect_pauli_words([pauli_string(observable, targets, self.qubit_count)])[0])
+        m, t = self._observable_matrix(observable, targets)
+        return self._expect_matrix(m, t)
+
+    def expectation_op_squared(self, observable: Observable, *targets) -> float:
+        """src/result_types.jl:168-194."""
+        if is_pauli_word(observable) or isinstance(observable, HObs):
+            return 1.0
+        m, t = self._observable_matrix(observable, list(targets), squared=True)
+        return self._expect_matrix(m, t)
+
+    def samples(self, seed: int | None = None, alias_mode: int = L.ALIAS_PARALLEL, wsum: float = 0.0,
+                shots: int | None = None) -> np.ndarray:
+        """Alias-method shots (src/result_types.jl:56-80), drawn on the GPU from
+        Philox-4x32-10 uniforms keyed by ``seed``."""
+        n = self.shots if shots is None else int(shots)
+        L.call("bsim_alias_build", self._h, int(alias_mode), float(wsum))
+        out = np.empty(n, dtype=np.int64)
+        L.call("bsim_sample", self._h, n, C.c_uint64(_seed() if seed is None else int(seed) & (2 ** 64 - 1)), L.i64ptr(out))
+        return out
+
+    def alias_table(self):
+        n = 2 ** self.qubit_count
+        ap = np.empty(n, dtype=np.float64)
+        al = np.empty(n, dtype=np.int32)
+        L.call("bsim_alias_download", self._h, ap.ctypes.data_as(C.POINTER(C.c_double)),
+               al.ctypes.data_as(C.POINTER(C.c_int32)))
+        return ap, al
+
+    def sample_from_draws(self, index: np.ndarray, u: np.ndarray) -> np.ndarray:
+        index = np.ascontiguousarray(index, dtype=np.int64)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        out = np.empty(index.size, dtype=np.int64)
+        L.call("bsim_sample_from_draws", self._h, index.size, L.i64ptr(index),
+               u.ctypes.data_as(C.POINTER(C.c_double)), L.i64ptr(out))
+        return out
+
+    # -- observables ------------------------------------------------------------
+    def apply_observables(self, observables):
+        """apply_observables! (src/sv_simulator.jl:256-265): diagonalizing gates
+        on a copy of the state."""
+        if self._observed is not None:
+            raise RuntimeError("observables have already been applied.")
+        ops = []
+        for obs, targets in observables:
+            ops.extend(diagonalizing_gates(obs, list(targets)))
+        cp = type(self)(self.qubit_count, self.shots)
+        L.call("bsim_state_copy", cp._h, self._h)
+        cp.evolve(ops)
+        self._observed = cp
+        return self
+
+    def state_with_observables(self):
+        if self._observed is None:
+            raise RuntimeError("observables have not been applied.")
+        return self._observed._result_state()
+
+
+def diagonalizing_gates(obs: Observable, targets):
+    """src/observables.jl:220-241."""
+    if isinstance(obs, IObs) or isinstance(obs, ZObs):
+        return []
+    if isinstance(obs, HObs):
+        return [Instruction(Ry(-np.pi / 4.0), t) for t in targets]
+    if isinstance(obs, XObs):
+        return [Instruction(HGate(), t) for t in targets]
+    if isinstance(obs, YObs):
+        return [Instruction(Unitary(1 / np.sqrt(2) * np.array([[1.0, -1j], [1.0, 1j]])), t) for t in targets]
+    if isinstance(obs, HermitianObservable):
+        vecs = np.linalg.eigh(obs.mat)[1]
+        if obs.mat.shape[0] == 2 ** len(targets):
+            return [Instruction(Unitary(vecs), targets)]
+        if obs.mat.shape[0] == 2 and len(targets) > 1:
+            return [Instruction(Unitary(vecs), t) for t in targets]
+    if isinstance(obs, TensorProduct):
+        out = []
+        for f, t in zip(obs.factors, targets):
+            out.extend(diagonalizing_gates(f, [t] if np.ndim(t) == 0 else list(t)))
+        return out
+    raise TypeError(f"no diagonalizing gates for {obs!r}")
+
+
+class StateVectorSimulator(_Simulator):
+    """StateVectorSimulator{ComplexF64, <device vector>} (src/sv_simulator.jl:6-42)."""
+
+    kind = L.BSIM_SV
+    device_id = "braket_sv_v2"
+
+    def state_vector(self) -> np.ndarray:
+        return self._download()
+
+    _result_state = state_vector
+
+    def apply_gate(self, gate: Gate, *targets):
+        """apply_gate!(gate, state_vector, targets...) -- one kernel launch."""
+        call = kernel_call(gate, targets)
+        self._apply_call(call, call.targets, 0)
+        return self
+
+    def density_matrix(self) -> np.ndarray:
+        v = self.state_vector()
+        return np.outer(v, np.conj(v))
+
+    def __repr__(self):
+        return f"StateVectorSimulator(qubit_count={self.qubit_count}, shots={self.shots})"
+
+
+class DensityMatrixSimulator(_Simulator):
+    """DensityMatrixSimulator{ComplexF64, <device matrix>} (src/dm_simulator.jl:6-40).
+
+    ``noise_mode`` "reference" reproduces apply_noise!(::Kraus) exactly
+    (only row/column blocks that agree on the non-target bits are updated,
+    src/noise_kernels.jl:105-196); "physical" applies the channel to every
+    block."""
+
+    kind = L.BSIM_DM
+    device_id = "braket_dm_v2"
+
+    def __init__(self, qubit_count: int, shots: int = 0, noise_mode: str = "reference"):
+        super().__init__(qubit_count, shots)
+        if noise_mode not in ("reference", "physical"):
+            raise ValueError("noise_mode is 'reference' or 'physical'")
+        self.noise_mode = noise_mode
+
+    def similar(self, shots=None):
+        return type(self)(self.qubit_count, self.shots if shots is None else shots, self.noise_mode)
+
+    def density_matrix(self) -> np.ndarray:
+        """The stored (column-major) matrix, as the reference returns it."""
+        n = 2 ** self.qubit_count
+        return self._download().reshape((n, n), order="F")
+
+    _result_state = density_matrix
+
+    def apply_gate(self, gate: Gate, *targets):
+        """_evolve_op!(dms, gate, targets...) (src/dm_simulator.jl:130-140):
+        U on the column bits, then conj(U) on the row bits of vec(rho)."""
+        call = kernel_call(gate, targets)
+        n = self.qubit_count
+        self._apply_call(call, list(call.targets), 0)
+        call2 = KernelCall(list(call.targets), call.matrix, list(call.controls), list(call.cvals), call.diagonal,
+                           call.double_excitation)
+        call2.controls = [c + n for c in call.controls]
+        call2.targets = [t + n for t in call.targets]
+        self._apply_call(call2, call2.targets, 1)
+        return self
+
+    def apply_noise(self, noise: Noise, *targets):
+        """apply_noise! (src/noise_kernels.jl:7-23, :102-196)."""
+        t = np.array(targets, dtype=np.int32)
+        if isinstance(noise, PhaseFlip):
+            L.call("bsim_dm_phaseflip", self._h, int(t[0]), float(noise.probability))
+            return self
+        ks = noise.kraus_rep()
+        flat = np.concatenate([L.cmat(k).ravel(order="F") for k in ks])
+        L.call("bsim_dm_kraus", self._h, len(t), L.iptr(t), len(ks), L.dptr(flat),
+               1 if self.noise_mode == "physical" else 0)
+        return self
+
+    def __repr__(self):
+        return f"DensityMatrixSimulator(qubit_count={self.qubit_count}, shots={self.shots})"

@@ -1,0 +1,81 @@
+// Shared definitions of libbsim: the state handle, error reporting and the
+// complex128 device arithmetic every kernel uses.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "bsim.h"
+
+namespace bsim {
+
+// ---- error handling --------------------------------------------------
+
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define BSIM_HIP(call)                                  \
+    do {                                                \
+        hipError_t _e = (call);                         \
+        if (_e != hipSuccess) return ::bsim::hip_fail(_e, #call); \
+    } while (0)
+
+// ---- state handle ----------------------------------------------------
+
+struct Workspace {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace bsim
+
+struct bsim_state {
+    int nq = 0;          // reference qubit count
+    int kind = BSIM_SV;  // BSIM_SV / BSIM_DM
+    int vq = 0;          // vector qubits: nq (SV) or 2 nq (DM)
+    int64_t len = 0;     // 2^vq complex elements
+    double2* d = nullptr;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    // reduction partials, gate scratch copy, alias table, planner buffers
+    bsim::Workspace red, res, scratch, ap, alias, alias_work, ops, lam;
+    bool alias_ready = false;
+};
+
+namespace bsim {
+
+int ensure_ws(Workspace& w, size_t bytes);
+void free_ws(Workspace& w);
+int check_state(const bsim_state* h);
+int device_cu_count();
+
+// ---- device helpers ----------------------------------------------------
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cfma(double2 a, double2 b, double2 c) {  // a*b + c
+    return make_double2(fma(a.x, b.x, fma(-a.y, b.y, c.x)), fma(a.x, b.y, fma(a.y, b.x, c.y)));
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+
+// Insert a zero bit at `bit` (src/gate_kernels.jl:34-38).
+__host__ __device__ __forceinline__ uint64_t pad_bit(uint64_t ix, int bit) {
+    uint64_t lo = ix & ((1ull << bit) - 1);
+    return ((ix - lo) << 1) | lo;
+}
+
+// Host-side helpers ---------------------------------------------------------
+inline int64_t grid_for(int64_t work, int block, int64_t cap) {
+    int64_t g = (work + block - 1) / block;
+    if (g > cap) g = cap;
+    if (g < 1) g = 1;
+    return g;
+}
+
+}  // namespace bsim

@@ -1,0 +1,92 @@
+// Fused evolution: instruction stream -> elementary vector ops -> planned
+// HBM passes (LDS tiles) -> register sweeps.  Shared with the adjoint sweep.
+#pragma once
+
+#include <vector>
+
+#include "common.h"
+#include "gates.h"
+
+namespace bsim {
+
+enum EKind { E_DENSE = 0, E_DIAG = 1, E_KRAUS_REF = 2 };
+
+// One elementary op on the 2^vq vector.
+struct EOp {
+    int kind = E_DENSE;
+    GateSpec g;                  // dense / diag description (vector bits)
+    // Kraus (reference semantics): channel qubits + matrices, applied by its own kernel
+    int kq = 0;
+    int ktargets[8] = {0};
+    int nkraus = 0;
+    std::vector<double2> kraus;
+    // differentiation (set by the adjoint path)
+    int nparams = 0;
+    int pidx[3] = { -1, -1, -1 };
+    std::vector<double2> dm[3];
+    uint64_t touched() const;    // every vector bit the op reads or writes
+    bool fusable() const;        // fits a tile sweep (dense k <= 2, diag k <= 4)
+};
+
+// Translate bsim_op instructions into elementary ops on the vector.
+// DM gates become U on column bits + conj(U) on row bits; physical Kraus
+// becomes one superoperator; PhaseFlip a 2-bit diagonal.
+int lower_ops(int nq, int kind, const bsim_op* ops, int n_ops, std::vector<EOp>& out, bool keep_params);
+
+// Device-side tile program ------------------------------------------------
+
+struct TOp {
+    int32_t kind;      // 0 dense1, 1 dense2, 2 diag
+    int32_t a, b;      // dense: register bits (a < b for dense2)
+    int32_t nd;        // diag: bit count
+    int32_t dbit[4];   // diag: vector bit of diagonal index bit i
+    uint64_t cmask, cval;
+    double2 m[16];
+};
+
+struct Sweep {
+    int32_t sb[4];     // local bit indices of the 4 register bits, ascending
+    int32_t op_begin, op_end;
+};
+
+struct Pass {
+    int32_t T;                // tile bits
+    int32_t nsweeps;
+    int32_t sweep_begin;
+    int32_t lbits[16];        // vector bit of local bit i (ascending)
+    uint64_t lmask;
+};
+
+struct Plan {
+    // sequence of steps: a step is either a tile pass or a single op launch
+    struct Step {
+        bool tile;
+        int pass;              // index into passes
+        int op;                // index into eops (single launch)
+    };
+    std::vector<Step> steps;
+    std::vector<Pass> passes;
+    std::vector<Sweep> sweeps;
+    std::vector<TOp> tops;
+    std::vector<int> top_src;  // eop index each TOp came from
+    int64_t fused_ops = 0;
+};
+
+struct TileConfig {
+    int tile_bits = 12;
+    int low_bits = 4;
+};
+TileConfig& tile_config();
+
+int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan& plan);
+
+// Launch a plan on d (ops buffer reused).  `eops` provides single launches.
+int run_plan(bsim_state* h, double2* d, const std::vector<EOp>& eops, const Plan& plan);
+
+// Single elementary op launch (per-gate path).
+int launch_eop(bsim_state* h, double2* d, const EOp& e);
+
+// Kraus channel with reference (block-diagonal) semantics.
+int launch_kraus_ref(bsim_state* h, double2* d, const EOp& e);
+
+}  // namespace bsim

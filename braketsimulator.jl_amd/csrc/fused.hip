@@ -1,0 +1,685 @@
+// Fused evolution (evolve!, src/sv_simulator.jl:140-148 and
+// src/dm_simulator.jl:170-179): the instruction stream is planned on the
+// host into HBM passes.  A pass loads a tile of 2^T amplitudes -- the low
+// `low_bits` vector bits (contiguous HBM runs) plus up to T-low_bits other
+// bits chosen by the planner -- into LDS, applies every planned gate whose
+// target bits lie in the tile, and writes the tile back: one HBM read and
+// one write for many gates.  Inside a pass gates are grouped in sweeps of 4
+// register bits: each thread pulls a 16-amplitude sub-cube from LDS into
+// registers, applies all gates of the sweep there, and pushes it back.
+// Diagonal gates and controls read the global index and need no tile bit.
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "fused.h"
+
+namespace bsim {
+
+TileConfig& tile_config() {
+    static TileConfig cfg;
+    return cfg;
+}
+
+uint64_t EOp::touched() const {
+    if (kind == E_KRAUS_REF) return ~0ull;
+    uint64_t t = g.cmask;
+    for (int b = 0; b < g.k; ++b) t |= 1ull << g.tb[b];
+    return t;
+}
+
+bool EOp::fusable() const {
+    if (kind == E_DENSE) return g.k <= 2;
+    if (kind == E_DIAG) return g.k <= 4;
+    return false;
+}
+
+static bool is_diagonal(const double* m, int k) {
+    int D = 1 << k;
+    for (int c = 0; c < D; ++c)
+        for (int r = 0; r < D; ++r)
+            if (r != c && (m[2 * (r + D * c)] != 0.0 || m[2 * (r + D * c) + 1] != 0.0)) return false;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// lowering
+// ---------------------------------------------------------------------------
+
+int lower_ops(int nq, int kind, const bsim_op* ops, int n_ops, std::vector<EOp>& out, bool keep_params) {
+    const int vq = kind == BSIM_SV ? nq : 2 * nq;
+    out.clear();
+    out.reserve((size_t)n_ops * (kind == BSIM_SV ? 1 : 2));
+    std::vector<double> tmp;
+    for (int i = 0; i < n_ops; ++i) {
+        const bsim_op& op = ops[i];
+        int k = op.n_targets, nc = op.n_controls;
+        if (k < 1 || k > BSIM_MAX_TARGETS) return fail(BSIM_EINVAL, "op: n_targets must be 1..8");
+        if (nc < 0 || nc > BSIM_MAX_CONTROLS) return fail(BSIM_EINVAL, "op: n_controls must be 0..8");
+        if (!op.matrix) return fail(BSIM_EINVAL, "op: matrix is null");
+        for (int b = 0; b < k; ++b)
+            if (op.targets[b] < 0 || op.targets[b] >= nq) return fail(BSIM_EINVAL, "op: target out of range");
+        for (int c = 0; c < nc; ++c)
+            if (op.controls[c] < 0 || op.controls[c] >= nq) return fail(BSIM_EINVAL, "op: control out of range");
+        if (op.type == BSIM_OP_GATE || op.type == BSIM_OP_DIAG) {
+            bool diag = op.type == BSIM_OP_DIAG;
+            const double* m = op.matrix;
+            if (!diag && k <= 4 && is_diagonal(m, k)) {
+                int D = 1 << k;
+                tmp.resize(2 * (size_t)D);
+                for (int j = 0; j < D; ++j) {
+                    tmp[2 * j] = m[2 * (j + D * j)];
+                    tmp[2 * j + 1] = m[2 * (j + D * j) + 1];
+                }
+                m = tmp.data();
+                diag = true;
+            }
+            int passes = kind == BSIM_SV ? 1 : 2;
+            for (int p = 0; p < passes; ++p) {
+                // DM: pass 0 = U on the column bits (vector qubit q), pass 1 =
+                // conj(U) on the row bits (vector qubit nq + q), dm_simulator.jl:136-138
+                int t[8], c[8];
+                for (int b = 0; b < k; ++b) t[b] = op.targets[b] + (p ? nq : 0);
+                for (int b = 0; b < nc; ++b) c[b] = op.controls[b] + (p ? nq : 0);
+                EOp e;
+                e.kind = diag ? E_DIAG : E_DENSE;
+                if (int rc = make_spec(vq, k, t, nc, c, op.control_values, m, diag, p == 1, false, e.g)) return rc;
+                if (keep_params && p == 0) {
+                    e.nparams = op.n_params;
+                    if (e.nparams < 0 || e.nparams > 3) return fail(BSIM_EINVAL, "op: n_params must be 0..3");
+                    int D = 1 << k;
+                    for (int j = 0; j < e.nparams; ++j) {
+                        e.pidx[j] = op.param_index[j];
+                        if (!op.dmatrix[j]) return fail(BSIM_EINVAL, "op: dmatrix is null");
+                        e.dm[j].resize((size_t)D * D);
+                        for (int x = 0; x < D * D; ++x)
+                            e.dm[j][x] = make_double2(op.dmatrix[j][2 * x], op.dmatrix[j][2 * x + 1]);
+                    }
+                }
+                out.push_back(std::move(e));
+            }
+        } else if (op.type == BSIM_OP_KRAUS_REF || op.type == BSIM_OP_KRAUS_PHYS || op.type == BSIM_OP_PHASEFLIP) {
+            if (kind != BSIM_DM) return fail(BSIM_EINVAL, "noise ops need a density matrix");
+            if (nc) return fail(BSIM_EINVAL, "noise ops take no controls");
+            uint64_t used = 0;
+            for (int b = 0; b < k; ++b) {
+                if (used & (1ull << op.targets[b])) return fail(BSIM_EINVAL, "op: repeated target");
+                used |= 1ull << op.targets[b];
+            }
+            if (op.type == BSIM_OP_PHASEFLIP) {
+                if (k != 1) return fail(BSIM_EINVAL, "PhaseFlip acts on one qubit");
+                double p = op.matrix[0];
+                double same = (1.0 - p) + p, diff = (1.0 - p) - p;  // noise_kernels.jl:18-20
+                double d[8] = { same, 0, diff, 0, diff, 0, same, 0 };
+                int t[2] = { op.targets[0] + nq, op.targets[0] };  // (row bit, column bit)
+                EOp e;
+                e.kind = E_DIAG;
+                if (int rc = make_spec(vq, 2, t, 0, nullptr, nullptr, d, true, false, false, e.g)) return rc;
+                out.push_back(std::move(e));
+            } else if (op.type == BSIM_OP_KRAUS_REF) {
+                if (k > 4) return fail(BSIM_ENOTSUP, "Kraus channels act on at most 4 qubits");
+                if (op.n_kraus < 1) return fail(BSIM_EINVAL, "Kraus channel without matrices");
+                EOp e;
+                e.kind = E_KRAUS_REF;
+                e.kq = k;
+                for (int b = 0; b < k; ++b) e.ktargets[b] = op.targets[b];
+                e.nkraus = op.n_kraus;
+                size_t D = (size_t)1 << k, n = (size_t)op.n_kraus * D * D;
+                e.kraus.resize(n);
+                for (size_t x = 0; x < n; ++x) e.kraus[x] = make_double2(op.matrix[2 * x], op.matrix[2 * x + 1]);
+                out.push_back(std::move(e));
+            } else {
+                // physical channel: superoperator sum_m K_m (column bits) (x) conj(K_m) (row bits)
+                if (k > 3) return fail(BSIM_ENOTSUP, "physical Kraus channels act on at most 3 qubits");
+                if (op.n_kraus < 1) return fail(BSIM_EINVAL, "Kraus channel without matrices");
+                int D = 1 << k, DD = D * D;
+                // channel matrix bit b -> qubit: k=1 (q); k=2 t1 = MSB (noise_kernels.jl:145-155);
+                // k>2 sorted endian targets (largest qubit number first)
+                int qb[8];
+                if (k == 1) qb[0] = op.targets[0];
+                else if (k == 2) { qb[0] = op.targets[1]; qb[1] = op.targets[0]; }
+                else {
+                    for (int b = 0; b < k; ++b) qb[b] = op.targets[b];
+                    std::sort(qb, qb + k, [](int a, int b) { return a > b; });
+                }
+                std::vector<double> sup(2 * (size_t)DD * DD, 0.0);
+                for (int m = 0; m < op.n_kraus; ++m) {
+                    const double* K = op.matrix + 2 * (size_t)m * DD;
+                    for (int Jc = 0; Jc < DD; ++Jc)
+                        for (int Jr = 0; Jr < DD; ++Jr) {
+                            int r = Jr % D, c = Jr / D, rp = Jc % D, cp = Jc / D;
+                            // Sup[(r,c),(r',c')] += conj(K[r,r']) K[c,c']
+                            double ar = K[2 * (r + D * rp)], ai = -K[2 * (r + D * rp) + 1];
+                            double br = K[2 * (c + D * cp)], bi = K[2 * (c + D * cp) + 1];
+                            size_t idx = 2 * ((size_t)Jr + (size_t)DD * Jc);
+                            sup[idx] += ar * br - ai * bi;
+                            sup[idx + 1] += ar * bi + ai * br;
+                        }
+                }
+                int t[8];
+                for (int b = 0; b < k; ++b) {
+                    t[b] = qb[b] + nq;  // row bit of the channel's matrix bit b
+                    t[k + b] = qb[b];   // column bit
+                }
+                EOp e;
+                e.kind = E_DENSE;
+                if (int rc = make_spec(vq, 2 * k, t, 0, nullptr, nullptr, sup.data(), false, false, false, e.g))
+                    return rc;
+                out.push_back(std::move(e));
+            }
+        } else {
+            return fail(BSIM_EINVAL, "unknown op type");
+        }
+    }
+    return BSIM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// planner
+// ---------------------------------------------------------------------------
+
+static inline int popc(uint64_t x) { return __builtin_popcountll(x); }
+
+static uint64_t target_mask(const EOp& e) {
+    uint64_t t = 0;
+    for (int b = 0; b < e.g.k; ++b) t |= 1ull << e.g.tb[b];
+    return t;
+}
+
+int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan& plan) {
+    plan = Plan();
+    const size_t n = eops.size();
+    int T = std::min(cfg.tile_bits, vq);
+    int L = std::min(cfg.low_bits, T);
+    const uint64_t all = vq >= 64 ? ~0ull : ((1ull << vq) - 1);
+    if (T < 4) {
+        for (size_t i = 0; i < n; ++i) plan.steps.push_back({ false, -1, (int)i });
+        return BSIM_OK;
+    }
+    std::vector<char> done(n, 0);
+    size_t head = 0;
+    std::vector<int> inc;
+    while (true) {
+        while (head < n && done[head]) ++head;
+        if (head >= n) break;
+        if (!eops[head].fusable()) {
+            plan.steps.push_back({ false, -1, (int)head });
+            done[head] = 1;
+            continue;
+        }
+        // ---- pass: choose the tile bits and the ops it absorbs
+        uint64_t S = (L > 0) ? ((1ull << L) - 1) : 0;
+        uint64_t blocked = 0;
+        inc.clear();
+        for (size_t i = head; i < n; ++i) {
+            if (done[i]) continue;
+            const EOp& e = eops[i];
+            uint64_t t = e.touched();
+            if (t & blocked) {
+                blocked |= t;
+            } else if (!e.fusable()) {
+                blocked |= t;
+            } else if (e.kind == E_DENSE) {
+                uint64_t need = target_mask(e);
+                if ((need & ~S) == 0 || popc(S | need) <= T) {
+                    S |= need;
+                    inc.push_back((int)i);
+                } else {
+                    blocked |= t;
+                }
+            } else {
+                inc.push_back((int)i);
+            }
+            if ((blocked & all) == all) break;
+        }
+        // fill the tile with the lowest free bits (longer contiguous runs)
+        for (int b = 0; b < vq && popc(S) < T; ++b) S |= 1ull << b;
+        Pass P;
+        memset(&P, 0, sizeof(P));
+        P.T = T;
+        P.lmask = S;
+        int pos[64];
+        for (int b = 0, l = 0; b < vq; ++b)
+            if (S & (1ull << b)) {
+                P.lbits[l] = b;
+                pos[b] = l++;
+            }
+        P.sweep_begin = (int)plan.sweeps.size();
+        // ---- sweeps: groups of ops whose dense targets fit 4 register bits
+        std::vector<int> rem = inc;
+        std::vector<int> next;
+        while (!rem.empty()) {
+            uint32_t W = 0;  // local bits
+            uint64_t blk = 0;
+            std::vector<int> take;
+            next.clear();
+            for (int i : rem) {
+                const EOp& e = eops[i];
+                uint64_t t = e.touched();
+                if (t & blk) {
+                    blk |= t;
+                    next.push_back(i);
+                    continue;
+                }
+                if (e.kind == E_DENSE) {
+                    uint32_t need = 0;
+                    for (int b = 0; b < e.g.k; ++b) need |= 1u << pos[e.g.tb[b]];
+                    if (popc(W | need) <= 4) {
+                        W |= need;
+                        take.push_back(i);
+                    } else {
+                        blk |= t;
+                        next.push_back(i);
+                    }
+                } else {
+                    take.push_back(i);
+                }
+            }
+            for (int b = 0; b < T && popc(W) < 4; ++b) W |= 1u << b;
+            Sweep sw;
+            int nb = 0;
+            for (int b = 0; b < T; ++b)
+                if (W & (1u << b)) sw.sb[nb++] = b;
+            sw.op_begin = (int)plan.tops.size();
+            for (int i : take) {
+                const EOp& e = eops[i];
+                TOp op;
+                memset(&op, 0, sizeof(op));
+                op.cmask = e.g.cmask;
+                op.cval = e.g.cval;
+                if (e.kind == E_DIAG) {
+                    op.kind = 2;
+                    op.nd = e.g.k;
+                    for (int b = 0; b < e.g.k; ++b) op.dbit[b] = e.g.tb[b];
+                    for (int j = 0; j < (1 << e.g.k); ++j) op.m[j] = e.g.m[j];
+                } else {
+                    auto reg = [&](int vbit) {
+                        int l = pos[vbit];
+                        for (int r = 0; r < 4; ++r)
+                            if (sw.sb[r] == l) return r;
+                        return -1;
+                    };
+                    if (e.g.k == 1) {
+                        op.kind = 0;
+                        op.a = reg(e.g.tb[0]);
+                        for (int j = 0; j < 4; ++j) op.m[j] = e.g.m[j];
+                    } else {
+                        op.kind = 1;
+                        int a = reg(e.g.tb[0]), b = reg(e.g.tb[1]);
+                        if (a < b) {
+                            op.a = a;
+                            op.b = b;
+                            for (int j = 0; j < 16; ++j) op.m[j] = e.g.m[j];
+                        } else {
+                            op.a = b;
+                            op.b = a;
+                            auto sw2 = [](int x) { return ((x & 1) << 1) | ((x >> 1) & 1); };
+                            for (int r = 0; r < 4; ++r)
+                                for (int c = 0; c < 4; ++c) op.m[r + 4 * c] = e.g.m[sw2(r) + 4 * sw2(c)];
+                        }
+                    }
+                }
+                plan.tops.push_back(op);
+                plan.top_src.push_back(i);
+            }
+            sw.op_end = (int)plan.tops.size();
+            plan.sweeps.push_back(sw);
+            plan.fused_ops += (int64_t)take.size();
+            rem.swap(next);
+        }
+        P.nsweeps = (int)plan.sweeps.size() - P.sweep_begin;
+        for (int i : inc) done[i] = 1;
+        plan.steps.push_back({ true, (int)plan.passes.size(), -1 });
+        plan.passes.push_back(P);
+    }
+    return BSIM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// tile kernel
+// ---------------------------------------------------------------------------
+
+constexpr int TB = 256;
+
+__device__ __forceinline__ int phys(int j) { return j + (j >> 4); }
+
+__device__ __forceinline__ uint64_t gsum(int j, const uint64_t (&gb)[4]) {
+    return ((j & 1) ? gb[0] : 0) | ((j & 2) ? gb[1] : 0) | ((j & 4) ? gb[2] : 0) | ((j & 8) ? gb[3] : 0);
+}
+
+template <int A>
+__device__ __forceinline__ void sw_dense1(double2 (&v)[16], const TOp* __restrict__ op, uint64_t gbase,
+                                          const uint64_t (&gb)[4]) {
+    const double2 m00 = op->m[0], m10 = op->m[1], m01 = op->m[2], m11 = op->m[3];
+    const uint64_t cm = op->cmask, cv = op->cval;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        if (j & (1 << A)) continue;
+        const int k = j | (1 << A);
+        const double2 a = v[j], b = v[k];
+        double2 x = cfma(m01, b, cmul(m00, a));
+        double2 y = cfma(m11, b, cmul(m10, a));
+        if (cm) {
+            bool on = ((gbase | gsum(j, gb)) & cm) == cv;
+            x = on ? x : a;
+            y = on ? y : b;
+        }
+        v[j] = x;
+        v[k] = y;
+    }
+}
+
+template <int A, int B>
+__device__ __forceinline__ void sw_dense2(double2 (&v)[16], const TOp* __restrict__ op, uint64_t gbase,
+                                          const uint64_t (&gb)[4]) {
+    double2 m[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) m[i] = op->m[i];
+    const uint64_t cm = op->cmask, cv = op->cval;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        if (j & ((1 << A) | (1 << B))) continue;
+        const int id[4] = { j, j | (1 << A), j | (1 << B), j | (1 << A) | (1 << B) };
+        double2 a[4] = { v[id[0]], v[id[1]], v[id[2]], v[id[3]] };
+        double2 r[4];
+#pragma unroll
+        for (int row = 0; row < 4; ++row) {
+            double2 acc = cmul(m[row], a[0]);
+#pragma unroll
+            for (int c = 1; c < 4; ++c) acc = cfma(m[row + 4 * c], a[c], acc);
+            r[row] = acc;
+        }
+        bool on = true;
+        if (cm) on = ((gbase | gsum(j, gb)) & cm) == cv;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[id[q]] = on ? r[q] : a[q];
+    }
+}
+
+__device__ __forceinline__ void sw_diag(double2 (&v)[16], const TOp* __restrict__ op, uint64_t gbase,
+                                        const uint64_t (&gb)[4]) {
+    const int nd = op->nd;
+    const int d0 = op->dbit[0], d1 = op->dbit[1], d2 = op->dbit[2], d3 = op->dbit[3];
+    const uint64_t cm = op->cmask, cv = op->cval;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        uint64_t g = gbase | gsum(j, gb);
+        int idx = (int)((g >> d0) & 1);
+        if (nd > 1) idx |= (int)((g >> d1) & 1) << 1;
+        if (nd > 2) idx |= (int)((g >> d2) & 1) << 2;
+        if (nd > 3) idx |= (int)((g >> d3) & 1) << 3;
+        double2 f = op->m[idx];
+        double2 x = cmul(f, v[j]);
+        if (cm) x = ((g & cm) == cv) ? x : v[j];
+        v[j] = x;
+    }
+}
+
+__device__ __forceinline__ void run_op(double2 (&v)[16], const TOp* __restrict__ op, uint64_t gbase,
+                                       const uint64_t (&gb)[4]) {
+    switch (op->kind * 16 + op->a * 4 + op->b) {
+        case 0 * 16 + 0: sw_dense1<0>(v, op, gbase, gb); break;
+        case 0 * 16 + 4: sw_dense1<1>(v, op, gbase, gb); break;
+        case 0 * 16 + 8: sw_dense1<2>(v, op, gbase, gb); break;
+        case 0 * 16 + 12: sw_dense1<3>(v, op, gbase, gb); break;
+        case 16 + 0 * 4 + 1: sw_dense2<0, 1>(v, op, gbase, gb); break;
+        case 16 + 0 * 4 + 2: sw_dense2<0, 2>(v, op, gbase, gb); break;
+        case 16 + 0 * 4 + 3: sw_dense2<0, 3>(v, op, gbase, gb); break;
+        case 16 + 1 * 4 + 2: sw_dense2<1, 2>(v, op, gbase, gb); break;
+        case 16 + 1 * 4 + 3: sw_dense2<1, 3>(v, op, gbase, gb); break;
+        case 16 + 2 * 4 + 3: sw_dense2<2, 3>(v, op, gbase, gb); break;
+        default: sw_diag(v, op, gbase, gb); break;
+    }
+}
+
+__global__ __launch_bounds__(TB) void k_tile(double2* __restrict__ sv, int vq, const Pass* __restrict__ pass,
+                                             const Sweep* __restrict__ sweeps, const TOp* __restrict__ tops) {
+    extern __shared__ double2 tile[];
+    const int T = pass->T;
+    const int nloc = 1 << T;
+    const uint64_t lmask = pass->lmask;
+    // global base: deposit the block index into the non-tile bits
+    uint64_t w = blockIdx.x, base = 0;
+    for (int bit = 0, k = 0; bit < vq; ++bit)
+        if (!((lmask >> bit) & 1)) {
+            base |= ((w >> k) & 1) << bit;
+            ++k;
+        }
+    // local j = tid + TB*i: the low 8 local bits come from tid
+    uint64_t s_t = 0;
+    const int tlow = T < 8 ? T : 8;
+    for (int b = 0; b < tlow; ++b)
+        if ((threadIdx.x >> b) & 1) s_t |= 1ull << pass->lbits[b];
+    for (int i = 0; (i << 8) < nloc; ++i) {
+        int j = (int)threadIdx.x + (i << 8);
+        if (j < nloc) {
+            uint64_t s_i = 0;
+            for (int b = 8; b < T; ++b)
+                if ((i >> (b - 8)) & 1) s_i |= 1ull << pass->lbits[b];
+            tile[phys(j)] = sv[base | s_t | s_i];
+        }
+    }
+    __syncthreads();
+    const int nsub = nloc >> 4;
+    for (int s = 0; s < pass->nsweeps; ++s) {
+        const Sweep* sw = sweeps + pass->sweep_begin + s;
+        const int sb0 = sw->sb[0], sb1 = sw->sb[1], sb2 = sw->sb[2], sb3 = sw->sb[3];
+        const int smask = (1 << sb0) | (1 << sb1) | (1 << sb2) | (1 << sb3);
+        const uint64_t gb[4] = { 1ull << pass->lbits[sb0], 1ull << pass->lbits[sb1], 1ull << pass->lbits[sb2],
+                                 1ull << pass->lbits[sb3] };
+        const int ob = sw->op_begin, oe = sw->op_end;
+        for (int c = threadIdx.x; c < nsub; c += TB) {
+            int lb = 0;
+            for (int bit = 0, k = 0; bit < T; ++bit)
+                if (!((smask >> bit) & 1)) {
+                    lb |= ((c >> k) & 1) << bit;
+                    ++k;
+                }
+            uint64_t gbase = base;
+            for (int bit = 0; bit < T; ++bit)
+                if ((lb >> bit) & 1) gbase |= 1ull << pass->lbits[bit];
+            double2 v[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                int off = lb | ((j & 1) << sb0) | (((j >> 1) & 1) << sb1) | (((j >> 2) & 1) << sb2) |
+                          (((j >> 3) & 1) << sb3);
+                v[j] = tile[phys(off)];
+            }
+            for (int o = ob; o < oe; ++o) run_op(v, tops + o, gbase, gb);
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                int off = lb | ((j & 1) << sb0) | (((j >> 1) & 1) << sb1) | (((j >> 2) & 1) << sb2) |
+                          (((j >> 3) & 1) << sb3);
+                tile[phys(off)] = v[j];
+            }
+        }
+        __syncthreads();
+    }
+    for (int i = 0; (i << 8) < nloc; ++i) {
+        int j = (int)threadIdx.x + (i << 8);
+        if (j < nloc) {
+            uint64_t s_i = 0;
+            for (int b = 8; b < T; ++b)
+                if ((i >> (b - 8)) & 1) s_i |= 1ull << pass->lbits[b];
+            sv[base | s_t | s_i] = tile[phys(j)];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// execution
+// ---------------------------------------------------------------------------
+
+int launch_eop(bsim_state* h, double2* d, const EOp& e) {
+    if (e.kind == E_KRAUS_REF) return launch_kraus_ref(h, d, e);
+    return launch_gate(d, h->vq, e.g, h->scratch, h->stream);
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// ---- optional kernel timing (HIP events on the launching stream) ----------
+struct Profiler {
+    bool on = false;
+    int64_t launches = 0;
+    double total_ms = 0;
+    std::vector<hipEvent_t> pool;  // reused start/stop pairs
+    size_t used = 0;
+};
+static Profiler& prof() {
+    static Profiler p;
+    return p;
+}
+
+static int prof_mark(hipStream_t s, hipEvent_t* out) {
+    Profiler& p = prof();
+    if (p.used == p.pool.size()) {
+        hipEvent_t e;
+        BSIM_HIP(hipEventCreate(&e));
+        p.pool.push_back(e);
+    }
+    *out = p.pool[p.used++];
+    BSIM_HIP(hipEventRecord(*out, s));
+    return BSIM_OK;
+}
+
+// after the stream has drained: fold the recorded pairs into the totals
+static int prof_collect() {
+    Profiler& p = prof();
+    for (size_t i = 0; i + 1 < p.used; i += 2) {
+        float ms = 0;
+        BSIM_HIP(hipEventElapsedTime(&ms, p.pool[i], p.pool[i + 1]));
+        p.total_ms += ms;
+        p.launches += 1;
+    }
+    p.used = 0;
+    return BSIM_OK;
+}
+
+int run_plan(bsim_state* h, double2* d, const std::vector<EOp>& eops, const Plan& plan) {
+    size_t off_p = 0;
+    size_t off_s = align256(sizeof(Pass) * plan.passes.size());
+    size_t off_o = off_s + align256(sizeof(Sweep) * plan.sweeps.size());
+    size_t total = off_o + align256(sizeof(TOp) * plan.tops.size()) + 256;
+    if (!plan.passes.empty()) {
+        if (int rc = ensure_ws(h->ops, total)) return rc;
+        char* base = (char*)h->ops.ptr;
+        BSIM_HIP(hipMemcpyAsync(base + off_p, plan.passes.data(), sizeof(Pass) * plan.passes.size(),
+                                hipMemcpyHostToDevice, h->stream));
+        BSIM_HIP(hipMemcpyAsync(base + off_s, plan.sweeps.data(), sizeof(Sweep) * plan.sweeps.size(),
+                                hipMemcpyHostToDevice, h->stream));
+        if (!plan.tops.empty())
+            BSIM_HIP(hipMemcpyAsync(base + off_o, plan.tops.data(), sizeof(TOp) * plan.tops.size(),
+                                    hipMemcpyHostToDevice, h->stream));
+    }
+    char* base = (char*)h->ops.ptr;
+    for (const Plan::Step& st : plan.steps) {
+        if (st.tile) {
+            const Pass& P = plan.passes[st.pass];
+            unsigned grid = (unsigned)(1ull << (h->vq - P.T));
+            size_t shmem = sizeof(double2) * (size_t)((1 << P.T) + ((1 << P.T) >> 4));
+            hipEvent_t e0, e1;
+            if (prof().on)
+                if (int rc = prof_mark(h->stream, &e0)) return rc;
+            hipLaunchKernelGGL(k_tile, dim3(grid), dim3(TB), shmem, h->stream, d, h->vq,
+                               (const Pass*)(base + off_p) + st.pass, (const Sweep*)(base + off_s),
+                               (const TOp*)(base + off_o));
+            BSIM_HIP(hipGetLastError());
+            if (prof().on)
+                if (int rc = prof_mark(h->stream, &e1)) return rc;
+        } else {
+            if (int rc = launch_eop(h, d, eops[st.op])) return rc;
+        }
+    }
+    return BSIM_OK;
+}
+
+static bool g_shmem_set = false;
+
+static int prepare_kernel_attrs() {
+    if (g_shmem_set) return BSIM_OK;
+    // allow > 64 KiB of dynamic LDS for 13-bit tiles
+    BSIM_HIP(hipFuncSetAttribute((const void*)k_tile, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    g_shmem_set = true;
+    return BSIM_OK;
+}
+
+}  // namespace bsim
+
+using namespace bsim;
+
+extern "C" {
+
+int bsim_set_tile_config(int tile_bits, int low_bits) {
+    TileConfig& c = tile_config();
+    if (tile_bits) {
+        if (tile_bits < 6 || tile_bits > 13) return fail(BSIM_EINVAL, "tile_bits must be 6..13");
+        c.tile_bits = tile_bits;
+    }
+    if (low_bits) {
+        if (low_bits < 0 || low_bits > c.tile_bits) return fail(BSIM_EINVAL, "low_bits must be 0..tile_bits");
+        c.low_bits = low_bits;
+    }
+    if (c.low_bits > c.tile_bits) c.low_bits = c.tile_bits;
+    return BSIM_OK;
+}
+
+int bsim_get_tile_config(int* tile_bits, int* low_bits) {
+    if (tile_bits) *tile_bits = tile_config().tile_bits;
+    if (low_bits) *low_bits = tile_config().low_bits;
+    return BSIM_OK;
+}
+
+int bsim_set_profiling(int enable) {
+    prof().on = enable != 0;
+    return BSIM_OK;
+}
+
+int bsim_kernel_stats(int reset, int64_t* launches, double* total_ms) {
+    if (launches) *launches = prof().launches;
+    if (total_ms) *total_ms = prof().total_ms;
+    if (reset) {
+        prof().launches = 0;
+        prof().total_ms = 0;
+    }
+    return BSIM_OK;
+}
+
+int bsim_plan_stats(int vector_qubits, int is_dm, const bsim_op* ops, int n_ops, int64_t* stats) {
+    if (!stats || (n_ops > 0 && !ops)) return fail(BSIM_EINVAL, "null argument");
+    int nq = is_dm ? vector_qubits / 2 : vector_qubits;
+    std::vector<EOp> eops;
+    if (int rc = lower_ops(nq, is_dm ? BSIM_DM : BSIM_SV, ops, n_ops, eops, false)) return rc;
+    Plan plan;
+    if (int rc = make_plan(is_dm ? 2 * nq : nq, eops, tile_config(), plan)) return rc;
+    int64_t singles = 0;
+    for (auto& s : plan.steps) singles += s.tile ? 0 : 1;
+    stats[0] = (int64_t)plan.passes.size();
+    stats[1] = singles;
+    stats[2] = (int64_t)plan.sweeps.size();
+    stats[3] = plan.fused_ops;
+    return BSIM_OK;
+}
+
+int bsim_evolve(bsim_state* h, const bsim_op* ops, int n_ops, int mode) {
+    if (int rc = check_state(h)) return rc;
+    if (n_ops < 0 || (n_ops > 0 && !ops)) return fail(BSIM_EINVAL, "bad op list");
+    std::vector<EOp> eops;
+    if (int rc = lower_ops(h->nq, h->kind, ops, n_ops, eops, false)) return rc;
+    h->alias_ready = false;
+    if (mode == 1) {
+        for (const EOp& e : eops)
+            if (int rc = launch_eop(h, h->d, e)) return rc;
+    } else {
+        if (int rc = prepare_kernel_attrs()) return rc;
+        Plan plan;
+        if (int rc = make_plan(h->vq, eops, tile_config(), plan)) return rc;
+        if (int rc = run_plan(h, h->d, eops, plan)) return rc;
+    }
+    BSIM_HIP(hipStreamSynchronize(h->stream));
+    if (prof().on)
+        if (int rc = prof_collect()) return rc;
+    return BSIM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,386 @@
+// Per-gate kernels: one HBM pass per gate, the drop-in for the reference's
+// apply_gate! / apply_controlled_gate! loops (src/gate_kernels.jl:489-775,
+// src/custom_gates.jl:364-416).
+//
+// Kernel families:
+//  - k_1q_lane:   single-target gates on amplitude bits 0..5.  Every lane
+//                 loads one consecutive amplitude (fully coalesced 1 KiB per
+//                 wavefront load) and receives its partner from lane
+//                 (lane ^ 2^e) by DPP (e = 0, 1), ds_swizzle (e = 2..4) or
+//                 ds_bpermute (e = 5).
+//  - k_dense<K>:  K = 1, 2 targets with the matrix in kernel arguments;
+//                 threads own one amplitude group (strided pairs/quads whose
+//                 low halves are consecutive across lanes).
+//  - k_dense_lds: K = 3..6 targets; matrix and amplitude groups staged in
+//                 LDS, one output amplitude per thread.
+//  - k_diag:      diagonal gates, one coalesced pass over the control
+//                 subspace.
+#include <string.h>
+
+#include <algorithm>
+
+#include "gates.h"
+
+namespace bsim {
+
+// ---------------------------------------------------------------------------
+// spec construction
+// ---------------------------------------------------------------------------
+
+int make_spec(int vq, int k, const int* targets, int nc, const int* controls, const int* cvals, const double* m,
+              bool diag, bool conj, bool sorted, GateSpec& g) {
+    if (k < 1 || k > 8) return fail(BSIM_ENOTSUP, "gate target count must be 1..8");
+    if (nc < 0 || nc > BSIM_MAX_CONTROLS) return fail(BSIM_ENOTSUP, "at most 8 controls");
+    if (!m) return fail(BSIM_EINVAL, "matrix pointer is null");
+    uint64_t used = 0;
+    g = GateSpec();
+    g.k = k;
+    g.diag = diag;
+    for (int b = 0; b < k; ++b) {
+        int q = targets[b];
+        if (q < 0 || q >= vq) return fail(BSIM_EINVAL, "target qubit out of range");
+        int e = vq - 1 - q;
+        if (used & (1ull << e)) return fail(BSIM_EINVAL, "repeated target/control qubit");
+        used |= 1ull << e;
+        g.tb[b] = e;
+    }
+    if (sorted) std::sort(g.tb, g.tb + k);
+    for (int c = 0; c < nc; ++c) {
+        int q = controls[c];
+        if (q < 0 || q >= vq) return fail(BSIM_EINVAL, "control qubit out of range");
+        int e = vq - 1 - q;
+        if (used & (1ull << e)) return fail(BSIM_EINVAL, "repeated target/control qubit");
+        used |= 1ull << e;
+        g.cmask |= 1ull << e;
+        if (cvals == nullptr || cvals[c]) g.cval |= 1ull << e;
+    }
+    size_t n = diag ? ((size_t)1 << k) : ((size_t)1 << (2 * k));
+    g.m.resize(n);
+    for (size_t i = 0; i < n; ++i) g.m[i] = make_double2(m[2 * i], conj ? -m[2 * i + 1] : m[2 * i + 1]);
+    return BSIM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+
+struct InsArgs {
+    int nins;       // number of bits inserted (targets + controls), ascending
+    int ins[16];
+    uint64_t cval;  // control values OR-ed into every group base
+    int64_t ngroups;
+};
+
+__device__ __forceinline__ uint64_t group_base(uint64_t g, const InsArgs& a) {
+    for (int i = 0; i < a.nins; ++i) g = pad_bit(g, a.ins[i]);
+    return g | a.cval;
+}
+
+template <int K>
+struct SmallMat {
+    double2 m[1 << (2 * K)];
+};
+
+template <int K>
+__global__ __launch_bounds__(256) void k_dense(double2* __restrict__ sv, InsArgs a, SmallMat<K> M, int t0, int t1) {
+    constexpr int D = 1 << K;
+    const uint64_t b0 = 1ull << t0;
+    const uint64_t b1 = K > 1 ? (1ull << t1) : 0;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < a.ngroups; g += stride) {
+        uint64_t base = group_base((uint64_t)g, a);
+        uint64_t off[D];
+        double2 v[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            off[j] = base | ((j & 1) ? b0 : 0) | ((j & 2) ? b1 : 0);
+            v[j] = sv[off[j]];
+        }
+        double2 r[D];
+#pragma unroll
+        for (int row = 0; row < D; ++row) {
+            double2 acc = cmul(M.m[row], v[0]);
+#pragma unroll
+            for (int col = 1; col < D; ++col) acc = cfma(M.m[row + D * col], v[col], acc);
+            r[row] = acc;
+        }
+#pragma unroll
+        for (int j = 0; j < D; ++j) sv[off[j]] = r[j];
+    }
+}
+
+// Lane exchange of a double2 with lane ^ (1 << E).
+template <int E>
+__device__ __forceinline__ double2 xor_lane(double2 v) {
+    int w[4];
+    memcpy(w, &v, 16);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if constexpr (E == 0) {
+            w[i] = __builtin_amdgcn_update_dpp(0, w[i], 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+        } else if constexpr (E == 1) {
+            w[i] = __builtin_amdgcn_update_dpp(0, w[i], 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+        } else if constexpr (E <= 4) {
+            w[i] = __builtin_amdgcn_ds_swizzle(w[i], ((1 << E) << 10) | 0x1F);    // bitmask mode: xor
+        } else {
+            w[i] = __shfl_xor(w[i], 32, 64);
+        }
+    }
+    double2 r;
+    memcpy(&r, w, 16);
+    return r;
+}
+
+template <int E>
+__global__ __launch_bounds__(256) void k_1q_lane(double2* __restrict__ sv, int64_t len, uint64_t cmask, uint64_t cval,
+                                                 double2 m00, double2 m10, double2 m01, double2 m11) {
+    // len and the grid stride are multiples of 64: whole wavefronts stay
+    // active together, so the lane exchange always has its partner.
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride) {
+        double2 a = sv[i];
+        double2 b = xor_lane<E>(a);
+        if (((uint64_t)i & cmask) == cval) {
+            double2 out = ((i >> E) & 1) ? cfma(m10, b, cmul(m11, a)) : cfma(m01, b, cmul(m00, a));
+            sv[i] = out;
+        }
+    }
+}
+
+// K = 3..6: matrix (D x D) and a block's amplitude groups in LDS.
+__global__ __launch_bounds__(256) void k_dense_lds(double2* __restrict__ sv, InsArgs a, const double2* __restrict__ mg,
+                                                   int K, int4 tlo, int2 thi) {
+    extern __shared__ double2 smem[];
+    const int D = 1 << K;
+    double2* M = smem;            // D*D
+    double2* A = smem + D * D;    // 256 amplitudes (256/D groups)
+    for (int i = threadIdx.x; i < D * D; i += blockDim.x) M[i] = mg[i];
+    int tb[6] = { tlo.x, tlo.y, tlo.z, tlo.w, thi.x, thi.y };
+    const int gpb = 256 / D;  // groups per block
+    int local_g = threadIdx.x / D, r = threadIdx.x % D;
+    int64_t nblk = (a.ngroups + gpb - 1) / gpb;
+    for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+        int64_t g = blk * gpb + local_g;
+        bool valid = g < a.ngroups;
+        uint64_t off = 0;
+        if (valid) {
+            uint64_t base = group_base((uint64_t)g, a);
+            off = base;
+            for (int b = 0; b < K; ++b)
+                if ((r >> b) & 1) off |= 1ull << tb[b];
+        }
+        __syncthreads();
+        if (valid) A[threadIdx.x] = sv[off];
+        __syncthreads();
+        if (valid) {
+            const double2* col = A + local_g * D;
+            double2 acc = cmul(M[r], col[0]);
+            for (int c = 1; c < D; ++c) acc = cfma(M[r + D * c], col[c], acc);
+            sv[off] = acc;
+        }
+    }
+}
+
+// Diagonal: amplitude idx (in the control subspace) times d[j(idx)].
+__global__ __launch_bounds__(256) void k_diag(double2* __restrict__ sv, InsArgs a, const double2* __restrict__ dg,
+                                              int K, int4 tlo, int4 thi) {
+    int tb[8] = { tlo.x, tlo.y, tlo.z, tlo.w, thi.x, thi.y, thi.z, thi.w };
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < a.ngroups; g += stride) {
+        uint64_t idx = group_base((uint64_t)g, a);
+        int j = 0;
+        for (int b = 0; b < K; ++b) j |= (int)((idx >> tb[b]) & 1) << b;
+        sv[idx] = cmul(dg[j], sv[idx]);
+    }
+}
+
+// src/custom_gates.jl:390-416
+__global__ __launch_bounds__(256) void k_double_exc(double2* __restrict__ sv, int64_t ngroups, int4 ord, int4 e,
+                                                    double c, double s) {
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < ngroups; g += stride) {
+        uint64_t p = pad_bit(pad_bit(pad_bit(pad_bit((uint64_t)g, ord.x), ord.y), ord.z), ord.w);
+        uint64_t i0011 = p ^ (1ull << e.z) ^ (1ull << e.w);
+        uint64_t i1100 = p ^ (1ull << e.x) ^ (1ull << e.y);
+        double2 a = sv[i0011], b = sv[i1100];
+        sv[i0011] = make_double2(c * a.x - s * b.x, c * a.y - s * b.y);
+        sv[i1100] = make_double2(s * a.x + c * b.x, s * a.y + c * b.y);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+
+static InsArgs ins_args(int vq, const GateSpec& g, bool include_targets) {
+    InsArgs a;
+    memset(&a, 0, sizeof(a));
+    int n = 0;
+    uint64_t mask = g.cmask;
+    if (include_targets)
+        for (int b = 0; b < g.k; ++b) mask |= 1ull << g.tb[b];
+    for (int bit = 0; bit < vq; ++bit)
+        if (mask & (1ull << bit)) a.ins[n++] = bit;
+    a.nins = n;
+    a.cval = g.cval;
+    a.ngroups = (int64_t)1 << (vq - n);
+    return a;
+}
+
+static int64_t cap_blocks() { return (int64_t)device_cu_count() * 64; }
+
+int launch_gate(double2* d, int vq, const GateSpec& g, Workspace& scratch, hipStream_t s) {
+    if (g.diag) {
+        if (g.k > 8) return fail(BSIM_ENOTSUP, "diagonal gate on more than 8 targets");
+        InsArgs a = ins_args(vq, g, false);
+        size_t n = (size_t)1 << g.k;
+        if (int rc = ensure_ws(scratch, sizeof(double2) * n)) return rc;
+        BSIM_HIP(hipMemcpyAsync(scratch.ptr, g.m.data(), sizeof(double2) * n, hipMemcpyHostToDevice, s));
+        int tb[8] = {0};
+        for (int b = 0; b < g.k; ++b) tb[b] = g.tb[b];
+        int64_t grid = grid_for(a.ngroups, 256, cap_blocks());
+        hipLaunchKernelGGL(k_diag, dim3((unsigned)grid), dim3(256), 0, s, d, a, (const double2*)scratch.ptr, g.k,
+                           make_int4(tb[0], tb[1], tb[2], tb[3]), make_int4(tb[4], tb[5], tb[6], tb[7]));
+        BSIM_HIP(hipGetLastError());
+        return BSIM_OK;
+    }
+    if (g.k > vq) return fail(BSIM_EINVAL, "more targets than qubits");
+    if (g.k == 1 && g.tb[0] < 6 && vq >= 6) {
+        int64_t len = (int64_t)1 << vq;
+        int64_t grid = grid_for(len, 256, cap_blocks());
+        uint64_t cm = g.cmask, cv = g.cval;
+        const double2* m = g.m.data();
+        switch (g.tb[0]) {
+#define LANE_CASE(E)                                                                                            \
+    case E:                                                                                                     \
+        hipLaunchKernelGGL(k_1q_lane<E>, dim3((unsigned)grid), dim3(256), 0, s, d, len, cm, cv, m[0], m[1], m[2], \
+                           m[3]);                                                                               \
+        break;
+            LANE_CASE(0)
+            LANE_CASE(1)
+            LANE_CASE(2)
+            LANE_CASE(3)
+            LANE_CASE(4)
+            LANE_CASE(5)
+#undef LANE_CASE
+        }
+        BSIM_HIP(hipGetLastError());
+        return BSIM_OK;
+    }
+    InsArgs a = ins_args(vq, g, true);
+    int64_t grid = grid_for(a.ngroups, 256, cap_blocks());
+    if (g.k == 1) {
+        SmallMat<1> M;
+        for (int i = 0; i < 4; ++i) M.m[i] = g.m[i];
+        hipLaunchKernelGGL(k_dense<1>, dim3((unsigned)grid), dim3(256), 0, s, d, a, M, g.tb[0], 0);
+    } else if (g.k == 2) {
+        SmallMat<2> M;
+        for (int i = 0; i < 16; ++i) M.m[i] = g.m[i];
+        hipLaunchKernelGGL(k_dense<2>, dim3((unsigned)grid), dim3(256), 0, s, d, a, M, g.tb[0], g.tb[1]);
+    } else if (g.k <= 6) {
+        size_t D = (size_t)1 << g.k;
+        if (int rc = ensure_ws(scratch, sizeof(double2) * D * D)) return rc;
+        BSIM_HIP(hipMemcpyAsync(scratch.ptr, g.m.data(), sizeof(double2) * D * D, hipMemcpyHostToDevice, s));
+        int tb[6] = {0};
+        for (int b = 0; b < g.k; ++b) tb[b] = g.tb[b];
+        int64_t gpb = 256 / (int64_t)D;
+        int64_t nblk = grid_for((a.ngroups + gpb - 1) / gpb, 1, cap_blocks());
+        size_t shmem = sizeof(double2) * (D * D + 256);
+        static bool attr_set = false;
+        if (!attr_set && shmem > 64 * 1024) {
+            BSIM_HIP(hipFuncSetAttribute((const void*)k_dense_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         160 * 1024));
+            attr_set = true;
+        }
+        hipLaunchKernelGGL(k_dense_lds, dim3((unsigned)nblk), dim3(256), shmem, s, d, a, (const double2*)scratch.ptr,
+                           g.k, make_int4(tb[0], tb[1], tb[2], tb[3]), make_int2(tb[4], tb[5]));
+    } else {
+        return fail(BSIM_ENOTSUP, "dense gates act on at most 6 targets");
+    }
+    BSIM_HIP(hipGetLastError());
+    return BSIM_OK;
+}
+
+int launch_double_excitation(double2* d, int vq, const int* e, double c, double sn, hipStream_t s) {
+    int ord[4] = { e[0], e[1], e[2], e[3] };
+    std::sort(ord, ord + 4);
+    int64_t ng = (int64_t)1 << (vq - 4);
+    int64_t grid = grid_for(ng, 256, cap_blocks());
+    hipLaunchKernelGGL(k_double_exc, dim3((unsigned)grid), dim3(256), 0, s, d, ng,
+                       make_int4(ord[0], ord[1], ord[2], ord[3]), make_int4(e[0], e[1], e[2], e[3]), c, sn);
+    BSIM_HIP(hipGetLastError());
+    return BSIM_OK;
+}
+
+}  // namespace bsim
+
+using namespace bsim;
+
+extern "C" {
+
+int bsim_apply_1q(bsim_state* h, int q, const double* m, int conj) {
+    if (int rc = check_state(h)) return rc;
+    GateSpec g;
+    if (int rc = make_spec(h->vq, 1, &q, 0, nullptr, nullptr, m, false, conj != 0, false, g)) return rc;
+    h->alias_ready = false;
+    return launch_gate(h->d, h->vq, g, h->scratch, h->stream);
+}
+
+int bsim_apply_2q(bsim_state* h, int t1, int t2, const double* m, int conj) {
+    if (int rc = check_state(h)) return rc;
+    int t[2] = { t1, t2 };
+    GateSpec g;
+    if (int rc = make_spec(h->vq, 2, t, 0, nullptr, nullptr, m, false, conj != 0, false, g)) return rc;
+    h->alias_ready = false;
+    return launch_gate(h->d, h->vq, g, h->scratch, h->stream);
+}
+
+int bsim_apply_controlled(bsim_state* h, int nc, const int* controls, const int* cvals, int nt, const int* targets,
+                          const double* m, int conj) {
+    if (int rc = check_state(h)) return rc;
+    if (nt < 1 || nt > 2) return fail(BSIM_ENOTSUP, "controlled gates take 1 or 2 targets");
+    if (nc < 1 || !controls) return fail(BSIM_EINVAL, "controlled gate without controls");
+    GateSpec g;
+    if (int rc = make_spec(h->vq, nt, targets, nc, controls, cvals, m, false, conj != 0, false, g)) return rc;
+    h->alias_ready = false;
+    return launch_gate(h->d, h->vq, g, h->scratch, h->stream);
+}
+
+int bsim_apply_kq(bsim_state* h, int k, const int* targets, const double* m, int conj) {
+    if (int rc = check_state(h)) return rc;
+    if (k < 1 || k > 6) return fail(BSIM_ENOTSUP, "k-qubit dense gates take 1..6 targets");
+    GateSpec g;
+    if (int rc = make_spec(h->vq, k, targets, 0, nullptr, nullptr, m, false, conj != 0, true, g)) return rc;
+    h->alias_ready = false;
+    if (int rc = launch_gate(h->d, h->vq, g, h->scratch, h->stream)) return rc;
+    // the matrix was staged from g.m, which ends with this call
+    BSIM_HIP(hipStreamSynchronize(h->stream));
+    return BSIM_OK;
+}
+
+int bsim_apply_diagonal(bsim_state* h, int k, const int* targets, const double* dvals, int conj) {
+    if (int rc = check_state(h)) return rc;
+    GateSpec g;
+    if (int rc = make_spec(h->vq, k, targets, 0, nullptr, nullptr, dvals, true, conj != 0, false, g)) return rc;
+    h->alias_ready = false;
+    if (int rc = launch_gate(h->d, h->vq, g, h->scratch, h->stream)) return rc;
+    BSIM_HIP(hipStreamSynchronize(h->stream));
+    return BSIM_OK;
+}
+
+int bsim_apply_double_excitation(bsim_state* h, const int* t, double angle) {
+    if (int rc = check_state(h)) return rc;
+    if (!t) return fail(BSIM_EINVAL, "targets is null");
+    int e[4];
+    uint64_t used = 0;
+    for (int i = 0; i < 4; ++i) {
+        if (t[i] < 0 || t[i] >= h->vq) return fail(BSIM_EINVAL, "target qubit out of range");
+        e[i] = h->vq - 1 - t[i];
+        if (used & (1ull << e[i])) return fail(BSIM_EINVAL, "repeated target qubit");
+        used |= 1ull << e[i];
+    }
+    h->alias_ready = false;
+    return launch_double_excitation(h->d, h->vq, e, cos(angle / 2.0), sin(angle / 2.0), h->stream);
+}
+
+}  // extern "C"

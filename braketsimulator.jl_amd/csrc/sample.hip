@@ -1,0 +1,471 @@
+// Shot sampling (src/result_types.jl:8-80): alias table built on the device
+// and one table lookup per shot.
+//
+// Two table builds:
+//  - BSIM_ALIAS_REFERENCE: make_alias_table! exactly (result_types.jl:8-54):
+//    same classification order, LIFO pairing and `(ap[l] - 1) + ap[s]`
+//    update, run by one device thread (the algorithm is sequential).
+//  - BSIM_ALIAS_PARALLEL: an exact-integer form of the sweep construction
+//    (Vose's alias method in the sweeping form used by Huebschle-Schneider &
+//    Sanders, "Parallel Weighted Random Sampling", 2019).  Weights become
+//    integers W_i = floor(p_i * N * C / wsum) (C = 2^32 per bucket; the
+//    rounding remainder goes to the largest weight), lights (W <= C) and
+//    heavies are swept in index order.  With inclusive prefix sums D (light
+//    deficits C - W) and E (heavy excesses W - C):
+//      light i: ap = W_i / C, alias = first heavy k with E[k] > D[i] - d_i;
+//      heavy i: ap = (C + E[i] - D[k]) / C with k the first light where
+//               D[k] >= E[i]; alias = the next heavy.
+//    Integer sums make the table independent of the reduction order, so a
+//    CPU restatement reproduces it bit for bit.
+// Shots draw Philox-4x32-10 (Salmon et al., SC'11) with counter = shot
+// index and key = seed: index = top n bits of (x1:x0), u = (x3:x2 >> 11)
+// * 2^-53, shot = u < ap[index] ? index : alias[index].
+#include <vector>
+
+#include "common.h"
+#include "reduce.h"
+
+namespace bsim {
+
+constexpr uint64_t ALIAS_C = 1ull << 32;
+
+// ---- Philox-4x32-10 ------------------------------------------------------
+__host__ __device__ __forceinline__ void philox4x32_10(uint32_t ctr[4], uint32_t k0, uint32_t k1) {
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)M0 * ctr[0];
+        uint64_t p1 = (uint64_t)M1 * ctr[2];
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ ctr[1] ^ k0;
+        uint32_t n2 = hi0 ^ ctr[3] ^ k1;
+        ctr[0] = n0;
+        ctr[1] = lo1;
+        ctr[2] = n2;
+        ctr[3] = lo0;
+        k0 += W0;
+        k1 += W1;
+    }
+}
+
+// ---- reference-order build (one thread) -----------------------------------
+__global__ void k_alias_reference(const double* __restrict__ w, int64_t n, double wsum, double* __restrict__ ap,
+                                  int32_t* __restrict__ alias, int32_t* __restrict__ larges,
+                                  int32_t* __restrict__ smalls) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double ac = (double)n / wsum;
+    for (int64_t i = 0; i < n; ++i) {
+        ap[i] = w[i] * ac;
+        alias[i] = 0;
+    }
+    int64_t kl = 0, ks = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        double ai = ap[i];
+        if (ai > 1.0) larges[kl++] = (int32_t)i;
+        else if (ai < 1.0) smalls[ks++] = (int32_t)i;
+    }
+    while (kl > 0 && ks > 0) {
+        int32_t s = smalls[--ks];
+        int32_t l = larges[--kl];
+        alias[s] = l;
+        double al = ap[l] = (ap[l] - 1.0) + ap[s];
+        if (al > 1.0) larges[kl++] = l;
+        else smalls[ks++] = l;
+    }
+    for (int64_t i = 0; i < ks; ++i) ap[smalls[i]] = 1.0;
+}
+
+// ---- parallel exact build ---------------------------------------------------
+__global__ void k_sum_p(const double* __restrict__ p, int64_t n, double* __restrict__ partials) {
+    __shared__ double sh[256];
+    double s = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        s += p[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
+}
+
+__global__ void k_sum_finish(const double* __restrict__ partials, int nb, double* __restrict__ out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double s = 0;
+        for (int b = 0; b < nb; ++b) s += partials[b];
+        out[0] = s;
+    }
+}
+
+// W_i = floor(p_i * scale); total and max by integer atomics (order-free)
+__global__ void k_quantize(const double* __restrict__ p, int64_t n, const double* __restrict__ wsum_dev,
+                           double wsum_host, uint64_t* __restrict__ W, unsigned long long* __restrict__ acc) {
+    const double wsum = wsum_host > 0 ? wsum_host : wsum_dev[0];
+    const double scale = (double)n * (double)ALIAS_C / wsum;
+    unsigned long long tot = 0, mx = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double x = p[i] * scale;
+        uint64_t w = x > 0 ? (uint64_t)x : 0;
+        W[i] = w;
+        tot += w;
+        mx = w > mx ? w : mx;
+    }
+    atomicAdd(&acc[0], tot);
+    atomicMax(&acc[1], mx);
+}
+
+__global__ void k_argmax(const uint64_t* __restrict__ W, int64_t n, unsigned long long* __restrict__ acc) {
+    const unsigned long long mx = acc[1];
+    unsigned long long best = ~0ull;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        if (W[i] == mx && (unsigned long long)i < best) best = (unsigned long long)i;
+    atomicMin(&acc[2], best);
+}
+
+__global__ void k_fix_remainder(uint64_t* __restrict__ W, int64_t n, const unsigned long long* __restrict__ acc) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        uint64_t target = (uint64_t)n * ALIAS_C;
+        uint64_t tot = acc[0];
+        W[acc[2]] += target - tot;  // modular arithmetic handles either sign
+    }
+}
+
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_TILE = SCAN_ITEMS * SCAN_BLOCK;
+
+__device__ __forceinline__ void split_dw(uint64_t w, uint64_t& d, uint64_t& e) {
+    d = w <= ALIAS_C ? ALIAS_C - w : 0;
+    e = w > ALIAS_C ? w - ALIAS_C : 0;
+}
+
+// block-wide inclusive scan of (d, e) pairs held one per thread
+__device__ __forceinline__ void block_scan2(uint64_t& d, uint64_t& e, uint64_t* shd, uint64_t* she) {
+    shd[threadIdx.x] = d;
+    she[threadIdx.x] = e;
+    __syncthreads();
+    for (int o = 1; o < SCAN_BLOCK; o <<= 1) {
+        uint64_t ad = 0, ae = 0;
+        if ((int)threadIdx.x >= o) {
+            ad = shd[threadIdx.x - o];
+            ae = she[threadIdx.x - o];
+        }
+        __syncthreads();
+        shd[threadIdx.x] += ad;
+        she[threadIdx.x] += ae;
+        __syncthreads();
+    }
+    d = shd[threadIdx.x];
+    e = she[threadIdx.x];
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(const uint64_t* __restrict__ W, int64_t n,
+                                                           uint64_t* __restrict__ tsum) {
+    __shared__ uint64_t shd[SCAN_BLOCK], she[SCAN_BLOCK];
+    int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    uint64_t d = 0, e = 0;
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        int64_t i = base + k;
+        if (i < n) {
+            uint64_t a, b;
+            split_dw(W[i], a, b);
+            d += a;
+            e += b;
+        }
+    }
+    block_scan2(d, e, shd, she);
+    if (threadIdx.x == SCAN_BLOCK - 1) {
+        tsum[2 * blockIdx.x] = d;
+        tsum[2 * blockIdx.x + 1] = e;
+    }
+}
+
+// exclusive scan of tile sums in place (one block, sequential chunks)
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tilesums(uint64_t* __restrict__ tsum, int64_t ntiles) {
+    __shared__ uint64_t shd[SCAN_BLOCK], she[SCAN_BLOCK];
+    int64_t per = (ntiles + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    int64_t b0 = (int64_t)threadIdx.x * per;
+    uint64_t d = 0, e = 0;
+    for (int64_t k = 0; k < per; ++k) {
+        int64_t i = b0 + k;
+        if (i < ntiles) {
+            d += tsum[2 * i];
+            e += tsum[2 * i + 1];
+        }
+    }
+    uint64_t di = d, ei = e;
+    block_scan2(di, ei, shd, she);
+    uint64_t od = di - d, oe = ei - e;  // exclusive prefix of this thread's chunk
+    for (int64_t k = 0; k < per; ++k) {
+        int64_t i = b0 + k;
+        if (i < ntiles) {
+            uint64_t td = tsum[2 * i], te = tsum[2 * i + 1];
+            tsum[2 * i] = od;
+            tsum[2 * i + 1] = oe;
+            od += td;
+            oe += te;
+        }
+    }
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(const uint64_t* __restrict__ W, int64_t n,
+                                                           const uint64_t* __restrict__ tsum,
+                                                           uint64_t* __restrict__ Dinc, uint64_t* __restrict__ Einc) {
+    __shared__ uint64_t shd[SCAN_BLOCK], she[SCAN_BLOCK];
+    int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    uint64_t dl[SCAN_ITEMS], el[SCAN_ITEMS];
+    uint64_t d = 0, e = 0;
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        int64_t i = base + k;
+        uint64_t a = 0, b = 0;
+        if (i < n) split_dw(W[i], a, b);
+        dl[k] = a;
+        el[k] = b;
+        d += a;
+        e += b;
+    }
+    uint64_t di = d, ei = e;
+    block_scan2(di, ei, shd, she);
+    uint64_t od = tsum[2 * blockIdx.x] + di - d, oe = tsum[2 * blockIdx.x + 1] + ei - e;
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        int64_t i = base + k;
+        od += dl[k];
+        oe += el[k];
+        if (i < n) {
+            Dinc[i] = od;
+            Einc[i] = oe;
+        }
+    }
+}
+
+// first index k in [lo, n) with a[k] > v (a non-decreasing), n if none
+__device__ __forceinline__ int64_t first_greater(const uint64_t* __restrict__ a, int64_t lo, int64_t n, uint64_t v) {
+    int64_t hi = n;
+    while (lo < hi) {
+        int64_t mid = lo + ((hi - lo) >> 1);
+        if (a[mid] > v) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+// first index k with a[k] >= v
+__device__ __forceinline__ int64_t first_geq(const uint64_t* __restrict__ a, int64_t n, uint64_t v) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        int64_t mid = lo + ((hi - lo) >> 1);
+        if (a[mid] >= v) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+__global__ void k_alias_sweep(const uint64_t* __restrict__ W, int64_t n, const uint64_t* __restrict__ Dinc,
+                              const uint64_t* __restrict__ Einc, double* __restrict__ ap, int32_t* __restrict__ alias) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t w = W[i];
+        if (w <= ALIAS_C) {
+            ap[i] = (double)w / (double)ALIAS_C;
+            if (w == ALIAS_C) {
+                alias[i] = (int32_t)i;
+            } else {
+                uint64_t dex = Dinc[i] - (ALIAS_C - w);
+                int64_t k = first_greater(Einc, 0, n, dex);
+                alias[i] = (int32_t)(k < n ? k : i);
+            }
+        } else {
+            uint64_t e = Einc[i];
+            int64_t k = first_geq(Dinc, n, e);
+            uint64_t r = k < n ? ALIAS_C + e - Dinc[k] : ALIAS_C;
+            int64_t nx = first_greater(Einc, i + 1, n, e);
+            if (nx >= n) {
+                r = ALIAS_C;
+                nx = i;
+            }
+            ap[i] = (double)r / (double)ALIAS_C;
+            alias[i] = (int32_t)nx;
+        }
+    }
+}
+
+// ---- lookups ---------------------------------------------------------------
+__global__ void k_sample(const double* __restrict__ ap, const int32_t* __restrict__ alias, int nq, int64_t shots,
+                         uint32_t k0, uint32_t k1, int64_t* __restrict__ out) {
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < shots; s += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t c[4] = { (uint32_t)s, (uint32_t)((uint64_t)s >> 32), 0u, 0u };
+        philox4x32_10(c, k0, k1);
+        uint64_t hi = ((uint64_t)c[1] << 32) | c[0];
+        int64_t idx = (int64_t)(hi >> (64 - nq));
+        double u = (double)((((uint64_t)c[3] << 32) | c[2]) >> 11) * 0x1.0p-53;
+        out[s] = u < ap[idx] ? idx : (int64_t)alias[idx];
+    }
+}
+
+__global__ void k_sample_draws(const double* __restrict__ ap, const int32_t* __restrict__ alias, int64_t shots,
+                               const int64_t* __restrict__ idx, const double* __restrict__ u,
+                               int64_t* __restrict__ out) {
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < shots; s += (int64_t)gridDim.x * blockDim.x) {
+        int64_t i = idx[s];
+        out[s] = u[s] < ap[i] ? i : (int64_t)alias[i];
+    }
+}
+
+__global__ void k_shots_to_bits(const int64_t* __restrict__ shots, int64_t n, int nq, int nm,
+                                const int* __restrict__ measured, int8_t* __restrict__ out) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n * nm; t += (int64_t)gridDim.x * blockDim.x) {
+        int64_t s = t / nm;
+        int j = (int)(t % nm);
+        out[t] = (int8_t)((shots[s] >> (nq - 1 - measured[j])) & 1);
+    }
+}
+
+static int alias_build(bsim_state* h, int mode, double wsum) {
+    const int64_t n = (int64_t)1 << h->nq;
+    if (h->nq > 31) return fail(BSIM_ENOTSUP, "alias tables hold at most 2^31 entries");
+    if (int rc = ensure_ws(h->ap, sizeof(double) * (size_t)n)) return rc;
+    if (int rc = ensure_ws(h->alias, sizeof(int32_t) * (size_t)n)) return rc;
+    // work: p (8n) | W (8n) | Dinc (8n) | Einc (8n) | tile sums | scalars
+    int64_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    size_t bytes = sizeof(uint64_t) * ((size_t)4 * n + 2 * (size_t)ntiles + 4096 + 64);
+    if (mode == BSIM_ALIAS_REFERENCE) bytes = sizeof(double) * (size_t)n + sizeof(int32_t) * 2 * (size_t)n + 65536;
+    if (int rc = ensure_ws(h->alias_work, bytes)) return rc;
+    char* base = (char*)h->alias_work.ptr;
+    double* p = (double*)base;
+    if (int rc = probabilities_device(h, p)) return rc;
+    double* ap = (double*)h->ap.ptr;
+    int32_t* al = (int32_t*)h->alias.ptr;
+    const int64_t cap = (int64_t)device_cu_count() * 32;
+    if (mode == BSIM_ALIAS_REFERENCE) {
+        int32_t* larges = (int32_t*)(base + sizeof(double) * n);
+        int32_t* smalls = larges + n;
+        double* scal = (double*)(smalls + n);
+        if (wsum <= 0) {
+            hipLaunchKernelGGL(k_sum_p, dim3(256), dim3(256), 0, h->stream, p, n, scal + 8);
+            hipLaunchKernelGGL(k_sum_finish, dim3(1), dim3(64), 0, h->stream, scal + 8, 256, scal);
+            BSIM_HIP(hipMemcpyAsync(&wsum, scal, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+            BSIM_HIP(hipStreamSynchronize(h->stream));
+        }
+        hipLaunchKernelGGL(k_alias_reference, dim3(1), dim3(64), 0, h->stream, p, n, wsum, ap, al, larges, smalls);
+        BSIM_HIP(hipGetLastError());
+    } else if (mode == BSIM_ALIAS_PARALLEL) {
+        uint64_t* W = (uint64_t*)(base + sizeof(double) * n);
+        uint64_t* Dinc = W + n;
+        uint64_t* Einc = Dinc + n;
+        uint64_t* tsum = Einc + n;
+        double* scal = (double*)(tsum + 2 * ntiles);
+        unsigned long long* acc = (unsigned long long*)(scal + 512);
+        if (wsum <= 0) {
+            hipLaunchKernelGGL(k_sum_p, dim3(256), dim3(256), 0, h->stream, p, n, scal + 8);
+            hipLaunchKernelGGL(k_sum_finish, dim3(1), dim3(64), 0, h->stream, scal + 8, 256, scal);
+        }
+        unsigned long long init[4] = { 0ull, 0ull, ~0ull, 0ull };
+        BSIM_HIP(hipMemcpyAsync(acc, init, sizeof(init), hipMemcpyHostToDevice, h->stream));
+        int64_t g = grid_for(n, 256, cap);
+        hipLaunchKernelGGL(k_quantize, dim3((unsigned)g), dim3(256), 0, h->stream, p, n, (const double*)scal, wsum, W,
+                           acc);
+        hipLaunchKernelGGL(k_argmax, dim3((unsigned)g), dim3(256), 0, h->stream, W, n, acc);
+        hipLaunchKernelGGL(k_fix_remainder, dim3(1), dim3(64), 0, h->stream, W, n, acc);
+        hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)ntiles), dim3(SCAN_BLOCK), 0, h->stream, W, n, tsum);
+        hipLaunchKernelGGL(k_scan_tilesums, dim3(1), dim3(SCAN_BLOCK), 0, h->stream, tsum, ntiles);
+        hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)ntiles), dim3(SCAN_BLOCK), 0, h->stream, W, n, tsum, Dinc,
+                           Einc);
+        hipLaunchKernelGGL(k_alias_sweep, dim3((unsigned)grid_for(n, 256, cap * 8)), dim3(256), 0, h->stream, W, n,
+                           Dinc, Einc, ap, al);
+        BSIM_HIP(hipGetLastError());
+        // `init` lives on this frame: make sure the copy has been consumed
+        BSIM_HIP(hipStreamSynchronize(h->stream));
+    } else {
+        return fail(BSIM_EINVAL, "unknown alias mode");
+    }
+    h->alias_ready = true;
+    return BSIM_OK;
+}
+
+}  // namespace bsim
+
+using namespace bsim;
+
+extern "C" {
+
+int bsim_alias_build(bsim_state* h, int mode, double wsum) {
+    if (int rc = check_state(h)) return rc;
+    if (int rc = alias_build(h, mode, wsum)) return rc;
+    BSIM_HIP(hipStreamSynchronize(h->stream));
+    return BSIM_OK;
+}
+
+int bsim_alias_download(bsim_state* h, double* ap, int32_t* alias) {
+    if (int rc = check_state(h)) return rc;
+    if (!h->alias_ready) return fail(BSIM_EINVAL, "no alias table: call bsim_alias_build after the last change");
+    int64_t n = (int64_t)1 << h->nq;
+    if (ap) BSIM_HIP(hipMemcpyAsync(ap, h->ap.ptr, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+    if (alias) BSIM_HIP(hipMemcpyAsync(alias, h->alias.ptr, sizeof(int32_t) * n, hipMemcpyDeviceToHost, h->stream));
+    BSIM_HIP(hipStreamSynchronize(h->stream));
+    return BSIM_OK;
+}
+
+int bsim_sample(bsim_state* h, int64_t shots, uint64_t seed, int64_t* out) {
+    if (int rc = check_state(h)) return rc;
+    if (shots < 0 || (shots > 0 && !out)) return fail(BSIM_EINVAL, "bad shots/out");
+    if (!h->alias_ready) return fail(BSIM_EINVAL, "no alias table: call bsim_alias_build after the last change");
+    if (shots == 0) return BSIM_OK;
+    if (int rc = ensure_ws(h->scratch, sizeof(int64_t) * (size_t)shots)) return rc;
+    int64_t* d = (int64_t*)h->scratch.ptr;
+    int64_t g = grid_for(shots, 256, (int64_t)device_cu_count() * 16);
+    hipLaunchKernelGGL(k_sample, dim3((unsigned)g), dim3(256), 0, h->stream, (const double*)h->ap.ptr,
+                       (const int32_t*)h->alias.ptr, h->nq, shots, (uint32_t)seed, (uint32_t)(seed >> 32), d);
+    BSIM_HIP(hipGetLastError());
+    BSIM_HIP(hipMemcpyAsync(out, d, sizeof(int64_t) * shots, hipMemcpyDeviceToHost, h->stream));
+    BSIM_HIP(hipStreamSynchronize(h->stream));
+    return BSIM_OK;
+}
+
+int bsim_sample_from_draws(bsim_state* h, int64_t shots, const int64_t* index, const double* u, int64_t* out) {
+    if (int rc = check_state(h)) return rc;
+    if (shots < 0 || (shots > 0 && (!index || !u || !out))) return fail(BSIM_EINVAL, "bad draws");
+    if (!h->alias_ready) return fail(BSIM_EINVAL, "no alias table: call bsim_alias_build after the last change");
+    int64_t n = (int64_t)1 << h->nq;
+    for (int64_t s = 0; s < shots; ++s)
+        if (index[s] < 0 || index[s] >= n) return fail(BSIM_EINVAL, "draw index out of range");
+    if (shots == 0) return BSIM_OK;
+    if (int rc = ensure_ws(h->scratch, (sizeof(int64_t) * 2 + sizeof(double)) * (size_t)shots)) return rc;
+    int64_t* di = (int64_t*)h->scratch.ptr;
+    double* du = (double*)(di + shots);
+    int64_t* dout = (int64_t*)(du + shots);
+    BSIM_HIP(hipMemcpyAsync(di, index, sizeof(int64_t) * shots, hipMemcpyHostToDevice, h->stream));
+    BSIM_HIP(hipMemcpyAsync(du, u, sizeof(double) * shots, hipMemcpyHostToDevice, h->stream));
+    int64_t g = grid_for(shots, 256, (int64_t)device_cu_count() * 16);
+    hipLaunchKernelGGL(k_sample_draws, dim3((unsigned)g), dim3(256), 0, h->stream, (const double*)h->ap.ptr,
+                       (const int32_t*)h->alias.ptr, shots, di, du, dout);
+    BSIM_HIP(hipGetLastError());
+    BSIM_HIP(hipMemcpyAsync(out, dout, sizeof(int64_t) * shots, hipMemcpyDeviceToHost, h->stream));
+    BSIM_HIP(hipStreamSynchronize(h->stream));
+    return BSIM_OK;
+}
+
+int bsim_shots_to_bits(bsim_state* h, int64_t shots, const int64_t* shot_index, int nm, const int* measured,
+                       int8_t* out) {
+    if (int rc = check_state(h)) return rc;
+    if (shots < 0 || nm < 0 || (shots > 0 && nm > 0 && (!shot_index || !measured || !out)))
+        return fail(BSIM_EINVAL, "bad arguments");
+    for (int j = 0; j < nm; ++j)
+        if (measured[j] < 0 || measured[j] >= h->nq) return fail(BSIM_EINVAL, "measured qubit out of range");
+    if (shots == 0 || nm == 0) return BSIM_OK;
+    size_t bytes = sizeof(int64_t) * shots + sizeof(int) * nm + (size_t)shots * nm + 256;
+    if (int rc = ensure_ws(h->scratch, bytes)) return rc;
+    int64_t* ds = (int64_t*)h->scratch.ptr;
+    int* dm = (int*)(ds + shots);
+    int8_t* dout = (int8_t*)(dm + nm);
+    BSIM_HIP(hipMemcpyAsync(ds, shot_index, sizeof(int64_t) * shots, hipMemcpyHostToDevice, h->stream));
+    BSIM_HIP(hipMemcpyAsync(dm, measured, sizeof(int) * nm, hipMemcpyHostToDevice, h->stream));
+    int64_t g = grid_for(shots * nm, 256, (int64_t)device_cu_count() * 16);
+    hipLaunchKernelGGL(k_shots_to_bits, dim3((unsigned)g), dim3(256), 0, h->stream, ds, shots, h->nq, nm, dm, dout);
+    BSIM_HIP(hipGetLastError());
+    BSIM_HIP(hipMemcpyAsync(out, dout, (size_t)shots * nm, hipMemcpyDeviceToHost, h->stream));
+    BSIM_HIP(hipStreamSynchronize(h->stream));
+    return BSIM_OK;
+}
+
+}  // extern "C"

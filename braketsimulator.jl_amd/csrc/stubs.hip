@@ -1,0 +1,13 @@
+// Entry points whose engines are not built yet report BSIM_ENOTSUP.
+#include "common.h"
+
+using namespace bsim;
+
+extern "C" {
+
+int bsim_adjoint_gradient(bsim_state*, const bsim_op*, int, int, const char*, const double*, int, double*, double*) {
+    return fail(BSIM_ENOTSUP, "not built");
+}
+int bsim_batch_run(const bsim_task*, int, double*) { return fail(BSIM_ENOTSUP, "not built"); }
+
+}

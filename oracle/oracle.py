@@ -1,0 +1,350 @@
+"""Python side of the oracle -- TEST INFRASTRUCTURE ONLY.
+
+Loads ``liboracle.so`` (refcpu.c, the C restatement of the reference's
+kernels) and adds numpy checkers that do not depend on it:
+
+* ``np_apply`` / ``np_state``: brute-force state-vector simulation by
+  tensor contraction in plain kron order (qubit 0 = most significant bit),
+  used to pin refcpu's index arithmetic on small registers;
+* ``qft_state``: the closed form of the RY-product + textbook QFT circuit,
+  ``sqrt(N) * ifft(a)``.
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's cpu_baseline leg may
+import this module.  The product path (libbsim, bsim) never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_i64p = C.POINTER(C.c_int64)
+_i32p = C.POINTER(C.c_int32)
+_u32p = C.POINTER(C.c_uint32)
+
+_SIGS = {
+    "ref_apply_1q": [_dp, C.c_int, C.c_int, _dp],
+    "ref_apply_2q": [_dp, C.c_int, C.c_int, C.c_int, _dp],
+    "ref_apply_ctrl_1c1t": [_dp, C.c_int, C.c_int, C.c_int, C.c_int, _dp],
+    "ref_apply_ctrl_1c2t": [_dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp],
+    "ref_apply_ctrl_2c1t": [_dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp],
+    "ref_apply_kq": [_dp, C.c_int, C.c_int, _ip, _dp],
+    "ref_apply_double_excitation": [_dp, C.c_int, _ip, C.c_double, C.c_double],
+    "ref_apply_hermitian_2q": [_dp, C.c_int, C.c_int, C.c_int, _dp],
+    "ref_dm_phaseflip": [_dp, C.c_int, C.c_int, C.c_double],
+    "ref_dm_kraus_ref": [_dp, C.c_int, C.c_int, _ip, C.c_int, _dp],
+    "ref_dm_kraus_physical": [_dp, C.c_int, C.c_int, _ip, C.c_int, _dp],
+    "ref_dm_apply_hermitian": [_dp, C.c_int, C.c_int, _ip, _dp],
+    "ref_sv_probabilities": [_dp, C.c_int, _dp],
+    "ref_dm_probabilities": [_dp, C.c_int, _dp],
+    "ref_dot": [_dp, _dp, C.c_int64, _dp],
+    "ref_marginal_probability": [_dp, C.c_int, C.c_int, _ip, _dp],
+    "ref_make_alias_table": [_dp, C.c_int64, C.c_double, _dp, _i64p, _i64p, _i64p],
+    "ref_alias_lookup": [_dp, _i64p, C.c_int64, _i64p, _dp, _i64p],
+    "ref_philox4x32_10": [_u32p, C.c_uint32, C.c_uint32, _u32p],
+    "ref_shot_draws": [C.c_uint64, C.c_int, C.c_int64, C.c_int64, _i64p, _dp],
+    "ref_alias_exact": [_dp, C.c_int64, C.c_double, _dp, _i32p],
+    "ref_splitmix64_doubles": [C.c_uint64, C.c_int64, _dp],
+}
+
+_lib = None
+
+
+def build() -> Path:
+    r = subprocess.run(["make", "-C", str(HERE), "liboracle.so"], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"oracle build failed:\n{r.stdout}\n{r.stderr}")
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        l = C.CDLL(str(LIB))
+        for name, args in _SIGS.items():
+            f = getattr(l, name)
+            f.argtypes = args
+            f.restype = C.c_double if name == "ref_dm_trace_re" else None
+        _lib = l
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _cm(m):
+    return np.asfortranarray(np.asarray(m, dtype=np.complex128))
+
+
+# ---------------------------------------------------------------------------
+# kernel-level calls (the same description the host gives libbsim)
+# ---------------------------------------------------------------------------
+
+def apply_call(vec: np.ndarray, vq: int, call, conj: bool = False, shift: int = 0):
+    """Apply one reference kernel (described by a bsim.gates.KernelCall) to
+    ``vec`` (length 2^vq), with qubits shifted by ``shift`` (the DM row pass)."""
+    L = lib()
+    m = np.asarray(call.matrix, dtype=np.complex128)
+    if call.diagonal:
+        m = np.diag(m)
+    if conj:
+        m = np.conj(m)
+    m = _cm(m)
+    t = [x + shift for x in call.targets]
+    c = [x + shift for x in call.controls]
+    if call.double_excitation is not None:
+        orig = np.array(list(reversed(t)), dtype=np.int32)
+        a = call.double_excitation
+        L.ref_apply_double_excitation(_d(vec), vq, orig.ctypes.data_as(_ip), math.cos(a / 2.0), math.sin(a / 2.0))
+        return
+    if c:
+        if len(c) == 1 and len(t) == 1:
+            L.ref_apply_ctrl_1c1t(_d(vec), vq, int(call.cvals[0]), c[0], t[0], _d(m))
+        elif len(c) == 1 and len(t) == 2:
+            L.ref_apply_ctrl_1c2t(_d(vec), vq, int(call.cvals[0]), c[0], t[0], t[1], _d(m))
+        elif len(c) == 2 and len(t) == 1:
+            L.ref_apply_ctrl_2c1t(_d(vec), vq, int(call.cvals[0]), int(call.cvals[1]), c[0], c[1], t[0], _d(m))
+        else:
+            raise NotImplementedError("the reference has no kernel for this control arity")
+        return
+    if len(t) == 1:
+        L.ref_apply_1q(_d(vec), vq, t[0], _d(m))
+    elif len(t) == 2:
+        L.ref_apply_2q(_d(vec), vq, t[0], t[1], _d(m))
+    else:
+        ta = np.array(t, dtype=np.int32)
+        L.ref_apply_kq(_d(vec), vq, len(t), ta.ctypes.data_as(_ip), _d(m))
+
+
+def evolve_sv(n: int, instructions, psi: np.ndarray | None = None) -> np.ndarray:
+    """evolve! (src/sv_simulator.jl:140-148) on the oracle."""
+    from bsim.gates import Gate, kernel_call  # host gate -> kernel description
+
+    if psi is None:
+        psi = np.zeros(2 ** n, dtype=np.complex128)
+        psi[0] = 1
+    psi = np.ascontiguousarray(psi, dtype=np.complex128)
+    for ins in instructions:
+        op = ins.operator
+        if not isinstance(op, Gate):
+            continue
+        apply_call(psi, n, kernel_call(op, ins.target))
+    return psi
+
+
+def evolve_dm(n: int, instructions, mode: str = "reference", rho: np.ndarray | None = None) -> np.ndarray:
+    """evolve! (src/dm_simulator.jl:170-179) on the oracle.  Returns the
+    column-major stored matrix flattened (length 4^n)."""
+    from bsim.gates import Gate, kernel_call
+    from bsim.noises import Noise, PhaseFlip
+
+    L = lib()
+    if rho is None:
+        rho = np.zeros(4 ** n, dtype=np.complex128)
+        rho[0] = 1
+    rho = np.ascontiguousarray(rho, dtype=np.complex128)
+    for ins in instructions:
+        op = ins.operator
+        if isinstance(op, Gate):
+            call = kernel_call(op, ins.target)
+            apply_call(rho, 2 * n, call, conj=False, shift=0)   # column bits
+            apply_call(rho, 2 * n, call, conj=True, shift=n)    # row bits, conj(U)
+        elif isinstance(op, Noise):
+            t = np.array(ins.target, dtype=np.int32)
+            if isinstance(op, PhaseFlip):
+                L.ref_dm_phaseflip(_d(rho), n, int(t[0]), float(op.probability))
+                continue
+            ks = op.kraus_rep()
+            flat = np.ascontiguousarray(np.concatenate([_cm(k).ravel(order="F") for k in ks]))
+            f = L.ref_dm_kraus_physical if mode == "physical" else L.ref_dm_kraus_ref
+            f(_d(rho), n, len(t), t.ctypes.data_as(_ip), len(ks), _d(flat))
+    return rho
+
+
+def sv_probabilities(psi: np.ndarray, n: int) -> np.ndarray:
+    out = np.empty(2 ** n)
+    lib().ref_sv_probabilities(_d(psi), n, _d(out))
+    return out
+
+
+def dm_probabilities(rho: np.ndarray, n: int) -> np.ndarray:
+    out = np.empty(2 ** n)
+    lib().ref_dm_probabilities(_d(rho), n, _d(out))
+    return out
+
+
+def dot(a: np.ndarray, b: np.ndarray) -> complex:
+    out = np.empty(2)
+    lib().ref_dot(_d(a), _d(b), a.size, _d(out))
+    return complex(out[0], out[1])
+
+
+_PAULI = {
+    "X": np.array([[0, 1], [1, 0]], dtype=np.complex128),
+    "Y": np.array([[0, -1j], [1j, 0]], dtype=np.complex128),
+    "Z": np.array([[1, 0], [0, -1]], dtype=np.complex128),
+}
+
+
+def sv_expect_pauli(psi: np.ndarray, n: int, word: str) -> float:
+    """real(dot(psi, O psi)) with the Pauli gates applied to a copy
+    (src/sv_simulator.jl:240-248, src/result_types.jl:196-221)."""
+    L = lib()
+    k = psi.copy()
+    for q, ch in enumerate(word):
+        if ch in _PAULI:
+            L.ref_apply_1q(_d(k), n, q, _d(_cm(_PAULI[ch])))
+    return dot(psi, k).real
+
+
+def dm_expect_pauli(rho: np.ndarray, n: int, word: str) -> float:
+    """Pauli gates on the column bits of a copy, then real(tr)
+    (src/dm_simulator.jl:181-189, :253-260)."""
+    L = lib()
+    k = rho.copy()
+    for q, ch in enumerate(word):
+        if ch in _PAULI:
+            L.ref_apply_1q(_d(k), 2 * n, q, _d(_cm(_PAULI[ch])))
+    N = 2 ** n
+    return float(np.real(np.trace(k.reshape((N, N), order="F"))))
+
+
+def sv_expect_matrix(psi: np.ndarray, n: int, m: np.ndarray, targets) -> float:
+    """Hermitian observable on a copy: 1q (:162-178), 2q t1 = MSB
+    (:179-198), k > 2 sorted (:200-230)."""
+    L = lib()
+    k = psi.copy()
+    mm = _cm(m)
+    t = list(targets)
+    if len(t) == 1:
+        L.ref_apply_1q(_d(k), n, t[0], _d(mm))
+    elif len(t) == 2:
+        L.ref_apply_hermitian_2q(_d(k), n, t[0], t[1], _d(mm))
+    else:
+        ta = np.array(t, dtype=np.int32)
+        L.ref_apply_kq(_d(k), n, len(t), ta.ctypes.data_as(_ip), _d(mm))
+    return dot(psi, k).real
+
+
+def dm_expect_matrix(rho: np.ndarray, n: int, m: np.ndarray, targets) -> float:
+    L = lib()
+    k = rho.copy()
+    ta = np.array(list(targets), dtype=np.int32)
+    L.ref_dm_apply_hermitian(_d(k), n, len(ta), ta.ctypes.data_as(_ip), _d(_cm(m)))
+    N = 2 ** n
+    return float(np.real(np.trace(k.reshape((N, N), order="F"))))
+
+
+def marginal(probs: np.ndarray, n: int, targets) -> np.ndarray:
+    t = np.array(list(targets), dtype=np.int32)
+    out = np.empty(2 ** len(t))
+    lib().ref_marginal_probability(_d(probs), n, len(t), t.ctypes.data_as(_ip), _d(out))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# sampling checkers
+# ---------------------------------------------------------------------------
+
+def alias_reference(p: np.ndarray, wsum: float):
+    """make_alias_table! (src/result_types.jl:8-54); alias returned 0-based."""
+    n = p.size
+    ap = np.empty(n)
+    alias = np.zeros(n, dtype=np.int64)
+    lw = np.zeros(n, dtype=np.int64)
+    sw = np.zeros(n, dtype=np.int64)
+    lib().ref_make_alias_table(_d(np.ascontiguousarray(p)), n, float(wsum), _d(ap), alias.ctypes.data_as(_i64p),
+                               lw.ctypes.data_as(_i64p), sw.ctypes.data_as(_i64p))
+    return ap, alias - 1
+
+
+def alias_exact(p: np.ndarray, wsum: float):
+    n = p.size
+    ap = np.empty(n)
+    alias = np.empty(n, dtype=np.int32)
+    lib().ref_alias_exact(_d(np.ascontiguousarray(p)), n, float(wsum), _d(ap), alias.ctypes.data_as(_i32p))
+    return ap, alias
+
+
+def shot_draws(seed: int, nq: int, n: int, first: int = 0):
+    idx = np.empty(n, dtype=np.int64)
+    u = np.empty(n)
+    lib().ref_shot_draws(C.c_uint64(seed), nq, first, n, idx.ctypes.data_as(_i64p), _d(u))
+    return idx, u
+
+
+def lookup(ap: np.ndarray, alias: np.ndarray, idx: np.ndarray, u: np.ndarray) -> np.ndarray:
+    """Shot rule of src/result_types.jl:71: u < ap[i] ? i : alias[i]."""
+    return np.where(u < ap[idx], idx, alias[idx]).astype(np.int64)
+
+
+def philox(ctr, key):
+    i = (C.c_uint32 * 4)(*ctr)
+    o = (C.c_uint32 * 4)()
+    lib().ref_philox4x32_10(i, key[0], key[1], o)
+    return list(o)
+
+
+def splitmix_doubles(seed: int, n: int) -> np.ndarray:
+    out = np.empty(n)
+    lib().ref_splitmix64_doubles(C.c_uint64(seed), n, _d(out))
+    return out
+
+
+def alias_distribution(ap: np.ndarray, alias: np.ndarray) -> np.ndarray:
+    """The distribution an alias table samples: (ap_k + sum over buckets b
+    aliasing to k of (1 - ap_b)) / N."""
+    n = ap.size
+    q = ap.astype(np.float64).copy()
+    np.add.at(q, alias, 1.0 - ap)
+    return q / n
+
+
+# ---------------------------------------------------------------------------
+# numpy brute force (independent of refcpu)
+# ---------------------------------------------------------------------------
+
+def np_apply(psi: np.ndarray, n: int, m: np.ndarray, qubits) -> np.ndarray:
+    """Apply ``m`` (kron order: qubits[0] is its most significant bit) to
+    the qubits of a state whose qubit 0 is the most significant bit."""
+    k = len(qubits)
+    t = psi.reshape([2] * n)
+    mt = np.asarray(m, dtype=np.complex128).reshape([2] * (2 * k))
+    out = np.tensordot(mt, t, axes=(list(range(k, 2 * k)), list(qubits)))
+    rest = [q for q in range(n) if q not in qubits]
+    order = list(qubits) + rest
+    inv = np.argsort(order)
+    return np.transpose(out, inv).reshape(-1)
+
+
+def np_controlled(m: np.ndarray, nc: int, cvals) -> np.ndarray:
+    """Full matrix of a gate with nc leading controls (kron order)."""
+    kt = int(round(math.log2(m.shape[0])))
+    d = 2 ** (nc + kt)
+    full = np.eye(d, dtype=np.complex128)
+    idx = 0
+    for b, v in enumerate(cvals):
+        idx |= int(v) << (nc + kt - 1 - b)
+    full[idx:idx + 2 ** kt, idx:idx + 2 ** kt] = m
+    return full
+
+
+def qft_state(thetas: np.ndarray) -> np.ndarray:
+    """RY(theta_q) on every qubit of |0...0>, then the textbook QFT with
+    qubit 0 as the most significant bit: sqrt(N) * ifft(a)."""
+    a = np.ones(1, dtype=np.complex128)
+    for th in thetas:
+        a = np.kron(a, np.array([math.cos(th / 2), math.sin(th / 2)], dtype=np.complex128))
+    return math.sqrt(a.size) * np.fft.ifft(a)

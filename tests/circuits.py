@@ -1,0 +1,201 @@
+"""Helpers shared by the tests: fixture ops -> Instructions, the GRCS-16
+QASM fixture reader, random circuits."""
+from __future__ import annotations
+
+import math
+import re
+
+import numpy as np
+
+import bsim
+from bsim import gates as G
+from bsim import noises as Nz
+from bsim.simulators import Instruction
+
+_P = {"I": np.eye(2), "X": np.array([[0, 1], [1, 0]]), "Y": np.array([[0, -1j], [1j, 0]]),
+      "Z": np.diag([1.0, -1.0])}
+
+
+def _kron_spec(spec: str):
+    body = spec.split(":", 1)[1]
+    zero = body.endswith("+zero")
+    body = body.replace("+zero", "")
+    m = np.ones((1, 1), dtype=np.complex128)
+    for ch in body:
+        m = np.kron(m, _P[ch])
+    return m, zero
+
+
+def _cmatrix(rows):
+    return np.array([[complex(x[0], x[1]) for x in r] for r in rows], dtype=np.complex128)
+
+
+def fixture_instruction(op):
+    name, args, targets = op
+    if name == "GPhase1":
+        return Instruction(G.GPhase(1, args[0]), targets)
+    if name == "Unitary":
+        m = _kron_spec(args)[0] if isinstance(args, str) else _cmatrix(args)
+        return Instruction(G.Unitary(m), targets)
+    if name == "Kraus":
+        if isinstance(args, str):
+            m, zero = _kron_spec(args)
+            ms = [m] + ([0.0 * np.ones_like(m)] if zero else [])
+        else:
+            ms = [_cmatrix(a) for a in args]
+        return Instruction(Nz.Kraus(ms), targets)
+    if name in G.GATES:
+        return Instruction(G.GATES[name](*args), targets)
+    if name in Nz.NOISES:
+        return Instruction(Nz.NOISES[name](*args), targets)
+    raise KeyError(name)
+
+
+def fixture_ops(case):
+    return [fixture_instruction(o) for o in case["ops"]]
+
+
+def grcs16(path):
+    """Instruction list of the GRCS-16 OpenQASM fixture (h, cz, t, rx(pi/2),
+    ry(pi/2); index lists, inclusive ranges and whole-register forms)."""
+    ins = []
+    n = 16
+
+    def qubits(spec):
+        spec = spec.strip()
+        if spec == "q":
+            return list(range(n))
+        inner = spec[spec.index("[") + 1: spec.rindex("]")]
+        if inner.startswith("{"):
+            return [int(x) for x in inner.strip("{}").split(",")]
+        if ":" in inner:
+            a, b = inner.split(":")
+            return list(range(int(a), int(b) + 1))
+        return [int(inner)]
+
+    for line in path.read_text().splitlines():
+        line = line.strip().rstrip(";")
+        if not line or line.startswith(("OPENQASM", "qubit", "#")):
+            continue
+        m = re.match(r"(\w+)(\((.*?)\))?\s+(.*)", line)
+        g, _, arg, rest = m.groups()
+        if g == "cz":
+            a, b = [qubits(x)[0] for x in rest.split(",")]
+            ins.append(Instruction(G.CZ(), [a, b]))
+            continue
+        for q in qubits(rest):
+            if g == "h":
+                ins.append(Instruction(G.H(), [q]))
+            elif g == "t":
+                ins.append(Instruction(G.T(), [q]))
+            elif g == "rx":
+                ins.append(Instruction(G.Rx(math.pi / 2), [q]))
+            elif g == "ry":
+                ins.append(Instruction(G.Ry(math.pi / 2), [q]))
+            else:
+                raise ValueError(line)
+    return ins
+
+
+ONE_Q = ["H", "X", "Y", "Z", "S", "Si", "T", "Ti", "V", "Vi"]
+ONE_Q_ANG = [("Rx", 1), ("Ry", 1), ("Rz", 1), ("PhaseShift", 1), ("U", 3), ("GPi", 1), ("GPi2", 1), ("PRx", 2)]
+TWO_Q = ["Swap", "ISwap", "ECR", "CNot", "CY", "CZ", "CV"]
+TWO_Q_ANG = [("XX", 1), ("YY", 1), ("ZZ", 1), ("XY", 1), ("PSwap", 1), ("CPhaseShift", 1), ("CPhaseShift00", 1),
+             ("CPhaseShift01", 1), ("CPhaseShift10", 1), ("MS", 3), ("SingleExcitation", 1),
+             ("SingleExcitationPlus", 1), ("SingleExcitationMinus", 1), ("FermionicSWAP", 1)]
+
+
+def random_unitary(k, rng):
+    a = rng.normal(size=(2 ** k, 2 ** k)) + 1j * rng.normal(size=(2 ** k, 2 ** k))
+    q, r = np.linalg.qr(a)
+    return q * (np.diag(r) / np.abs(np.diag(r)))
+
+
+def random_circuit(n, n_ops, rng, kinds=("1q", "2q", "ctrl", "kq", "diag", "pow")):
+    """A random mix of every gate family the reference dispatches."""
+    ins = []
+    for _ in range(n_ops):
+        kind = kinds[rng.integers(len(kinds))]
+        if kind == "1q" or n < 2:
+            if rng.random() < 0.5:
+                g = G.GATES[ONE_Q[rng.integers(len(ONE_Q))]]()
+            else:
+                name, na = ONE_Q_ANG[rng.integers(len(ONE_Q_ANG))]
+                g = G.GATES[name](*rng.uniform(-math.pi, math.pi, na))
+            ins.append(Instruction(g, [int(rng.integers(n))]))
+        elif kind == "2q":
+            t = rng.choice(n, 2, replace=False)
+            if rng.random() < 0.4:
+                g = G.GATES[TWO_Q[rng.integers(len(TWO_Q))]]()
+            elif rng.random() < 0.5:
+                name, na = TWO_Q_ANG[rng.integers(len(TWO_Q_ANG))]
+                g = G.GATES[name](*rng.uniform(-math.pi, math.pi, na))
+            else:
+                g = G.Unitary(random_unitary(2, rng))
+            ins.append(Instruction(g, [int(x) for x in t]))
+        elif kind == "ctrl" and n >= 3:
+            t = [int(x) for x in rng.choice(n, 3, replace=False)]
+            r = rng.random()
+            if r < 0.25:
+                ins.append(Instruction(G.CCNot(), t))
+            elif r < 0.5:
+                ins.append(Instruction(G.CSwap(), t))
+            elif r < 0.75:
+                g = G.Control(G.Rx(float(rng.uniform(-3, 3))), (int(rng.integers(2)),))
+                ins.append(Instruction(g, t[:2]))
+            else:
+                g = G.Control(G.Ry(float(rng.uniform(-3, 3))), (int(rng.integers(2)), int(rng.integers(2))))
+                ins.append(Instruction(g, t))
+        elif kind == "kq" and n >= 3:
+            k = int(rng.integers(3, min(n, 5) + 1))
+            t = [int(x) for x in rng.choice(n, k, replace=False)]
+            ins.append(Instruction(G.Unitary(random_unitary(k, rng)), t))
+        elif kind == "diag" and n >= 3:
+            k = int(rng.integers(1, min(n, 4) + 1))
+            t = [int(x) for x in rng.choice(n, k, replace=False)]
+            if rng.random() < 0.5:
+                ins.append(Instruction(G.MultiRZ(float(rng.uniform(-3, 3))), t))
+            elif rng.random() < 0.5:
+                bits = tuple(int(x) for x in rng.integers(0, 2, k))
+                ins.append(Instruction(G.Control(G.GPhase(k, float(rng.uniform(-3, 3))), bits), t))
+            else:
+                ins.append(Instruction(G.GPhase(k, float(rng.uniform(-3, 3))), t))
+        elif kind == "pow":
+            name = ["X", "H", "S", "T", "V", "Rx", "U", "ISwap"][rng.integers(8)]
+            p = float(rng.choice([-1.0, 0.5, 2.0, 3.0, 0.3]))
+            if name in ("Rx",):
+                g = G.Rx(float(rng.uniform(-3, 3)), pow_exponent=p)
+            elif name == "U":
+                g = G.U(*rng.uniform(-3, 3, 3), pow_exponent=p)
+            else:
+                g = G.GATES[name](pow_exponent=p)
+            k = g.n_qubits
+            t = [int(x) for x in rng.choice(n, k, replace=False)]
+            ins.append(Instruction(g, t))
+        else:
+            ins.append(Instruction(G.H(), [int(rng.integers(n))]))
+    return ins
+
+
+def random_noise(n, rng):
+    r = rng.random()
+    q = int(rng.integers(n))
+    if r < 0.2:
+        return Instruction(Nz.Depolarizing(float(rng.uniform(0, 0.3))), [q])
+    if r < 0.35:
+        return Instruction(Nz.AmplitudeDamping(float(rng.uniform(0, 0.3))), [q])
+    if r < 0.5:
+        return Instruction(Nz.PhaseFlip(float(rng.uniform(0, 0.3))), [q])
+    if r < 0.6:
+        return Instruction(Nz.GeneralizedAmplitudeDamping(float(rng.uniform(0, 0.3)), float(rng.uniform(0, 1))), [q])
+    if r < 0.7:
+        return Instruction(Nz.PhaseDamping(float(rng.uniform(0, 0.3))), [q])
+    if r < 0.8 and n >= 2:
+        t = [int(x) for x in rng.choice(n, 2, replace=False)]
+        return Instruction(Nz.TwoQubitDepolarizing(float(rng.uniform(0, 0.2))), t)
+    if r < 0.9 and n >= 2:
+        t = [int(x) for x in rng.choice(n, 2, replace=False)]
+        return Instruction(Nz.TwoQubitDephasing(float(rng.uniform(0, 0.2))), t)
+    k1 = random_unitary(1, rng) * math.sqrt(0.7)
+    k2 = random_unitary(1, rng) * math.sqrt(0.3)
+    return Instruction(Nz.Kraus([k1, k2]), [q])

@@ -1,0 +1,146 @@
+"""Host-side logic and the C ABI's shape: no GPU work is issued here."""
+import ctypes
+import math
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+import bsim
+from bsim import _lib as L
+from bsim import gates as G
+from bsim.gates import kernel_call
+from bsim.simulators import Instruction, lower_instructions
+from bsim import workloads as W
+
+
+def _declared_symbols():
+    text = (ROOT / "include" / "bsim.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(bsim_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(str(L.LIB_PATH))
+    declared = _declared_symbols()
+    assert len(declared) >= 30
+    missing = [s for s in declared if not hasattr(lib, s)]
+    assert not missing, missing
+    # the ctypes table covers the same set
+    assert sorted(L.exported_symbols()) == declared
+
+
+def test_version_and_error_string_without_gpu():
+    lib = L.load()
+    assert lib.bsim_version() == 10000
+    buf = ctypes.create_string_buffer(64)
+    lib.bsim_last_error(buf, 64)
+
+
+def test_struct_layout_matches_header():
+    # bsim_op: 4 ints, 3 x 8 ints, pointer, int + 3 ints, 3 pointers
+    assert ctypes.sizeof(L.Op) == 4 * 4 + 3 * 8 * 4 + 8 + 4 * 4 + 3 * 8
+
+
+def test_gate_matrices_unitary_and_pow_rules():
+    # test/test_gate_operations.jl:40-221: inv(g) * g == I and g^p consistency
+    rng = np.random.default_rng(0)
+    for name, cls in G.GATES.items():
+        nang = {"U": 3, "MS": 3, "PRx": 2}.get(name, 0)
+        try:
+            g = cls(*rng.uniform(-2, 2, nang)) if nang else cls()
+        except ValueError:
+            g = cls(float(rng.uniform(-2, 2)))
+        m = g.matrix_rep()
+        np.testing.assert_allclose(m @ m.conj().T, np.eye(m.shape[0]), atol=1e-12, err_msg=name)
+        gi = type(g).__new__(type(g))
+        gi.__dict__.update(g.__dict__)
+        gi.pow_exponent = -1.0
+        np.testing.assert_allclose(gi.matrix_rep() @ m, np.eye(m.shape[0]), atol=1e-10, err_msg=name)
+        g2 = type(g).__new__(type(g))
+        g2.__dict__.update(g.__dict__)
+        g2.pow_exponent = 2.0
+        np.testing.assert_allclose(g2.matrix_rep(), m @ m, atol=1e-10, err_msg=name)
+
+
+def test_dispatch_conventions():
+    # 2q dense: first target is matrix bit 0
+    c = kernel_call(G.Swap(), [3, 1])
+    assert c.targets == [3, 1] and not c.controls
+    # CNot -> controlled X with control = first target
+    c = kernel_call(G.CNot(), [2, 5])
+    assert c.controls == [2] and c.targets == [5] and c.cvals == [1]
+    np.testing.assert_array_equal(c.matrix, [[0, 1], [1, 0]])
+    # CCNot: two controls
+    c = kernel_call(G.CCNot(), [0, 1, 2])
+    assert c.controls == [0, 1] and c.targets == [2]
+    # CSwap: one control, two targets in given order
+    c = kernel_call(G.CSwap(), [2, 0, 1])
+    assert c.controls == [2] and c.targets == [0, 1]
+    # k-qubit fallback: matrix bit j is the j-th smallest endian target
+    c = kernel_call(G.Unitary(np.eye(8)), [0, 2, 1])
+    assert c.targets == [2, 1, 0]
+    # Control bit values
+    c = kernel_call(G.Control(G.X(), (0,)), [1, 0])
+    assert c.controls == [1] and c.cvals == [0]
+
+
+def test_lowering_builds_ops():
+    ins = [Instruction(G.H(), [0]), Instruction(G.CNot(), [0, 1]), Instruction(G.CPhaseShift(0.3), [1, 2]),
+           Instruction(G.Unitary(np.eye(8)), [0, 1, 2])]
+    ol = lower_instructions(ins, L.BSIM_SV)
+    assert ol.n == 4
+    assert ol.arr[0].type == L.OP_GATE and ol.arr[0].n_targets == 1
+    assert ol.arr[1].n_controls == 1 and ol.arr[1].controls[0] == 0 and ol.arr[1].targets[0] == 1
+    assert ol.arr[2].type == L.OP_GATE  # diagonal matrices are detected by the library
+    assert ol.arr[3].n_targets == 3 and list(ol.arr[3].targets)[:3] == [2, 1, 0]
+
+
+def test_plan_stats_brickwall_fuses():
+    """The planner (host C++) packs the 30-qubit brick-wall into far fewer
+    HBM passes than gates."""
+    ins = W.brickwall(30, 20, seed=0)
+    ol = lower_instructions(ins, L.BSIM_SV)
+    stats = (ctypes.c_int64 * 4)()
+    L.call("bsim_plan_stats", 30, 0, ol.ptr, ol.n, stats)
+    passes, singles, sweeps, fused = list(stats)
+    assert singles == 0
+    assert fused == len(ins)
+    assert passes < len(ins) / 10, (passes, len(ins))
+
+
+def test_plan_stats_dm_and_kq():
+    ins = W.brickwall(6, 4, seed=1, noise=True)
+    ol = lower_instructions(ins, L.BSIM_DM)
+    stats = (ctypes.c_int64 * 4)()
+    L.call("bsim_plan_stats", 12, 1, ol.ptr, ol.n, stats)
+    assert stats[1] > 0  # reference-semantics Kraus ops run as their own launches
+
+
+def test_bad_op_rejected_without_gpu():
+    ins = [Instruction(G.H(), [5])]
+    ol = lower_instructions(ins, L.BSIM_SV)
+    stats = (ctypes.c_int64 * 4)()
+    rc = L.load().bsim_plan_stats(3, 0, ol.ptr, ol.n, stats)
+    assert rc == -1 and "range" in L.last_error()
+
+
+def test_derivative_matrices_accuracy():
+    th = 0.37
+    d = G.derivative_matrices(G.Ry(th), [0])[0]
+    exact = 0.5 * np.array([[-math.sin(th / 2), -math.cos(th / 2)], [math.cos(th / 2), -math.sin(th / 2)]])
+    np.testing.assert_allclose(d, exact, atol=1e-13)
+    du = G.derivative_matrices(G.U(0.3, 0.2, 0.1), [0])
+    assert len(du) == 3
+
+
+def test_workload_sizes():
+    ins, th = W.ry_qft(20)
+    assert len(ins) == 20 + 20 + 190 + 10
+    bw = W.brickwall(30, 20)
+    assert len(bw) == 600 + 290
+    h, pm, th = W.hea(24, 8)
+    assert len(pm) == 384 and len(h) == 8 * (48 + 23)
+    words, c = W.tfim(24)
+    assert len(words) == 23 + 24
