@@ -118,7 +118,7 @@ int lower_ops(int nq, int kind, const bsim_op* ops, int n_ops, std::vector<EOp>&
                 if (int rc = make_spec(vq, 2, t, 0, nullptr, nullptr, d, true, false, false, e.g)) return rc;
                 out.push_back(std::move(e));
             } else if (op.type == BSIM_OP_KRAUS_REF) {
-                if (k > 4) return fail(BSIM_ENOTSUP, "Kraus channels act on at most 4 qubits");
+                if (k > 5) return fail(BSIM_ENOTSUP, "Kraus channels act on at most 5 qubits");
                 if (op.n_kraus < 1) return fail(BSIM_EINVAL, "Kraus channel without matrices");
                 EOp e;
                 e.kind = E_KRAUS_REF;
@@ -191,7 +191,8 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
     plan = Plan();
     const size_t n = eops.size();
     int T = std::min(cfg.tile_bits, vq);
-    int L = std::min(cfg.low_bits, T);
+    // keep two free tile bits so that any 2-target gate fits a pass
+    int L = std::max(0, std::min(cfg.low_bits, T - 2));
     const uint64_t all = vq >= 64 ? ~0ull : ((1ull << vq) - 1);
     if (T < 4) {
         for (size_t i = 0; i < n; ++i) plan.steps.push_back({ false, -1, (int)i });
@@ -232,6 +233,11 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
                 inc.push_back((int)i);
             }
             if ((blocked & all) == all) break;
+        }
+        if (inc.empty()) {  // cannot happen with L <= T-2; never loop without progress
+            plan.steps.push_back({ false, -1, (int)head });
+            done[head] = 1;
+            continue;
         }
         // fill the tile with the lowest free bits (longer contiguous runs)
         for (int b = 0; b < vq && popc(S) < T; ++b) S |= 1ull << b;
@@ -617,10 +623,10 @@ int bsim_set_tile_config(int tile_bits, int low_bits) {
         c.tile_bits = tile_bits;
     }
     if (low_bits) {
-        if (low_bits < 0 || low_bits > c.tile_bits) return fail(BSIM_EINVAL, "low_bits must be 0..tile_bits");
+        if (low_bits < 0 || low_bits > c.tile_bits - 2) return fail(BSIM_EINVAL, "low_bits must be 0..tile_bits-2");
         c.low_bits = low_bits;
     }
-    if (c.low_bits > c.tile_bits) c.low_bits = c.tile_bits;
+    if (c.low_bits > c.tile_bits - 2) c.low_bits = c.tile_bits - 2;
     return BSIM_OK;
 }
 

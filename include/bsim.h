@@ -141,7 +141,7 @@ int bsim_evolve(bsim_state* h, const bsim_op* ops, int n_ops, int mode);
 int bsim_plan_stats(int vector_qubits, int is_dm, const bsim_op* ops, int n_ops, int64_t* stats);
 
 /* Tile configuration of the fused path (tile_bits 6..13, contiguous low
- * bits 0..tile_bits); 0 keeps the current value. */
+ * bits 0..tile_bits-2); 0 keeps the current value. */
 int bsim_set_tile_config(int tile_bits, int low_bits);
 int bsim_get_tile_config(int* tile_bits, int* low_bits);
 
@@ -158,7 +158,8 @@ int bsim_kernel_stats(int reset, int64_t* launches, double* total_ms);
  * mode 0 = reference semantics (only blocks whose row and column agree on
  * every non-target bit; 2-qubit block order (00, t2, t1, 11), t1 = MSB;
  * k > 2: sorted endian targets), mode 1 = physical channel on every block
- * of the stored matrix (S <- sum conj(K) S K^T, S = rho^T).  k <= 4. */
+ * of the stored matrix (S <- sum conj(K) S K^T, S = rho^T).  k <= 5 in
+ * mode 0, k <= 3 in mode 1. */
 int bsim_dm_kraus(bsim_state* h, int k, const int* targets, int n_kraus, const double* kraus, int mode);
 /* apply_noise!(::PhaseFlip, dm, q): src/noise_kernels.jl:7-23. */
 int bsim_dm_phaseflip(bsim_state* h, int q, double p);

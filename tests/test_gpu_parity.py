@@ -86,7 +86,7 @@ def test_sv_random_vs_oracle(n, seed, mode):
     assert np.max(np.abs(got - exp)) < TOL
 
 
-@pytest.mark.parametrize("tile,low", [(6, 0), (8, 2), (10, 3), (12, 4), (13, 5), (12, 12)])
+@pytest.mark.parametrize("tile,low", [(6, 0), (8, 2), (10, 3), (12, 4), (13, 5), (12, 10)])
 def test_tile_configs(tile, low):
     _say(f"tile config {tile} {low}")
     n = 16
