@@ -342,176 +342,11 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
     return BSIM_OK;
 }
 
-// ---------------------------------------------------------------------------
-// tile kernel
-// ---------------------------------------------------------------------------
+}  // namespace bsim
 
-constexpr int TB = 256;
+#include "tile_kernel.h"
 
-__device__ __forceinline__ int phys(int j) { return j + (j >> 4); }
-
-__device__ __forceinline__ uint64_t gsum(int j, const uint64_t (&gb)[4]) {
-    return ((j & 1) ? gb[0] : 0) | ((j & 2) ? gb[1] : 0) | ((j & 4) ? gb[2] : 0) | ((j & 8) ? gb[3] : 0);
-}
-
-template <int A>
-__device__ __forceinline__ void sw_dense1(double2 (&v)[16], const TOp* __restrict__ op, uint64_t gbase,
-                                          const uint64_t (&gb)[4]) {
-    const double2 m00 = op->m[0], m10 = op->m[1], m01 = op->m[2], m11 = op->m[3];
-    const uint64_t cm = op->cmask, cv = op->cval;
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        if (j & (1 << A)) continue;
-        const int k = j | (1 << A);
-        const double2 a = v[j], b = v[k];
-        double2 x = cfma(m01, b, cmul(m00, a));
-        double2 y = cfma(m11, b, cmul(m10, a));
-        if (cm) {
-            bool on = ((gbase | gsum(j, gb)) & cm) == cv;
-            x = on ? x : a;
-            y = on ? y : b;
-        }
-        v[j] = x;
-        v[k] = y;
-    }
-}
-
-template <int A, int B>
-__device__ __forceinline__ void sw_dense2(double2 (&v)[16], const TOp* __restrict__ op, uint64_t gbase,
-                                          const uint64_t (&gb)[4]) {
-    double2 m[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) m[i] = op->m[i];
-    const uint64_t cm = op->cmask, cv = op->cval;
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        if (j & ((1 << A) | (1 << B))) continue;
-        const int id[4] = { j, j | (1 << A), j | (1 << B), j | (1 << A) | (1 << B) };
-        double2 a[4] = { v[id[0]], v[id[1]], v[id[2]], v[id[3]] };
-        double2 r[4];
-#pragma unroll
-        for (int row = 0; row < 4; ++row) {
-            double2 acc = cmul(m[row], a[0]);
-#pragma unroll
-            for (int c = 1; c < 4; ++c) acc = cfma(m[row + 4 * c], a[c], acc);
-            r[row] = acc;
-        }
-        bool on = true;
-        if (cm) on = ((gbase | gsum(j, gb)) & cm) == cv;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) v[id[q]] = on ? r[q] : a[q];
-    }
-}
-
-__device__ __forceinline__ void sw_diag(double2 (&v)[16], const TOp* __restrict__ op, uint64_t gbase,
-                                        const uint64_t (&gb)[4]) {
-    const int nd = op->nd;
-    const int d0 = op->dbit[0], d1 = op->dbit[1], d2 = op->dbit[2], d3 = op->dbit[3];
-    const uint64_t cm = op->cmask, cv = op->cval;
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        uint64_t g = gbase | gsum(j, gb);
-        int idx = (int)((g >> d0) & 1);
-        if (nd > 1) idx |= (int)((g >> d1) & 1) << 1;
-        if (nd > 2) idx |= (int)((g >> d2) & 1) << 2;
-        if (nd > 3) idx |= (int)((g >> d3) & 1) << 3;
-        double2 f = op->m[idx];
-        double2 x = cmul(f, v[j]);
-        if (cm) x = ((g & cm) == cv) ? x : v[j];
-        v[j] = x;
-    }
-}
-
-__device__ __forceinline__ void run_op(double2 (&v)[16], const TOp* __restrict__ op, uint64_t gbase,
-                                       const uint64_t (&gb)[4]) {
-    switch (op->kind * 16 + op->a * 4 + op->b) {
-        case 0 * 16 + 0: sw_dense1<0>(v, op, gbase, gb); break;
-        case 0 * 16 + 4: sw_dense1<1>(v, op, gbase, gb); break;
-        case 0 * 16 + 8: sw_dense1<2>(v, op, gbase, gb); break;
-        case 0 * 16 + 12: sw_dense1<3>(v, op, gbase, gb); break;
-        case 16 + 0 * 4 + 1: sw_dense2<0, 1>(v, op, gbase, gb); break;
-        case 16 + 0 * 4 + 2: sw_dense2<0, 2>(v, op, gbase, gb); break;
-        case 16 + 0 * 4 + 3: sw_dense2<0, 3>(v, op, gbase, gb); break;
-        case 16 + 1 * 4 + 2: sw_dense2<1, 2>(v, op, gbase, gb); break;
-        case 16 + 1 * 4 + 3: sw_dense2<1, 3>(v, op, gbase, gb); break;
-        case 16 + 2 * 4 + 3: sw_dense2<2, 3>(v, op, gbase, gb); break;
-        default: sw_diag(v, op, gbase, gb); break;
-    }
-}
-
-__global__ __launch_bounds__(TB) void k_tile(double2* __restrict__ sv, int vq, const Pass* __restrict__ pass,
-                                             const Sweep* __restrict__ sweeps, const TOp* __restrict__ tops) {
-    extern __shared__ double2 tile[];
-    const int T = pass->T;
-    const int nloc = 1 << T;
-    const uint64_t lmask = pass->lmask;
-    // global base: deposit the block index into the non-tile bits
-    uint64_t w = blockIdx.x, base = 0;
-    for (int bit = 0, k = 0; bit < vq; ++bit)
-        if (!((lmask >> bit) & 1)) {
-            base |= ((w >> k) & 1) << bit;
-            ++k;
-        }
-    // local j = tid + TB*i: the low 8 local bits come from tid
-    uint64_t s_t = 0;
-    const int tlow = T < 8 ? T : 8;
-    for (int b = 0; b < tlow; ++b)
-        if ((threadIdx.x >> b) & 1) s_t |= 1ull << pass->lbits[b];
-    for (int i = 0; (i << 8) < nloc; ++i) {
-        int j = (int)threadIdx.x + (i << 8);
-        if (j < nloc) {
-            uint64_t s_i = 0;
-            for (int b = 8; b < T; ++b)
-                if ((i >> (b - 8)) & 1) s_i |= 1ull << pass->lbits[b];
-            tile[phys(j)] = sv[base | s_t | s_i];
-        }
-    }
-    __syncthreads();
-    const int nsub = nloc >> 4;
-    for (int s = 0; s < pass->nsweeps; ++s) {
-        const Sweep* sw = sweeps + pass->sweep_begin + s;
-        const int sb0 = sw->sb[0], sb1 = sw->sb[1], sb2 = sw->sb[2], sb3 = sw->sb[3];
-        const int smask = (1 << sb0) | (1 << sb1) | (1 << sb2) | (1 << sb3);
-        const uint64_t gb[4] = { 1ull << pass->lbits[sb0], 1ull << pass->lbits[sb1], 1ull << pass->lbits[sb2],
-                                 1ull << pass->lbits[sb3] };
-        const int ob = sw->op_begin, oe = sw->op_end;
-        for (int c = threadIdx.x; c < nsub; c += TB) {
-            int lb = 0;
-            for (int bit = 0, k = 0; bit < T; ++bit)
-                if (!((smask >> bit) & 1)) {
-                    lb |= ((c >> k) & 1) << bit;
-                    ++k;
-                }
-            uint64_t gbase = base;
-            for (int bit = 0; bit < T; ++bit)
-                if ((lb >> bit) & 1) gbase |= 1ull << pass->lbits[bit];
-            double2 v[16];
-#pragma unroll
-            for (int j = 0; j < 16; ++j) {
-                int off = lb | ((j & 1) << sb0) | (((j >> 1) & 1) << sb1) | (((j >> 2) & 1) << sb2) |
-                          (((j >> 3) & 1) << sb3);
-                v[j] = tile[phys(off)];
-            }
-            for (int o = ob; o < oe; ++o) run_op(v, tops + o, gbase, gb);
-#pragma unroll
-            for (int j = 0; j < 16; ++j) {
-                int off = lb | ((j & 1) << sb0) | (((j >> 1) & 1) << sb1) | (((j >> 2) & 1) << sb2) |
-                          (((j >> 3) & 1) << sb3);
-                tile[phys(off)] = v[j];
-            }
-        }
-        __syncthreads();
-    }
-    for (int i = 0; (i << 8) < nloc; ++i) {
-        int j = (int)threadIdx.x + (i << 8);
-        if (j < nloc) {
-            uint64_t s_i = 0;
-            for (int b = 8; b < T; ++b)
-                if ((i >> (b - 8)) & 1) s_i |= 1ull << pass->lbits[b];
-            sv[base | s_t | s_i] = tile[phys(j)];
-        }
-    }
-}
+namespace bsim {
 
 // ---------------------------------------------------------------------------
 // execution

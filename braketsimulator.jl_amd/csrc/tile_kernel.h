@@ -1,0 +1,234 @@
+// The fused tile kernel (one HBM pass of a planned instruction block).
+//
+// Block = 256 threads, one tile of 2^T amplitudes in LDS (padded by one
+// element per 16 so that strided sub-cube reads spread over the banks).
+//   1. load: local index j = tid + 256 i; the global index is the block's
+//      base plus the scatter of j over the tile's vector bits (two small LDS
+//      tables: local bits 0..7 and 8..T-1).  Local bits 0..low_bits-1 are
+//      vector bits 0..low_bits-1, so each wavefront load covers contiguous
+//      runs of 2^low_bits amplitudes.
+//   2. sweeps: a sweep names 4 local bits; each thread gathers a
+//      16-amplitude sub-cube into registers, applies the sweep's gates there
+//      (dense 1- and 2-qubit on the 4 bits, diagonal/controls on any vector
+//      bit through the global index) and writes it back.
+//   3. store: the load pattern in reverse.
+// Every register index is a compile-time constant (sfor below), so the
+// sub-cube never leaves the VGPR file.
+#pragma once
+
+#include <type_traits>
+
+#include "fused.h"
+
+namespace bsim {
+
+constexpr int TB = 256;
+
+template <int I, int N, typename F>
+__device__ __forceinline__ void sfor(F&& f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        sfor<I + 1, N>(f);
+    }
+}
+
+__device__ __forceinline__ int phys(int j) { return j + (j >> 4); }
+
+struct c2 {
+    double x, y;
+};
+// The 16-amplitude sub-cube is one 32-wide vector value, so it stays in
+// VGPRs: every element access below uses a compile-time index.
+typedef double dv32 __attribute__((ext_vector_type(32)));
+template <int J>
+__device__ __forceinline__ c2 getv(const dv32& v) {
+    return { v[2 * J], v[2 * J + 1] };
+}
+template <int J>
+__device__ __forceinline__ void setv(dv32& v, c2 a) {
+    v[2 * J] = a.x;
+    v[2 * J + 1] = a.y;
+}
+__device__ __forceinline__ c2 to_c2(double2 a) { return { a.x, a.y }; }
+__device__ __forceinline__ double2 to_d2(c2 a) { return make_double2(a.x, a.y); }
+__device__ __forceinline__ c2 cmul(c2 a, c2 b) { return { a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x }; }
+__device__ __forceinline__ c2 cfma(c2 a, c2 b, c2 c) {
+    return { fma(a.x, b.x, fma(-a.y, b.y, c.x)), fma(a.x, b.y, fma(a.y, b.x, c.y)) };
+}
+
+struct SubCube {
+    dv32 v;
+    uint64_t gbase;            // global index of register 0
+    uint64_t g0, g1, g2, g3;   // global bit of register bits 0..3
+    template <int J>
+    __device__ __forceinline__ uint64_t gidx() const {
+        return gbase | ((J & 1) ? g0 : 0) | ((J & 2) ? g1 : 0) | ((J & 4) ? g2 : 0) | ((J & 8) ? g3 : 0);
+    }
+};
+
+template <int A>
+__device__ __forceinline__ void sw_dense1(SubCube& s, const TOp* __restrict__ op) {
+    const c2 m00 = to_c2(op->m[0]), m10 = to_c2(op->m[1]), m01 = to_c2(op->m[2]), m11 = to_c2(op->m[3]);
+    const uint64_t cm = op->cmask, cv = op->cval;
+    sfor<0, 16>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        if constexpr ((j & (1 << A)) == 0) {
+            constexpr int k = j | (1 << A);
+            const c2 a = getv<j>(s.v), b = getv<k>(s.v);
+            c2 x = cfma(m01, b, cmul(m00, a));
+            c2 y = cfma(m11, b, cmul(m10, a));
+            if (cm) {
+                const bool on = (s.template gidx<j>() & cm) == cv;
+                x = on ? x : a;
+                y = on ? y : b;
+            }
+            setv<j>(s.v, x);
+            setv<k>(s.v, y);
+        }
+    });
+}
+
+template <int A, int B>
+__device__ __forceinline__ void sw_dense2(SubCube& s, const TOp* __restrict__ op) {
+    const uint64_t cm = op->cmask, cv = op->cval;
+    sfor<0, 16>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        if constexpr ((j & ((1 << A) | (1 << B))) == 0) {
+            constexpr int i1 = j | (1 << A), i2 = j | (1 << B), i3 = j | (1 << A) | (1 << B);
+            const c2 a0 = getv<j>(s.v), a1 = getv<i1>(s.v), a2 = getv<i2>(s.v), a3 = getv<i3>(s.v);
+            auto row = [&](int r) {
+                c2 acc = cmul(to_c2(op->m[r]), a0);
+                acc = cfma(to_c2(op->m[r + 4]), a1, acc);
+                acc = cfma(to_c2(op->m[r + 8]), a2, acc);
+                return cfma(to_c2(op->m[r + 12]), a3, acc);
+            };
+            const c2 r0 = row(0), r1 = row(1), r2 = row(2), r3 = row(3);
+            bool on = true;
+            if (cm) on = (s.template gidx<j>() & cm) == cv;
+            setv<j>(s.v, on ? r0 : a0);
+            setv<i1>(s.v, on ? r1 : a1);
+            setv<i2>(s.v, on ? r2 : a2);
+            setv<i3>(s.v, on ? r3 : a3);
+        }
+    });
+}
+
+// Diagonal index of register j = (index bits read from the thread's base)
+// | (index bits carried by the register bits of j).  The second part is the
+// same for every thread, so it is built from uniform values.
+__device__ __forceinline__ void sw_diag(SubCube& s, const TOp* __restrict__ op) {
+    const int nd = op->nd;
+    int base_idx = 0, r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+    for (int i = 0; i < nd; ++i) {
+        const uint64_t bit = 1ull << op->dbit[i];
+        base_idx |= (s.gbase & bit) ? (1 << i) : 0;
+        r0 |= (s.g0 == bit) ? (1 << i) : 0;
+        r1 |= (s.g1 == bit) ? (1 << i) : 0;
+        r2 |= (s.g2 == bit) ? (1 << i) : 0;
+        r3 |= (s.g3 == bit) ? (1 << i) : 0;
+    }
+    const uint64_t cm = op->cmask, cv = op->cval;
+    const uint64_t tb = s.gbase & cm;
+    sfor<0, 16>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        const int idx = base_idx | ((j & 1) ? r0 : 0) | ((j & 2) ? r1 : 0) | ((j & 4) ? r2 : 0) | ((j & 8) ? r3 : 0);
+        const c2 f = to_c2(op->m[idx]);
+        const c2 x = cmul(f, getv<j>(s.v));
+        const uint64_t uj = (((j & 1) ? s.g0 : 0) | ((j & 2) ? s.g1 : 0) | ((j & 4) ? s.g2 : 0) | ((j & 8) ? s.g3 : 0)) & cm;
+        setv<j>(s.v, (cm == 0 || (tb | uj) == cv) ? x : getv<j>(s.v));
+    });
+}
+
+__device__ __forceinline__ void run_op(SubCube& s, const TOp* __restrict__ op) {
+    switch (op->kind * 16 + op->a * 4 + op->b) {
+        case 0: sw_dense1<0>(s, op); break;
+        case 4: sw_dense1<1>(s, op); break;
+        case 8: sw_dense1<2>(s, op); break;
+        case 12: sw_dense1<3>(s, op); break;
+        case 16 + 1: sw_dense2<0, 1>(s, op); break;
+        case 16 + 2: sw_dense2<0, 2>(s, op); break;
+        case 16 + 3: sw_dense2<0, 3>(s, op); break;
+        case 16 + 6: sw_dense2<1, 2>(s, op); break;
+        case 16 + 7: sw_dense2<1, 3>(s, op); break;
+        case 16 + 11: sw_dense2<2, 3>(s, op); break;
+        default: sw_diag(s, op); break;
+    }
+}
+
+__global__ __launch_bounds__(TB) void k_tile(double2* __restrict__ sv, int vq, const Pass* __restrict__ pass,
+                                             const Sweep* __restrict__ sweeps, const TOp* __restrict__ tops) {
+    extern __shared__ double2 tile[];
+    __shared__ uint64_t scat_lo[256];   // scatter of local bits 0..7
+    __shared__ uint64_t scat_hi[64];    // scatter of local bits 8..13
+    const int T = pass->T;
+    const int nloc = 1 << T;
+    const uint64_t lmask = pass->lmask;
+    {
+        const int t = threadIdx.x;
+        uint64_t s = 0;
+        for (int b = 0; b < 8 && b < T; ++b)
+            if ((t >> b) & 1) s |= 1ull << pass->lbits[b];
+        scat_lo[t] = s;
+        if (t < 64) {
+            uint64_t h = 0;
+            for (int b = 8; b < T; ++b)
+                if ((t >> (b - 8)) & 1) h |= 1ull << pass->lbits[b];
+            scat_hi[t] = h;
+        }
+    }
+    // global base: deposit the block index into the non-tile bits
+    uint64_t w = blockIdx.x, base = 0;
+    for (int bit = 0, k = 0; bit < vq; ++bit)
+        if (!((lmask >> bit) & 1)) {
+            base |= ((w >> k) & 1) << bit;
+            ++k;
+        }
+    __syncthreads();
+    const uint64_t s_t = base | scat_lo[threadIdx.x & 255];
+    for (int i = 0; (i << 8) < nloc; ++i) {
+        const int j = (int)threadIdx.x + (i << 8);
+        if (j < nloc) tile[phys(j)] = sv[s_t | scat_hi[i]];
+    }
+    __syncthreads();
+    const int nsub = nloc >> 4;
+    for (int sidx = 0; sidx < pass->nsweeps; ++sidx) {
+        const Sweep* sw = sweeps + pass->sweep_begin + sidx;
+        const int sb0 = sw->sb[0], sb1 = sw->sb[1], sb2 = sw->sb[2], sb3 = sw->sb[3];
+        const int smask = (1 << sb0) | (1 << sb1) | (1 << sb2) | (1 << sb3);
+        const int ob = sw->op_begin, oe = sw->op_end;
+        for (int c = threadIdx.x; c < nsub; c += TB) {
+            int lb = 0;
+            for (int bit = 0, k = 0; bit < T; ++bit)
+                if (!((smask >> bit) & 1)) {
+                    lb |= ((c >> k) & 1) << bit;
+                    ++k;
+                }
+            SubCube s;
+            s.gbase = base | scat_lo[lb & 255] | scat_hi[lb >> 8];
+            auto gl = [&](int l) { return scat_lo[(1 << l) & 255] | scat_hi[(1 << l) >> 8]; };
+            s.g0 = gl(sb0);
+            s.g1 = gl(sb1);
+            s.g2 = gl(sb2);
+            s.g3 = gl(sb3);
+            const int o0 = 1 << sb0, o1 = 1 << sb1, o2 = 1 << sb2, o3 = 1 << sb3;
+            sfor<0, 16>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                const int off = lb | ((j & 1) ? o0 : 0) | ((j & 2) ? o1 : 0) | ((j & 4) ? o2 : 0) | ((j & 8) ? o3 : 0);
+                setv<j>(s.v, to_c2(tile[phys(off)]));
+            });
+            for (int o = ob; o < oe; ++o) run_op(s, tops + o);
+            sfor<0, 16>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                const int off = lb | ((j & 1) ? o0 : 0) | ((j & 2) ? o1 : 0) | ((j & 4) ? o2 : 0) | ((j & 8) ? o3 : 0);
+                tile[phys(off)] = to_d2(getv<j>(s.v));
+            });
+        }
+        __syncthreads();
+    }
+    for (int i = 0; (i << 8) < nloc; ++i) {
+        const int j = (int)threadIdx.x + (i << 8);
+        if (j < nloc) sv[s_t | scat_hi[i]] = tile[phys(j)];
+    }
+}
+
+}  // namespace bsim
