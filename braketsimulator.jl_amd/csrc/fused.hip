@@ -439,8 +439,10 @@ static bool g_shmem_set = false;
 
 static int prepare_kernel_attrs() {
     if (g_shmem_set) return BSIM_OK;
-    // allow > 64 KiB of dynamic LDS for 13-bit tiles
-    BSIM_HIP(hipFuncSetAttribute((const void*)k_tile, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    // allow > 64 KiB of dynamic LDS: a padded 13-bit tile (the kernel's
+    // static scatter tables take another 2.5 KiB of the 160 KiB)
+    BSIM_HIP(hipFuncSetAttribute((const void*)k_tile, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)(sizeof(double2) * ((1 << 13) + (1 << 9)))));
     g_shmem_set = true;
     return BSIM_OK;
 }
