@@ -93,6 +93,7 @@ _SIGS = {
     "bsim_plan_stats": (C.c_int, [C.c_int, C.c_int, C.POINTER(Op), C.c_int, _i64p]),
     "bsim_set_tile_config": (C.c_int, [C.c_int, C.c_int]),
     "bsim_get_tile_config": (C.c_int, [_ip, _ip]),
+    "bsim_set_fusion_limit": (C.c_int, [C.c_int, _ip]),
     "bsim_set_profiling": (C.c_int, [C.c_int]),
     "bsim_kernel_stats": (C.c_int, [C.c_int, _i64p, _dp]),
     "bsim_dm_kraus": (C.c_int, [C.c_void_p, C.c_int, _ip, C.c_int, _dp, C.c_int]),

@@ -36,11 +36,13 @@ int lower_ops(int nq, int kind, const bsim_op* ops, int n_ops, std::vector<EOp>&
 // Device-side tile program ------------------------------------------------
 
 struct TOp {
-    int32_t kind;      // 0 dense1, 1 dense2, 2 diag
+    int32_t kind;      // 0 dense1, 1 dense2, 2 diag, 3 sign parity
     int32_t a, b;      // dense: register bits (a < b for dense2)
     int32_t nd;        // diag: bit count
     int32_t dbit[4];   // diag: vector bit of diagonal index bit i
     uint64_t cmask, cval;
+    uint64_t pm;       // sign: amplitude i is negated when popcount(i & (i >> pd) & pm) is odd
+    int32_t pd, pad_;
     double2 m[16];
 };
 
@@ -75,6 +77,10 @@ struct Plan {
 struct TileConfig {
     int tile_bits = 12;
     int low_bits = 4;
+    // FP64 work budget of one pass, in operations per amplitude (dense 1q
+    // = 8, dense 2q = 32, diagonal = 4, sign = 1); 0 = unlimited.  Bounds
+    // the fusion width so that a pass stays HBM-bound.
+    int max_cost = 0;
 };
 TileConfig& tile_config();
 

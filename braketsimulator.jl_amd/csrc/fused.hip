@@ -187,6 +187,49 @@ static uint64_t target_mask(const EOp& e) {
     return t;
 }
 
+static bool is_pm1(double2 z, double s) { return z.x == s && z.y == 0.0; }
+
+// A diagonal whose entries are +1 except -1 where two bits are both set (a
+// CZ in either spelling: 2-bit diag(1,1,1,-1), or Z controlled on value 1),
+// or a single-bit diag(1,-1): sign = parity(i & (i >> pd) & pm).
+static bool sign_pair(const EOp& e, uint64_t& pm, int& pd) {
+    const GateSpec& g = e.g;
+    if (g.k == 1 && is_pm1(g.m[0], 1.0) && is_pm1(g.m[1], -1.0)) {
+        if (g.cmask == 0) {
+            pm = 1ull << g.tb[0];
+            pd = 0;
+            return true;
+        }
+        if (popc(g.cmask) == 1 && g.cval == g.cmask) {
+            int c = __builtin_ctzll(g.cmask), t = g.tb[0];
+            int lo = c < t ? c : t, hi = c < t ? t : c;
+            pm = 1ull << lo;
+            pd = hi - lo;
+            return true;
+        }
+        return false;
+    }
+    if (g.k == 2 && g.cmask == 0 && is_pm1(g.m[0], 1.0) && is_pm1(g.m[1], 1.0) && is_pm1(g.m[2], 1.0) &&
+        is_pm1(g.m[3], -1.0)) {
+        int lo = g.tb[0] < g.tb[1] ? g.tb[0] : g.tb[1], hi = g.tb[0] < g.tb[1] ? g.tb[1] : g.tb[0];
+        pm = 1ull << lo;
+        pd = hi - lo;
+        return true;
+    }
+    return false;
+}
+
+// FP64 operations per amplitude of one op inside a tile sweep
+static int op_cost(const EOp& e) {
+    if (e.kind == E_DENSE) return e.g.k == 1 ? 8 : 32;
+    if (e.kind == E_DIAG) {
+        uint64_t pm;
+        int pd;
+        return sign_pair(e, pm, pd) ? 1 : 4;
+    }
+    return 0;
+}
+
 int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan& plan) {
     plan = Plan();
     const size_t n = eops.size();
@@ -212,25 +255,30 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
         // ---- pass: choose the tile bits and the ops it absorbs
         uint64_t S = (L > 0) ? ((1ull << L) - 1) : 0;
         uint64_t blocked = 0;
+        int cost = 0;
         inc.clear();
         for (size_t i = head; i < n; ++i) {
             if (done[i]) continue;
             const EOp& e = eops[i];
             uint64_t t = e.touched();
+            int c = op_cost(e);
+            bool over = cfg.max_cost > 0 && !inc.empty() && cost + c > cfg.max_cost;
             if (t & blocked) {
                 blocked |= t;
-            } else if (!e.fusable()) {
+            } else if (!e.fusable() || over) {
                 blocked |= t;
             } else if (e.kind == E_DENSE) {
                 uint64_t need = target_mask(e);
                 if ((need & ~S) == 0 || popc(S | need) <= T) {
                     S |= need;
                     inc.push_back((int)i);
+                    cost += c;
                 } else {
                     blocked |= t;
                 }
             } else {
                 inc.push_back((int)i);
+                cost += c;
             }
             if ((blocked & all) == all) break;
         }
@@ -294,7 +342,20 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
                 memset(&op, 0, sizeof(op));
                 op.cmask = e.g.cmask;
                 op.cval = e.g.cval;
-                if (e.kind == E_DIAG) {
+                uint64_t pm = 0;
+                int pd = 0;
+                if (e.kind == E_DIAG && sign_pair(e, pm, pd)) {
+                    // consecutive sign ops with the same bit distance multiply:
+                    // their pair masks add modulo 2
+                    if (plan.tops.size() > (size_t)sw.op_begin && plan.tops.back().kind == 3 &&
+                        plan.tops.back().pd == pd) {
+                        plan.tops.back().pm ^= pm;
+                        continue;
+                    }
+                    op.kind = 3;
+                    op.pm = pm;
+                    op.pd = pd;
+                } else if (e.kind == E_DIAG) {
                     op.kind = 2;
                     op.nd = e.g.k;
                     for (int b = 0; b < e.g.k; ++b) op.dbit[b] = e.g.tb[b];
@@ -470,6 +531,13 @@ int bsim_set_tile_config(int tile_bits, int low_bits) {
 int bsim_get_tile_config(int* tile_bits, int* low_bits) {
     if (tile_bits) *tile_bits = tile_config().tile_bits;
     if (low_bits) *low_bits = tile_config().low_bits;
+    return BSIM_OK;
+}
+
+int bsim_set_fusion_limit(int max_cost, int* old) {
+    if (max_cost < 0) return fail(BSIM_EINVAL, "max_cost must be >= 0");
+    if (old) *old = tile_config().max_cost;
+    tile_config().max_cost = max_cost;
     return BSIM_OK;
 }
 

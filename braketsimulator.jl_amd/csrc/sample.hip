@@ -22,12 +22,11 @@
 // * 2^-53, shot = u < ap[index] ? index : alias[index].
 #include <vector>
 
+#include "alias_kernels.h"
 #include "common.h"
 #include "reduce.h"
 
 namespace bsim {
-
-constexpr uint64_t ALIAS_C = 1ull << 32;
 
 // ---- Philox-4x32-10 ------------------------------------------------------
 __host__ __device__ __forceinline__ void philox4x32_10(uint32_t ctr[4], uint32_t k0, uint32_t k1) {
@@ -132,164 +131,6 @@ __global__ void k_fix_remainder(uint64_t* __restrict__ W, int64_t n, const unsig
     }
 }
 
-constexpr int SCAN_ITEMS = 8;
-constexpr int SCAN_BLOCK = 256;
-constexpr int SCAN_TILE = SCAN_ITEMS * SCAN_BLOCK;
-
-__device__ __forceinline__ void split_dw(uint64_t w, uint64_t& d, uint64_t& e) {
-    d = w <= ALIAS_C ? ALIAS_C - w : 0;
-    e = w > ALIAS_C ? w - ALIAS_C : 0;
-}
-
-// block-wide inclusive scan of (d, e) pairs held one per thread
-__device__ __forceinline__ void block_scan2(uint64_t& d, uint64_t& e, uint64_t* shd, uint64_t* she) {
-    shd[threadIdx.x] = d;
-    she[threadIdx.x] = e;
-    __syncthreads();
-    for (int o = 1; o < SCAN_BLOCK; o <<= 1) {
-        uint64_t ad = 0, ae = 0;
-        if ((int)threadIdx.x >= o) {
-            ad = shd[threadIdx.x - o];
-            ae = she[threadIdx.x - o];
-        }
-        __syncthreads();
-        shd[threadIdx.x] += ad;
-        she[threadIdx.x] += ae;
-        __syncthreads();
-    }
-    d = shd[threadIdx.x];
-    e = she[threadIdx.x];
-    __syncthreads();
-}
-
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(const uint64_t* __restrict__ W, int64_t n,
-                                                           uint64_t* __restrict__ tsum) {
-    __shared__ uint64_t shd[SCAN_BLOCK], she[SCAN_BLOCK];
-    int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
-    uint64_t d = 0, e = 0;
-    for (int k = 0; k < SCAN_ITEMS; ++k) {
-        int64_t i = base + k;
-        if (i < n) {
-            uint64_t a, b;
-            split_dw(W[i], a, b);
-            d += a;
-            e += b;
-        }
-    }
-    block_scan2(d, e, shd, she);
-    if (threadIdx.x == SCAN_BLOCK - 1) {
-        tsum[2 * blockIdx.x] = d;
-        tsum[2 * blockIdx.x + 1] = e;
-    }
-}
-
-// exclusive scan of tile sums in place (one block, sequential chunks)
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tilesums(uint64_t* __restrict__ tsum, int64_t ntiles) {
-    __shared__ uint64_t shd[SCAN_BLOCK], she[SCAN_BLOCK];
-    int64_t per = (ntiles + SCAN_BLOCK - 1) / SCAN_BLOCK;
-    int64_t b0 = (int64_t)threadIdx.x * per;
-    uint64_t d = 0, e = 0;
-    for (int64_t k = 0; k < per; ++k) {
-        int64_t i = b0 + k;
-        if (i < ntiles) {
-            d += tsum[2 * i];
-            e += tsum[2 * i + 1];
-        }
-    }
-    uint64_t di = d, ei = e;
-    block_scan2(di, ei, shd, she);
-    uint64_t od = di - d, oe = ei - e;  // exclusive prefix of this thread's chunk
-    for (int64_t k = 0; k < per; ++k) {
-        int64_t i = b0 + k;
-        if (i < ntiles) {
-            uint64_t td = tsum[2 * i], te = tsum[2 * i + 1];
-            tsum[2 * i] = od;
-            tsum[2 * i + 1] = oe;
-            od += td;
-            oe += te;
-        }
-    }
-}
-
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(const uint64_t* __restrict__ W, int64_t n,
-                                                           const uint64_t* __restrict__ tsum,
-                                                           uint64_t* __restrict__ Dinc, uint64_t* __restrict__ Einc) {
-    __shared__ uint64_t shd[SCAN_BLOCK], she[SCAN_BLOCK];
-    int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
-    uint64_t dl[SCAN_ITEMS], el[SCAN_ITEMS];
-    uint64_t d = 0, e = 0;
-    for (int k = 0; k < SCAN_ITEMS; ++k) {
-        int64_t i = base + k;
-        uint64_t a = 0, b = 0;
-        if (i < n) split_dw(W[i], a, b);
-        dl[k] = a;
-        el[k] = b;
-        d += a;
-        e += b;
-    }
-    uint64_t di = d, ei = e;
-    block_scan2(di, ei, shd, she);
-    uint64_t od = tsum[2 * blockIdx.x] + di - d, oe = tsum[2 * blockIdx.x + 1] + ei - e;
-    for (int k = 0; k < SCAN_ITEMS; ++k) {
-        int64_t i = base + k;
-        od += dl[k];
-        oe += el[k];
-        if (i < n) {
-            Dinc[i] = od;
-            Einc[i] = oe;
-        }
-    }
-}
-
-// first index k in [lo, n) with a[k] > v (a non-decreasing), n if none
-__device__ __forceinline__ int64_t first_greater(const uint64_t* __restrict__ a, int64_t lo, int64_t n, uint64_t v) {
-    int64_t hi = n;
-    while (lo < hi) {
-        int64_t mid = lo + ((hi - lo) >> 1);
-        if (a[mid] > v) hi = mid;
-        else lo = mid + 1;
-    }
-    return lo;
-}
-// first index k with a[k] >= v
-__device__ __forceinline__ int64_t first_geq(const uint64_t* __restrict__ a, int64_t n, uint64_t v) {
-    int64_t lo = 0, hi = n;
-    while (lo < hi) {
-        int64_t mid = lo + ((hi - lo) >> 1);
-        if (a[mid] >= v) hi = mid;
-        else lo = mid + 1;
-    }
-    return lo;
-}
-
-__global__ void k_alias_sweep(const uint64_t* __restrict__ W, int64_t n, const uint64_t* __restrict__ Dinc,
-                              const uint64_t* __restrict__ Einc, double* __restrict__ ap, int32_t* __restrict__ alias) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        uint64_t w = W[i];
-        if (w <= ALIAS_C) {
-            ap[i] = (double)w / (double)ALIAS_C;
-            if (w == ALIAS_C) {
-                alias[i] = (int32_t)i;
-            } else {
-                uint64_t dex = Dinc[i] - (ALIAS_C - w);
-                int64_t k = first_greater(Einc, 0, n, dex);
-                alias[i] = (int32_t)(k < n ? k : i);
-            }
-        } else {
-            uint64_t e = Einc[i];
-            int64_t k = first_geq(Dinc, n, e);
-            uint64_t r = k < n ? ALIAS_C + e - Dinc[k] : ALIAS_C;
-            int64_t nx = first_greater(Einc, i + 1, n, e);
-            if (nx >= n) {
-                r = ALIAS_C;
-                nx = i;
-            }
-            ap[i] = (double)r / (double)ALIAS_C;
-            alias[i] = (int32_t)nx;
-        }
-    }
-}
-
 // ---- lookups ---------------------------------------------------------------
 __global__ void k_sample(const double* __restrict__ ap, const int32_t* __restrict__ alias, int nq, int64_t shots,
                          uint32_t k0, uint32_t k1, int64_t* __restrict__ out) {
@@ -328,7 +169,7 @@ static int alias_build(bsim_state* h, int mode, double wsum) {
     if (int rc = ensure_ws(h->alias, sizeof(int32_t) * (size_t)n)) return rc;
     // work: p (8n) | W (8n) | Dinc (8n) | Einc (8n) | tile sums | scalars
     int64_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
-    size_t bytes = sizeof(uint64_t) * ((size_t)4 * n + 2 * (size_t)ntiles + 4096 + 64);
+    size_t bytes = sizeof(uint64_t) * ((size_t)4 * n + 4 * (size_t)ntiles + 4096 + 64) + sizeof(int32_t) * ((size_t)2 * n + 2);
     if (mode == BSIM_ALIAS_REFERENCE) bytes = sizeof(double) * (size_t)n + sizeof(int32_t) * 2 * (size_t)n + 65536;
     if (int rc = ensure_ws(h->alias_work, bytes)) return rc;
     char* base = (char*)h->alias_work.ptr;
@@ -351,10 +192,13 @@ static int alias_build(bsim_state* h, int mode, double wsum) {
         BSIM_HIP(hipGetLastError());
     } else if (mode == BSIM_ALIAS_PARALLEL) {
         uint64_t* W = (uint64_t*)(base + sizeof(double) * n);
-        uint64_t* Dinc = W + n;
-        uint64_t* Einc = Dinc + n;
-        uint64_t* tsum = Einc + n;
-        double* scal = (double*)(tsum + 2 * ntiles);
+        uint64_t* Lval = W + n;
+        uint64_t* Hval = Lval + n;
+        int32_t* Lidx = (int32_t*)(Hval + n);
+        int32_t* Hidx = Lidx + n;
+        Acc4* tsum = (Acc4*)(Hidx + n + (n & 1));
+        Acc4* total = tsum + ntiles;
+        double* scal = (double*)(total + 2);
         unsigned long long* acc = (unsigned long long*)(scal + 512);
         if (wsum <= 0) {
             hipLaunchKernelGGL(k_sum_p, dim3(256), dim3(256), 0, h->stream, p, n, scal + 8);
@@ -368,11 +212,15 @@ static int alias_build(bsim_state* h, int mode, double wsum) {
         hipLaunchKernelGGL(k_argmax, dim3((unsigned)g), dim3(256), 0, h->stream, W, n, acc);
         hipLaunchKernelGGL(k_fix_remainder, dim3(1), dim3(64), 0, h->stream, W, n, acc);
         hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)ntiles), dim3(SCAN_BLOCK), 0, h->stream, W, n, tsum);
-        hipLaunchKernelGGL(k_scan_tilesums, dim3(1), dim3(SCAN_BLOCK), 0, h->stream, tsum, ntiles);
-        hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)ntiles), dim3(SCAN_BLOCK), 0, h->stream, W, n, tsum, Dinc,
-                           Einc);
-        hipLaunchKernelGGL(k_alias_sweep, dim3((unsigned)grid_for(n, 256, cap * 8)), dim3(256), 0, h->stream, W, n,
-                           Dinc, Einc, ap, al);
+        hipLaunchKernelGGL(k_scan_tilesums, dim3(1), dim3(SCAN_BLOCK), 0, h->stream, tsum, ntiles, total);
+        hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)ntiles), dim3(SCAN_BLOCK), 0, h->stream, W, n,
+                           (const Acc4*)tsum, Lidx, Lval, Hidx, Hval);
+        int64_t runs = (n + ALIAS_RUN - 1) / ALIAS_RUN;
+        int64_t gr = grid_for(runs, 256, cap * 8);
+        hipLaunchKernelGGL(k_alias_lights, dim3((unsigned)gr), dim3(256), 0, h->stream, W, Lidx, Lval, Hidx, Hval,
+                           (const Acc4*)total, ap, al);
+        hipLaunchKernelGGL(k_alias_heavies, dim3((unsigned)gr), dim3(256), 0, h->stream, Lval, Hidx, Hval,
+                           (const Acc4*)total, ap, al);
         BSIM_HIP(hipGetLastError());
         // `init` lives on this frame: make sure the copy has been consumed
         BSIM_HIP(hipStreamSynchronize(h->stream));

@@ -55,6 +55,9 @@ __device__ __forceinline__ c2 cmul(c2 a, c2 b) { return { a.x * b.x - a.y * b.y,
 __device__ __forceinline__ c2 cfma(c2 a, c2 b, c2 c) {
     return { fma(a.x, b.x, fma(-a.y, b.y, c.x)), fma(a.x, b.y, fma(a.y, b.x, c.y)) };
 }
+// Component-wise select: `on ? a : b` on the struct itself becomes a select
+// between two stack addresses, i.e. scratch memory.
+__device__ __forceinline__ c2 csel(bool on, c2 a, c2 b) { return { on ? a.x : b.x, on ? a.y : b.y }; }
 
 struct SubCube {
     dv32 v;
@@ -79,8 +82,8 @@ __device__ __forceinline__ void sw_dense1(SubCube& s, const TOp* __restrict__ op
             c2 y = cfma(m11, b, cmul(m10, a));
             if (cm) {
                 const bool on = (s.template gidx<j>() & cm) == cv;
-                x = on ? x : a;
-                y = on ? y : b;
+                x = csel(on, x, a);
+                y = csel(on, y, b);
             }
             setv<j>(s.v, x);
             setv<k>(s.v, y);
@@ -105,10 +108,10 @@ __device__ __forceinline__ void sw_dense2(SubCube& s, const TOp* __restrict__ op
             const c2 r0 = row(0), r1 = row(1), r2 = row(2), r3 = row(3);
             bool on = true;
             if (cm) on = (s.template gidx<j>() & cm) == cv;
-            setv<j>(s.v, on ? r0 : a0);
-            setv<i1>(s.v, on ? r1 : a1);
-            setv<i2>(s.v, on ? r2 : a2);
-            setv<i3>(s.v, on ? r3 : a3);
+            setv<j>(s.v, csel(on, r0, a0));
+            setv<i1>(s.v, csel(on, r1, a1));
+            setv<i2>(s.v, csel(on, r2, a2));
+            setv<i3>(s.v, csel(on, r3, a3));
         }
     });
 }
@@ -135,12 +138,26 @@ __device__ __forceinline__ void sw_diag(SubCube& s, const TOp* __restrict__ op) 
         const c2 f = to_c2(op->m[idx]);
         const c2 x = cmul(f, getv<j>(s.v));
         const uint64_t uj = (((j & 1) ? s.g0 : 0) | ((j & 2) ? s.g1 : 0) | ((j & 4) ? s.g2 : 0) | ((j & 8) ? s.g3 : 0)) & cm;
-        setv<j>(s.v, (cm == 0 || (tb | uj) == cv) ? x : getv<j>(s.v));
+        setv<j>(s.v, csel(cm == 0 || (tb | uj) == cv, x, getv<j>(s.v)));
+    });
+}
+
+// +-1 diagonal (CZ layers): negate where popcount(i & (i >> pd) & pm) is odd
+__device__ __forceinline__ void sw_sign(SubCube& s, const TOp* __restrict__ op) {
+    const uint64_t pm = op->pm;
+    const int pd = op->pd;
+    sfor<0, 16>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        const uint64_t g = s.template gidx<j>();
+        const bool neg = __popcll(g & (g >> pd) & pm) & 1;
+        const c2 a = getv<j>(s.v);
+        setv<j>(s.v, { neg ? -a.x : a.x, neg ? -a.y : a.y });
     });
 }
 
 __device__ __forceinline__ void run_op(SubCube& s, const TOp* __restrict__ op) {
     switch (op->kind * 16 + op->a * 4 + op->b) {
+        case 48: sw_sign(s, op); break;
         case 0: sw_dense1<0>(s, op); break;
         case 4: sw_dense1<1>(s, op); break;
         case 8: sw_dense1<2>(s, op); break;
@@ -185,9 +202,21 @@ __global__ __launch_bounds__(TB) void k_tile(double2* __restrict__ sv, int vq, c
         }
     __syncthreads();
     const uint64_t s_t = base | scat_lo[threadIdx.x & 255];
-    for (int i = 0; (i << 8) < nloc; ++i) {
-        const int j = (int)threadIdx.x + (i << 8);
-        if (j < nloc) tile[phys(j)] = sv[s_t | scat_hi[i]];
+    const int iters = nloc >> 8;  // rows of 256 amplitudes (T >= 8)
+    if (iters == 0) {
+        if ((int)threadIdx.x < nloc) tile[phys(threadIdx.x)] = sv[s_t];
+    }
+    // issue up to 16 loads per thread before the first LDS write
+    for (int i0 = 0; i0 < iters; i0 += 16) {
+        dv32 buf;
+        sfor<0, 16>([&](auto J) {
+            constexpr int jj = decltype(J)::value;
+            if (i0 + jj < iters) setv<jj>(buf, to_c2(sv[s_t | scat_hi[i0 + jj]]));
+        });
+        sfor<0, 16>([&](auto J) {
+            constexpr int jj = decltype(J)::value;
+            if (i0 + jj < iters) tile[phys((int)threadIdx.x + ((i0 + jj) << 8))] = to_d2(getv<jj>(buf));
+        });
     }
     __syncthreads();
     const int nsub = nloc >> 4;
@@ -225,9 +254,26 @@ __global__ __launch_bounds__(TB) void k_tile(double2* __restrict__ sv, int vq, c
         }
         __syncthreads();
     }
-    for (int i = 0; (i << 8) < nloc; ++i) {
-        const int j = (int)threadIdx.x + (i << 8);
-        if (j < nloc) sv[s_t | scat_hi[i]] = tile[phys(j)];
+    if (iters == 0) {
+        if ((int)threadIdx.x < nloc) sv[s_t] = tile[phys(threadIdx.x)];
+    }
+    // read up to 16 rows from LDS, then stream them out (non-temporal: the
+    // pass never re-reads what it wrote)
+    for (int i0 = 0; i0 < iters; i0 += 16) {
+        dv32 buf;
+        sfor<0, 16>([&](auto J) {
+            constexpr int jj = decltype(J)::value;
+            if (i0 + jj < iters) setv<jj>(buf, to_c2(tile[phys((int)threadIdx.x + ((i0 + jj) << 8))]));
+        });
+        sfor<0, 16>([&](auto J) {
+            constexpr int jj = decltype(J)::value;
+            if (i0 + jj < iters) {
+                double* p = (double*)(sv + (s_t | scat_hi[i0 + jj]));
+                const c2 a = getv<jj>(buf);
+                __builtin_nontemporal_store(a.x, p);
+                __builtin_nontemporal_store(a.y, p + 1);
+            }
+        });
     }
 }
 

@@ -144,6 +144,10 @@ int bsim_plan_stats(int vector_qubits, int is_dm, const bsim_op* ops, int n_ops,
  * bits 0..tile_bits-2); 0 keeps the current value. */
 int bsim_set_tile_config(int tile_bits, int low_bits);
 int bsim_get_tile_config(int* tile_bits, int* low_bits);
+/* Fusion width of the fused path: FP64 operations per amplitude one pass
+ * may spend (1-qubit gate 8, 2-qubit 32, diagonal 4, CZ-type sign 1);
+ * 0 = unlimited.  Returns the previous value in *old (may be NULL). */
+int bsim_set_fusion_limit(int max_cost, int* old);
 
 /* Kernel timing of the fused path (no reference counterpart; measurement
  * only): when enabled, every tile-pass launch is bracketed by HIP events on
