@@ -77,6 +77,8 @@ def main():
     ap.add_argument("--depth", type=int, default=20)
     ap.add_argument("--shots", type=int, default=1000000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--adjoint-qubits", type=int, default=24, help="0 skips the adjoint leg")
+    ap.add_argument("--ps-sample", type=int, default=24, help="parameters timed by parameter shift")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -182,10 +184,57 @@ def main():
         },
         "z_checksum": float(np.sum(z)),
     }
+    del sim
+    if a.adjoint_qubits > 0:
+        rec["adjoint"] = adjoint_leg(a.adjoint_qubits, a.ps_sample)
     if not a.no_cpu:
         log("cpu baseline (oracle) ...")
         rec["cpu_baseline"] = cpu_baseline(n, ins)
     print(json.dumps(rec), flush=True)
+
+
+def adjoint_leg(nq: int, ps_sample: int):
+    """24-qubit hardware-efficient ansatz (8 layers, 384 parameters) with the
+    open-chain transverse-field Ising H: full adjoint gradient vs parameter
+    shift through the plain GPU path on a sample of parameters."""
+    import numpy as np
+
+    from bsim import adjoint as A
+    from bsim import workloads as W
+    from bsim.simulators import StateVectorSimulator
+
+    ins, pmap, th = W.hea(nq, 8, seed=0)
+    words, coeffs = W.tfim(nq)
+    sim = StateVectorSimulator(nq, 0)
+    log(f"adjoint: {nq} qubits, {len(th)} parameters")
+    A.adjoint_gradient(sim, ins, pmap, words, coeffs)  # warm-up
+    reps = 3
+    t = time.perf_counter()
+    for _ in range(reps):
+        ev, grad = A.adjoint_gradient(sim, ins, pmap, words, coeffs)
+    t_adj = (time.perf_counter() - t) / reps
+    # parameter shift on the first `ps_sample` parameters (2 evolutions each)
+    sample = list(range(min(ps_sample, len(th))))
+    t = time.perf_counter()
+    ps = np.zeros(len(sample))
+    for k, p in enumerate(sample):
+        tp = th.copy(); tp[p] += np.pi / 2
+        tm = th.copy(); tm[p] -= np.pi / 2
+        ps[k] = 0.5 * (A.energy(sim, A.with_params(ins, pmap, tp), words, coeffs)
+                       - A.energy(sim, A.with_params(ins, pmap, tm), words, coeffs))
+    t_ps_per_param = (time.perf_counter() - t) / len(sample)
+    t_ps = t_ps_per_param * len(th)
+    return {
+        "workload": f"HEA {nq} qubits x 8 layers (RY RZ + CNOT ladder), {len(th)} params, TFIM H (47 Pauli words)",
+        "expval": ev,
+        "adjoint_s": round(t_adj, 4),
+        "grads_per_s": round(1.0 / t_adj, 3),
+        "param_shift_s_extrapolated": round(t_ps, 3),
+        "param_shift_grads_per_s": round(1.0 / t_ps, 4),
+        "speedup_vs_param_shift": round(t_ps / t_adj, 1),
+        "max_abs_diff_vs_param_shift": float(np.max(np.abs(grad[sample] - ps))),
+        "param_shift_sample": f"first {len(sample)} of {len(th)} parameters, 2 fused evolutions + <H> each",
+    }
 
 
 if __name__ == "__main__":

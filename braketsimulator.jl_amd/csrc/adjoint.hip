@@ -1,0 +1,352 @@
+// Adjoint differentiation of <H> for a Pauli-sum H (new: the reference has
+// the AdjointGradient type only, src/results.jl:90-160, and rejects the
+// pragma, src/pragmas.jl:67-68).
+//
+//   forward:  psi = U_L ... U_1 |0>          (fused tile passes)
+//   lambda = H psi,  <H> = Re <psi|lambda>
+//   backward, k = L..1:
+//       psi    <- U_k^dagger psi             (now psi_{k-1})
+//       g_p    += 2 Re <lambda| dU_k/dtheta_p |psi>
+//       lambda <- U_k^dagger lambda
+// One kernel per op reads (psi, lambda) once, un-applies U_k to both and
+// reduces the <lambda|dU|psi> products of the op's parameters in the same
+// pass (wavefront shuffle tree -> LDS -> per-block partials -> one ordered
+// sum), so the gradient is deterministic.
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "fused.h"
+#include "reduce.h"
+
+namespace bsim {
+
+constexpr int AB = 256;  // adjoint block
+
+__device__ __forceinline__ double wave_sum_adj(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+template <int P>
+__device__ __forceinline__ void block_sum_adj(double (&v)[P > 0 ? P : 1], double* lds, double* out) {
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+        double s = wave_sum_adj(v[i]);
+        if (lane == 0) lds[i * 8 + wid] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+            double s = 0;
+            for (int w = 0; w < AB / 64; ++w) s += lds[i * 8 + w];
+            out[i] = s;
+        }
+    }
+}
+
+struct AdjArgs {
+    int nins;
+    int ins[16];
+    uint64_t cval;
+    int64_t ngroups;
+    int t0, t1;
+};
+
+template <int K>
+struct AdjMats {
+    double2 udag[1 << (2 * K)];
+    double2 d[3][1 << (2 * K)];
+};
+
+// dense K = 1, 2 (with controls)
+template <int K, int P>
+__global__ __launch_bounds__(AB) void k_adj_dense(double2* __restrict__ psi, double2* __restrict__ lam, AdjArgs a,
+                                                  AdjMats<K> M, double* __restrict__ partials) {
+    constexpr int D = 1 << K;
+    __shared__ double lds[3 * 8];
+    double acc[P > 0 ? P : 1];
+#pragma unroll
+    for (int p = 0; p < (P > 0 ? P : 1); ++p) acc[p] = 0;
+    const uint64_t b0 = 1ull << a.t0, b1 = K > 1 ? (1ull << a.t1) : 0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < a.ngroups; g += stride) {
+        uint64_t base = (uint64_t)g;
+        for (int i = 0; i < a.nins; ++i) base = pad_bit(base, a.ins[i]);
+        base |= a.cval;
+        uint64_t off[D];
+        double2 x[D], l[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            off[j] = base | ((j & 1) ? b0 : 0) | ((j & 2) ? b1 : 0);
+            x[j] = psi[off[j]];
+            l[j] = lam[off[j]];
+        }
+        double2 xn[D], ln[D];
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+            double2 s = cmul(M.udag[r], x[0]), t = cmul(M.udag[r], l[0]);
+#pragma unroll
+            for (int c = 1; c < D; ++c) {
+                s = cfma(M.udag[r + D * c], x[c], s);
+                t = cfma(M.udag[r + D * c], l[c], t);
+            }
+            xn[r] = s;
+            ln[r] = t;
+        }
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+#pragma unroll
+            for (int r = 0; r < D; ++r) {
+                double2 mu = cmul(M.d[p][r], xn[0]);
+#pragma unroll
+                for (int c = 1; c < D; ++c) mu = cfma(M.d[p][r + D * c], xn[c], mu);
+                acc[p] += l[r].x * mu.x + l[r].y * mu.y;  // Re(conj(lambda) mu)
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            psi[off[j]] = xn[j];
+            lam[off[j]] = ln[j];
+        }
+    }
+    if (P > 0) block_sum_adj<P>(acc, lds, partials + (size_t)blockIdx.x * 3);
+}
+
+// diagonal, k <= 4 target bits anywhere, with controls
+template <int P>
+__global__ __launch_bounds__(AB) void k_adj_diag(double2* __restrict__ psi, double2* __restrict__ lam, AdjArgs a,
+                                                 int k, int4 tb, AdjMats<2> M, double* __restrict__ partials) {
+    __shared__ double lds[3 * 8];
+    double acc[P > 0 ? P : 1];
+#pragma unroll
+    for (int p = 0; p < (P > 0 ? P : 1); ++p) acc[p] = 0;
+    const int t[4] = { tb.x, tb.y, tb.z, tb.w };
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < a.ngroups; g += stride) {
+        uint64_t idx = (uint64_t)g;
+        for (int i = 0; i < a.nins; ++i) idx = pad_bit(idx, a.ins[i]);
+        idx |= a.cval;
+        int j = 0;
+        for (int b = 0; b < k; ++b) j |= (int)((idx >> t[b]) & 1) << b;
+        const double2 x = cmul(M.udag[j], psi[idx]);
+        const double2 l = lam[idx];
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const double2 mu = cmul(M.d[p][j], x);
+            acc[p] += l.x * mu.x + l.y * mu.y;
+        }
+        psi[idx] = x;
+        lam[idx] = cmul(M.udag[j], l);
+    }
+    if (P > 0) block_sum_adj<P>(acc, lds, partials + (size_t)blockIdx.x * 3);
+}
+
+// lambda_i = sum_w c_w phase_w(i) psi_{i ^ f_w}
+struct PWord {
+    uint64_t f, y, z;
+    int32_t ny, pad_;
+    double c;
+};
+
+__device__ __forceinline__ double2 pphase(uint64_t i, const PWord& w) {
+    int neg = (__popcll(~i & w.y) + __popcll(i & w.z)) & 1;
+    double s = neg ? -w.c : w.c;
+    switch (w.ny & 3) {
+        case 0: return make_double2(s, 0);
+        case 1: return make_double2(0, s);
+        case 2: return make_double2(-s, 0);
+        default: return make_double2(0, -s);
+    }
+}
+
+__global__ __launch_bounds__(AB) void k_pauli_sum(const double2* __restrict__ psi, double2* __restrict__ lam,
+                                                  int64_t len, const PWord* __restrict__ words, int nw) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride) {
+        double2 acc = make_double2(0, 0);
+        for (int w = 0; w < nw; ++w) acc = cfma(pphase((uint64_t)i, words[w]), psi[(uint64_t)i ^ words[w].f], acc);
+        lam[i] = acc;
+    }
+}
+
+__global__ void k_adj_finish(const double* __restrict__ partials, int nblocks, int np, double* __restrict__ out) {
+    int p = threadIdx.x;
+    if (p >= np) return;
+    double s = 0;
+    for (int b = 0; b < nblocks; ++b) s += partials[(size_t)b * 3 + p];
+    out[p] = s;
+}
+
+static void conj_transpose(const std::vector<double2>& m, int D, double2* out) {
+    for (int r = 0; r < D; ++r)
+        for (int c = 0; c < D; ++c) out[r + D * c] = make_double2(m[c + D * r].x, -m[c + D * r].y);
+}
+
+static AdjArgs adj_args(int vq, const GateSpec& g, bool include_targets) {
+    AdjArgs a;
+    memset(&a, 0, sizeof(a));
+    uint64_t mask = g.cmask;
+    if (include_targets)
+        for (int b = 0; b < g.k; ++b) mask |= 1ull << g.tb[b];
+    int n = 0;
+    for (int bit = 0; bit < vq; ++bit)
+        if (mask & (1ull << bit)) a.ins[n++] = bit;
+    a.nins = n;
+    a.cval = g.cval;
+    a.ngroups = (int64_t)1 << (vq - n);
+    a.t0 = g.k > 0 ? g.tb[0] : 0;
+    a.t1 = g.k > 1 ? g.tb[1] : 0;
+    return a;
+}
+
+// One backward step for eops[k]; the op's parameter products land in res[0..P).
+static int adjoint_step(bsim_state* h, double2* psi, double2* lam, const EOp& e, double* res, int grid) {
+    const GateSpec& g = e.g;
+    const int P = e.nparams;
+    double* partials = (double*)h->red.ptr;
+    const int vq = h->vq;
+    if (e.kind == E_DENSE && g.k <= 2) {
+        AdjArgs a = adj_args(vq, g, true);
+        const int D = 1 << g.k;
+        if (g.k == 1) {
+            AdjMats<1> M;
+            memset(&M, 0, sizeof(M));
+            conj_transpose(g.m, D, M.udag);
+            for (int p = 0; p < P; ++p)
+                for (int x = 0; x < D * D; ++x) M.d[p][x] = e.dm[p][x];
+            switch (P) {
+                case 0: hipLaunchKernelGGL((k_adj_dense<1, 0>), dim3(grid), dim3(AB), 0, h->stream, psi, lam, a, M, partials); break;
+                case 1: hipLaunchKernelGGL((k_adj_dense<1, 1>), dim3(grid), dim3(AB), 0, h->stream, psi, lam, a, M, partials); break;
+                case 2: hipLaunchKernelGGL((k_adj_dense<1, 2>), dim3(grid), dim3(AB), 0, h->stream, psi, lam, a, M, partials); break;
+                default: hipLaunchKernelGGL((k_adj_dense<1, 3>), dim3(grid), dim3(AB), 0, h->stream, psi, lam, a, M, partials); break;
+            }
+        } else {
+            AdjMats<2> M;
+            memset(&M, 0, sizeof(M));
+            conj_transpose(g.m, D, M.udag);
+            for (int p = 0; p < P; ++p)
+                for (int x = 0; x < D * D; ++x) M.d[p][x] = e.dm[p][x];
+            switch (P) {
+                case 0: hipLaunchKernelGGL((k_adj_dense<2, 0>), dim3(grid), dim3(AB), 0, h->stream, psi, lam, a, M, partials); break;
+                case 1: hipLaunchKernelGGL((k_adj_dense<2, 1>), dim3(grid), dim3(AB), 0, h->stream, psi, lam, a, M, partials); break;
+                case 2: hipLaunchKernelGGL((k_adj_dense<2, 2>), dim3(grid), dim3(AB), 0, h->stream, psi, lam, a, M, partials); break;
+                default: hipLaunchKernelGGL((k_adj_dense<2, 3>), dim3(grid), dim3(AB), 0, h->stream, psi, lam, a, M, partials); break;
+            }
+        }
+    } else if (e.kind == E_DIAG && g.k <= 4) {
+        AdjArgs a = adj_args(vq, g, false);
+        AdjMats<2> M;  // 16 diagonal entries fit the 4x4 slots
+        memset(&M, 0, sizeof(M));
+        const int D = 1 << g.k;
+        for (int j = 0; j < D; ++j) M.udag[j] = make_double2(g.m[j].x, -g.m[j].y);
+        for (int p = 0; p < P; ++p)
+            for (int j = 0; j < D; ++j) M.d[p][j] = e.dm[p][j + D * j];  // dU of a diagonal gate is diagonal
+        int4 tb = make_int4(g.tb[0], g.k > 1 ? g.tb[1] : 0, g.k > 2 ? g.tb[2] : 0, g.k > 3 ? g.tb[3] : 0);
+        switch (P) {
+            case 0: hipLaunchKernelGGL((k_adj_diag<0>), dim3(grid), dim3(AB), 0, h->stream, psi, lam, a, g.k, tb, M, partials); break;
+            case 1: hipLaunchKernelGGL((k_adj_diag<1>), dim3(grid), dim3(AB), 0, h->stream, psi, lam, a, g.k, tb, M, partials); break;
+            case 2: hipLaunchKernelGGL((k_adj_diag<2>), dim3(grid), dim3(AB), 0, h->stream, psi, lam, a, g.k, tb, M, partials); break;
+            default: hipLaunchKernelGGL((k_adj_diag<3>), dim3(grid), dim3(AB), 0, h->stream, psi, lam, a, g.k, tb, M, partials); break;
+        }
+    } else {
+        if (P > 0) return fail(BSIM_ENOTSUP, "adjoint: parametrized gates act on at most 2 dense or 4 diagonal targets");
+        // un-apply a parameter-free gate to both vectors with the per-gate kernels
+        GateSpec inv = g;
+        if (g.diag) {
+            for (auto& z : inv.m) z.y = -z.y;
+        } else {
+            conj_transpose(g.m, 1 << g.k, inv.m.data());
+        }
+        if (int rc = launch_gate(psi, vq, inv, h->scratch, h->stream)) return rc;
+        if (int rc = launch_gate(lam, vq, inv, h->scratch, h->stream)) return rc;
+        // the staged matrix is reused by the second launch: wait before it changes
+        BSIM_HIP(hipStreamSynchronize(h->stream));
+        return BSIM_OK;
+    }
+    BSIM_HIP(hipGetLastError());
+    if (P > 0) {
+        hipLaunchKernelGGL(k_adj_finish, dim3(1), dim3(64), 0, h->stream, (const double*)partials, grid, P, res);
+        BSIM_HIP(hipGetLastError());
+    }
+    return BSIM_OK;
+}
+
+}  // namespace bsim
+
+using namespace bsim;
+
+extern "C" {
+
+int bsim_adjoint_gradient(bsim_state* h, const bsim_op* ops, int n_ops, int n_words, const char* paulis,
+                          const double* coeffs, int n_params, double* grad, double* expval) {
+    if (int rc = check_state(h)) return rc;
+    if (h->kind != BSIM_SV) return fail(BSIM_ENOTSUP, "adjoint gradients need a state vector");
+    if (n_ops < 0 || (n_ops > 0 && !ops) || n_words < 1 || !paulis || !coeffs || n_params < 0 ||
+        (n_params > 0 && !grad))
+        return fail(BSIM_EINVAL, "bad adjoint arguments");
+    std::vector<EOp> eops;
+    if (int rc = lower_ops(h->nq, h->kind, ops, n_ops, eops, true)) return rc;
+    for (const EOp& e : eops)
+        for (int p = 0; p < e.nparams; ++p)
+            if (e.pidx[p] < 0 || e.pidx[p] >= n_params) return fail(BSIM_EINVAL, "param_index out of range");
+    std::vector<PWord> words(n_words);
+    for (int w = 0; w < n_words; ++w) {
+        PauliMask pm;
+        if (int rc = parse_pauli(h->nq, paulis + (size_t)w * h->nq, pm)) return rc;
+        words[w] = { pm.x | pm.y, pm.y, pm.z, pm.ny, 0, coeffs[w] };
+    }
+    const int64_t len = h->len;
+    // buffers: lambda and a working copy of psi (the handle keeps psi_L)
+    if (int rc = ensure_ws(h->lam, sizeof(double2) * (size_t)len * 2)) return rc;
+    double2* lam = (double2*)h->lam.ptr;
+    double2* work = lam + len;
+    const int grid = 2048;
+    if (int rc = ensure_ws(h->red, sizeof(double) * (size_t)grid * 3 + 64)) return rc;
+    int nslots = 0;
+    for (const EOp& e : eops) nslots += e.nparams > 0 ? 1 : 0;
+    if (int rc = ensure_ws(h->res, sizeof(double) * (3 * (size_t)nslots + 8) + sizeof(PWord) * (size_t)n_words))
+        return rc;
+    double* res = (double*)h->res.ptr;
+    PWord* dwords = (PWord*)(res + 3 * (size_t)nslots + 8);
+    // forward
+    if (int rc = bsim_state_reset(h)) return rc;
+    Plan plan;
+    if (int rc = make_plan(h->vq, eops, tile_config(), plan)) return rc;
+    if (int rc = prepare_tile_kernel()) return rc;
+    if (int rc = run_plan(h, h->d, eops, plan)) return rc;
+    // lambda = H psi, <H>
+    BSIM_HIP(hipMemcpyAsync(dwords, words.data(), sizeof(PWord) * n_words, hipMemcpyHostToDevice, h->stream));
+    int64_t g = grid_for(len, AB, (int64_t)device_cu_count() * 32);
+    hipLaunchKernelGGL(k_pauli_sum, dim3((unsigned)g), dim3(AB), 0, h->stream, h->d, lam, len, dwords, n_words);
+    BSIM_HIP(hipGetLastError());
+    if (int rc = dot_device(h->d, lam, len, h->red, h->stream, res + 3 * (size_t)nslots)) return rc;
+    BSIM_HIP(hipMemcpyAsync(work, h->d, sizeof(double2) * (size_t)len, hipMemcpyDeviceToDevice, h->stream));
+    // backward
+    int slot = nslots;
+    for (int k = (int)eops.size() - 1; k >= 0; --k) {
+        const EOp& e = eops[k];
+        double* r = nullptr;
+        if (e.nparams > 0) r = res + 3 * (size_t)(--slot);
+        if (int rc = adjoint_step(h, work, lam, e, r, grid)) return rc;
+    }
+    std::vector<double> hres(3 * (size_t)nslots + 2);
+    BSIM_HIP(hipMemcpyAsync(hres.data(), res, sizeof(double) * hres.size(), hipMemcpyDeviceToHost, h->stream));
+    BSIM_HIP(hipStreamSynchronize(h->stream));
+    for (int p = 0; p < n_params; ++p) grad[p] = 0.0;
+    slot = 0;
+    for (const EOp& e : eops) {
+        if (e.nparams == 0) continue;
+        for (int p = 0; p < e.nparams; ++p) grad[e.pidx[p]] += 2.0 * hres[3 * (size_t)slot + p];
+        ++slot;
+    }
+    if (expval) *expval = hres[3 * (size_t)nslots];
+    h->alias_ready = false;
+    return BSIM_OK;
+}
+
+}  // extern "C"

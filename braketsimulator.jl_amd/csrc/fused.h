@@ -89,6 +89,9 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
 // Launch a plan on d (ops buffer reused).  `eops` provides single launches.
 int run_plan(bsim_state* h, double2* d, const std::vector<EOp>& eops, const Plan& plan);
 
+// One-time launch attributes of the tile kernel (dynamic LDS above 64 KiB).
+int prepare_tile_kernel();
+
 // Single elementary op launch (per-gate path).
 int launch_eop(bsim_state* h, double2* d, const EOp& e);
 

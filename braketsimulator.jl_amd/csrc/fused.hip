@@ -498,7 +498,7 @@ int run_plan(bsim_state* h, double2* d, const std::vector<EOp>& eops, const Plan
 
 static bool g_shmem_set = false;
 
-static int prepare_kernel_attrs() {
+int prepare_tile_kernel() {
     if (g_shmem_set) return BSIM_OK;
     // allow > 64 KiB of dynamic LDS: a padded 13-bit tile (the kernel's
     // static scatter tables take another 2.5 KiB of the 160 KiB)
@@ -582,7 +582,7 @@ int bsim_evolve(bsim_state* h, const bsim_op* ops, int n_ops, int mode) {
         for (const EOp& e : eops)
             if (int rc = launch_eop(h, h->d, e)) return rc;
     } else {
-        if (int rc = prepare_kernel_attrs()) return rc;
+        if (int rc = prepare_tile_kernel()) return rc;
         Plan plan;
         if (int rc = make_plan(h->vq, eops, tile_config(), plan)) return rc;
         if (int rc = run_plan(h, h->d, eops, plan)) return rc;

@@ -5,9 +5,6 @@ using namespace bsim;
 
 extern "C" {
 
-int bsim_adjoint_gradient(bsim_state*, const bsim_op*, int, int, const char*, const double*, int, double*, double*) {
-    return fail(BSIM_ENOTSUP, "not built");
-}
 int bsim_batch_run(const bsim_task*, int, double*) { return fail(BSIM_ENOTSUP, "not built"); }
 
 }

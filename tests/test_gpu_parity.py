@@ -299,3 +299,91 @@ def test_errors_are_reported():
         sim.apply_gate(G.H(), 7)
     with pytest.raises(bsim.BsimError):
         sim.samples(shots=10, seed=1, alias_mode=99)
+
+
+# ---- adjoint differentiation ---------------------------------------------------
+
+def _oracle_energy(n, ins, words, coeffs):
+    psi = O.evolve_sv(n, ins)
+    return sum(c * O.sv_expect_pauli(psi, n, w) for w, c in zip(words, coeffs))
+
+
+@pytest.mark.parametrize("n,layers", [(4, 2), (8, 3), (12, 4)])
+def test_adjoint_vs_parameter_shift(n, layers):
+    """Adjoint gradient of the hardware-efficient ansatz vs parameter shift
+    through the plain GPU path (1e-10) and vs the oracle's parameter shift."""
+    from bsim import adjoint as A
+
+    _say(f"adjoint n={n}")
+    ins, pmap, th = W.hea(n, layers, seed=n)
+    words, coeffs = W.tfim(n)
+    sim = StateVectorSimulator(n, 0)
+    ev, grad = A.adjoint_gradient(sim, ins, pmap, words, coeffs)
+    assert abs(ev - _oracle_energy(n, ins, words, coeffs)) < TOL
+    ps = A.parameter_shift_gradient(sim, ins, pmap, th, words, coeffs)
+    assert np.max(np.abs(grad - ps)) < TOL
+    if n <= 8:
+        ops = []
+        for p in range(len(th)):
+            tp = th.copy(); tp[p] += math.pi / 2
+            tm = th.copy(); tm[p] -= math.pi / 2
+            ops.append(0.5 * (_oracle_energy(n, A.with_params(ins, pmap, tp), words, coeffs)
+                              - _oracle_energy(n, A.with_params(ins, pmap, tm), words, coeffs)))
+        assert np.max(np.abs(grad - np.array(ops))) < TOL
+    fd = A.finite_difference_gradient(sim, ins, pmap, th, words, coeffs, h=1e-6)
+    assert np.max(np.abs(grad - fd)) < 1e-7  # FD roundoff ~ eps/h
+
+
+def test_adjoint_general_gates():
+    """U(theta, phi, lambda), controlled rotations, ZZ and CPhaseShift
+    parameters, checked against central differences of the oracle."""
+    from bsim import adjoint as A
+
+    n = 6
+    rng = np.random.default_rng(3)
+    ins, pmap = [], {}
+    p = 0
+    for _ in range(5):
+        for q in range(n):
+            ins.append(Instruction(G.U(*rng.uniform(-3, 3, 3)), [q]))
+            for j in range(3):
+                pmap[(len(ins) - 1, j)] = p
+                p += 1
+        for q in range(n - 1):
+            g = [G.ZZ(float(rng.uniform(-3, 3))), G.CPhaseShift(float(rng.uniform(-3, 3))),
+                 G.Control(G.Rx(float(rng.uniform(-3, 3))), (1,)), G.XX(float(rng.uniform(-3, 3)))][q % 4]
+            ins.append(Instruction(g, [q, q + 1]))
+            pmap[(len(ins) - 1, 0)] = p
+            p += 1
+        ins.append(Instruction(G.CNot(), [0, n - 1]))
+    th = np.zeros(p)
+    for (i, j), k in pmap.items():
+        th[k] = ins[i].operator.angles[j]
+    words = ["ZZIIII", "IXIXII", "YIIIIZ", "IIZYXI"]
+    coeffs = [0.7, -0.3, 1.1, 0.25]
+    sim = StateVectorSimulator(n, 0)
+    ev, grad = A.adjoint_gradient(sim, ins, pmap, words, coeffs)
+    h = 1e-5
+    fd = np.zeros(p)
+    for k in range(p):
+        tp = th.copy(); tp[k] += h
+        tm = th.copy(); tm[k] -= h
+        fd[k] = (_oracle_energy(n, A.with_params(ins, pmap, tp), words, coeffs)
+                 - _oracle_energy(n, A.with_params(ins, pmap, tm), words, coeffs)) / (2 * h)
+    assert abs(ev - _oracle_energy(n, ins, words, coeffs)) < TOL
+    assert np.max(np.abs(grad - fd)) < 1e-8
+
+
+def test_adjoint_torch_function():
+    import torch
+    from bsim import adjoint as A
+
+    n = 5
+    ins, pmap, th = W.hea(n, 2, seed=1)
+    words, coeffs = W.tfim(n)
+    sim = StateVectorSimulator(n, 0)
+    theta = torch.tensor(th, dtype=torch.float64, requires_grad=True)
+    e = A.ExpectationFunction.apply(theta, sim, ins, pmap, words, coeffs)
+    e.backward()
+    _, grad = A.adjoint_gradient(sim, ins, pmap, words, coeffs)
+    assert np.max(np.abs(theta.grad.numpy() - grad)) < 1e-12
