@@ -174,12 +174,23 @@ __global__ __launch_bounds__(AB) void k_pauli_sum(const double2* __restrict__ ps
     }
 }
 
-__global__ void k_adj_finish(const double* __restrict__ partials, int nblocks, int np, double* __restrict__ out) {
-    int p = threadIdx.x;
-    if (p >= np) return;
-    double s = 0;
-    for (int b = 0; b < nblocks; ++b) s += partials[(size_t)b * 3 + p];
-    out[p] = s;
+// One block per parameter slot: partials[slot][block][3] -> out[slot][3].
+// Each thread sums a fixed strided subset, then a fixed tree: deterministic.
+__global__ __launch_bounds__(256) void k_adj_finish(const double* __restrict__ partials, int nblocks,
+                                                    double* __restrict__ out) {
+    __shared__ double sh[3][256];
+    const double* p = partials + (size_t)blockIdx.x * nblocks * 3;
+    double s[3] = { 0, 0, 0 };
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+        for (int q = 0; q < 3; ++q) s[q] += p[(size_t)b * 3 + q];
+    for (int q = 0; q < 3; ++q) sh[q][threadIdx.x] = s[q];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int q = 0; q < 3; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) out[(size_t)blockIdx.x * 3 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
 static void conj_transpose(const std::vector<double2>& m, int D, double2* out) {
@@ -204,11 +215,11 @@ static AdjArgs adj_args(int vq, const GateSpec& g, bool include_targets) {
     return a;
 }
 
-// One backward step for eops[k]; the op's parameter products land in res[0..P).
-static int adjoint_step(bsim_state* h, double2* psi, double2* lam, const EOp& e, double* res, int grid) {
+// One backward step for eops[k]; the op's per-block parameter products land
+// in partials[block][0..P) (the caller's slot of the partials buffer).
+static int adjoint_step(bsim_state* h, double2* psi, double2* lam, const EOp& e, double* partials, int grid) {
     const GateSpec& g = e.g;
     const int P = e.nparams;
-    double* partials = (double*)h->red.ptr;
     const int vq = h->vq;
     if (e.kind == E_DENSE && g.k <= 2) {
         AdjArgs a = adj_args(vq, g, true);
@@ -269,10 +280,6 @@ static int adjoint_step(bsim_state* h, double2* psi, double2* lam, const EOp& e,
         return BSIM_OK;
     }
     BSIM_HIP(hipGetLastError());
-    if (P > 0) {
-        hipLaunchKernelGGL(k_adj_finish, dim3(1), dim3(64), 0, h->stream, (const double*)partials, grid, P, res);
-        BSIM_HIP(hipGetLastError());
-    }
     return BSIM_OK;
 }
 
@@ -305,10 +312,15 @@ int bsim_adjoint_gradient(bsim_state* h, const bsim_op* ops, int n_ops, int n_wo
     if (int rc = ensure_ws(h->lam, sizeof(double2) * (size_t)len * 2)) return rc;
     double2* lam = (double2*)h->lam.ptr;
     double2* work = lam + len;
-    const int grid = 2048;
-    if (int rc = ensure_ws(h->red, sizeof(double) * (size_t)grid * 3 + 64)) return rc;
+    const int grid = 1024;
     int nslots = 0;
     for (const EOp& e : eops) nslots += e.nparams > 0 ? 1 : 0;
+    // partials: [slot][block][3]; sized so that the <H> dot product (which
+    // uses the same workspace first, up to 4096 x 2 doubles) never regrows it
+    // after `partials` is taken
+    size_t red_doubles = std::max((size_t)(nslots + 1) * grid * 3, (size_t)4096 * 2) + 64;
+    if (int rc = ensure_ws(h->red, sizeof(double) * red_doubles)) return rc;
+    double* partials = (double*)h->red.ptr;
     if (int rc = ensure_ws(h->res, sizeof(double) * (3 * (size_t)nslots + 8) + sizeof(PWord) * (size_t)n_words))
         return rc;
     double* res = (double*)h->res.ptr;
@@ -330,9 +342,13 @@ int bsim_adjoint_gradient(bsim_state* h, const bsim_op* ops, int n_ops, int n_wo
     int slot = nslots;
     for (int k = (int)eops.size() - 1; k >= 0; --k) {
         const EOp& e = eops[k];
-        double* r = nullptr;
-        if (e.nparams > 0) r = res + 3 * (size_t)(--slot);
+        double* r = partials;  // unused by parameter-free ops
+        if (e.nparams > 0) r = partials + (size_t)(--slot) * grid * 3;
         if (int rc = adjoint_step(h, work, lam, e, r, grid)) return rc;
+    }
+    if (nslots > 0) {
+        hipLaunchKernelGGL(k_adj_finish, dim3(nslots), dim3(256), 0, h->stream, (const double*)partials, grid, res);
+        BSIM_HIP(hipGetLastError());
     }
     std::vector<double> hres(3 * (size_t)nslots + 2);
     BSIM_HIP(hipMemcpyAsync(hres.data(), res, sizeof(double) * hres.size(), hipMemcpyDeviceToHost, h->stream));

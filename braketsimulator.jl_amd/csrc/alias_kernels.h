@@ -154,26 +154,44 @@ __device__ __forceinline__ int64_t gallop(const uint64_t* __restrict__ a, int64_
     return lo;
 }
 
+// Lanes of a wavefront take consecutive items (coalesced).  The first lane's
+// answer, found by bisection and broadcast, is a lower bound for every other
+// lane (answers are monotone in the item), so each lane gallops from it over
+// a few cached entries.
+__device__ __forceinline__ int64_t wave_first_lane_i64(int64_t v) {
+    return ((int64_t)__shfl(int(v >> 32), 0, 64) << 32) | (uint32_t)__shfl(int(v), 0, 64);
+}
+
 __global__ void k_alias_lights(const uint64_t* __restrict__ W, const int32_t* __restrict__ Lidx,
                                const uint64_t* __restrict__ Lval, const int32_t* __restrict__ Hidx,
                                const uint64_t* __restrict__ Hval, const Acc4* __restrict__ total,
                                double* __restrict__ ap, int32_t* __restrict__ alias) {
     const int64_t nl = (int64_t)total->nl, nh = (int64_t)total->nh;
-    const int64_t nruns = (nl + ALIAS_RUN - 1) / ALIAS_RUN;
-    for (int64_t run = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; run < nruns;
-         run += (int64_t)gridDim.x * blockDim.x) {
-        int64_t r = 0;
-        for (int64_t l = run * ALIAS_RUN; l < nl && l < (run + 1) * ALIAS_RUN; ++l) {
-            const int32_t i = Lidx[l];
-            const uint64_t w = W[i];
+    const int lane = threadIdx.x & 63;
+    const int64_t nwaves = (nl + 63) / 64;
+    const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / 64);
+    for (int64_t wv = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); wv < nwaves; wv += wstride) {
+        const int64_t l = wv * 64 + lane;
+        const bool valid = l < nl;
+        int32_t i = 0;
+        uint64_t w = ALIAS_C, dex = 0;
+        if (valid) {
+            i = Lidx[l];
+            w = W[i];
+            dex = Lval[l] - (ALIAS_C - w);  // deficits of the lights before this one
+        }
+        // lane 0's query is the smallest of the wave
+        int64_t r0 = 0;
+        if (lane == 0 && valid) r0 = gallop<true>(Hval, nh, 0, dex);
+        r0 = wave_first_lane_i64(r0);
+        if (valid) {
             ap[i] = (double)w / (double)ALIAS_C;
             if (w == ALIAS_C) {
                 alias[i] = i;
-                continue;
+            } else {
+                const int64_t r = gallop<true>(Hval, nh, r0, dex);
+                alias[i] = r < nh ? Hidx[r] : i;
             }
-            const uint64_t dex = Lval[l] - (ALIAS_C - w);  // deficits before this light
-            r = gallop<true>(Hval, nh, r, dex);
-            alias[i] = r < nh ? Hidx[r] : i;
         }
     }
 }
@@ -182,14 +200,23 @@ __global__ void k_alias_heavies(const uint64_t* __restrict__ Lval, const int32_t
                                 const uint64_t* __restrict__ Hval, const Acc4* __restrict__ total,
                                 double* __restrict__ ap, int32_t* __restrict__ alias) {
     const int64_t nl = (int64_t)total->nl, nh = (int64_t)total->nh;
-    const int64_t nruns = (nh + ALIAS_RUN - 1) / ALIAS_RUN;
-    for (int64_t run = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; run < nruns;
-         run += (int64_t)gridDim.x * blockDim.x) {
-        int64_t l = 0;
-        for (int64_t h = run * ALIAS_RUN; h < nh && h < (run + 1) * ALIAS_RUN; ++h) {
-            const int32_t i = Hidx[h];
-            const uint64_t e = Hval[h];
-            l = gallop<false>(Lval, nl, l, e);  // first light whose prefix deficit reaches e
+    const int lane = threadIdx.x & 63;
+    const int64_t nwaves = (nh + 63) / 64;
+    const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / 64);
+    for (int64_t wv = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); wv < nwaves; wv += wstride) {
+        const int64_t h = wv * 64 + lane;
+        const bool valid = h < nh;
+        int32_t i = 0;
+        uint64_t e = 0;
+        if (valid) {
+            i = Hidx[h];
+            e = Hval[h];
+        }
+        int64_t l0 = 0;
+        if (lane == 0 && valid) l0 = gallop<false>(Lval, nl, 0, e);
+        l0 = wave_first_lane_i64(l0);
+        if (valid) {
+            const int64_t l = gallop<false>(Lval, nl, l0, e);  // first light whose prefix deficit reaches e
             uint64_t rr = l < nl ? ALIAS_C + e - Lval[l] : ALIAS_C;
             int32_t nx = i;
             if (h + 1 < nh) nx = Hidx[h + 1];

@@ -15,8 +15,10 @@ struct PauliMask {
 int parse_pauli(int nq, const char* word, PauliMask& pm);
 // probabilities into a device buffer of 2^nq doubles
 int probabilities_device(bsim_state* h, double* dp);
-// (tot, per-bit sums) -> dout[0..35] bit sums, dout[36] total
-int zsums_device(bsim_state* h, double* dout);
+// Per-bit probability sums and the total into dout[0..37).  *layout = -1:
+// dout[b] is the sum over bit b; layout L >= 0: bits b < L at dout[16 + b],
+// bits b >= L at dout[b - L]; dout[36] is the total.
+int zsums_device(bsim_state* h, double* dout, int* layout);
 // <P> (re, im) into dout2 (device)
 int pauli_expect_device(bsim_state* h, const PauliMask& pm, double* dout2);
 // conj(a).b (re, im) into dout2 (device)

@@ -44,13 +44,21 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds /* NV * 1
     __syncthreads();
 }
 
-// partials[b * nv + i] -> out[i] (sum in block order)
-__global__ void k_finish(const double* __restrict__ partials, int nblocks, int nv, double* __restrict__ out) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nv) return;
+// partials[b * nv + i] -> out[i]; launched with one 256-thread block per
+// value i: fixed strided partial sums, then a fixed tree (deterministic).
+__global__ __launch_bounds__(256) void k_finish(const double* __restrict__ partials, int nblocks, int nv,
+                                                double* __restrict__ out) {
+    __shared__ double sh[256];
+    const int i = blockIdx.x;
     double s = 0;
-    for (int b = 0; b < nblocks; ++b) s += partials[(size_t)b * nv + i];
-    out[i] = s;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += partials[(size_t)b * nv + i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[i] = sh[0];
 }
 
 constexpr int RBLOCK = 256;
@@ -102,17 +110,21 @@ __global__ __launch_bounds__(RBLOCK) void k_zsums_sv(const double2* __restrict__
     const int64_t len = (int64_t)1 << vq;
     const int64_t stride = (int64_t)1 << lstride;
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (int64_t i = tid; i < len; i += stride) {
+    // Layout of v: [0..16) sums of bits lstride+b (the bits of the iteration
+    // count k), [16..36) sums of bits 0..19 (thread constants), [36] total.
+    const int hi_bits = vq > lstride ? vq - lstride : 0;
+    int64_t k = 0;
+    for (int64_t i = tid; i < len; i += stride, ++k) {
         double2 a = sv[i];
         double p = a.x * a.x + a.y * a.y;
         v[36] += p;
 #pragma unroll
-        for (int b = 0; b < 36; ++b)
-            if (b >= lstride && b < vq && ((i >> b) & 1)) v[b] += p;
+        for (int b = 0; b < 16; ++b)
+            if (b < hi_bits && ((k >> b) & 1)) v[b] += p;
     }
 #pragma unroll
-    for (int b = 0; b < 36; ++b)
-        if (b < lstride && ((tid >> b) & 1)) v[b] = v[36];
+    for (int b = 0; b < 20; ++b)
+        if (b < lstride && ((tid >> b) & 1)) v[16 + b] = v[36];
     block_sum<37>(v, lds);
     if (threadIdx.x == 0)
         for (int i = 0; i < 37; ++i) partials[(size_t)blockIdx.x * 37 + i] = v[i];
@@ -139,7 +151,7 @@ __global__ __launch_bounds__(RBLOCK) void k_zsums_dm(const double2* __restrict__
         for (int i = 0; i < 37; ++i) partials[(size_t)blockIdx.x * 37 + i] = v[i];
 }
 
-int zsums_device(bsim_state* h, double* dout /* 37 doubles */) {
+int zsums_device(bsim_state* h, double* dout /* 37 doubles */, int* layout) {
     int vq = h->kind == BSIM_SV ? h->vq : h->nq;
     int64_t len = (int64_t)1 << vq;
     int grid = red_grid(len);
@@ -153,8 +165,9 @@ int zsums_device(bsim_state* h, double* dout /* 37 doubles */) {
     else
         hipLaunchKernelGGL(k_zsums_dm, dim3(grid), dim3(RBLOCK), 0, h->stream, h->d, vq, partials);
     BSIM_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, h->stream, partials, grid, 37, dout);
+    hipLaunchKernelGGL(k_finish, dim3(37), dim3(256), 0, h->stream, partials, grid, 37, dout);
     BSIM_HIP(hipGetLastError());
+    if (layout) *layout = h->kind == BSIM_SV ? lstride : -1;
     return BSIM_OK;
 }
 
@@ -240,7 +253,7 @@ int pauli_expect_device(bsim_state* h, const PauliMask& pm, double* dout2) {
         hipLaunchKernelGGL(k_pauli_dm, dim3(grid), dim3(RBLOCK), 0, h->stream, h->d, n, f, pm.y, pm.z, pm.ny,
                            partials);
     BSIM_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, h->stream, partials, grid, 2, dout2);
+    hipLaunchKernelGGL(k_finish, dim3(2), dim3(256), 0, h->stream, partials, grid, 2, dout2);
     BSIM_HIP(hipGetLastError());
     return BSIM_OK;
 }
@@ -307,7 +320,7 @@ int dot_device(const double2* a, const double2* b, int64_t len, Workspace& red, 
     double* partials = (double*)red.ptr;
     hipLaunchKernelGGL(k_dot, dim3(grid), dim3(RBLOCK), 0, s, a, b, len, partials);
     BSIM_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, s, partials, grid, 2, dout2);
+    hipLaunchKernelGGL(k_finish, dim3(2), dim3(256), 0, s, partials, grid, 2, dout2);
     BSIM_HIP(hipGetLastError());
     return BSIM_OK;
 }
@@ -438,13 +451,16 @@ int bsim_expect_z_all(bsim_state* h, double* out) {
     if (!out) return fail(BSIM_EINVAL, "out is null");
     if (int rc = ensure_ws(h->res, 4096)) return rc;
     double* dres = (double*)h->res.ptr;
-    if (int rc = zsums_device(h, dres)) return rc;
+    int layout = -1;
+    if (int rc = zsums_device(h, dres, &layout)) return rc;
     double host[37];
     BSIM_HIP(hipMemcpyAsync(host, dres, sizeof(host), hipMemcpyDeviceToHost, h->stream));
     BSIM_HIP(hipStreamSynchronize(h->stream));
     for (int q = 0; q < h->nq; ++q) {
         int b = h->nq - 1 - q;
-        out[q] = host[36] - 2.0 * host[b];
+        // SV layout: bits below `layout` at 16 + b, bits above at b - layout
+        double s = layout < 0 ? host[b] : (b < layout ? host[16 + b] : host[b - layout]);
+        out[q] = host[36] - 2.0 * s;
     }
     return BSIM_OK;
 }
@@ -513,7 +529,7 @@ int bsim_expect_matrix(bsim_state* h, int k, const int* targets, const double* m
                                    make_int4(ins[0], ins[1], 0, 0), g.tb[0], g.tb[1], (const double2*)h->scratch.ptr,
                                    partials);
             BSIM_HIP(hipGetLastError());
-            hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, h->stream, partials, grid, 2, dres);
+            hipLaunchKernelGGL(k_finish, dim3(2), dim3(256), 0, h->stream, partials, grid, 2, dres);
             BSIM_HIP(hipGetLastError());
         } else {
             // copy, apply, dot (the reference's apply_observable on a deepcopy)
@@ -548,7 +564,7 @@ int bsim_expect_matrix(bsim_state* h, int k, const int* targets, const double* m
         hipLaunchKernelGGL(k_expect_dm_matrix, dim3(grid), dim3(RBLOCK), 0, h->stream, h->d, h->nq, k, dord,
                            (const double2*)h->scratch.ptr, partials);
         BSIM_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, h->stream, partials, grid, 2, dres);
+        hipLaunchKernelGGL(k_finish, dim3(2), dim3(256), 0, h->stream, partials, grid, 2, dres);
         BSIM_HIP(hipGetLastError());
         BSIM_HIP(hipMemcpyAsync(res, dres, sizeof(res), hipMemcpyDeviceToHost, h->stream));
         BSIM_HIP(hipStreamSynchronize(h->stream));

@@ -215,8 +215,7 @@ static int alias_build(bsim_state* h, int mode, double wsum) {
         hipLaunchKernelGGL(k_scan_tilesums, dim3(1), dim3(SCAN_BLOCK), 0, h->stream, tsum, ntiles, total);
         hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)ntiles), dim3(SCAN_BLOCK), 0, h->stream, W, n,
                            (const Acc4*)tsum, Lidx, Lval, Hidx, Hval);
-        int64_t runs = (n + ALIAS_RUN - 1) / ALIAS_RUN;
-        int64_t gr = grid_for(runs, 256, cap * 8);
+        int64_t gr = grid_for(n, 256, cap * 8);  // one item per lane
         hipLaunchKernelGGL(k_alias_lights, dim3((unsigned)gr), dim3(256), 0, h->stream, W, Lidx, Lval, Hidx, Hval,
                            (const Acc4*)total, ap, al);
         hipLaunchKernelGGL(k_alias_heavies, dim3((unsigned)gr), dim3(256), 0, h->stream, Lval, Hidx, Hval,
