@@ -79,6 +79,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--adjoint-qubits", type=int, default=24, help="0 skips the adjoint leg")
     ap.add_argument("--ps-sample", type=int, default=24, help="parameters timed by parameter shift")
+    ap.add_argument("--batch", type=int, default=4096, help="circuits in the batch leg (0 skips it)")
+    ap.add_argument("--sweep-sets", type=int, default=256, help="16-qubit ansatz parameter sets with gradients")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -187,6 +189,8 @@ def main():
     del sim
     if a.adjoint_qubits > 0:
         rec["adjoint"] = adjoint_leg(a.adjoint_qubits, a.ps_sample)
+    if a.batch > 0:
+        rec["batch"] = batch_leg(a.batch, a.sweep_sets)
     if not a.no_cpu:
         log("cpu baseline (oracle) ...")
         rec["cpu_baseline"] = cpu_baseline(n, ins)
@@ -235,6 +239,82 @@ def adjoint_leg(nq: int, ps_sample: int):
         "max_abs_diff_vs_param_shift": float(np.max(np.abs(grad[sample] - ps))),
         "param_shift_sample": f"first {len(sample)} of {len(th)} parameters, 2 fused evolutions + <H> each",
     }
+
+
+def batch_leg(count: int, sweep_sets: int, shots: int = 1000):
+    """Uneven batch: ``count`` brick-wall circuits, qubits uniform in 8..22,
+    depth uniform in 10..40, <Z_q> + 1000 shots each, in one persistent
+    launch, against the per-circuit launch path on the same circuits; then
+    ``sweep_sets`` parameter sets of the 16-qubit 8-layer ansatz, each with
+    its adjoint gradient."""
+    import numpy as np
+
+    from bsim import _lib as L
+    from bsim import adjoint as A
+    from bsim import workloads as W
+    from bsim.batch import run_batch
+    from bsim.simulators import StateVectorSimulator
+
+    spec = W.batch_tasks(count, seed=0)
+    circuits = [(n, W.brickwall_ops(n, d, s)) for n, d, s in spec]
+    seeds = [s for _, _, s in spec]
+    gates = sum(c[1].n for c in circuits)
+    log(f"batch: {count} circuits, {gates} gates")
+    run_batch(circuits[:64], shots, seeds[:64])  # warm-up
+    t = time.perf_counter()
+    res = run_batch(circuits, shots, seeds)
+    t_batch = time.perf_counter() - t
+    lat = np.array([r.latency_us for r in res]) * 1e-3
+    # per-circuit launches: reset, fused evolve, <Z>, parallel alias, shots
+    log("batch: per-circuit launch path")
+    sims = {n: StateVectorSimulator(n, shots) for n in sorted({c[0] for c in circuits})}
+    out = np.empty(shots, dtype=np.int64)
+    done = np.empty(count)
+    z_diff = 0.0
+    t = time.perf_counter()
+    for i, ((n, ops), seed) in enumerate(zip(circuits, seeds)):
+        sim = sims[n]
+        L.call("bsim_state_reset", sim.handle)
+        L.call("bsim_evolve", sim.handle, ops.ptr, ops.n, 0)
+        z = sim.expect_z_all()
+        L.call("bsim_alias_build", sim.handle, L.ALIAS_PARALLEL, 0.0)
+        L.call("bsim_sample", sim.handle, shots, ctypes.c_uint64(seed), L.i64ptr(out))
+        done[i] = time.perf_counter() - t
+        z_diff = max(z_diff, float(np.max(np.abs(z - res[i].expect_z))))
+    t_single = done[-1]
+    del sims
+    rec = {
+        "workload": f"{count} brick-wall SV circuits, qubits U{{8..22}}, depth U{{10..40}}, <Z_i> + {shots} shots each",
+        "gates": int(gates),
+        "persistent_s": round(t_batch, 4),
+        "persistent_circuits_per_s": round(count / t_batch, 1),
+        "persistent_kernel_ms": round(float(lat.max()), 3),
+        "persistent_latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
+        "persistent_latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
+        "per_circuit_s": round(t_single, 4),
+        "per_circuit_circuits_per_s": round(count / t_single, 1),
+        "per_circuit_latency_ms_p50": round(float(np.percentile(done, 50)) * 1e3, 3),
+        "per_circuit_latency_ms_p99": round(float(np.percentile(done, 99)) * 1e3, 3),
+        "speedup": round(t_single / t_batch, 2),
+        "max_abs_z_diff_vs_per_circuit": z_diff,
+        "latency_note": "completion time from the start of the batch; persistent = device clock from the launch",
+    }
+    if sweep_sets > 0:
+        words, coeffs = W.tfim(16)
+        sim = StateVectorSimulator(16, 0)
+        sets = [W.hea(16, 8, seed=k) for k in range(sweep_sets)]
+        A.adjoint_gradient(sim, sets[0][0], sets[0][1], words, coeffs)
+        log(f"batch: {sweep_sets} parameter sets x 16-qubit ansatz gradients")
+        t = time.perf_counter()
+        for ins, pmap, _ in sets:
+            A.adjoint_gradient(sim, ins, pmap, words, coeffs)
+        t_sw = time.perf_counter() - t
+        rec["sweep"] = {
+            "workload": f"{sweep_sets} parameter sets, HEA 16 qubits x 8 layers ({len(sets[0][2])} params), TFIM H",
+            "s": round(t_sw, 4),
+            "grads_per_s": round(sweep_sets / t_sw, 1),
+        }
+    return rec
 
 
 if __name__ == "__main__":

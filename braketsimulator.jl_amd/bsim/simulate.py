@@ -79,9 +79,34 @@ def simulate(sim, program: Program, shots: int = 0, seed=None, evolve_mode: int 
     return TaskResult(meas, mq, list(program.results), values, shots)
 
 
-def simulate_batch(sim, programs, shots: int = 0, seed=None):
-    """Batch simulate (src/BraketSimulator.jl:347-395).  State-vector
-    batches run as one persistent launch (bsim_batch_run) when every result
-    is ``Expectation(Z)`` on all qubits or shots; otherwise program by
-    program on ``sim``."""
+def simulate_batch(sim, programs, shots: int = 0, seed=None, persistent: bool = True):
+    """Batch simulate (src/BraketSimulator.jl:347-395).  A state-vector batch
+    with shots whose programs are gate lists of 4..26 qubits runs as one
+    persistent launch (bsim_batch_run; shots by inverse CDF, see bsim.batch);
+    any other batch runs program by program on ``sim``."""
+    from .batch import run_batch
+    from .gates import Gate
+
+    def eligible(p):
+        n = p.qubit_count()
+        ok = 4 <= n <= 26
+        for ins in p.instructions + p.basis_rotation_instructions:
+            ok = ok and (isinstance(ins.operator, (Gate, Measure)) or ins.operator is None)
+        return ok
+
+    if persistent and shots > 0 and isinstance(sim, StateVectorSimulator) and programs and all(map(eligible, programs)):
+        circuits, mqs = [], []
+        for p in programs:
+            n = p.qubit_count()
+            ops = [i for i in p.instructions + p.basis_rotation_instructions if not isinstance(i.operator, Measure)]
+            circuits.append((n, ops))
+            mqs.append(_measured_qubits(p, n))
+        seeds = None if seed is None else [seed + i for i in range(len(programs))]
+        res = run_batch(circuits, shots, seeds)
+        out = []
+        for p, (n, _), mq, r in zip(programs, circuits, mqs, res):
+            sh = np.asarray(mq, dtype=np.int64)
+            meas = ((r.samples[:, None] >> (n - 1 - sh)[None, :]) & 1).astype(np.int8)
+            out.append(TaskResult(meas, mq, list(p.results), [0.0 for _ in p.results], shots))
+        return out
     return [simulate(sim, p, shots, seed=None if seed is None else seed + i) for i, p in enumerate(programs)]

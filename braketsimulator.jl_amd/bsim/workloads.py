@@ -16,17 +16,14 @@ _M64 = (1 << 64) - 1
 
 
 def splitmix64_doubles(seed: int, n: int) -> np.ndarray:
-    """n doubles in [0, 1) from SplitMix64 seeded with ``seed``."""
-    out = np.empty(n, dtype=np.float64)
-    x = seed & _M64
-    for i in range(n):
-        x = (x + 0x9E3779B97F4A7C15) & _M64
-        z = x
-        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
-        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
-        z ^= z >> 31
-        out[i] = (z >> 11) * 2.0 ** -53
-    return out
+    """n doubles in [0, 1) from SplitMix64 seeded with ``seed`` (state i is
+    seed + (i + 1) * 0x9E3779B97F4A7C15 mod 2^64, mixed)."""
+    with np.errstate(over="ignore"):
+        x = np.uint64(seed & _M64) + np.arange(1, n + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+        z = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z ^= z >> np.uint64(31)
+    return (z >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
 
 
 def ry_qft(n: int = 20, seed: int = 0):
@@ -116,3 +113,69 @@ def batch_tasks(count: int = 4096, seed: int = 0, qmin: int = 8, qmax: int = 22,
         d = dmin + int(u[2 * i + 1] * (dmax - dmin + 1))
         out.append((n, d, seed + 1 + i))
     return out
+
+
+class PackedOps:
+    """A bsim_op array built directly in numpy (same layout as the ctypes
+    ``Op``), with the matrix buffer it points into; ``n`` and ``ptr`` as for
+    ``OpList``."""
+
+    def __init__(self, rec: np.ndarray, mats: np.ndarray):
+        self.rec, self.mats = rec, mats
+        self.n = len(rec)
+
+    @property
+    def ptr(self):
+        import ctypes as C
+
+        from . import _lib as L
+        return C.cast(C.c_void_p(self.rec.ctypes.data), C.POINTER(L.Op))
+
+
+def _op_dtype():
+    from . import _lib as L
+    f = {name: getattr(L.Op, name).offset for name, _ in L.Op._fields_}
+    return np.dtype({"names": ["type", "n_targets", "n_controls", "target0", "control0", "cval0", "matrix"],
+                     "formats": ["<i4", "<i4", "<i4", "<i4", "<i4", "<i4", "<u8"],
+                     "offsets": [f["type"], f["n_targets"], f["n_controls"], f["targets"], f["controls"],
+                                 f["control_values"], f["matrix"]],
+                     "itemsize": __import__("ctypes").sizeof(L.Op)})
+
+
+def brickwall_ops(n: int, depth: int, seed: int = 0) -> PackedOps:
+    """``brickwall(n, depth, seed)`` lowered straight into a bsim_op array:
+    U(theta, phi, lambda) = [[c, -e^{i lam} s], [e^{i phi} s, e^{i(phi+lam)} c]]
+    (c, s = cos, sin of theta/2) as a 1-target gate, CZ(q, q+1) as Z on q+1
+    controlled by q -- the kernel calls ``lower_instructions`` makes."""
+    from . import _lib as L
+    ang = splitmix64_doubles(seed, 3 * n * depth).reshape(depth, n, 3) * (2 * math.pi)
+    th, ph, lam = ang[..., 0], ang[..., 1], ang[..., 2]
+    c, s = np.cos(th / 2), np.sin(th / 2)
+    m = np.empty((depth, n, 4), dtype=np.complex128)   # column-major 2x2
+    m[..., 0] = c
+    m[..., 1] = np.exp(1j * ph) * s
+    m[..., 2] = -np.exp(1j * lam) * s
+    m[..., 3] = np.exp(1j * (lam + ph)) * c
+    mats = np.concatenate([m.reshape(-1, 4), np.array([[1, 0, 0, -1]], dtype=np.complex128)])
+    zaddr = mats.ctypes.data + 64 * (len(mats) - 1)
+    count = depth * n + sum(len(range(l % 2, n - 1, 2)) for l in range(depth))
+    rec = np.zeros(count, dtype=_op_dtype())
+    k = 0
+    for layer in range(depth):
+        sl = slice(k, k + n)
+        rec["type"][sl] = L.OP_GATE
+        rec["n_targets"][sl] = 1
+        rec["target0"][sl] = np.arange(n)
+        rec["matrix"][sl] = mats.ctypes.data + 64 * (layer * n + np.arange(n, dtype=np.uint64))
+        k += n
+        qs = np.arange(layer % 2, n - 1, 2)
+        sl = slice(k, k + len(qs))
+        rec["type"][sl] = L.OP_GATE
+        rec["n_targets"][sl] = 1
+        rec["n_controls"][sl] = 1
+        rec["target0"][sl] = qs + 1
+        rec["control0"][sl] = qs
+        rec["cval0"][sl] = 1
+        rec["matrix"][sl] = zaddr
+        k += len(qs)
+    return PackedOps(rec, mats)

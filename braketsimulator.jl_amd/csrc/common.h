@@ -70,6 +70,26 @@ __host__ __device__ __forceinline__ uint64_t pad_bit(uint64_t ix, int bit) {
     return ((ix - lo) << 1) | lo;
 }
 
+// Philox-4x32-10 (Salmon et al., SC'11): ten rounds on ctr with key (k0, k1).
+__host__ __device__ __forceinline__ void philox4x32_10(uint32_t ctr[4], uint32_t k0, uint32_t k1) {
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)M0 * ctr[0];
+        uint64_t p1 = (uint64_t)M1 * ctr[2];
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ ctr[1] ^ k0;
+        uint32_t n2 = hi0 ^ ctr[3] ^ k1;
+        ctr[0] = n0;
+        ctr[1] = lo1;
+        ctr[2] = n2;
+        ctr[3] = lo0;
+        k0 += W0;
+        k1 += W1;
+    }
+}
+
 // Host-side helpers ---------------------------------------------------------
 inline int64_t grid_for(int64_t work, int block, int64_t cap) {
     int64_t g = (work + block - 1) / block;

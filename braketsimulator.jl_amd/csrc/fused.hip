@@ -409,6 +409,19 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
 
 namespace bsim {
 
+// One launch = one pass: block b runs tile b.
+__global__ __launch_bounds__(TB) void k_tile(double2* __restrict__ sv, int vq, const Pass* __restrict__ pass,
+                                             const Sweep* __restrict__ sweeps, const TOp* __restrict__ tops) {
+    extern __shared__ double2 tile[];
+    __shared__ uint64_t scat_lo[256];   // scatter of local bits 0..7
+    __shared__ uint64_t scat_hi[64];    // scatter of local bits 8..13
+    tile_body(sv, vq, pass, sweeps, tops, blockIdx.x, false, tile, scat_lo, scat_hi);
+}
+
+}  // namespace bsim
+
+namespace bsim {
+
 // ---------------------------------------------------------------------------
 // execution
 // ---------------------------------------------------------------------------

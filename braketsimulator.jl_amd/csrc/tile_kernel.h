@@ -172,11 +172,13 @@ __device__ __forceinline__ void run_op(SubCube& s, const TOp* __restrict__ op) {
     }
 }
 
-__global__ __launch_bounds__(TB) void k_tile(double2* __restrict__ sv, int vq, const Pass* __restrict__ pass,
-                                             const Sweep* __restrict__ sweeps, const TOp* __restrict__ tops) {
-    extern __shared__ double2 tile[];
-    __shared__ uint64_t scat_lo[256];   // scatter of local bits 0..7
-    __shared__ uint64_t scat_hi[64];    // scatter of local bits 8..13
+// One tile of one pass.  `init`: the state is |0...0> and is not read (the
+// tile is built in LDS).  LDS: tile (2^T + 2^T/16 amplitudes), scat_lo[256],
+// scat_hi[64].
+__device__ __forceinline__ void tile_body(double2* __restrict__ sv, int vq, const Pass* __restrict__ pass,
+                                          const Sweep* __restrict__ sweeps, const TOp* __restrict__ tops,
+                                          uint64_t tile_index, bool init, double2* __restrict__ tile,
+                                          uint64_t* __restrict__ scat_lo, uint64_t* __restrict__ scat_hi) {
     const int T = pass->T;
     const int nloc = 1 << T;
     const uint64_t lmask = pass->lmask;
@@ -194,7 +196,7 @@ __global__ __launch_bounds__(TB) void k_tile(double2* __restrict__ sv, int vq, c
         }
     }
     // global base: deposit the block index into the non-tile bits
-    uint64_t w = blockIdx.x, base = 0;
+    uint64_t w = tile_index, base = 0;
     for (int bit = 0, k = 0; bit < vq; ++bit)
         if (!((lmask >> bit) & 1)) {
             base |= ((w >> k) & 1) << bit;
@@ -203,11 +205,14 @@ __global__ __launch_bounds__(TB) void k_tile(double2* __restrict__ sv, int vq, c
     __syncthreads();
     const uint64_t s_t = base | scat_lo[threadIdx.x & 255];
     const int iters = nloc >> 8;  // rows of 256 amplitudes (T >= 8)
-    if (iters == 0) {
+    if (init) {
+        for (int j = threadIdx.x; j < nloc; j += TB)
+            tile[phys(j)] = make_double2((base == 0 && j == 0) ? 1.0 : 0.0, 0.0);
+    } else if (iters == 0) {
         if ((int)threadIdx.x < nloc) tile[phys(threadIdx.x)] = sv[s_t];
     }
     // issue up to 16 loads per thread before the first LDS write
-    for (int i0 = 0; i0 < iters; i0 += 16) {
+    for (int i0 = 0; i0 < iters && !init; i0 += 16) {
         dv32 buf;
         sfor<0, 16>([&](auto J) {
             constexpr int jj = decltype(J)::value;

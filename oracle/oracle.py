@@ -54,6 +54,7 @@ _SIGS = {
     "ref_shot_draws": [C.c_uint64, C.c_int, C.c_int64, C.c_int64, _i64p, _dp],
     "ref_alias_exact": [_dp, C.c_int64, C.c_double, _dp, _i32p],
     "ref_splitmix64_doubles": [C.c_uint64, C.c_int64, _dp],
+    "ref_cdf_shots": [_dp, C.c_int, C.c_int64, C.c_uint64, _i64p],
 }
 
 _lib = None
@@ -283,6 +284,14 @@ def shot_draws(seed: int, nq: int, n: int, first: int = 0):
     u = np.empty(n)
     lib().ref_shot_draws(C.c_uint64(seed), nq, first, n, idx.ctypes.data_as(_i64p), _d(u))
     return idx, u
+
+
+def cdf_shots(psi: np.ndarray, nq: int, shots: int, seed: int) -> np.ndarray:
+    """Batched-execution shot rule: inverse CDF over exact integer weights."""
+    out = np.empty(shots, dtype=np.int64)
+    a = np.ascontiguousarray(psi, dtype=np.complex128)
+    lib().ref_cdf_shots(_d(a), nq, shots, C.c_uint64(seed), out.ctypes.data_as(_i64p))
+    return out
 
 
 def lookup(ap: np.ndarray, alias: np.ndarray, idx: np.ndarray, u: np.ndarray) -> np.ndarray:

@@ -647,3 +647,35 @@ void ref_splitmix64_doubles(uint64_t seed, int64_t n, double* out) {
         out[i] = (double)(z >> 11) * 0x1.0p-53;
     }
 }
+
+/* Batched-execution shots (no counterpart in the reference, whose batch
+ * calls the alias sampler per circuit, src/BraketSimulator.jl:347-395):
+ * inverse CDF over the exact integer weights W_i = floor(p_i * 2^n * 2^32)
+ * with p_i = re^2 + im^2 of the final amplitudes; shot s draws
+ * Philox-4x32-10(counter (s, 0, 0, 0), key seed), r = (x1:x0),
+ * t = floor(r * sum(W) / 2^64), and returns the first i with
+ * W_0 + ... + W_i > t. */
+void ref_cdf_shots(const double* amp, int nq, int64_t shots, uint64_t seed, int64_t* out) {
+    const int64_t n = (int64_t)1 << nq;
+    uint64_t* P = (uint64_t*)malloc(sizeof(uint64_t) * n);
+    const double scale = (double)n * 4294967296.0;
+    uint64_t run = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        double re = amp[2 * i], im = amp[2 * i + 1];
+        double re2 = re * re, im2 = im * im;
+        double p = re2 + im2;
+        double x = p * scale;
+        run += x > 0 ? (uint64_t)x : 0;
+        P[i] = run;
+    }
+    for (int64_t s = 0; s < shots; ++s) {
+        uint32_t in[4] = { (uint32_t)s, (uint32_t)((uint64_t)s >> 32), 0, 0 }, o[4];
+        ref_philox4x32_10(in, (uint32_t)seed, (uint32_t)(seed >> 32), o);
+        uint64_t r = ((uint64_t)o[1] << 32) | o[0];
+        uint64_t t = (uint64_t)(((unsigned __int128)r * run) >> 64);
+        int64_t lo = 0, hi = n - 1;
+        while (lo < hi) { int64_t m = (lo + hi) >> 1; if (P[m] > t) hi = m; else lo = m + 1; }
+        out[s] = lo;
+    }
+    free(P);
+}

@@ -315,3 +315,30 @@ def test_shot_rule():
     idx = np.array([1, 1, 2, 2, 0])
     u = np.array([0.1, 0.3, 0.49, 0.5, 0.99])
     np.testing.assert_array_equal(O.lookup(ap, al, idx, u), [1, 3, 2, 0, 0])
+
+
+def test_cdf_shots_special_cases():
+    """Inverse-CDF shots of the batch path: a basis state always returns its
+    index, and the uniform superposition (every weight exactly 2^32) returns
+    the top n bits of (x1:x0) -- the alias path's Philox index draw."""
+    n = 6
+    psi = np.zeros(2 ** n, dtype=np.complex128)
+    psi[37] = 1j
+    assert (O.cdf_shots(psi, n, 500, 3) == 37).all()
+    uni = np.full(2 ** n, 2 ** (-n / 2), dtype=np.complex128)
+    idx, _ = O.shot_draws(11, n, 2000)
+    np.testing.assert_array_equal(O.cdf_shots(uni, n, 2000, 11), idx)
+
+
+def test_cdf_shots_frequencies():
+    """Empirical frequencies of 400k shots of a random 4-qubit state within
+    5 sigma of |psi_i|^2, and zero-probability states never drawn."""
+    rng = np.random.default_rng(2)
+    psi = rng.normal(size=16) + 1j * rng.normal(size=16)
+    psi[[3, 9]] = 0
+    psi /= np.linalg.norm(psi)
+    p = np.abs(psi) ** 2
+    s = O.cdf_shots(psi, 4, 400000, 5)
+    f = np.bincount(s, minlength=16) / len(s)
+    assert f[3] == 0 and f[9] == 0
+    assert (np.abs(f - p) < 5 * np.sqrt(p * (1 - p) / len(s)) + 1e-12).all()
