@@ -14,6 +14,7 @@ namespace bsim {
 // ---- error handling --------------------------------------------------
 
 void set_error(const std::string& msg);
+const std::string& last_error_text();  // this thread's last message
 int fail(int code, const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
 

@@ -14,6 +14,7 @@ namespace bsim {
 static thread_local std::string g_last_error;
 
 void set_error(const std::string& msg) { g_last_error = msg; }
+const std::string& last_error_text() { return g_last_error; }
 
 int fail(int code, const std::string& msg) {
     set_error(msg);
