@@ -81,6 +81,8 @@ def main():
     ap.add_argument("--ps-sample", type=int, default=24, help="parameters timed by parameter shift")
     ap.add_argument("--batch", type=int, default=4096, help="circuits in the batch leg (0 skips it)")
     ap.add_argument("--sweep-sets", type=int, default=256, help="16-qubit ansatz parameter sets with gradients")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="profiling aid: one fused evolve of the workload and one 8 GiB device copy, no JSON")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -95,6 +97,9 @@ def main():
     from bsim.simulators import StateVectorSimulator, lower_instructions
 
     n, depth = a.qubits, a.depth
+    if a.pmc_probe:
+        pmc_probe(n, depth)
+        return
     ins = W.brickwall(n, depth, seed=0)
     n_gates = len(ins)
     ol = lower_instructions(ins, L.BSIM_SV)
@@ -180,7 +185,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(n, depth),
+            "traffic_source": "profiles/k_tile_brickwall_30q.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
             "avg_kernel_ms": round(avg_kernel_ms, 4),
             "launches_timed": int(launches.value),
         },
@@ -239,6 +245,41 @@ def adjoint_leg(nq: int, ps_sample: int):
         "max_abs_diff_vs_param_shift": float(np.max(np.abs(grad[sample] - ps))),
         "param_shift_sample": f"first {len(sample)} of {len(th)} parameters, 2 fused evolutions + <H> each",
     }
+
+
+def measured_traffic(n: int, depth: int):
+    """HBM bytes per k_tile launch measured by hardware counters for the
+    30-qubit depth-20 workload (committed profile); None for other shapes."""
+    f = ROOT / "profiles" / "k_tile_brickwall_30q.md"
+    if (n, depth) != (30, 20) or not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
+def pmc_probe(n: int, depth: int):
+    """Kernels for hardware-counter runs: the fused passes of one evolve
+    (k_tile) and a plain device copy of 2^29 doubles (4 GiB read + 4 GiB
+    written), whose FETCH_SIZE / WRITE_SIZE calibrate the counters."""
+    import torch
+
+    from bsim import _lib as L
+    from bsim import workloads as W
+    from bsim.simulators import StateVectorSimulator, lower_instructions
+
+    ol = lower_instructions(W.brickwall(n, depth, seed=0), L.BSIM_SV)
+    sim = StateVectorSimulator(n, 0)
+    L.call("bsim_evolve", sim.handle, ol.ptr, ol.n, 0)
+    L.call("bsim_synchronize", sim.handle)
+    log(f"pmc probe: evolve {n}q done")
+    del sim
+    src = torch.ones(2 ** 29, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log("pmc probe: copy of 4 GiB done")
 
 
 def batch_leg(count: int, sweep_sets: int, shots: int = 1000):
