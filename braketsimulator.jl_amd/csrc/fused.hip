@@ -391,6 +391,36 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
                 plan.top_src.push_back(i);
             }
             sw.op_end = (int)plan.tops.size();
+            // index tests split into a per-thread part and a part fixed by the register
+            uint64_t gb[4], G = 0;
+            for (int r = 0; r < 4; ++r) {
+                gb[r] = 1ull << P.lbits[sw.sb[r]];
+                G |= gb[r];
+            }
+            auto rj = [&](int j) {
+                uint64_t x = 0;
+                for (int r = 0; r < 4; ++r)
+                    if ((j >> r) & 1) x |= gb[r];
+                return x;
+            };
+            for (int o = sw.op_begin; o < sw.op_end; ++o) {
+                TOp& op = plan.tops[o];
+                if (op.kind == 3) {
+                    uint32_t q = 0;
+                    for (int j = 0; j < 16; ++j) {
+                        const uint64_t x = rj(j);
+                        q |= (uint32_t)(popc(x & (x >> op.pd) & op.pm) & 1) << j;
+                    }
+                    op.jmask = q;
+                    for (int r = 0; r < 4; ++r) op.sgm[r] = ((gb[r] >> op.pd) & op.pm) ^ ((gb[r] & op.pm) << op.pd);
+                } else if (op.cmask) {
+                    op.tmask = op.cmask & ~G;
+                    op.tval = op.cval & ~G;
+                    uint32_t q = 0;
+                    for (int j = 0; j < 16; ++j) q |= (uint32_t)((rj(j) & op.cmask) == (op.cval & G)) << j;
+                    op.jmask = q;
+                }
+            }
             plan.sweeps.push_back(sw);
             plan.fused_ops += (int64_t)take.size();
             rem.swap(next);

@@ -56,7 +56,9 @@ __global__ void k_alias_reference(const double* __restrict__ w, int64_t n, doubl
 }
 
 // ---- parallel exact build ---------------------------------------------------
-__global__ void k_sum_p(const double* __restrict__ p, int64_t n, double* __restrict__ partials) {
+constexpr int SUM_BLOCKS = 2048;  // partial sums of k_sum_p
+
+__global__ __launch_bounds__(256) void k_sum_p(const double* __restrict__ p, int64_t n, double* __restrict__ partials) {
     __shared__ double sh[256];
     double s = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -70,17 +72,26 @@ __global__ void k_sum_p(const double* __restrict__ p, int64_t n, double* __restr
     if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
 }
 
-__global__ void k_sum_finish(const double* __restrict__ partials, int nb, double* __restrict__ out) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        double s = 0;
-        for (int b = 0; b < nb; ++b) s += partials[b];
-        out[0] = s;
+// one 256-thread block: fixed strided sums, then a fixed tree (deterministic)
+__global__ __launch_bounds__(256) void k_sum_finish(const double* __restrict__ partials, int nb,
+                                                    double* __restrict__ out) {
+    __shared__ double sh[256];
+    double s = 0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) s += partials[b];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) out[0] = sh[0];
 }
 
-// W_i = floor(p_i * scale); total and max by integer atomics (order-free)
-__global__ void k_quantize(const double* __restrict__ p, int64_t n, const double* __restrict__ wsum_dev,
-                           double wsum_host, uint64_t* __restrict__ W, unsigned long long* __restrict__ acc) {
+// W_i = floor(p_i * scale); total and max by integer atomics (order-free),
+// one per wave after a wave reduction
+__global__ __launch_bounds__(256) void k_quantize(const double* __restrict__ p, int64_t n,
+                                                  const double* __restrict__ wsum_dev, double wsum_host,
+                                                  uint64_t* __restrict__ W, unsigned long long* __restrict__ acc) {
     const double wsum = wsum_host > 0 ? wsum_host : wsum_dev[0];
     const double scale = (double)n * (double)ALIAS_C / wsum;
     unsigned long long tot = 0, mx = 0;
@@ -91,16 +102,28 @@ __global__ void k_quantize(const double* __restrict__ p, int64_t n, const double
         tot += w;
         mx = w > mx ? w : mx;
     }
-    atomicAdd(&acc[0], tot);
-    atomicMax(&acc[1], mx);
+    for (int o = 32; o > 0; o >>= 1) {
+        tot += __shfl_xor(tot, o);
+        const unsigned long long m = __shfl_xor(mx, o);
+        mx = m > mx ? m : mx;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&acc[0], tot);
+        atomicMax(&acc[1], mx);
+    }
 }
 
-__global__ void k_argmax(const uint64_t* __restrict__ W, int64_t n, unsigned long long* __restrict__ acc) {
+__global__ __launch_bounds__(256) void k_argmax(const uint64_t* __restrict__ W, int64_t n,
+                                                unsigned long long* __restrict__ acc) {
     const unsigned long long mx = acc[1];
     unsigned long long best = ~0ull;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         if (W[i] == mx && (unsigned long long)i < best) best = (unsigned long long)i;
-    atomicMin(&acc[2], best);
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long b = __shfl_xor(best, o);
+        best = b < best ? b : best;
+    }
+    if ((threadIdx.x & 63) == 0 && best != ~0ull) atomicMin(&acc[2], best);
 }
 
 __global__ void k_fix_remainder(uint64_t* __restrict__ W, int64_t n, const unsigned long long* __restrict__ acc) {
@@ -163,8 +186,9 @@ static int alias_build(bsim_state* h, int mode, double wsum) {
         int32_t* smalls = larges + n;
         double* scal = (double*)(smalls + n);
         if (wsum <= 0) {
-            hipLaunchKernelGGL(k_sum_p, dim3(256), dim3(256), 0, h->stream, p, n, scal + 8);
-            hipLaunchKernelGGL(k_sum_finish, dim3(1), dim3(64), 0, h->stream, scal + 8, 256, scal);
+            const int gs = (int)grid_for(n, 256, SUM_BLOCKS);
+            hipLaunchKernelGGL(k_sum_p, dim3(gs), dim3(256), 0, h->stream, p, n, scal + 8);
+            hipLaunchKernelGGL(k_sum_finish, dim3(1), dim3(256), 0, h->stream, scal + 8, gs, scal);
             BSIM_HIP(hipMemcpyAsync(&wsum, scal, sizeof(double), hipMemcpyDeviceToHost, h->stream));
             BSIM_HIP(hipStreamSynchronize(h->stream));
         }
@@ -179,10 +203,11 @@ static int alias_build(bsim_state* h, int mode, double wsum) {
         Acc4* tsum = (Acc4*)(Hidx + n + (n & 1));
         Acc4* total = tsum + ntiles;
         double* scal = (double*)(total + 2);
-        unsigned long long* acc = (unsigned long long*)(scal + 512);
+        unsigned long long* acc = (unsigned long long*)(scal + 8 + SUM_BLOCKS);
         if (wsum <= 0) {
-            hipLaunchKernelGGL(k_sum_p, dim3(256), dim3(256), 0, h->stream, p, n, scal + 8);
-            hipLaunchKernelGGL(k_sum_finish, dim3(1), dim3(64), 0, h->stream, scal + 8, 256, scal);
+            const int gs = (int)grid_for(n, 256, SUM_BLOCKS);
+            hipLaunchKernelGGL(k_sum_p, dim3(gs), dim3(256), 0, h->stream, p, n, scal + 8);
+            hipLaunchKernelGGL(k_sum_finish, dim3(1), dim3(256), 0, h->stream, scal + 8, gs, scal);
         }
         unsigned long long init[4] = { 0ull, 0ull, ~0ull, 0ull };
         BSIM_HIP(hipMemcpyAsync(acc, init, sizeof(init), hipMemcpyHostToDevice, h->stream));

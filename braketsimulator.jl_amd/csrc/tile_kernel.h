@@ -72,7 +72,9 @@ struct SubCube {
 template <int A>
 __device__ __forceinline__ void sw_dense1(SubCube& s, const TOp* __restrict__ op) {
     const c2 m00 = to_c2(op->m[0]), m10 = to_c2(op->m[1]), m01 = to_c2(op->m[2]), m11 = to_c2(op->m[3]);
-    const uint64_t cm = op->cmask, cv = op->cval;
+    const bool ctl = op->cmask != 0;
+    const bool tok = (s.gbase & op->tmask) == op->tval;  // this thread's part of the control test
+    const uint32_t jm = op->jmask;                         // the registers' part
     sfor<0, 16>([&](auto J) {
         constexpr int j = decltype(J)::value;
         if constexpr ((j & (1 << A)) == 0) {
@@ -80,8 +82,8 @@ __device__ __forceinline__ void sw_dense1(SubCube& s, const TOp* __restrict__ op
             const c2 a = getv<j>(s.v), b = getv<k>(s.v);
             c2 x = cfma(m01, b, cmul(m00, a));
             c2 y = cfma(m11, b, cmul(m10, a));
-            if (cm) {
-                const bool on = (s.template gidx<j>() & cm) == cv;
+            if (ctl) {
+                const bool on = tok && ((jm >> j) & 1u);
                 x = csel(on, x, a);
                 y = csel(on, y, b);
             }
@@ -93,7 +95,9 @@ __device__ __forceinline__ void sw_dense1(SubCube& s, const TOp* __restrict__ op
 
 template <int A, int B>
 __device__ __forceinline__ void sw_dense2(SubCube& s, const TOp* __restrict__ op) {
-    const uint64_t cm = op->cmask, cv = op->cval;
+    const bool ctl = op->cmask != 0;
+    const bool tok = (s.gbase & op->tmask) == op->tval;
+    const uint32_t jm = op->jmask;
     sfor<0, 16>([&](auto J) {
         constexpr int j = decltype(J)::value;
         if constexpr ((j & ((1 << A) | (1 << B))) == 0) {
@@ -106,8 +110,7 @@ __device__ __forceinline__ void sw_dense2(SubCube& s, const TOp* __restrict__ op
                 return cfma(to_c2(op->m[r + 12]), a3, acc);
             };
             const c2 r0 = row(0), r1 = row(1), r2 = row(2), r3 = row(3);
-            bool on = true;
-            if (cm) on = (s.template gidx<j>() & cm) == cv;
+            const bool on = !ctl || (tok && ((jm >> j) & 1u));
             setv<j>(s.v, csel(on, r0, a0));
             setv<i1>(s.v, csel(on, r1, a1));
             setv<i2>(s.v, csel(on, r2, a2));
@@ -142,16 +145,25 @@ __device__ __forceinline__ void sw_diag(SubCube& s, const TOp* __restrict__ op) 
     });
 }
 
-// +-1 diagonal (CZ layers): negate where popcount(i & (i >> pd) & pm) is odd
+// +-1 diagonal (CZ layers): negate where P(i) = parity(i & (i >> pd) & pm)
+// is 1.  P is a quadratic form over GF(2): for i = gbase ^ r_j,
+// P(i) = P(gbase) ^ P(r_j) ^ sum over the register bits of j of
+// parity(gbase & sgm[bit]); the planner stores P(r_j) for all 16 j in jmask.
 __device__ __forceinline__ void sw_sign(SubCube& s, const TOp* __restrict__ op) {
-    const uint64_t pm = op->pm;
-    const int pd = op->pd;
+    const uint64_t a = s.gbase;
+    uint32_t m = op->jmask;
+    if (__popcll(a & (a >> op->pd) & op->pm) & 1) m ^= 0xFFFFu;
+    if (__popcll(a & op->sgm[0]) & 1) m ^= 0xAAAAu;
+    if (__popcll(a & op->sgm[1]) & 1) m ^= 0xCCCCu;
+    if (__popcll(a & op->sgm[2]) & 1) m ^= 0xF0F0u;
+    if (__popcll(a & op->sgm[3]) & 1) m ^= 0xFF00u;
     sfor<0, 16>([&](auto J) {
         constexpr int j = decltype(J)::value;
-        const uint64_t g = s.template gidx<j>();
-        const bool neg = __popcll(g & (g >> pd) & pm) & 1;
-        const c2 a = getv<j>(s.v);
-        setv<j>(s.v, { neg ? -a.x : a.x, neg ? -a.y : a.y });
+        const uint64_t f = (uint64_t)((m << (31 - j)) & 0x80000000u) << 32;  // sign bit of register j
+        c2 v = getv<j>(s.v);
+        v.x = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, v.x) ^ f);
+        v.y = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, v.y) ^ f);
+        setv<j>(s.v, v);
     });
 }
 
@@ -228,33 +240,34 @@ __device__ __forceinline__ void tile_body(double2* __restrict__ sv, int vq, cons
     for (int sidx = 0; sidx < pass->nsweeps; ++sidx) {
         const Sweep* sw = sweeps + pass->sweep_begin + sidx;
         const int sb0 = sw->sb[0], sb1 = sw->sb[1], sb2 = sw->sb[2], sb3 = sw->sb[3];
-        const int smask = (1 << sb0) | (1 << sb1) | (1 << sb2) | (1 << sb3);
         const int ob = sw->op_begin, oe = sw->op_end;
+        // global bits of the register bits (uniform)
+        auto gl = [&](int l) { return scat_lo[(1 << l) & 255] | scat_hi[(1 << l) >> 8]; };
+        const uint64_t g0 = gl(sb0), g1 = gl(sb1), g2 = gl(sb2), g3 = gl(sb3);
+        // local offsets of the registers: the register bits are disjoint from the
+        // sub-cube's base, so phys(lb | o) = phys(lb) + phys(o)
+        const int o0 = 1 << sb0, o1 = 1 << sb1, o2 = 1 << sb2, o3 = 1 << sb3;
         for (int c = threadIdx.x; c < nsub; c += TB) {
-            int lb = 0;
-            for (int bit = 0, k = 0; bit < T; ++bit)
-                if (!((smask >> bit) & 1)) {
-                    lb |= ((c >> k) & 1) << bit;
-                    ++k;
-                }
+            // base of sub-cube c: zero bits inserted at the (ascending) register bits
+            auto ins = [](int x, int b) { const int lo = x & ((1 << b) - 1); return ((x - lo) << 1) | lo; };
+            const int lb = ins(ins(ins(ins(c, sb0), sb1), sb2), sb3);
+            const int pl = phys(lb);
             SubCube s;
             s.gbase = base | scat_lo[lb & 255] | scat_hi[lb >> 8];
-            auto gl = [&](int l) { return scat_lo[(1 << l) & 255] | scat_hi[(1 << l) >> 8]; };
-            s.g0 = gl(sb0);
-            s.g1 = gl(sb1);
-            s.g2 = gl(sb2);
-            s.g3 = gl(sb3);
-            const int o0 = 1 << sb0, o1 = 1 << sb1, o2 = 1 << sb2, o3 = 1 << sb3;
+            s.g0 = g0;
+            s.g1 = g1;
+            s.g2 = g2;
+            s.g3 = g3;
             sfor<0, 16>([&](auto J) {
                 constexpr int j = decltype(J)::value;
-                const int off = lb | ((j & 1) ? o0 : 0) | ((j & 2) ? o1 : 0) | ((j & 4) ? o2 : 0) | ((j & 8) ? o3 : 0);
-                setv<j>(s.v, to_c2(tile[phys(off)]));
+                const int o = ((j & 1) ? o0 : 0) + ((j & 2) ? o1 : 0) + ((j & 4) ? o2 : 0) + ((j & 8) ? o3 : 0);
+                setv<j>(s.v, to_c2(tile[pl + phys(o)]));
             });
             for (int o = ob; o < oe; ++o) run_op(s, tops + o);
             sfor<0, 16>([&](auto J) {
                 constexpr int j = decltype(J)::value;
-                const int off = lb | ((j & 1) ? o0 : 0) | ((j & 2) ? o1 : 0) | ((j & 4) ? o2 : 0) | ((j & 8) ? o3 : 0);
-                tile[phys(off)] = to_d2(getv<j>(s.v));
+                const int o = ((j & 1) ? o0 : 0) + ((j & 2) ? o1 : 0) + ((j & 4) ? o2 : 0) + ((j & 8) ? o3 : 0);
+                tile[pl + phys(o)] = to_d2(getv<j>(s.v));
             });
         }
         __syncthreads();
