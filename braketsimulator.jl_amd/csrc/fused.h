@@ -41,14 +41,16 @@ struct TOp {
     int32_t nd;        // diag: bit count
     int32_t dbit[4];   // diag: vector bit of diagonal index bit i
     uint64_t cmask, cval;
-    uint64_t pm;       // sign: amplitude i is negated when popcount(i & (i >> pd) & pm) is odd
-    int32_t pd, pad_;
+    // sign: amplitude i is negated when sum_k popcount(i & (i >> pd[k]) & pm[k]) is odd
+    uint64_t pm[4];
+    int32_t pd[4];
+    int32_t npairs, pad_;
     // Split of the index tests over a sweep (computed by the planner): a
     // register j of a sub-cube has index gbase | r_j, gbase the sub-cube's
     // base and r_j the global bits of j's register bits.
     //   dense with controls: on_j = ((gbase & tmask) == tval) && bit j of jmask;
     //   sign: negate_j = bit j of (jmask ^ P(gbase) * 0xFFFF ^ sum_i parity(gbase & sgm[i]) * pattern_i),
-    //         P(x) = parity(x & (x >> pd) & pm), pattern_i = registers with bit i set.
+    //         P(x) = sum_k parity(x & (x >> pd[k]) & pm[k]), pattern_i = registers with bit i set.
     uint64_t tmask, tval;
     uint64_t sgm[4];
     uint32_t jmask;

@@ -11,6 +11,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <complex>
 #include <vector>
 
 #include "fused.h"
@@ -230,6 +231,164 @@ static int op_cost(const EOp& e) {
     return 0;
 }
 
+// ---- fusion of a sweep's ops on at most two register bits ----------------
+//
+// A run of 1- and 2-register ops (dense or diagonal, controls included when
+// they are register bits too) that together touch at most two register bits
+// is one 4x4 (or 2x2) unitary: the product of the ops' embeddings.  Applied
+// as one dense op it does the arithmetic of the two 1-qubit gates it
+// replaces, and absorbs the CZ-type signs and diagonals between them.
+
+struct SweepBlock {
+    uint32_t regs = 0;     // register bits (bit r = register r)
+    std::vector<int> ops;  // eop indices, in program order
+};
+
+// register bits an op reads, writes or conditions on; *outside: whether it
+// also touches vector bits that are not registers of the sweep
+template <class Reg>
+static uint32_t op_regs(const EOp& e, Reg& reg, bool* outside) {
+    uint32_t m = 0;
+    *outside = false;
+    auto add = [&](int vbit) {
+        const int r = reg(vbit);
+        if (r < 0) *outside = true;
+        else m |= 1u << r;
+    };
+    for (int b = 0; b < e.g.k; ++b) add(e.g.tb[b]);
+    for (uint64_t c = e.g.cmask; c; c &= c - 1) add(__builtin_ctzll(c));
+    return m;
+}
+
+// per-amplitude VALU cost of an op as a separate sweep op (for the fusion rule)
+static int sweep_cost(const EOp& e) {
+    uint64_t pm;
+    int pd;
+    if (e.kind == E_DIAG) return sign_pair(e, pm, pd) ? 8 : 8;
+    return e.g.k == 1 ? (e.g.cmask ? 14 : 12) : 20;
+}
+
+// Embedding of op e into the block basis: block index bit p <-> register R[p].
+template <class Reg>
+static void embed(const EOp& e, Reg& reg, const int* R, int nb, std::vector<std::complex<double>>& E) {
+    const int D = 1 << nb;
+    E.assign((size_t)D * D, 0.0);
+    int tpos[8], cpos[8], cbit[8], nc = 0;
+    for (int b = 0; b < e.g.k; ++b) {
+        const int r = reg(e.g.tb[b]);
+        for (int p = 0; p < nb; ++p)
+            if (R[p] == r) tpos[b] = p;
+    }
+    for (uint64_t c = e.g.cmask; c; c &= c - 1) {
+        const int v = __builtin_ctzll(c), r = reg(v);
+        for (int p = 0; p < nb; ++p)
+            if (R[p] == r) cpos[nc] = p;
+        cbit[nc++] = (int)((e.g.cval >> v) & 1);
+    }
+    uint32_t tmask = 0;
+    for (int b = 0; b < e.g.k; ++b) tmask |= 1u << tpos[b];
+    for (int c = 0; c < D; ++c)
+        for (int r = 0; r < D; ++r) {
+            if ((r & ~tmask) != (c & ~tmask)) continue;  // bits the op leaves alone
+            bool on = true;
+            for (int i = 0; i < nc; ++i) on = on && (((r >> cpos[i]) & 1) == cbit[i]);
+            if (!on) {
+                if (r == c) E[r + (size_t)D * c] = 1.0;
+                continue;
+            }
+            int rs = 0, cs = 0;
+            for (int b = 0; b < e.g.k; ++b) {
+                rs |= ((r >> tpos[b]) & 1) << b;
+                cs |= ((c >> tpos[b]) & 1) << b;
+            }
+            double2 z;
+            if (e.kind == E_DIAG) {
+                if (rs != cs) continue;
+                z = e.g.m[rs];
+            } else {
+                z = e.g.m[rs + (1 << e.g.k) * cs];
+            }
+            E[r + (size_t)D * c] = std::complex<double>(z.x, z.y);
+        }
+}
+
+template <class Reg, class Emit>
+static void fuse_sweep(const std::vector<EOp>& eops, const std::vector<int>& take, Reg& reg, Emit& emit_one, Plan& plan) {
+    std::vector<SweepBlock> pend;
+    auto flush = [&](SweepBlock& b) {
+        const int nb = popc(b.regs);
+        int cost = 0;
+        for (int i : b.ops) cost += sweep_cost(eops[i]);
+        const int fused = nb == 1 ? 12 : 20;
+        if (b.ops.size() < 2 || cost <= fused) {
+            for (int i : b.ops) emit_one(i);
+            return;
+        }
+        int R[2], n = 0;
+        for (int r = 0; r < 4; ++r)
+            if (b.regs & (1u << r)) R[n++] = r;
+        const int D = 1 << nb;
+        std::vector<std::complex<double>> M((size_t)D * D, 0.0), E, T;
+        for (int d = 0; d < D; ++d) M[d + (size_t)D * d] = 1.0;
+        for (int i : b.ops) {  // M <- E_i M
+            embed(eops[i], reg, R, nb, E);
+            T.assign((size_t)D * D, 0.0);
+            for (int c = 0; c < D; ++c)
+                for (int k = 0; k < D; ++k) {
+                    const std::complex<double> mk = M[k + (size_t)D * c];
+                    if (mk == 0.0) continue;
+                    for (int r = 0; r < D; ++r) T[r + (size_t)D * c] += E[r + (size_t)D * k] * mk;
+                }
+            M.swap(T);
+        }
+        TOp op;
+        memset(&op, 0, sizeof(op));
+        op.kind = nb == 1 ? 0 : 1;
+        op.a = R[0];
+        op.b = nb == 2 ? R[1] : 0;
+        for (int j = 0; j < D * D; ++j) op.m[j] = make_double2(M[j].real(), M[j].imag());
+        plan.tops.push_back(op);
+        plan.top_src.push_back(b.ops.front());
+    };
+    for (int i : take) {
+        const EOp& e = eops[i];
+        bool outside = false;
+        const uint32_t R = op_regs(e, reg, &outside);
+        const bool fusable = !outside && (e.kind == E_DENSE || e.kind == E_DIAG) && popc(R) <= 2 && R != 0;
+        uint32_t U = R;
+        for (auto& b : pend)
+            if (b.regs & R) U |= b.regs;
+        if (fusable && popc(U) <= 2) {
+            SweepBlock nbk;
+            nbk.regs = U;
+            std::vector<SweepBlock> keep;
+            for (auto& b : pend) {
+                if (b.regs & R) nbk.ops.insert(nbk.ops.end(), b.ops.begin(), b.ops.end());
+                else keep.push_back(std::move(b));
+            }
+            nbk.ops.push_back(i);
+            keep.push_back(std::move(nbk));
+            pend.swap(keep);
+            continue;
+        }
+        std::vector<SweepBlock> keep;
+        for (auto& b : pend) {
+            if (b.regs & R) flush(b);
+            else keep.push_back(std::move(b));
+        }
+        pend.swap(keep);
+        if (fusable) {
+            SweepBlock nbk;
+            nbk.regs = R;
+            nbk.ops.push_back(i);
+            pend.push_back(std::move(nbk));
+        } else {
+            emit_one(i);
+        }
+    }
+    for (auto& b : pend) flush(b);
+}
+
 int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan& plan) {
     plan = Plan();
     const size_t n = eops.size();
@@ -294,6 +453,7 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
         P.T = T;
         P.lmask = S;
         int pos[64];
+        for (int b = 0; b < 64; ++b) pos[b] = -1;  // -1: not a tile bit
         for (int b = 0, l = 0; b < vq; ++b)
             if (S & (1ull << b)) {
                 P.lbits[l] = b;
@@ -336,7 +496,14 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
             for (int b = 0; b < T; ++b)
                 if (W & (1u << b)) sw.sb[nb++] = b;
             sw.op_begin = (int)plan.tops.size();
-            for (int i : take) {
+            auto reg = [&](int vbit) {
+                const int l = pos[vbit];
+                if (l < 0) return -1;
+                for (int r = 0; r < 4; ++r)
+                    if (sw.sb[r] == l) return r;
+                return -1;
+            };
+            auto emit_one = [&](int i) {
                 const EOp& e = eops[i];
                 TOp op;
                 memset(&op, 0, sizeof(op));
@@ -345,28 +512,33 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
                 uint64_t pm = 0;
                 int pd = 0;
                 if (e.kind == E_DIAG && sign_pair(e, pm, pd)) {
-                    // consecutive sign ops with the same bit distance multiply:
-                    // their pair masks add modulo 2
-                    if (plan.tops.size() > (size_t)sw.op_begin && plan.tops.back().kind == 3 &&
-                        plan.tops.back().pd == pd) {
-                        plan.tops.back().pm ^= pm;
-                        continue;
+                    // consecutive sign ops multiply: their parities add modulo 2
+                    // (pair masks of the same bit distance add, others are kept
+                    // as separate terms, up to 4)
+                    if (plan.tops.size() > (size_t)sw.op_begin && plan.tops.back().kind == 3) {
+                        TOp& prev = plan.tops.back();
+                        for (int k = 0; k < prev.npairs; ++k)
+                            if (prev.pd[k] == pd) {
+                                prev.pm[k] ^= pm;
+                                return;
+                            }
+                        if (prev.npairs < 4) {
+                            prev.pm[prev.npairs] = pm;
+                            prev.pd[prev.npairs] = pd;
+                            ++prev.npairs;
+                            return;
+                        }
                     }
                     op.kind = 3;
-                    op.pm = pm;
-                    op.pd = pd;
+                    op.pm[0] = pm;
+                    op.pd[0] = pd;
+                    op.npairs = 1;
                 } else if (e.kind == E_DIAG) {
                     op.kind = 2;
                     op.nd = e.g.k;
                     for (int b = 0; b < e.g.k; ++b) op.dbit[b] = e.g.tb[b];
                     for (int j = 0; j < (1 << e.g.k); ++j) op.m[j] = e.g.m[j];
                 } else {
-                    auto reg = [&](int vbit) {
-                        int l = pos[vbit];
-                        for (int r = 0; r < 4; ++r)
-                            if (sw.sb[r] == l) return r;
-                        return -1;
-                    };
                     if (e.g.k == 1) {
                         op.kind = 0;
                         op.a = reg(e.g.tb[0]);
@@ -389,7 +561,8 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
                 }
                 plan.tops.push_back(op);
                 plan.top_src.push_back(i);
-            }
+            };
+            fuse_sweep(eops, take, reg, emit_one, plan);
             sw.op_end = (int)plan.tops.size();
             // index tests split into a per-thread part and a part fixed by the register
             uint64_t gb[4], G = 0;
@@ -409,10 +582,16 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
                     uint32_t q = 0;
                     for (int j = 0; j < 16; ++j) {
                         const uint64_t x = rj(j);
-                        q |= (uint32_t)(popc(x & (x >> op.pd) & op.pm) & 1) << j;
+                        int par = 0;
+                        for (int k = 0; k < op.npairs; ++k) par ^= popc(x & (x >> op.pd[k]) & op.pm[k]) & 1;
+                        q |= (uint32_t)par << j;
                     }
                     op.jmask = q;
-                    for (int r = 0; r < 4; ++r) op.sgm[r] = ((gb[r] >> op.pd) & op.pm) ^ ((gb[r] & op.pm) << op.pd);
+                    for (int r = 0; r < 4; ++r) {
+                        op.sgm[r] = 0;
+                        for (int k = 0; k < op.npairs; ++k)
+                            op.sgm[r] ^= ((gb[r] >> op.pd[k]) & op.pm[k]) ^ ((gb[r] & op.pm[k]) << op.pd[k]);
+                    }
                 } else if (op.cmask) {
                     op.tmask = op.cmask & ~G;
                     op.tval = op.cval & ~G;

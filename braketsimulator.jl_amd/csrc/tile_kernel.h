@@ -145,14 +145,17 @@ __device__ __forceinline__ void sw_diag(SubCube& s, const TOp* __restrict__ op) 
     });
 }
 
-// +-1 diagonal (CZ layers): negate where P(i) = parity(i & (i >> pd) & pm)
-// is 1.  P is a quadratic form over GF(2): for i = gbase ^ r_j,
+// +-1 diagonal (CZ layers): negate where
+// P(i) = sum_k parity(i & (i >> pd[k]) & pm[k]) is 1.  P is a quadratic form
+// over GF(2): for i = gbase ^ r_j,
 // P(i) = P(gbase) ^ P(r_j) ^ sum over the register bits of j of
 // parity(gbase & sgm[bit]); the planner stores P(r_j) for all 16 j in jmask.
 __device__ __forceinline__ void sw_sign(SubCube& s, const TOp* __restrict__ op) {
     const uint64_t a = s.gbase;
     uint32_t m = op->jmask;
-    if (__popcll(a & (a >> op->pd) & op->pm) & 1) m ^= 0xFFFFu;
+    int par = 0;
+    for (int k = 0; k < op->npairs; ++k) par ^= __popcll(a & (a >> op->pd[k]) & op->pm[k]);
+    if (par & 1) m ^= 0xFFFFu;
     if (__popcll(a & op->sgm[0]) & 1) m ^= 0xAAAAu;
     if (__popcll(a & op->sgm[1]) & 1) m ^= 0xCCCCu;
     if (__popcll(a & op->sgm[2]) & 1) m ^= 0xF0F0u;

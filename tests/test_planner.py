@@ -1,0 +1,115 @@
+"""The fused planner on the CPU: libbsim's own host planner (lower_ops +
+make_plan) is run on random circuits and its plan is interpreted op by op
+(tests/native/plan_interp.cpp) on a full state vector; the result must match
+the oracle.  This pins the planner's pass/sweep ordering, the 2-register
+block fusion, the sign-op merging and the per-thread/per-register split of
+the index tests without a GPU."""
+import ctypes
+import os
+import shutil
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle as O
+from bsim import _lib as L
+from bsim import gates as G
+from bsim import workloads as W
+from bsim.simulators import Instruction, lower_instructions
+
+from circuits import grcs16, random_circuit
+from conftest import GOLDEN, ROOT
+
+HERE = Path(__file__).resolve().parent
+SRC = HERE / "native" / "plan_interp.cpp"
+OUT = HERE / "native" / "_build" / "libplan_interp.so"
+
+
+@pytest.fixture(scope="module")
+def interp():
+    lib = ROOT / "braketsimulator.jl_amd" / "lib" / "libbsim.so"
+    if not lib.exists():
+        pytest.skip("libbsim not built")
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    csrc = ROOT / "braketsimulator.jl_amd" / "csrc"
+    deps = [SRC, lib] + list(csrc.glob("*.h"))
+    if not OUT.exists() or OUT.stat().st_mtime < max(p.stat().st_mtime for p in deps):
+        OUT.parent.mkdir(exist_ok=True)
+        cmd = [hipcc, "-O1", "-std=c++17", "-fPIC", "-shared", "-I", str(ROOT / "include"), "-I", str(csrc),
+               str(SRC), "-o", str(OUT), "-L", str(lib.parent), "-lbsim", f"-Wl,-rpath,{lib.parent}"]
+        subprocess.run(cmd, check=True, capture_output=True)
+    L.load()
+    h = ctypes.CDLL(str(OUT))
+    h.planexec.restype = ctypes.c_int
+    h.planexec.argtypes = [ctypes.c_int, ctypes.POINTER(L.Op), ctypes.c_int, ctypes.c_void_p]
+    return h
+
+
+def _run(h, n, ins):
+    ol = lower_instructions(ins, L.BSIM_SV)
+    psi = np.zeros(2 ** n, dtype=np.complex128)
+    psi[0] = 1
+    rc = h.planexec(n, ol.ptr, ol.n, psi.ctypes.data)
+    assert rc == 0, f"planexec returned {rc} (-10/-11: an index-test split disagrees with the direct test)"
+    return psi
+
+
+@pytest.fixture
+def tile_config():
+    t, lo = ctypes.c_int(), ctypes.c_int()
+    L.load()
+    L.call("bsim_get_tile_config", ctypes.byref(t), ctypes.byref(lo))
+    yield
+    L.call("bsim_set_tile_config", t.value, lo.value)
+
+
+KINDS = ("1q", "2q", "ctrl", "diag", "pow")
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_planner_random_circuits(interp, seed):
+    rng = np.random.default_rng(100 + seed)
+    n = int(rng.integers(4, 14))
+    ins = random_circuit(n, 80, rng, kinds=KINDS)
+    assert np.max(np.abs(_run(interp, n, ins) - O.evolve_sv(n, ins))) < 1e-12
+
+
+@pytest.mark.parametrize("tile,low", [(6, 0), (8, 3), (12, 4), (12, 10)])
+def test_planner_tile_configs(interp, tile_config, tile, low):
+    L.call("bsim_set_tile_config", tile, low)
+    rng = np.random.default_rng(tile * 31 + low)
+    n = 13
+    ins = random_circuit(n, 70, rng, kinds=KINDS)
+    assert np.max(np.abs(_run(interp, n, ins) - O.evolve_sv(n, ins))) < 1e-12
+
+
+def test_planner_brickwall_and_grcs(interp):
+    ins = W.brickwall(14, 12, seed=4)
+    assert np.max(np.abs(_run(interp, 14, ins) - O.evolve_sv(14, ins))) < 1e-12
+    g = grcs16(GOLDEN / "grcs_16.qasm")
+    assert np.max(np.abs(_run(interp, 16, g) - O.evolve_sv(16, g))) < 1e-12
+
+
+def test_planner_cz_with_global_bits(interp):
+    """Minimal case of a planner bug found on the GPU: a CZ whose second bit
+    lies outside the tile must flush the 2-register block of its tile bit."""
+    h = np.pi / 2
+    ins = [Instruction(G.Rx(h), [2]), Instruction(G.Rx(h), [8]), Instruction(G.Rx(h), [13]),
+           Instruction(G.CZ(), [8, 4]), Instruction(G.Ry(h), [1]), Instruction(G.Rx(h), [3])]
+    assert np.max(np.abs(_run(interp, 14, ins) - O.evolve_sv(14, ins))) < 1e-12
+
+
+def test_planner_sign_terms(interp):
+    """CZ layers at several bit distances merge into one multi-term sign op;
+    every split sign table is checked against the direct parity inside the
+    interpreter."""
+    n = 12
+    ins = [Instruction(G.H(), [q]) for q in range(n)]
+    for d in (1, 2, 3, 5, 7):
+        ins += [Instruction(G.CZ(), [q, q + d]) for q in range(0, n - d, 2)]
+    ins += [Instruction(G.Rx(0.3 * q), [q]) for q in range(n)]
+    assert np.max(np.abs(_run(interp, n, ins) - O.evolve_sv(n, ins))) < 1e-12
