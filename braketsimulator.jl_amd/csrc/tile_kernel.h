@@ -109,12 +109,18 @@ __device__ __forceinline__ void sw_dense2(SubCube& s, const TOp* __restrict__ op
                 acc = cfma(to_c2(op->m[r + 8]), a2, acc);
                 return cfma(to_c2(op->m[r + 12]), a3, acc);
             };
-            const c2 r0 = row(0), r1 = row(1), r2 = row(2), r3 = row(3);
-            const bool on = !ctl || (tok && ((jm >> j) & 1u));
-            setv<j>(s.v, csel(on, r0, a0));
-            setv<i1>(s.v, csel(on, r1, a1));
-            setv<i2>(s.v, csel(on, r2, a2));
-            setv<i3>(s.v, csel(on, r3, a3));
+            c2 r0 = row(0), r1 = row(1), r2 = row(2), r3 = row(3);
+            if (ctl) {
+                const bool on = tok && ((jm >> j) & 1u);
+                r0 = csel(on, r0, a0);
+                r1 = csel(on, r1, a1);
+                r2 = csel(on, r2, a2);
+                r3 = csel(on, r3, a3);
+            }
+            setv<j>(s.v, r0);
+            setv<i1>(s.v, r1);
+            setv<i2>(s.v, r2);
+            setv<i3>(s.v, r3);
         }
     });
 }
