@@ -69,12 +69,17 @@ struct SubCube {
     }
 };
 
-template <int A>
-__device__ __forceinline__ void sw_dense1(SubCube& s, const TOp* __restrict__ op) {
+// Dense ops come in two instantiations: without controls (the common case)
+// and with the split control test; the choice is made once per op.
+template <int A, bool CTL>
+__device__ __forceinline__ void sw_dense1_t(SubCube& s, const TOp* __restrict__ op) {
     const c2 m00 = to_c2(op->m[0]), m10 = to_c2(op->m[1]), m01 = to_c2(op->m[2]), m11 = to_c2(op->m[3]);
-    const bool ctl = op->cmask != 0;
-    const bool tok = (s.gbase & op->tmask) == op->tval;  // this thread's part of the control test
-    const uint32_t jm = op->jmask;                         // the registers' part
+    bool tok = true;
+    uint32_t jm = 0xFFFFu;
+    if constexpr (CTL) {
+        tok = (s.gbase & op->tmask) == op->tval;  // this thread's part of the control test
+        jm = op->jmask;                           // the registers' part
+    }
     sfor<0, 16>([&](auto J) {
         constexpr int j = decltype(J)::value;
         if constexpr ((j & (1 << A)) == 0) {
@@ -82,7 +87,7 @@ __device__ __forceinline__ void sw_dense1(SubCube& s, const TOp* __restrict__ op
             const c2 a = getv<j>(s.v), b = getv<k>(s.v);
             c2 x = cfma(m01, b, cmul(m00, a));
             c2 y = cfma(m11, b, cmul(m10, a));
-            if (ctl) {
+            if constexpr (CTL) {
                 const bool on = tok && ((jm >> j) & 1u);
                 x = csel(on, x, a);
                 y = csel(on, y, b);
@@ -93,11 +98,20 @@ __device__ __forceinline__ void sw_dense1(SubCube& s, const TOp* __restrict__ op
     });
 }
 
-template <int A, int B>
-__device__ __forceinline__ void sw_dense2(SubCube& s, const TOp* __restrict__ op) {
-    const bool ctl = op->cmask != 0;
-    const bool tok = (s.gbase & op->tmask) == op->tval;
-    const uint32_t jm = op->jmask;
+template <int A>
+__device__ __forceinline__ void sw_dense1(SubCube& s, const TOp* __restrict__ op) {
+    if (op->cmask) sw_dense1_t<A, true>(s, op);
+    else sw_dense1_t<A, false>(s, op);
+}
+
+template <int A, int B, bool CTL>
+__device__ __forceinline__ void sw_dense2_t(SubCube& s, const TOp* __restrict__ op) {
+    bool tok = true;
+    uint32_t jm = 0xFFFFu;
+    if constexpr (CTL) {
+        tok = (s.gbase & op->tmask) == op->tval;
+        jm = op->jmask;
+    }
     sfor<0, 16>([&](auto J) {
         constexpr int j = decltype(J)::value;
         if constexpr ((j & ((1 << A) | (1 << B))) == 0) {
@@ -110,7 +124,7 @@ __device__ __forceinline__ void sw_dense2(SubCube& s, const TOp* __restrict__ op
                 return cfma(to_c2(op->m[r + 12]), a3, acc);
             };
             c2 r0 = row(0), r1 = row(1), r2 = row(2), r3 = row(3);
-            if (ctl) {
+            if constexpr (CTL) {
                 const bool on = tok && ((jm >> j) & 1u);
                 r0 = csel(on, r0, a0);
                 r1 = csel(on, r1, a1);
@@ -123,6 +137,12 @@ __device__ __forceinline__ void sw_dense2(SubCube& s, const TOp* __restrict__ op
             setv<i3>(s.v, r3);
         }
     });
+}
+
+template <int A, int B>
+__device__ __forceinline__ void sw_dense2(SubCube& s, const TOp* __restrict__ op) {
+    if (op->cmask) sw_dense2_t<A, B, true>(s, op);
+    else sw_dense2_t<A, B, false>(s, op);
 }
 
 // Diagonal index of register j = (index bits read from the thread's base)
