@@ -87,53 +87,6 @@ __global__ __launch_bounds__(256) void k_sum_finish(const double* __restrict__ p
     if (threadIdx.x == 0) out[0] = sh[0];
 }
 
-// W_i = floor(p_i * scale); total and max by integer atomics (order-free),
-// one per wave after a wave reduction
-__global__ __launch_bounds__(256) void k_quantize(const double* __restrict__ p, int64_t n,
-                                                  const double* __restrict__ wsum_dev, double wsum_host,
-                                                  uint64_t* __restrict__ W, unsigned long long* __restrict__ acc) {
-    const double wsum = wsum_host > 0 ? wsum_host : wsum_dev[0];
-    const double scale = (double)n * (double)ALIAS_C / wsum;
-    unsigned long long tot = 0, mx = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        double x = p[i] * scale;
-        uint64_t w = x > 0 ? (uint64_t)x : 0;
-        W[i] = w;
-        tot += w;
-        mx = w > mx ? w : mx;
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-        tot += __shfl_xor(tot, o);
-        const unsigned long long m = __shfl_xor(mx, o);
-        mx = m > mx ? m : mx;
-    }
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&acc[0], tot);
-        atomicMax(&acc[1], mx);
-    }
-}
-
-__global__ __launch_bounds__(256) void k_argmax(const uint64_t* __restrict__ W, int64_t n,
-                                                unsigned long long* __restrict__ acc) {
-    const unsigned long long mx = acc[1];
-    unsigned long long best = ~0ull;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        if (W[i] == mx && (unsigned long long)i < best) best = (unsigned long long)i;
-    for (int o = 32; o > 0; o >>= 1) {
-        const unsigned long long b = __shfl_xor(best, o);
-        best = b < best ? b : best;
-    }
-    if ((threadIdx.x & 63) == 0 && best != ~0ull) atomicMin(&acc[2], best);
-}
-
-__global__ void k_fix_remainder(uint64_t* __restrict__ W, int64_t n, const unsigned long long* __restrict__ acc) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        uint64_t target = (uint64_t)n * ALIAS_C;
-        uint64_t tot = acc[0];
-        W[acc[2]] += target - tot;  // modular arithmetic handles either sign
-    }
-}
-
 // ---- lookups ---------------------------------------------------------------
 __global__ void k_sample(const double* __restrict__ ap, const int32_t* __restrict__ alias, int nq, int64_t shots,
                          uint32_t k0, uint32_t k1, int64_t* __restrict__ out) {
@@ -170,9 +123,13 @@ static int alias_build(bsim_state* h, int mode, double wsum) {
     if (h->nq > 31) return fail(BSIM_ENOTSUP, "alias tables hold at most 2^31 entries");
     if (int rc = ensure_ws(h->ap, sizeof(double) * (size_t)n)) return rc;
     if (int rc = ensure_ws(h->alias, sizeof(int32_t) * (size_t)n)) return rc;
-    // work: p (8n) | W (8n) | Dinc (8n) | Einc (8n) | tile sums | scalars
-    int64_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
-    size_t bytes = sizeof(uint64_t) * ((size_t)4 * n + 4 * (size_t)ntiles + 4096 + 64) + sizeof(int32_t) * ((size_t)2 * n + 2);
+    // work: p (8n, later the lights' W) | W (8n) | Lval (8n) | Hval (8n) | Lidx (4n) |
+    // Hidx (4n) | tile sums | merge splits (2x) | totals | scalars
+    const int64_t ntiles = (n + AT_TILE - 1) / AT_TILE;
+    const int64_t nsplits = (n + MERGE_G - 1) / MERGE_G + 2;
+    size_t bytes = sizeof(uint64_t) * (size_t)4 * n + sizeof(int32_t) * (size_t)2 * n + 256 +
+                   sizeof(TileAcc) * (size_t)ntiles + sizeof(int64_t) * 2 * (size_t)nsplits + sizeof(Acc4) * 2 +
+                   sizeof(double) * (8 + SUM_BLOCKS) + 1024;
     if (mode == BSIM_ALIAS_REFERENCE) bytes = sizeof(double) * (size_t)n + sizeof(int32_t) * 2 * (size_t)n + 65536;
     if (int rc = ensure_ws(h->alias_work, bytes)) return rc;
     char* base = (char*)h->alias_work.ptr;
@@ -200,34 +157,35 @@ static int alias_build(bsim_state* h, int mode, double wsum) {
         uint64_t* Hval = Lval + n;
         int32_t* Lidx = (int32_t*)(Hval + n);
         int32_t* Hidx = Lidx + n;
-        Acc4* tsum = (Acc4*)(Hidx + n + (n & 1));
-        Acc4* total = tsum + ntiles;
+        char* q = (char*)(Hidx + n);
+        q = (char*)(((uintptr_t)q + 255) & ~(uintptr_t)255);
+        TileAcc* ta = (TileAcc*)q;
+        int64_t* splitsL = (int64_t*)(ta + ntiles);
+        int64_t* splitsH = splitsL + nsplits;
+        Acc4* total = (Acc4*)(splitsH + nsplits);
         double* scal = (double*)(total + 2);
-        unsigned long long* acc = (unsigned long long*)(scal + 8 + SUM_BLOCKS);
+        uint64_t* Lw = (uint64_t*)p;  // p is not read after the quantization
         if (wsum <= 0) {
             const int gs = (int)grid_for(n, 256, SUM_BLOCKS);
             hipLaunchKernelGGL(k_sum_p, dim3(gs), dim3(256), 0, h->stream, p, n, scal + 8);
             hipLaunchKernelGGL(k_sum_finish, dim3(1), dim3(256), 0, h->stream, scal + 8, gs, scal);
         }
-        unsigned long long init[4] = { 0ull, 0ull, ~0ull, 0ull };
-        BSIM_HIP(hipMemcpyAsync(acc, init, sizeof(init), hipMemcpyHostToDevice, h->stream));
-        int64_t g = grid_for(n, 256, cap);
-        hipLaunchKernelGGL(k_quantize, dim3((unsigned)g), dim3(256), 0, h->stream, p, n, (const double*)scal, wsum, W,
-                           acc);
-        hipLaunchKernelGGL(k_argmax, dim3((unsigned)g), dim3(256), 0, h->stream, W, n, acc);
-        hipLaunchKernelGGL(k_fix_remainder, dim3(1), dim3(64), 0, h->stream, W, n, acc);
-        hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)ntiles), dim3(SCAN_BLOCK), 0, h->stream, W, n, tsum);
-        hipLaunchKernelGGL(k_scan_tilesums, dim3(1), dim3(SCAN_BLOCK), 0, h->stream, tsum, ntiles, total);
-        hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)ntiles), dim3(SCAN_BLOCK), 0, h->stream, W, n,
-                           (const Acc4*)tsum, Lidx, Lval, Hidx, Hval);
-        int64_t gr = grid_for(n, 256, cap * 8);  // one item per lane
-        hipLaunchKernelGGL(k_alias_lights, dim3((unsigned)gr), dim3(256), 0, h->stream, W, Lidx, Lval, Hidx, Hval,
-                           (const Acc4*)total, ap, al);
-        hipLaunchKernelGGL(k_alias_heavies, dim3((unsigned)gr), dim3(256), 0, h->stream, Lval, Hidx, Hval,
-                           (const Acc4*)total, ap, al);
+        hipLaunchKernelGGL(k_quant_tiles, dim3((unsigned)ntiles), dim3(AT_BLOCK), 0, h->stream, p, n,
+                           (const double*)scal, wsum, W, ta);
+        hipLaunchKernelGGL(k_tiles_finish, dim3(1), dim3(1024), 0, h->stream, ta, ntiles, n, W, total);
+        hipLaunchKernelGGL(k_compact, dim3((unsigned)ntiles), dim3(AT_BLOCK), 0, h->stream, W, n, (const TileAcc*)ta,
+                           Lidx, Lval, Lw, Hidx, Hval);
+        const unsigned gp = (unsigned)grid_for(nsplits, 256, 1 << 20);
+        hipLaunchKernelGGL(k_partition<0>, dim3(gp), dim3(256), 0, h->stream, Lval, Lw, Hval, (const Acc4*)total,
+                           splitsL, nsplits);
+        hipLaunchKernelGGL(k_partition<1>, dim3(gp), dim3(256), 0, h->stream, Lval, Lw, Hval, (const Acc4*)total,
+                           splitsH, nsplits);
+        const unsigned gm = (unsigned)grid_for(n, 256, cap * 8);  // one A item per lane (at most n)
+        hipLaunchKernelGGL(k_merge<0>, dim3(gm), dim3(256), 0, h->stream, Lval, Lw, Lidx, Hval, Hidx,
+                           (const Acc4*)total, (const int64_t*)splitsL, ap, al);
+        hipLaunchKernelGGL(k_merge<1>, dim3(gm), dim3(256), 0, h->stream, Lval, Lw, Lidx, Hval, Hidx,
+                           (const Acc4*)total, (const int64_t*)splitsH, ap, al);
         BSIM_HIP(hipGetLastError());
-        // `init` lives on this frame: make sure the copy has been consumed
-        BSIM_HIP(hipStreamSynchronize(h->stream));
     } else {
         return fail(BSIM_EINVAL, "unknown alias mode");
     }
