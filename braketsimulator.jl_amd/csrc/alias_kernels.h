@@ -65,7 +65,8 @@ __device__ __forceinline__ void max_arg(uint64_t& mx, int64_t& arg, uint64_t w, 
     }
 }
 
-__global__ __launch_bounds__(AT_BLOCK) void k_quant_tiles(const double* __restrict__ p, int64_t n,
+template <class Src>
+__global__ __launch_bounds__(AT_BLOCK) void k_quant_tiles(Src p, int64_t n,
                                                           const double* __restrict__ wsum_dev, double wsum_host,
                                                           uint64_t* __restrict__ W, TileAcc* __restrict__ ta) {
     __shared__ uint64_t sh[4][7];
@@ -78,7 +79,7 @@ __global__ __launch_bounds__(AT_BLOCK) void k_quant_tiles(const double* __restri
     for (int r = 0; r < AT_ROWS; ++r) {
         const int64_t i = base + r * AT_BLOCK + t;
         if (i < n) {
-            const double x = p[i] * scale;
+            const double x = p(i) * scale;
             const uint64_t w = x > 0 ? (uint64_t)x : 0;
             W[i] = w;
             if (w <= ALIAS_C) {
@@ -211,42 +212,53 @@ __global__ __launch_bounds__(AT_BLOCK) void k_compact(const uint64_t* __restrict
                                                       const TileAcc* __restrict__ ta, int32_t* __restrict__ Lidx,
                                                       uint64_t* __restrict__ Lval, uint64_t* __restrict__ Lw,
                                                       int32_t* __restrict__ Hidx, uint64_t* __restrict__ Hval) {
-    __shared__ uint64_t sh[2][4][4];  // [row parity][wave][value]
+    __shared__ uint64_t sh[2][4][2];  // [row parity][wave][light, heavy]
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int64_t base = (int64_t)blockIdx.x * AT_TILE;
     const TileAcc a = ta[blockIdx.x];
     uint64_t od = a.d, oe = a.e, onl = a.nl, onh = a.nh;
+    // per row: light deficit + (light count << 48) and heavy excess + (heavy
+    // count << 48) in one 64-bit scan each (a row's deficits stay below 2^40)
+    constexpr int CS = 48;
+    constexpr uint64_t LOW = (1ull << CS) - 1;
     for (int r = 0; r < AT_ROWS; ++r) {
         const int64_t i = base + r * AT_BLOCK + t;
         const bool valid = i < n;
         const uint64_t w = valid ? W[i] : 0;
         const bool isL = valid && w <= ALIAS_C, isH = valid && w > ALIAS_C;
-        uint64_t v[4] = { isL ? ALIAS_C - w : 0, isH ? w - ALIAS_C : 0, isL ? 1u : 0u, isH ? 1u : 0u };
-        for (int q = 0; q < 4; ++q) v[q] = wave_scan_u64(v[q], lane);
+        const uint64_t vl = wave_scan_u64(isL ? (ALIAS_C - w) | (1ull << CS) : 0, lane);
+        const uint64_t vh = wave_scan_u64(isH ? (w - ALIAS_C) | (1ull << CS) : 0, lane);
         const int par = r & 1;
-        if (lane == 63)
-            for (int q = 0; q < 4; ++q) sh[par][wave][q] = v[q];
+        if (lane == 63) {
+            sh[par][wave][0] = vl;
+            sh[par][wave][1] = vh;
+        }
         __syncthreads();
-        uint64_t pre[4] = { 0, 0, 0, 0 }, row[4] = { 0, 0, 0, 0 };
-        for (int ww = 0; ww < 4; ++ww)
-            for (int q = 0; q < 4; ++q) {
-                if (ww < wave) pre[q] += sh[par][ww][q];
-                row[q] += sh[par][ww][q];
+        uint64_t pl = 0, ph = 0, rl = 0, rh = 0;
+        for (int ww = 0; ww < 4; ++ww) {
+            if (ww < wave) {
+                pl += sh[par][ww][0];
+                ph += sh[par][ww][1];
             }
+            rl += sh[par][ww][0];
+            rh += sh[par][ww][1];
+        }
         if (isL) {
-            const uint64_t pos = onl + pre[2] + v[2] - 1;
+            const uint64_t x = pl + vl;
+            const uint64_t pos = onl + (x >> CS) - 1;
             Lidx[pos] = (int32_t)i;
-            Lval[pos] = od + pre[0] + v[0];
+            Lval[pos] = od + (x & LOW);
             Lw[pos] = w;
         } else if (isH) {
-            const uint64_t pos = onh + pre[3] + v[3] - 1;
+            const uint64_t x = ph + vh;
+            const uint64_t pos = onh + (x >> CS) - 1;
             Hidx[pos] = (int32_t)i;
-            Hval[pos] = oe + pre[1] + v[1];
+            Hval[pos] = oe + (x & LOW);
         }
-        od += row[0];
-        oe += row[1];
-        onl += row[2];
-        onh += row[3];
+        od += rl & LOW;
+        oe += rh & LOW;
+        onl += rl >> CS;
+        onh += rh >> CS;
     }
 }
 
@@ -299,78 +311,105 @@ __global__ void k_partition(const uint64_t* __restrict__ Lval, const uint64_t* _
     }
 }
 
-// One wavefront per 64 consecutive A items: the wave's smallest answer r0
-// comes from the coarse splits, the 128 B items from r0 on are staged in LDS
-// (coalesced), and every lane finds its own answer there (a global search
-// only if it lies beyond the window).
+// One wavefront assigns the A items l0 .. l0+63 (those below lend): the
+// wave's smallest answer r0 comes from the coarse splits, the 128 B items from
+// r0 on are staged in LDS (`win`, coalesced), and every lane finds its own
+// answer there (a global search only if it lies beyond the window).  The
+// result (index, ap, alias) goes to `put`.
+template <int MODE, class Put>
+__device__ __forceinline__ void assign_wave(const uint64_t* __restrict__ Lval, const uint64_t* __restrict__ Lw,
+                                            const int32_t* __restrict__ Lidx, const uint64_t* __restrict__ Hval,
+                                            const int32_t* __restrict__ Hidx, int64_t nA, int64_t nB,
+                                            const int64_t* __restrict__ rc, int64_t l0, int64_t lend,
+                                            uint64_t* __restrict__ win, Put&& put) {
+    constexpr int WIN = 128;
+    const int lane = threadIdx.x & 63;
+    const int64_t l = l0 + lane;
+    const bool valid = l < lend;
+    const uint64_t a = valid ? key_a<MODE>(Lval, Lw, Hval, l) : 0;
+    // r0 = answer of item l0, in [rc[c], rc[c+1]]: every round the 64 lanes
+    // probe 64 evenly spaced points and the interval shrinks 64-fold
+    const uint64_t a0 = shfl_u64(a, 0);
+    const int64_t c = l0 / MERGE_G;
+    int64_t lo = rc[c], hi = rc[c + 1];
+    while (lo < hi) {
+        const int64_t step = (hi - lo + 63) / 64;
+        const int64_t pk = lo + (int64_t)lane * step;
+        const bool pred = pk < hi && a_first<MODE>(a0, key_b<MODE>(Lval, Hval, pk));
+        const uint64_t m = __ballot(pred);
+        if (m == 0) {  // past the last probe below hi
+            lo = lo + ((hi - 1 - lo) / step) * step + 1;
+        } else {  // between the last false probe and the first true one
+            const int k = __builtin_ctzll(m);
+            const int64_t first = lo + (int64_t)k * step;
+            if (k > 0) lo = first - step + 1;
+            hi = first;
+        }
+    }
+    const int64_t r0 = hi;
+    const int64_t nwin = nB - r0 < WIN ? nB - r0 : WIN;
+    for (int k = lane; k < WIN; k += 64) win[k] = k < nwin ? key_b<MODE>(Lval, Hval, r0 + k) : ~0ull;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (valid) {
+        int lo = 0, hi = (int)nwin;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (a_first<MODE>(a, win[mid])) hi = mid;
+            else lo = mid + 1;
+        }
+        int64_t r = r0 + lo;
+        uint64_t br = 0;  // B[r] if r < nB
+        if (lo < nwin) {
+            br = win[lo];
+        } else if (r < nB) {  // beyond the window: gallop, then bisect
+            int64_t step = 1, lo2 = r, hi2 = r;
+            while (hi2 < nB && !a_first<MODE>(a, key_b<MODE>(Lval, Hval, hi2))) {
+                lo2 = hi2 + 1;
+                hi2 = r + step;
+                step <<= 1;
+            }
+            if (hi2 > nB) hi2 = nB;
+            r = first_after<MODE>(Lval, Hval, a, lo2, hi2);
+            if (r < nB) br = key_b<MODE>(Lval, Hval, r);
+        }
+        if constexpr (MODE == 0) {  // light: alias = first heavy whose prefix excess exceeds a
+            const int32_t idx = Lidx[l];
+            const uint64_t w = Lw[l];
+            put(idx, (double)w / (double)ALIAS_C, (w == ALIAS_C || r >= nB) ? idx : Hidx[r]);
+        } else {  // heavy: residual from the first light whose prefix deficit reaches a
+            const int32_t idx = Hidx[l];
+            uint64_t rr = r < nB ? ALIAS_C + a - br : ALIAS_C;
+            int32_t nx = idx;
+            if (l + 1 < nA) nx = Hidx[l + 1];
+            else rr = ALIAS_C;
+            put(idx, (double)rr / (double)ALIAS_C, nx);
+        }
+    }
+    __builtin_amdgcn_wave_barrier();  // the window is rewritten by the next call
+}
+
+// One wavefront per 64 consecutive A items, all waves of the grid striding
+// over the list; results are written straight to the table.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_merge(const uint64_t* __restrict__ Lval, const uint64_t* __restrict__ Lw,
                                                const int32_t* __restrict__ Lidx, const uint64_t* __restrict__ Hval,
                                                const int32_t* __restrict__ Hidx, const Acc4* __restrict__ total,
                                                const int64_t* __restrict__ rc, double* __restrict__ ap,
                                                int32_t* __restrict__ alias) {
-    constexpr int WIN = 128;
-    __shared__ uint64_t win[4][WIN];
+    __shared__ uint64_t win[4][128];
     const int64_t nl = (int64_t)total->nl, nh = (int64_t)total->nh;
     const int64_t nA = MODE == 0 ? nl : nh, nB = MODE == 0 ? nh : nl;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wave = threadIdx.x >> 6;
     const int64_t nwaves = (nA + 63) / 64;
     const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / 64);
-    for (int64_t wv = (int64_t)blockIdx.x * (blockDim.x / 64) + wave; wv < nwaves; wv += wstride) {
-        const int64_t l = wv * 64 + lane;
-        const bool valid = l < nA;
-        const uint64_t a = valid ? key_a<MODE>(Lval, Lw, Hval, l) : 0;
-        // the wave's first item: search between its coarse split points
-        int64_t r0 = 0;
-        if (lane == 0) {
-            const int64_t c = (wv * 64) / MERGE_G;
-            r0 = first_after<MODE>(Lval, Hval, a, rc[c], rc[c + 1]);
-        }
-        r0 = (int64_t)shfl_u64((uint64_t)r0, 0);
-        const int64_t nwin = nB - r0 < WIN ? nB - r0 : WIN;
-        for (int k = lane; k < WIN; k += 64) win[wave][k] = k < nwin ? key_b<MODE>(Lval, Hval, r0 + k) : ~0ull;
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (valid) {
-            int lo = 0, hi = (int)nwin;
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (a_first<MODE>(a, win[wave][mid])) hi = mid;
-                else lo = mid + 1;
-            }
-            int64_t r = r0 + lo;
-            uint64_t br = 0;  // B[r] if r < nB
-            if (lo < nwin) {
-                br = win[wave][lo];
-            } else if (r < nB) {  // beyond the window: gallop, then bisect
-                int64_t step = 1, lo2 = r, hi2 = r;
-                while (hi2 < nB && !a_first<MODE>(a, key_b<MODE>(Lval, Hval, hi2))) {
-                    lo2 = hi2 + 1;
-                    hi2 = r + step;
-                    step <<= 1;
-                }
-                if (hi2 > nB) hi2 = nB;
-                r = first_after<MODE>(Lval, Hval, a, lo2, hi2);
-                if (r < nB) br = key_b<MODE>(Lval, Hval, r);
-            }
-            if constexpr (MODE == 0) {  // light: alias = first heavy whose prefix excess exceeds a
-                const int32_t idx = Lidx[l];
-                const uint64_t w = Lw[l];
-                ap[idx] = (double)w / (double)ALIAS_C;
-                alias[idx] = (w == ALIAS_C || r >= nB) ? idx : Hidx[r];
-            } else {  // heavy: residual from the first light whose prefix deficit reaches a
-                const int32_t idx = Hidx[l];
-                uint64_t rr = r < nB ? ALIAS_C + a - br : ALIAS_C;
-                int32_t nx = idx;
-                if (l + 1 < nA) nx = Hidx[l + 1];
-                else rr = ALIAS_C;
-                ap[idx] = (double)rr / (double)ALIAS_C;
-                alias[idx] = nx;
-            }
-        }
-        __builtin_amdgcn_wave_barrier();  // the window is rewritten by the next trip
-    }
+    auto put = [&](int32_t idx, double a, int32_t al) {
+        ap[idx] = a;
+        alias[idx] = al;
+    };
+    for (int64_t wv = (int64_t)blockIdx.x * (blockDim.x / 64) + wave; wv < nwaves; wv += wstride)
+        assign_wave<MODE>(Lval, Lw, Lidx, Hval, Hidx, nA, nB, rc, wv * 64, nA, win[wave], put);
 }
 
 }  // namespace bsim

@@ -65,6 +65,23 @@ __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_doub
 __device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
 __device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
 
+// |a|^2 with a fixed rounding (re*re rounded into the fma with im*im): every
+// kernel that turns amplitudes into probabilities uses this one expression,
+// so probability vectors and the alias tables built from them agree bit for
+// bit.
+__device__ __forceinline__ double sv_prob(double2 a) { return __fma_rn(a.x, a.x, __dmul_rn(a.y, a.y)); }
+
+// Probability sources of the alias build: a probability vector, or the
+// amplitudes of a state vector (no probability vector is written).
+struct ProbVec {
+    const double* p;
+    __device__ __forceinline__ double operator()(int64_t i) const { return p[i]; }
+};
+struct ProbAmp {
+    const double2* sv;
+    __device__ __forceinline__ double operator()(int64_t i) const { return sv_prob(sv[i]); }
+};
+
 // Insert a zero bit at `bit` (src/gate_kernels.jl:34-38).
 __host__ __device__ __forceinline__ uint64_t pad_bit(uint64_t ix, int bit) {
     uint64_t lo = ix & ((1ull << bit) - 1);

@@ -76,8 +76,7 @@ static int red_grid(int64_t work) {
 __global__ void k_prob_sv(const double2* __restrict__ sv, int64_t len, double* __restrict__ p) {
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride) {
-        double2 a = sv[i];
-        p[i] = a.x * a.x + a.y * a.y;
+        p[i] = sv_prob(sv[i]);
     }
 }
 

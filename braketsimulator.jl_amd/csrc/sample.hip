@@ -58,11 +58,12 @@ __global__ void k_alias_reference(const double* __restrict__ w, int64_t n, doubl
 // ---- parallel exact build ---------------------------------------------------
 constexpr int SUM_BLOCKS = 2048;  // partial sums of k_sum_p
 
-__global__ __launch_bounds__(256) void k_sum_p(const double* __restrict__ p, int64_t n, double* __restrict__ partials) {
+template <class Src>
+__global__ __launch_bounds__(256) void k_sum_p(Src p, int64_t n, double* __restrict__ partials) {
     __shared__ double sh[256];
     double s = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        s += p[i];
+        s += p(i);
     sh[threadIdx.x] = s;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
@@ -134,17 +135,19 @@ static int alias_build(bsim_state* h, int mode, double wsum) {
     if (int rc = ensure_ws(h->alias_work, bytes)) return rc;
     char* base = (char*)h->alias_work.ptr;
     double* p = (double*)base;
-    if (int rc = probabilities_device(h, p)) return rc;
+    // a state vector's parallel build reads the amplitudes directly
+    const bool from_amps = mode == BSIM_ALIAS_PARALLEL && h->kind == BSIM_SV;
+    if (!from_amps)
+        if (int rc = probabilities_device(h, p)) return rc;
     double* ap = (double*)h->ap.ptr;
     int32_t* al = (int32_t*)h->alias.ptr;
-    const int64_t cap = (int64_t)device_cu_count() * 32;
     if (mode == BSIM_ALIAS_REFERENCE) {
         int32_t* larges = (int32_t*)(base + sizeof(double) * n);
         int32_t* smalls = larges + n;
         double* scal = (double*)(smalls + n);
         if (wsum <= 0) {
             const int gs = (int)grid_for(n, 256, SUM_BLOCKS);
-            hipLaunchKernelGGL(k_sum_p, dim3(gs), dim3(256), 0, h->stream, p, n, scal + 8);
+            hipLaunchKernelGGL(k_sum_p<ProbVec>, dim3(gs), dim3(256), 0, h->stream, ProbVec{ p }, n, scal + 8);
             hipLaunchKernelGGL(k_sum_finish, dim3(1), dim3(256), 0, h->stream, scal + 8, gs, scal);
             BSIM_HIP(hipMemcpyAsync(&wsum, scal, sizeof(double), hipMemcpyDeviceToHost, h->stream));
             BSIM_HIP(hipStreamSynchronize(h->stream));
@@ -165,13 +168,24 @@ static int alias_build(bsim_state* h, int mode, double wsum) {
         Acc4* total = (Acc4*)(splitsH + nsplits);
         double* scal = (double*)(total + 2);
         uint64_t* Lw = (uint64_t*)p;  // p is not read after the quantization
-        if (wsum <= 0) {
-            const int gs = (int)grid_for(n, 256, SUM_BLOCKS);
-            hipLaunchKernelGGL(k_sum_p, dim3(gs), dim3(256), 0, h->stream, p, n, scal + 8);
-            hipLaunchKernelGGL(k_sum_finish, dim3(1), dim3(256), 0, h->stream, scal + 8, gs, scal);
+        const int gs = (int)grid_for(n, 256, SUM_BLOCKS);
+        if (from_amps) {
+            const ProbAmp src{ h->d };
+            if (wsum <= 0) {
+                hipLaunchKernelGGL(k_sum_p<ProbAmp>, dim3(gs), dim3(256), 0, h->stream, src, n, scal + 8);
+                hipLaunchKernelGGL(k_sum_finish, dim3(1), dim3(256), 0, h->stream, scal + 8, gs, scal);
+            }
+            hipLaunchKernelGGL(k_quant_tiles<ProbAmp>, dim3((unsigned)ntiles), dim3(AT_BLOCK), 0, h->stream, src, n,
+                               (const double*)scal, wsum, W, ta);
+        } else {
+            const ProbVec src{ p };
+            if (wsum <= 0) {
+                hipLaunchKernelGGL(k_sum_p<ProbVec>, dim3(gs), dim3(256), 0, h->stream, src, n, scal + 8);
+                hipLaunchKernelGGL(k_sum_finish, dim3(1), dim3(256), 0, h->stream, scal + 8, gs, scal);
+            }
+            hipLaunchKernelGGL(k_quant_tiles<ProbVec>, dim3((unsigned)ntiles), dim3(AT_BLOCK), 0, h->stream, src, n,
+                               (const double*)scal, wsum, W, ta);
         }
-        hipLaunchKernelGGL(k_quant_tiles, dim3((unsigned)ntiles), dim3(AT_BLOCK), 0, h->stream, p, n,
-                           (const double*)scal, wsum, W, ta);
         hipLaunchKernelGGL(k_tiles_finish, dim3(1), dim3(1024), 0, h->stream, ta, ntiles, n, W, total);
         hipLaunchKernelGGL(k_compact, dim3((unsigned)ntiles), dim3(AT_BLOCK), 0, h->stream, W, n, (const TileAcc*)ta,
                            Lidx, Lval, Lw, Hidx, Hval);
@@ -180,7 +194,7 @@ static int alias_build(bsim_state* h, int mode, double wsum) {
                            splitsL, nsplits);
         hipLaunchKernelGGL(k_partition<1>, dim3(gp), dim3(256), 0, h->stream, Lval, Lw, Hval, (const Acc4*)total,
                            splitsH, nsplits);
-        const unsigned gm = (unsigned)grid_for(n, 256, cap * 8);  // one A item per lane (at most n)
+        const unsigned gm = (unsigned)grid_for(n, 256, (int64_t)device_cu_count() * 256);  // one A item per lane
         hipLaunchKernelGGL(k_merge<0>, dim3(gm), dim3(256), 0, h->stream, Lval, Lw, Lidx, Hval, Hidx,
                            (const Acc4*)total, (const int64_t*)splitsL, ap, al);
         hipLaunchKernelGGL(k_merge<1>, dim3(gm), dim3(256), 0, h->stream, Lval, Lw, Lidx, Hval, Hidx,
