@@ -114,6 +114,7 @@ _SIGS = {
         [C.c_void_p, C.POINTER(Op), C.c_int, C.c_int, C.c_char_p, _dp, C.c_int, _dp, _dp],
     ),
     "bsim_batch_run": (C.c_int, [C.POINTER(Task), C.c_int, _dp]),
+    "bsim_set_checkpoint_interval": (C.c_int, [C.c_int, C.POINTER(C.c_int)]),
 }
 
 _lib = None

@@ -53,18 +53,25 @@ def with_params(instructions, param_map, theta):
     return out
 
 
-def adjoint_gradient(sim: StateVectorSimulator, instructions, param_map, words, coeffs, n_params=None):
+def adjoint_gradient(sim, instructions, param_map, words, coeffs, n_params=None, checkpoint_every=0):
     """(<H>, dH/dtheta) for the circuit from |0...0>.  ``param_map`` maps
-    (instruction index, angle index) -> parameter index.  The simulator
-    keeps the final state."""
+    (instruction index, angle index) -> parameter index.  On a
+    ``DensityMatrixSimulator`` the circuit may hold noise; the forward pass is
+    checkpointed every ``checkpoint_every`` ops (0: ceil(sqrt(ops))) and
+    recomputed segment by segment.  The simulator keeps the final state."""
     n_params = (max(param_map.values()) + 1 if param_map else 0) if n_params is None else n_params
-    ol = lower_instructions(list(instructions), L.BSIM_SV, param_map=param_map)
+    kind = getattr(sim, "kind", L.BSIM_SV)
+    ol = lower_instructions(list(instructions), kind, noise_mode=getattr(sim, "noise_mode", "reference"),
+                            param_map=param_map)
+    old = C.c_int()
+    L.call("bsim_set_checkpoint_interval", int(checkpoint_every), C.byref(old))
     grad = np.zeros(n_params, dtype=np.float64)
     c = np.ascontiguousarray(coeffs, dtype=np.float64)
     ev = C.c_double()
     L.call("bsim_adjoint_gradient", sim.handle, ol.ptr, ol.n, len(words), _words_buf(words, sim.qubit_count),
            c.ctypes.data_as(C.POINTER(C.c_double)), n_params, grad.ctypes.data_as(C.POINTER(C.c_double)),
            C.byref(ev))
+    L.call("bsim_set_checkpoint_interval", old.value, None)
     return ev.value, grad
 
 

@@ -283,6 +283,213 @@ static int adjoint_step(bsim_state* h, double2* psi, double2* lam, const EOp& e,
     return BSIM_OK;
 }
 
+
+// ---- density matrices: adjoint with checkpointed recomputation -------------
+//
+// The forward map of a noisy circuit is not invertible, so the backward sweep
+// cannot uncompute rho.  The forward pass keeps rho every `s` ops
+// (s = ceil(sqrt(D)) by default, D ops); the backward sweep rebuilds the
+// states of one segment from its checkpoint, then walks the segment in
+// reverse: for a parameterized gate U(theta) with derivative dU,
+//   d<H>/dtheta = 2 Re tr(Lambda dU rho U^dagger),
+// and Lambda <- E^dagger(Lambda) for every op E (U^dagger . U for gates, the
+// same channel with K^dagger for Kraus maps; PhaseFlip is self-adjoint).
+// Lambda starts as H.  Storage of every operator is the transpose (the
+// density-matrix convention), so tr(Lambda X) = conj(vec Lambda) . vec X for
+// Hermitian Lambda.
+
+// Lambda = H: stored element (r, c) = H[c][r] = sum_w [r == c ^ f_w] phase_w(c)
+__global__ __launch_bounds__(AB) void k_dm_pauli_op(double2* __restrict__ lam, int nq, const PWord* __restrict__ words,
+                                                    int nw) {
+    const int64_t N = (int64_t)1 << nq, len = N * N;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride) {
+        const uint64_t r = (uint64_t)i & (uint64_t)(N - 1), c = (uint64_t)i >> nq;
+        double2 acc = make_double2(0, 0);
+        for (int w = 0; w < nw; ++w)
+            if (r == (c ^ words[w].f)) acc = cadd(acc, pphase(c, words[w]));
+        lam[i] = acc;
+    }
+}
+
+static int g_checkpoint_every = 0;  // 0: ceil(sqrt(ops))
+
+struct DmParam {
+    int pidx;
+    GateSpec dcol, urow, proj;  // dU on the column bits, conj(U) on the row bits, control projector
+    bool ctl;
+};
+struct DmStep {
+    std::vector<EOp> fwd, adj;
+    std::vector<DmParam> params;
+};
+
+static int dm_steps(int nq, const bsim_op* ops, int n_ops, int n_params, std::vector<DmStep>& st) {
+    st.assign(n_ops, DmStep());
+    const int vq = 2 * nq;
+    std::vector<double> buf;
+    for (int i = 0; i < n_ops; ++i) {
+        const bsim_op& op = ops[i];
+        if (int rc = lower_ops(nq, BSIM_DM, &op, 1, st[i].fwd, false)) return rc;
+        bsim_op a = op;
+        a.n_params = 0;
+        const int k = op.n_targets, D = 1 << k;
+        if (op.type == BSIM_OP_GATE) {
+            buf.resize(2 * (size_t)D * D);
+            for (int r = 0; r < D; ++r)
+                for (int c = 0; c < D; ++c) {
+                    buf[2 * (r + (size_t)D * c)] = op.matrix[2 * (c + (size_t)D * r)];
+                    buf[2 * (r + (size_t)D * c) + 1] = -op.matrix[2 * (c + (size_t)D * r) + 1];
+                }
+            a.matrix = buf.data();
+        } else if (op.type == BSIM_OP_DIAG) {
+            buf.resize(2 * (size_t)D);
+            for (int j = 0; j < D; ++j) {
+                buf[2 * j] = op.matrix[2 * j];
+                buf[2 * j + 1] = -op.matrix[2 * j + 1];
+            }
+            a.matrix = buf.data();
+        } else if (op.type == BSIM_OP_KRAUS_REF || op.type == BSIM_OP_KRAUS_PHYS) {
+            buf.resize(2 * (size_t)op.n_kraus * D * D);
+            for (int m = 0; m < op.n_kraus; ++m) {
+                const double* K = op.matrix + 2 * (size_t)m * D * D;
+                double* A = buf.data() + 2 * (size_t)m * D * D;
+                for (int r = 0; r < D; ++r)
+                    for (int c = 0; c < D; ++c) {
+                        A[2 * (r + (size_t)D * c)] = K[2 * (c + (size_t)D * r)];
+                        A[2 * (r + (size_t)D * c) + 1] = -K[2 * (c + (size_t)D * r) + 1];
+                    }
+            }
+            a.matrix = buf.data();
+        }  // PhaseFlip: elementwise real factors, its own adjoint
+        if (int rc = lower_ops(nq, BSIM_DM, &a, 1, st[i].adj, false)) return rc;
+        if (op.n_params < 0 || op.n_params > 3) return fail(BSIM_EINVAL, "op: n_params must be 0..3");
+        if (op.n_params > 0 && op.type != BSIM_OP_GATE)
+            return fail(BSIM_EINVAL, "parameterized ops must be dense gates (BSIM_OP_GATE)");
+        for (int j = 0; j < op.n_params; ++j) {
+            DmParam P;
+            P.pidx = op.param_index[j];
+            if (P.pidx < 0 || P.pidx >= n_params) return fail(BSIM_EINVAL, "param_index out of range");
+            if (!op.dmatrix[j]) return fail(BSIM_EINVAL, "op: dmatrix is null");
+            int tc[8], tr[8], cc[8], cr[8];
+            for (int b = 0; b < k; ++b) {
+                tc[b] = op.targets[b];
+                tr[b] = op.targets[b] + nq;
+            }
+            for (int b = 0; b < op.n_controls; ++b) {
+                cc[b] = op.controls[b];
+                cr[b] = op.controls[b] + nq;
+            }
+            if (int rc = make_spec(vq, k, tc, op.n_controls, cc, op.control_values, op.dmatrix[j], false, false, false,
+                                   P.dcol))
+                return rc;
+            if (int rc = make_spec(vq, k, tr, op.n_controls, cr, op.control_values, op.matrix, false, true, false, P.urow))
+                return rc;
+            P.ctl = op.n_controls > 0;
+            if (P.ctl) {  // dU of a controlled gate vanishes off the control subspace
+                const int nc = op.n_controls, DC = 1 << nc;
+                std::vector<double> d(2 * (size_t)DC, 0.0);
+                for (int jj = 0; jj < DC; ++jj) {
+                    bool on = true;
+                    for (int b = 0; b < nc; ++b) on = on && (((jj >> b) & 1) == (op.control_values[b] ? 1 : 0));
+                    d[2 * jj] = on ? 1.0 : 0.0;
+                }
+                if (int rc = make_spec(vq, nc, cc, 0, nullptr, nullptr, d.data(), true, false, false, P.proj)) return rc;
+            }
+            st[i].params.push_back(std::move(P));
+        }
+    }
+    return BSIM_OK;
+}
+
+static int apply_eops(bsim_state* h, double2* d, const std::vector<EOp>& e) {
+    for (const EOp& x : e)
+        if (int rc = launch_eop(h, d, x)) return rc;
+    return BSIM_OK;
+}
+
+static int dm_adjoint(bsim_state* h, const bsim_op* ops, int n_ops, const std::vector<PWord>& words, int n_params,
+                      double* grad, double* expval) {
+    std::vector<DmStep> st;
+    if (int rc = dm_steps(h->nq, ops, n_ops, n_params, st)) return rc;
+    const int64_t len = h->len;
+    const size_t bytes = sizeof(double2) * (size_t)len;
+    const int D = n_ops;
+    int s = g_checkpoint_every > 0 ? g_checkpoint_every : 1;
+    if (g_checkpoint_every <= 0)
+        while ((int64_t)s * s < D) ++s;
+    const int nseg = D > 0 ? (D + s - 1) / s : 0;
+    int nslots = 0;
+    for (auto& x : st) nslots += (int)x.params.size();
+    // arena: checkpoints (nseg) | segment states (s) | Lambda | sigma
+    char* arena = nullptr;
+    const size_t nbuf = (size_t)nseg + (size_t)s + 2;
+    hipError_t e = hipMalloc((void**)&arena, bytes * nbuf);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(density-matrix adjoint checkpoints)");
+    struct Guard {
+        char* p;
+        ~Guard() { (void)hipFree(p); }
+    } guard{ arena };
+    auto buf = [&](size_t i) { return (double2*)(arena + bytes * i); };
+    double2* lam = buf(nbuf - 2);
+    double2* sig = buf(nbuf - 1);
+    if (int rc = ensure_ws(h->res, sizeof(double) * (2 * (size_t)nslots + 8) + sizeof(PWord) * words.size())) return rc;
+    double* res = (double*)h->res.ptr;
+    PWord* dwords = (PWord*)(res + 2 * (size_t)nslots + 8);
+    // forward, fused per segment, with a checkpoint before each segment
+    if (int rc = bsim_state_reset(h)) return rc;
+    if (int rc = prepare_tile_kernel()) return rc;
+    for (int c = 0; c < nseg; ++c) {
+        BSIM_HIP(hipMemcpyAsync(buf(c), h->d, bytes, hipMemcpyDeviceToDevice, h->stream));
+        std::vector<EOp> seg;
+        for (int k = c * s; k < std::min(D, (c + 1) * s); ++k) seg.insert(seg.end(), st[k].fwd.begin(), st[k].fwd.end());
+        Plan plan;
+        if (int rc = make_plan(h->vq, seg, tile_config(), plan)) return rc;
+        if (int rc = run_plan(h, h->d, seg, plan)) return rc;
+    }
+    // Lambda = H, <H> = tr(H rho)
+    BSIM_HIP(hipMemcpyAsync(dwords, words.data(), sizeof(PWord) * words.size(), hipMemcpyHostToDevice, h->stream));
+    const int64_t g = grid_for(len, AB, (int64_t)device_cu_count() * 32);
+    hipLaunchKernelGGL(k_dm_pauli_op, dim3((unsigned)g), dim3(AB), 0, h->stream, lam, h->nq, (const PWord*)dwords,
+                       (int)words.size());
+    BSIM_HIP(hipGetLastError());
+    if (int rc = dot_device(lam, h->d, len, h->red, h->stream, res + 2 * (size_t)nslots)) return rc;
+    // backward, segment by segment
+    int slot = nslots;
+    std::vector<int> slot_pidx(nslots);
+    for (int c = nseg - 1; c >= 0; --c) {
+        const int a = c * s, b = std::min(D, (c + 1) * s);
+        BSIM_HIP(hipMemcpyAsync(buf(nseg), buf(c), bytes, hipMemcpyDeviceToDevice, h->stream));
+        for (int k = a; k + 1 < b; ++k) {  // states before ops a+1 .. b-1
+            double2* nx = buf(nseg + (k - a) + 1);
+            BSIM_HIP(hipMemcpyAsync(nx, buf(nseg + (k - a)), bytes, hipMemcpyDeviceToDevice, h->stream));
+            if (int rc = apply_eops(h, nx, st[k].fwd)) return rc;
+        }
+        for (int k = b - 1; k >= a; --k) {
+            for (int j = (int)st[k].params.size() - 1; j >= 0; --j) {
+                const DmParam& P = st[k].params[j];
+                BSIM_HIP(hipMemcpyAsync(sig, buf(nseg + (k - a)), bytes, hipMemcpyDeviceToDevice, h->stream));
+                if (int rc = launch_gate(sig, h->vq, P.dcol, h->scratch, h->stream)) return rc;
+                if (P.ctl)
+                    if (int rc = launch_gate(sig, h->vq, P.proj, h->scratch, h->stream)) return rc;
+                if (int rc = launch_gate(sig, h->vq, P.urow, h->scratch, h->stream)) return rc;
+                --slot;
+                slot_pidx[slot] = P.pidx;
+                if (int rc = dot_device(lam, sig, len, h->red, h->stream, res + 2 * (size_t)slot)) return rc;
+            }
+            if (int rc = apply_eops(h, lam, st[k].adj)) return rc;
+        }
+    }
+    std::vector<double> hres(2 * (size_t)nslots + 2);
+    BSIM_HIP(hipMemcpyAsync(hres.data(), res, sizeof(double) * hres.size(), hipMemcpyDeviceToHost, h->stream));
+    BSIM_HIP(hipStreamSynchronize(h->stream));
+    for (int p = 0; p < n_params; ++p) grad[p] = 0.0;
+    for (int q = 0; q < nslots; ++q) grad[slot_pidx[q]] += 2.0 * hres[2 * (size_t)q];
+    if (expval) *expval = hres[2 * (size_t)nslots];
+    h->alias_ready = false;
+    return BSIM_OK;
+}
+
 }  // namespace bsim
 
 using namespace bsim;
@@ -292,10 +499,18 @@ extern "C" {
 int bsim_adjoint_gradient(bsim_state* h, const bsim_op* ops, int n_ops, int n_words, const char* paulis,
                           const double* coeffs, int n_params, double* grad, double* expval) {
     if (int rc = check_state(h)) return rc;
-    if (h->kind != BSIM_SV) return fail(BSIM_ENOTSUP, "adjoint gradients need a state vector");
     if (n_ops < 0 || (n_ops > 0 && !ops) || n_words < 1 || !paulis || !coeffs || n_params < 0 ||
         (n_params > 0 && !grad))
         return fail(BSIM_EINVAL, "bad adjoint arguments");
+    if (h->kind == BSIM_DM) {
+        std::vector<PWord> words(n_words);
+        for (int w = 0; w < n_words; ++w) {
+            PauliMask pm;
+            if (int rc = parse_pauli(h->nq, paulis + (size_t)w * h->nq, pm)) return rc;
+            words[w] = { pm.x | pm.y, pm.y, pm.z, pm.ny, 0, coeffs[w] };
+        }
+        return dm_adjoint(h, ops, n_ops, words, n_params, grad, expval);
+    }
     std::vector<EOp> eops;
     if (int rc = lower_ops(h->nq, h->kind, ops, n_ops, eops, true)) return rc;
     for (const EOp& e : eops)
@@ -362,6 +577,13 @@ int bsim_adjoint_gradient(bsim_state* h, const bsim_op* ops, int n_ops, int n_wo
     }
     if (expval) *expval = hres[3 * (size_t)nslots];
     h->alias_ready = false;
+    return BSIM_OK;
+}
+
+int bsim_set_checkpoint_interval(int ops, int* old) {
+    if (ops < 0) return fail(BSIM_EINVAL, "checkpoint interval must be >= 0");
+    if (old) *old = g_checkpoint_every;
+    g_checkpoint_every = ops;
     return BSIM_OK;
 }
 

@@ -221,9 +221,20 @@ int bsim_shots_to_bits(bsim_state* h, int64_t shots, const int64_t* shot_index, 
  * bsim_expect_pauli).  Forward evolve from |0>, lambda = H psi, backward
  * sweep un-applying U^dagger to psi and lambda with the <lambda|dU|psi>
  * products fused into the gate kernel.  grad has n_params doubles
- * (op.param_index accumulates).  *expval = <H>.  SV only. */
+ * (op.param_index accumulates).  *expval = <H>.
+ * On a density matrix (noise ops allowed; parameterized ops must be
+ * BSIM_OP_GATE) the forward map cannot be uncomputed: rho is checkpointed
+ * every s ops (s = ceil(sqrt(n_ops)) unless set below), each segment is
+ * recomputed from its checkpoint, and the operator Lambda (initially H) is
+ * propagated backwards through the adjoint maps (U^dagger . U; Kraus ops
+ * K^dagger in the same mode); d<H>/dtheta = 2 Re tr(Lambda dU rho U^dagger).
+ * The handle keeps the final state in both cases. */
 int bsim_adjoint_gradient(bsim_state* h, const bsim_op* ops, int n_ops, int n_words, const char* paulis,
                           const double* coeffs, int n_params, double* grad, double* expval);
+/* Checkpoint spacing (in ops) of the density-matrix adjoint; 0 = ceil(sqrt(n_ops)).
+ * Returns the previous value in *old (may be NULL).  No reference
+ * counterpart (the reference has no gradients). */
+int bsim_set_checkpoint_interval(int ops, int* old);
 /* <H> for a Pauli sum on the current state. */
 int bsim_expect_pauli_sum(bsim_state* h, int n_words, const char* paulis, const double* coeffs, double* out);
 
