@@ -80,6 +80,8 @@ def main():
     ap.add_argument("--adjoint-qubits", type=int, default=24, help="0 skips the adjoint leg")
     ap.add_argument("--ps-sample", type=int, default=24, help="parameters timed by parameter shift")
     ap.add_argument("--batch", type=int, default=4096, help="circuits in the batch leg (0 skips it)")
+    ap.add_argument("--dm-qubits", type=int, default=14, help="noisy density-matrix leg (0 skips it)")
+    ap.add_argument("--graph-reps", type=int, default=500, help="captured-evolution replays (0 skips the leg)")
     ap.add_argument("--sweep-sets", type=int, default=256, help="16-qubit ansatz parameter sets with gradients")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="profiling aid: one fused evolve of the workload and one 8 GiB device copy, no JSON")
@@ -197,6 +199,10 @@ def main():
         rec["adjoint"] = adjoint_leg(a.adjoint_qubits, a.ps_sample)
     if a.batch > 0:
         rec["batch"] = batch_leg(a.batch, a.sweep_sets)
+    if a.dm_qubits > 0:
+        rec["density_matrix"] = dm_leg(a.dm_qubits)
+    if a.graph_reps > 0:
+        rec["captured"] = graph_leg(a.graph_reps)
     if not a.no_cpu:
         log("cpu baseline (oracle) ...")
         rec["cpu_baseline"] = cpu_baseline(n, ins)
@@ -257,6 +263,104 @@ def measured_traffic(n: int, depth: int):
         if line.startswith("traffic_bytes_per_launch:"):
             return int(line.split(":")[1])
     return None
+
+
+def dm_leg(nq: int):
+    """Noisy density matrix: the brick-wall of the state-vector leg at nq
+    qubits, depth 10, with Depolarizing(0.01) after every U,
+    TwoQubitDepolarizing(0.02) after every CZ and AmplitudeDamping(0.005) on
+    every qubit per layer; <Z_i Z_i+1> for all pairs and the diagonal
+    probabilities, in both noise semantics.  Then the 12-qubit noisy
+    hardware-efficient ansatz (depolarizing 0.005) with its full adjoint
+    gradient through the checkpointed density-matrix adjoint."""
+    import numpy as np
+
+    from bsim import _lib as L
+    from bsim import adjoint as A
+    from bsim import workloads as W
+    from bsim.simulators import DensityMatrixSimulator, lower_instructions
+
+    ins = W.brickwall(nq, 10, seed=0, noise=True)
+    words = ["I" * i + "ZZ" + "I" * (nq - 2 - i) for i in range(nq - 1)]
+    out = {"workload": f"noisy brick-wall {nq} qubits depth 10 ({len(ins)} ops), "
+                       f"{2 ** (2 * nq) * 16 / 2 ** 30:.0f} GiB density matrix, <Z_i Z_i+1> + probabilities"}
+    for mode in ("reference", "physical"):
+        sim = DensityMatrixSimulator(nq, 0, noise_mode=mode)
+        ol = lower_instructions(ins, L.BSIM_DM, mode)
+        stats = (ctypes.c_int64 * 4)()
+        L.call("bsim_plan_stats", 2 * nq, 1, ol.ptr, ol.n, stats)
+        L.call("bsim_evolve", sim.handle, ol.ptr, ol.n, 0)  # warm-up
+        L.call("bsim_synchronize", sim.handle)
+        t = time.perf_counter()
+        L.call("bsim_state_reset", sim.handle)
+        L.call("bsim_evolve", sim.handle, ol.ptr, ol.n, 0)
+        L.call("bsim_synchronize", sim.handle)
+        t_ev = time.perf_counter() - t
+        t = time.perf_counter()
+        zz = sim.expect_pauli_words(words)
+        p = sim.probabilities()
+        t_res = time.perf_counter() - t
+        out[mode] = {"evolve_ms": round(t_ev * 1e3, 3), "results_ms": round(t_res * 1e3, 3),
+                     "passes": int(stats[0]), "single_launches": int(stats[1]),
+                     "zz_checksum": float(np.sum(zz)), "trace": float(np.sum(p))}
+        log(f"density matrix {nq}q {mode}: evolve {t_ev * 1e3:.1f} ms")
+        del sim
+    n12 = 12
+    hins, pmap, th = W.hea(n12, 8, seed=0, noise_p=0.005)
+    hw, hc = W.tfim(n12)
+    sim = DensityMatrixSimulator(n12, 0)
+    A.adjoint_gradient(sim, hins, pmap, hw, hc)  # warm-up
+    t = time.perf_counter()
+    ev, g = A.adjoint_gradient(sim, hins, pmap, hw, hc)
+    t_adj = time.perf_counter() - t
+    out["adjoint"] = {"workload": f"noisy HEA {n12} qubits x 8 layers (depolarizing 0.005), {len(th)} params, "
+                                  f"{len(hins)} ops, TFIM H, checkpoint every ceil(sqrt(ops))",
+                      "s": round(t_adj, 4), "grads_per_s": round(1.0 / t_adj, 3), "expval": ev}
+    log(f"density matrix adjoint {n12}q: {t_adj:.3f} s")
+    return out
+
+
+def graph_leg(reps: int):
+    """Launch-bound small circuits: a 12-qubit brick-wall of depth 20 (460
+    gates) evolved from |0...0> ``reps`` times, eagerly (bsim_evolve: lower,
+    plan, upload, launch per call) and as replays of one captured graph
+    (reset + passes).  Reports the wall time per evolution."""
+    import numpy as np
+
+    from bsim import _lib as L
+    from bsim import workloads as W
+    from bsim.simulators import StateVectorSimulator, lower_instructions
+
+    nq, depth = 12, 20
+    ins = W.brickwall(nq, depth, seed=5)
+    ol = lower_instructions(ins, L.BSIM_SV)
+    sim = StateVectorSimulator(nq, 0)
+    for _ in range(10):
+        L.call("bsim_state_reset", sim.handle)
+        L.call("bsim_evolve", sim.handle, ol.ptr, ol.n, 0)
+    L.call("bsim_synchronize", sim.handle)
+    t = time.perf_counter()
+    for _ in range(reps):
+        L.call("bsim_state_reset", sim.handle)
+        L.call("bsim_evolve", sim.handle, ol.ptr, ol.n, 0)
+    L.call("bsim_synchronize", sim.handle)
+    t_eager = (time.perf_counter() - t) / reps
+    eager = sim.state_vector()
+    g = sim.capture(ins, reset=True)
+    for _ in range(10):
+        g.launch()
+    L.call("bsim_synchronize", sim.handle)
+    t = time.perf_counter()
+    for _ in range(reps):
+        g.launch()
+    L.call("bsim_synchronize", sim.handle)
+    t_graph = (time.perf_counter() - t) / reps
+    same = bool(np.array_equal(sim.state_vector(), eager))
+    g.close()
+    log(f"captured {nq}q depth {depth}: eager {t_eager * 1e6:.1f} us, graph {t_graph * 1e6:.1f} us per evolve")
+    return {"workload": f"brick-wall {nq} qubits depth {depth} ({len(ins)} gates) from |0...0>, {reps} evolutions",
+            "eager_us": round(t_eager * 1e6, 2), "graph_us": round(t_graph * 1e6, 2),
+            "speedup": round(t_eager / t_graph, 2), "bit_identical": same}
 
 
 def pmc_probe(n: int, depth: int):

@@ -11,12 +11,14 @@ from .results import Amplitude, DensityMatrix, Expectation, Probability, Sample,
 from .simulate import Program, TaskResult, simulate, simulate_batch
 from .simulators import (
     Barrier,
+    CapturedEvolution,
     Delay,
     DensityMatrixSimulator,
     Instruction,
     Measure,
     Reset,
     StateVectorSimulator,
+    Superoperator,
     diagonalizing_gates,
     lower_instructions,
 )
@@ -25,5 +27,5 @@ __all__ = [
     "BsimError", "load", "gates", "noises", "observables", "StateVectorSimulator", "DensityMatrixSimulator",
     "Instruction", "Measure", "Barrier", "Reset", "Delay", "Program", "TaskResult", "simulate", "simulate_batch",
     "Expectation", "Variance", "Probability", "Sample", "StateVector", "DensityMatrix", "Amplitude", "calculate",
-    "diagonalizing_gates", "lower_instructions",
+    "diagonalizing_gates", "lower_instructions", "CapturedEvolution", "Superoperator",
 ]

@@ -25,6 +25,7 @@ OP_DIAG = 1
 OP_KRAUS_REF = 2
 OP_KRAUS_PHYS = 3
 OP_PHASEFLIP = 4
+OP_SUPEROP = 5
 
 ALIAS_PARALLEL = 0
 ALIAS_REFERENCE = 1
@@ -115,6 +116,11 @@ _SIGS = {
     ),
     "bsim_batch_run": (C.c_int, [C.POINTER(Task), C.c_int, _dp]),
     "bsim_set_checkpoint_interval": (C.c_int, [C.c_int, C.POINTER(C.c_int)]),
+    "bsim_dm_apply_superop": (C.c_int, [C.c_void_p, C.c_int, _ip, _dp]),
+    "bsim_graph_create": (C.c_int, [C.c_void_p, C.POINTER(Op), C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    "bsim_graph_launch": (C.c_int, [C.c_void_p]),
+    "bsim_graph_destroy": (C.c_int, [C.c_void_p]),
+    "bsim_autotune": (C.c_int, [C.c_void_p, C.POINTER(Op), C.c_int, _ip, _ip, _dp]),
 }
 
 _lib = None

@@ -57,6 +57,33 @@ class Delay:
 NO_OPS = (Measure, Barrier, Reset, Delay)
 
 
+class Superoperator:
+    """A linear map on the density matrix of k <= 3 qubits, given by its
+    4^k x 4^k Liouville matrix under column stacking of the physical rho:
+    rho'[r, c] = sum S[r + 2^k c, r' + 2^k c'] rho[r', c'], where r and c
+    index the targets' block with the first target as the most significant
+    bit, as in Braket's Kraus and unitary matrices (U rho U^dagger:
+    S = kron(conj(U), U); Kraus K_m: S = sum kron(conj(K_m), K_m)).  One
+    fused pass of bsim_dm_apply_superop; the reference applies a DM gate as
+    two passes (src/dm_simulator.jl:130-140)."""
+
+    def __init__(self, matrix):
+        m = np.asarray(matrix, dtype=np.complex128)
+        k = int(round(np.log2(m.shape[0]) / 2))
+        if m.ndim != 2 or m.shape != (4 ** k, 4 ** k) or not 1 <= k <= 3:
+            raise ValueError("a superoperator is a 4^k x 4^k matrix, 1 <= k <= 3")
+        self.matrix = m
+        self.qubit_count = k
+
+    @classmethod
+    def from_kraus(cls, kraus):
+        return cls(sum(np.kron(np.conj(K), K) for K in kraus))
+
+    @classmethod
+    def from_unitary(cls, u):
+        return cls(np.kron(np.conj(u), u))
+
+
 # ---- lowering to bsim_op arrays ----------------------------------------
 
 class OpList:
@@ -126,6 +153,10 @@ def lower_instructions(instructions, kind: int, noise_mode: str = "reference", p
                     params = [p for _, p in idx]
                     dmats = [allm[j] for j, _ in idx]
             ol.add_call(call, params, dmats)
+        elif isinstance(op, Superoperator):
+            if kind != L.BSIM_DM:
+                raise TypeError("superoperators need a DensityMatrixSimulator")
+            ol.add(L.OP_SUPEROP, list(reversed(ins.target)), op.matrix)  # bsim: matrix bit b <-> targets[b]
         elif isinstance(op, Noise):
             if kind != L.BSIM_DM:
                 raise TypeError("noise channels need a DensityMatrixSimulator")
@@ -217,6 +248,20 @@ class _Simulator:
         ol = lower_instructions(list(operations), self.kind, getattr(self, "noise_mode", "reference"))
         L.call("bsim_evolve", self._h, ol.ptr, ol.n, int(mode))
         return self
+
+    def capture(self, operations, reset: bool = True) -> "CapturedEvolution":
+        """Record evolve!(operations) -- preceded by the reset to |0...0>
+        when ``reset`` -- as one hipGraph on this state (bsim_graph_create)."""
+        return CapturedEvolution(self, operations, reset)
+
+    def autotune(self, operations):
+        """Time the fused evolution of ``operations`` for every tile
+        configuration and keep the fastest (bsim_autotune).  Returns
+        (tile_bits, low_bits, ms); the state is left unspecified."""
+        ol = lower_instructions(list(operations), self.kind, getattr(self, "noise_mode", "reference"))
+        t, lo, ms = C.c_int(), C.c_int(), C.c_double()
+        L.call("bsim_autotune", self._h, ol.ptr, ol.n, C.byref(t), C.byref(lo), C.byref(ms))
+        return t.value, lo.value, ms.value
 
     def _apply_call(self, call: KernelCall, vq_targets, conj: int):
         lib = L.load()
@@ -463,5 +508,57 @@ class DensityMatrixSimulator(_Simulator):
                1 if self.noise_mode == "physical" else 0)
         return self
 
+    def apply_superop(self, superop, *targets):
+        """One superoperator pass on ``targets`` (bsim_dm_apply_superop)."""
+        sop = superop if isinstance(superop, Superoperator) else Superoperator(superop)
+        if len(targets) != sop.qubit_count:
+            raise ValueError("superoperator arity and target count differ")
+        t = np.array(list(reversed(targets)), dtype=np.int32)  # bsim: matrix bit b <-> targets[b]
+        m = L.cmat(sop.matrix)
+        L.call("bsim_dm_apply_superop", self._h, len(t), L.iptr(t), L.dptr(m))
+        return self
+
     def __repr__(self):
         return f"DensityMatrixSimulator(qubit_count={self.qubit_count}, shots={self.shots})"
+
+
+class CapturedEvolution:
+    """An evolve! recorded once as a hipGraph and replayed with ``launch``
+    (bsim_graph_create / bsim_graph_launch / bsim_graph_destroy).  Only
+    circuits whose fused plan is made of tile passes are captured (gates on
+    at most 2 qubits, diagonals on at most 4 vector bits and, on a density
+    matrix, PhaseFlip and 1-qubit physical-mode noise); others raise
+    BsimError.  The graph holds its simulator and is destroyed
+    by ``close`` (or when collected)."""
+
+    def __init__(self, sim: _Simulator, operations, reset: bool = True):
+        self._g = C.c_void_p()
+        self._sim = sim
+        self._hval = sim.handle.value
+        ol = lower_instructions(list(operations), sim.kind, getattr(sim, "noise_mode", "reference"))
+        L.call("bsim_graph_create", sim.handle, ol.ptr, ol.n, int(bool(reset)), C.byref(self._g))
+
+    def launch(self) -> _Simulator:
+        if not self._g.value:
+            raise L.BsimError("captured evolution is closed")
+        if self._sim.handle is None or self._sim.handle.value != self._hval:
+            raise L.BsimError("the simulator's state was re-created after the capture")
+        L.call("bsim_graph_launch", self._g)
+        return self._sim
+
+    def close(self):
+        g = getattr(self, "_g", None)
+        if g is not None and g.value:
+            try:
+                L.load().bsim_graph_destroy(g)
+            except Exception:
+                pass
+            self._g = C.c_void_p()
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

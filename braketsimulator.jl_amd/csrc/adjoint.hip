@@ -361,6 +361,15 @@ static int dm_steps(int nq, const bsim_op* ops, int n_ops, int n_params, std::ve
                     }
             }
             a.matrix = buf.data();
+        } else if (op.type == BSIM_OP_SUPEROP) {  // adjoint map: S^dagger
+            const size_t DD = (size_t)D * D;
+            buf.resize(2 * DD * DD);
+            for (size_t r = 0; r < DD; ++r)
+                for (size_t c = 0; c < DD; ++c) {
+                    buf[2 * (r + DD * c)] = op.matrix[2 * (c + DD * r)];
+                    buf[2 * (r + DD * c) + 1] = -op.matrix[2 * (c + DD * r) + 1];
+                }
+            a.matrix = buf.data();
         }  // PhaseFlip: elementwise real factors, its own adjoint
         if (int rc = lower_ops(nq, BSIM_DM, &a, 1, st[i].adj, false)) return rc;
         if (op.n_params < 0 || op.n_params > 3) return fail(BSIM_EINVAL, "op: n_params must be 0..3");

@@ -122,6 +122,24 @@ int bsim_dm_kraus(bsim_state* h, int k, const int* targets, int n_kraus, const d
     return BSIM_OK;
 }
 
+int bsim_dm_apply_superop(bsim_state* h, int k, const int* targets, const double* S) {
+    if (int rc = check_state(h)) return rc;
+    if (h->kind != BSIM_DM) return fail(BSIM_EINVAL, "superoperators need a density matrix");
+    if (k < 1 || k > 3 || !targets || !S) return fail(BSIM_EINVAL, "bad superoperator arguments");
+    bsim_op op = {};
+    op.type = BSIM_OP_SUPEROP;
+    op.n_targets = k;
+    for (int b = 0; b < k; ++b) op.targets[b] = targets[b];
+    op.matrix = S;
+    std::vector<EOp> eops;
+    if (int rc = lower_ops(h->nq, h->kind, &op, 1, eops, false)) return rc;
+    h->alias_ready = false;
+    for (const EOp& e : eops)
+        if (int rc = launch_eop(h, h->d, e)) return rc;
+    BSIM_HIP(hipStreamSynchronize(h->stream));
+    return BSIM_OK;
+}
+
 int bsim_dm_phaseflip(bsim_state* h, int q, double p) {
     if (int rc = check_state(h)) return rc;
     if (h->kind != BSIM_DM) return fail(BSIM_EINVAL, "PhaseFlip needs a density matrix");

@@ -100,6 +100,13 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
 
 // Launch a plan on d (ops buffer reused).  `eops` provides single launches.
 int run_plan(bsim_state* h, double2* d, const std::vector<EOp>& eops, const Plan& plan);
+// The same in two parts: the plan's device tables (passes | sweeps | ops,
+// plan_table_bytes bytes), then the launches reading them (`timed`: tile
+// passes are bracketed by the profiling events when profiling is on).
+size_t plan_table_bytes(const Plan& plan);
+int upload_plan(const Plan& plan, char* dst, hipStream_t s);
+int launch_plan(bsim_state* h, double2* d, const std::vector<EOp>& eops, const Plan& plan, const char* tables,
+                bool timed);
 
 // One-time launch attributes of the tile kernel (dynamic LDS above 64 KiB).
 int prepare_tile_kernel();

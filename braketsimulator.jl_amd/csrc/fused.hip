@@ -169,6 +169,22 @@ int lower_ops(int nq, int kind, const bsim_op* ops, int n_ops, std::vector<EOp>&
                     return rc;
                 out.push_back(std::move(e));
             }
+        } else if (op.type == BSIM_OP_SUPEROP) {
+            // rho[r, c] <- sum S[(r, c), (r', c')] rho[r', c'], J = r + 2^k c: the
+            // row index r of the physical matrix lives on vector qubits t, its
+            // column index c on vector qubits nq + t (storage is rho^T)
+            if (kind != BSIM_DM) return fail(BSIM_EINVAL, "superoperators need a density matrix");
+            if (nc) return fail(BSIM_EINVAL, "superoperators take no controls");
+            if (k > 3) return fail(BSIM_ENOTSUP, "superoperators act on at most 3 qubits");
+            int t[6];
+            for (int b = 0; b < k; ++b) {
+                t[b] = op.targets[b];
+                t[k + b] = op.targets[b] + nq;
+            }
+            EOp e;
+            e.kind = E_DENSE;
+            if (int rc = make_spec(vq, 2 * k, t, 0, nullptr, nullptr, op.matrix, false, false, false, e.g)) return rc;
+            out.push_back(std::move(e));
         } else {
             return fail(BSIM_EINVAL, "unknown op type");
         }
@@ -680,42 +696,54 @@ static int prof_collect() {
     return BSIM_OK;
 }
 
-int run_plan(bsim_state* h, double2* d, const std::vector<EOp>& eops, const Plan& plan) {
-    size_t off_p = 0;
-    size_t off_s = align256(sizeof(Pass) * plan.passes.size());
-    size_t off_o = off_s + align256(sizeof(Sweep) * plan.sweeps.size());
-    size_t total = off_o + align256(sizeof(TOp) * plan.tops.size()) + 256;
-    if (!plan.passes.empty()) {
-        if (int rc = ensure_ws(h->ops, total)) return rc;
-        char* base = (char*)h->ops.ptr;
-        BSIM_HIP(hipMemcpyAsync(base + off_p, plan.passes.data(), sizeof(Pass) * plan.passes.size(),
-                                hipMemcpyHostToDevice, h->stream));
-        BSIM_HIP(hipMemcpyAsync(base + off_s, plan.sweeps.data(), sizeof(Sweep) * plan.sweeps.size(),
-                                hipMemcpyHostToDevice, h->stream));
-        if (!plan.tops.empty())
-            BSIM_HIP(hipMemcpyAsync(base + off_o, plan.tops.data(), sizeof(TOp) * plan.tops.size(),
-                                    hipMemcpyHostToDevice, h->stream));
-    }
-    char* base = (char*)h->ops.ptr;
+size_t plan_table_bytes(const Plan& plan) {
+    return align256(sizeof(Pass) * plan.passes.size()) + align256(sizeof(Sweep) * plan.sweeps.size()) +
+           align256(sizeof(TOp) * plan.tops.size()) + 256;
+}
+
+int upload_plan(const Plan& plan, char* base, hipStream_t s) {
+    if (plan.passes.empty()) return BSIM_OK;
+    const size_t off_s = align256(sizeof(Pass) * plan.passes.size());
+    const size_t off_o = off_s + align256(sizeof(Sweep) * plan.sweeps.size());
+    BSIM_HIP(hipMemcpyAsync(base, plan.passes.data(), sizeof(Pass) * plan.passes.size(), hipMemcpyHostToDevice, s));
+    BSIM_HIP(hipMemcpyAsync(base + off_s, plan.sweeps.data(), sizeof(Sweep) * plan.sweeps.size(), hipMemcpyHostToDevice,
+                            s));
+    if (!plan.tops.empty())
+        BSIM_HIP(hipMemcpyAsync(base + off_o, plan.tops.data(), sizeof(TOp) * plan.tops.size(), hipMemcpyHostToDevice,
+                                s));
+    return BSIM_OK;
+}
+
+int launch_plan(bsim_state* h, double2* d, const std::vector<EOp>& eops, const Plan& plan, const char* base,
+                bool timed) {
+    const size_t off_s = align256(sizeof(Pass) * plan.passes.size());
+    const size_t off_o = off_s + align256(sizeof(Sweep) * plan.sweeps.size());
     for (const Plan::Step& st : plan.steps) {
         if (st.tile) {
             const Pass& P = plan.passes[st.pass];
             unsigned grid = (unsigned)(1ull << (h->vq - P.T));
             size_t shmem = sizeof(double2) * (size_t)((1 << P.T) + ((1 << P.T) >> 4));
             hipEvent_t e0, e1;
-            if (prof().on)
+            if (timed && prof().on)
                 if (int rc = prof_mark(h->stream, &e0)) return rc;
             hipLaunchKernelGGL(k_tile, dim3(grid), dim3(TB), shmem, h->stream, d, h->vq,
-                               (const Pass*)(base + off_p) + st.pass, (const Sweep*)(base + off_s),
-                               (const TOp*)(base + off_o));
+                               (const Pass*)base + st.pass, (const Sweep*)(base + off_s), (const TOp*)(base + off_o));
             BSIM_HIP(hipGetLastError());
-            if (prof().on)
+            if (timed && prof().on)
                 if (int rc = prof_mark(h->stream, &e1)) return rc;
         } else {
             if (int rc = launch_eop(h, d, eops[st.op])) return rc;
         }
     }
     return BSIM_OK;
+}
+
+int run_plan(bsim_state* h, double2* d, const std::vector<EOp>& eops, const Plan& plan) {
+    if (!plan.passes.empty()) {
+        if (int rc = ensure_ws(h->ops, plan_table_bytes(plan))) return rc;
+        if (int rc = upload_plan(plan, (char*)h->ops.ptr, h->stream)) return rc;
+    }
+    return launch_plan(h, d, eops, plan, (const char*)h->ops.ptr, true);
 }
 
 static bool g_shmem_set = false;

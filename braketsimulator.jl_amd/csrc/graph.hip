@@ -1,0 +1,157 @@
+// Captured evolutions (hipGraph) and the tile autotuner.
+//
+// A captured evolution is the fused plan of an instruction list, its device
+// tables uploaded once, and the launches -- optionally preceded by the reset
+// to |0...0> -- captured from the state's stream into one hipGraph.
+// Replaying it issues the whole evolution as one graph launch, which is what
+// small, launch-bound circuits need (the reference's per-gate loop,
+// src/sv_simulator.jl:140-148, pays one dispatch per gate).  Only plans made
+// of tile passes are captured: single-op launches stage matrices through
+// host copies.
+//
+// The autotuner times the fused evolution of one instruction list for every
+// candidate tile width (tile bits) and contiguous-low-bit count on the
+// handle's own state, and keeps the fastest configuration.
+#include <float.h>
+
+#include <vector>
+
+#include "fused.h"
+
+struct bsim_graph {
+    bsim_state* h = nullptr;
+    char* tables = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+using namespace bsim;
+
+extern "C" {
+
+int bsim_graph_create(bsim_state* h, const bsim_op* ops, int n_ops, int reset, bsim_graph** out) {
+    if (int rc = check_state(h)) return rc;
+    if (!out || n_ops < 0 || (n_ops > 0 && !ops)) return fail(BSIM_EINVAL, "bad graph arguments");
+    *out = nullptr;
+    std::vector<EOp> eops;
+    if (int rc = lower_ops(h->nq, h->kind, ops, n_ops, eops, false)) return rc;
+    Plan plan;
+    if (int rc = make_plan(h->vq, eops, tile_config(), plan)) return rc;
+    for (const Plan::Step& st : plan.steps)
+        if (!st.tile) return fail(BSIM_ENOTSUP, "graph capture takes fusable ops only (dense <= 2 targets, diagonal <= 4)");
+    if (int rc = prepare_tile_kernel()) return rc;
+    bsim_graph* g = new bsim_graph();
+    g->h = h;
+    auto cleanup = [&](int rc) {
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        if (g->graph) (void)hipGraphDestroy(g->graph);
+        if (g->tables) (void)hipFree(g->tables);
+        delete g;
+        return rc;
+    };
+    hipError_t e = hipMalloc((void**)&g->tables, plan_table_bytes(plan));
+    if (e != hipSuccess) return cleanup(hip_fail(e, "hipMalloc(graph tables)"));
+    if (int rc = upload_plan(plan, g->tables, h->stream)) return cleanup(rc);
+    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return cleanup(hip_fail(e, "hipStreamSynchronize"));
+    if ((e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+        return cleanup(hip_fail(e, "hipStreamBeginCapture"));
+    int rc = BSIM_OK;
+    if (reset) rc = bsim_state_reset(h);  // a fill kernel, no memset node
+    if (rc == BSIM_OK) rc = launch_plan(h, h->d, eops, plan, g->tables, false);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(h->stream, &graph);
+    g->graph = graph;
+    if (rc != BSIM_OK) return cleanup(rc);
+    if (e != hipSuccess) return cleanup(hip_fail(e, "hipStreamEndCapture"));
+    if ((e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0)) != hipSuccess)
+        return cleanup(hip_fail(e, "hipGraphInstantiate"));
+    *out = g;
+    return BSIM_OK;
+}
+
+int bsim_graph_launch(bsim_graph* g) {
+    if (!g || !g->exec) return fail(BSIM_EINVAL, "null graph");
+    if (int rc = check_state(g->h)) return rc;
+    BSIM_HIP(hipGraphLaunch(g->exec, g->h->stream));
+    g->h->alias_ready = false;
+    return BSIM_OK;
+}
+
+int bsim_graph_destroy(bsim_graph* g) {
+    if (!g) return BSIM_OK;
+    if (g->h && g->h->stream) (void)hipStreamSynchronize(g->h->stream);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->tables) (void)hipFree(g->tables);
+    delete g;
+    return BSIM_OK;
+}
+
+int bsim_autotune(bsim_state* h, const bsim_op* ops, int n_ops, int* tile_bits, int* low_bits, double* best_ms) {
+    if (int rc = check_state(h)) return rc;
+    if (n_ops < 0 || (n_ops > 0 && !ops)) return fail(BSIM_EINVAL, "bad autotune arguments");
+    std::vector<EOp> eops;
+    if (int rc = lower_ops(h->nq, h->kind, ops, n_ops, eops, false)) return rc;
+    if (int rc = prepare_tile_kernel()) return rc;
+    const TileConfig keep = tile_config();
+    hipEvent_t e0, e1;
+    BSIM_HIP(hipEventCreate(&e0));
+    BSIM_HIP(hipEventCreate(&e1));
+    struct EvGuard {
+        hipEvent_t a, b;
+        ~EvGuard() {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } evg{ e0, e1 };
+    int bestT = keep.tile_bits, bestL = keep.low_bits;
+    double best = DBL_MAX;
+    const int Tmax = h->vq < 13 ? h->vq : 13;
+    for (int T = 10; T <= Tmax; ++T)
+        for (int L = 2; L <= 6 && L <= T - 2; ++L) {
+            tile_config().tile_bits = T;
+            tile_config().low_bits = L;
+            Plan plan;
+            if (int rc = make_plan(h->vq, eops, tile_config(), plan)) {
+                tile_config() = keep;
+                return rc;
+            }
+            double t = DBL_MAX;
+            for (int rep = 0; rep < 3; ++rep) {  // the first run warms up
+                if (int rc = bsim_state_reset(h)) {
+                    tile_config() = keep;
+                    return rc;
+                }
+                BSIM_HIP(hipEventRecord(e0, h->stream));
+                if (int rc = run_plan(h, h->d, eops, plan)) {
+                    tile_config() = keep;
+                    return rc;
+                }
+                BSIM_HIP(hipEventRecord(e1, h->stream));
+                BSIM_HIP(hipEventSynchronize(e1));
+                float ms = 0;
+                BSIM_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (rep > 0 && ms < t) t = ms;
+            }
+            if (t < best) {
+                best = t;
+                bestT = T;
+                bestL = L;
+            }
+        }
+    if (best == DBL_MAX) {  // no candidate fits this state: keep the current configuration
+        tile_config() = keep;
+        best = 0;
+    } else {
+        tile_config() = keep;
+        tile_config().tile_bits = bestT;
+        tile_config().low_bits = bestL;
+    }
+    h->alias_ready = false;
+    if (tile_bits) *tile_bits = tile_config().tile_bits;
+    if (low_bits) *low_bits = tile_config().low_bits;
+    if (best_ms) *best_ms = best;
+    return BSIM_OK;
+}
+
+}  // extern "C"

@@ -109,6 +109,7 @@ int bsim_apply_double_excitation(bsim_state* h, const int* targets4, double angl
 #define BSIM_OP_KRAUS_REF 2 /* DM only: Kraus channel, reference block-diagonal semantics */
 #define BSIM_OP_KRAUS_PHYS 3 /* DM only: Kraus channel on every block */
 #define BSIM_OP_PHASEFLIP 4 /* DM only: PhaseFlip(p) elementwise, matrix[0] = p */
+#define BSIM_OP_SUPEROP 5   /* DM only: 4^k x 4^k superoperator, k <= 3 (bsim_dm_apply_superop) */
 
 #define BSIM_MAX_TARGETS 8
 #define BSIM_MAX_CONTROLS 8
@@ -167,6 +168,14 @@ int bsim_kernel_stats(int reset, int64_t* launches, double* total_ms);
 int bsim_dm_kraus(bsim_state* h, int k, const int* targets, int n_kraus, const double* kraus, int mode);
 /* apply_noise!(::PhaseFlip, dm, q): src/noise_kernels.jl:7-23. */
 int bsim_dm_phaseflip(bsim_state* h, int q, double p);
+/* One superoperator pass: the fused form of a DM gate's two passes
+ * (apply_gate!(U) on the column bits, apply_gate!(conj U) on the row bits,
+ * src/dm_simulator.jl:130-140) and of a physical Kraus channel.  S is the
+ * 4^k x 4^k column-major Liouville matrix of the map on the physical rho
+ * under column stacking: rho'[r, c] = sum S[r + 2^k c, r' + 2^k c'] rho[r', c'],
+ * matrix bit b of r and c <-> qubit targets[b] (so U rho U^dagger has
+ * S = conj(U) (x) U, Kraus K_m have S = sum conj(K_m) (x) K_m).  k <= 3. */
+int bsim_dm_apply_superop(bsim_state* h, int k, const int* targets, const double* S);
 
 /* ---- results (src/result_types.jl, src/sv_simulator.jl:240-271,
  *      src/dm_simulator.jl:253-262) ------------------------------------ */
@@ -254,6 +263,32 @@ typedef struct bsim_task {
  * work queue.  latency_us (optional, n_tasks doubles) receives each task's
  * completion time from the launch start. */
 int bsim_batch_run(const bsim_task* tasks, int n_tasks, double* latency_us);
+
+/* ---- captured evolutions and tuning ------------------------------ */
+
+/* A captured evolve! (the per-simulation loop of src/sv_simulator.jl:140-148
+ * and src/dm_simulator.jl:170-179, recorded once and replayed): the fused
+ * plan of ops is built and its device tables uploaded once, then the launches
+ * -- preceded by bsim_state_reset when reset != 0 -- are captured from the
+ * state's stream into one hipGraph.  bsim_graph_launch replays it on the
+ * state's stream (asynchronously; bsim_synchronize waits).  Only plans made
+ * entirely of tile passes are captured (dense ops on <= 2 vector bits,
+ * diagonals on <= 4, with any controls): anything else returns
+ * BSIM_ENOTSUP.  The graph keeps h and the tile configuration of its
+ * creation; destroy it before the state. */
+typedef struct bsim_graph bsim_graph;
+int bsim_graph_create(bsim_state* h, const bsim_op* ops, int n_ops, int reset, bsim_graph** out);
+int bsim_graph_launch(bsim_graph* g);
+int bsim_graph_destroy(bsim_graph* g);
+
+/* Tile autotuner (the reference has no tunables besides CHUNK_SIZE,
+ * src/BraketSimulator.jl:85-86): times the fused evolution of ops on h's
+ * state (reset to |0...0> before every run, left in an unspecified state
+ * afterwards) for tile_bits 10..min(13, vector qubits) and contiguous low
+ * bits 2..min(6, tile_bits-2), keeps the fastest as the tile configuration
+ * and returns it (outputs may be NULL; *best_ms = 0 when no candidate fits
+ * and the configuration is unchanged). */
+int bsim_autotune(bsim_state* h, const bsim_op* ops, int n_ops, int* tile_bits, int* low_bits, double* best_ms);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,147 @@
+"""Captured evolutions (bsim_graph_create / launch / destroy) and the tile
+autotuner (bsim_autotune).
+
+A replayed graph runs the same fused plan as bsim_evolve, so every replay
+must be bit-identical to an eager evolve of the same circuit (and within
+1e-12 of the oracle).  Replays are separated by eager work on the same and
+on other handles, with synchronizations, and each one is checked.
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from bsim import BsimError
+from bsim import _lib as L
+from bsim import gates as G
+from bsim import noises as Nz
+from bsim import workloads as W
+from bsim.simulators import DensityMatrixSimulator, Instruction, StateVectorSimulator, lower_instructions
+
+from circuits import random_circuit, random_unitary
+
+pytestmark = pytest.mark.gpu
+
+FUSABLE = ("1q", "2q", "ctrl", "diag")  # every gate lands in a tile pass (dense <= 2 targets)
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture
+def keep_tile_config():
+    t, lo = ctypes.c_int(), ctypes.c_int()
+    L.call("bsim_get_tile_config", ctypes.byref(t), ctypes.byref(lo))
+    yield
+    L.call("bsim_set_tile_config", t.value, lo.value)
+
+
+def _plan_singles(n, kind, ins):
+    ol = lower_instructions(ins, kind, "physical")
+    st = (ctypes.c_int64 * 4)()
+    vq = n if kind == L.BSIM_SV else 2 * n
+    L.call("bsim_plan_stats", vq, int(kind == L.BSIM_DM), ol.ptr, ol.n, st)
+    return st[1]
+
+
+@pytest.mark.parametrize("n,depth", [(6, 4), (14, 10), (20, 6)])
+def test_graph_replays_match_eager_sv(n, depth):
+    ins = W.brickwall(n, depth, seed=n)
+    assert _plan_singles(n, L.BSIM_SV, ins) == 0
+    _say(f"graph sv brickwall n={n} depth={depth}")
+    eager = StateVectorSimulator(n, 0).evolve(ins).state_vector()
+    np.testing.assert_allclose(eager, O.evolve_sv(n, ins), atol=1e-12)
+    sim = StateVectorSimulator(n, 0)
+    other = StateVectorSimulator(n, 0)
+    with sim.capture(ins, reset=True) as g:
+        for rep in range(4):
+            g.launch()
+            got = sim.state_vector()  # synchronizes
+            assert np.array_equal(got, eager), f"replay {rep}"
+            # eager work between replays: on the captured handle and on another one
+            sim.evolve([Instruction(G.H(), [0]), Instruction(G.CNot(), [0, n - 1])])
+            other.evolve(ins[: len(ins) // 2])
+            assert abs(float(np.sum(other.probabilities())) - 1.0) < 1e-12
+
+
+def test_graph_without_reset_accumulates():
+    n = 12
+    rng = np.random.default_rng(7)
+    ins = random_circuit(n, 60, rng, kinds=FUSABLE)
+    assert _plan_singles(n, L.BSIM_SV, ins) == 0
+    _say("graph sv without reset")
+    ref = StateVectorSimulator(n, 0)
+    sim = StateVectorSimulator(n, 0)
+    g = sim.capture(ins, reset=False)
+    for rep in range(3):
+        ref.evolve(ins)
+        g.launch()
+        assert np.array_equal(sim.state_vector(), ref.state_vector()), f"replay {rep}"
+    g.close()
+    g.close()  # idempotent
+    with pytest.raises(BsimError):
+        g.launch()
+
+
+def test_graph_density_matrix_with_noise():
+    n = 6
+    ins = []
+    for q in range(n):
+        ins.append(Instruction(G.Ry(0.3 + 0.2 * q), [q]))
+        ins.append(Instruction(Nz.Depolarizing(0.02), [q]))
+    for q in range(n - 1):
+        ins.append(Instruction(G.CNot(), [q, q + 1]))
+    ins += [Instruction(Nz.PhaseFlip(0.1), [2]), Instruction(Nz.AmplitudeDamping(0.05), [4]),
+            Instruction(G.Rz(0.7), [5])]
+    assert _plan_singles(n, L.BSIM_DM, ins) == 0
+    _say("graph dm noisy physical")
+    eager = DensityMatrixSimulator(n, 0, noise_mode="physical").evolve(ins).density_matrix()
+    np.testing.assert_allclose(eager.ravel(order="F"), O.evolve_dm(n, ins, "physical"), atol=1e-12)
+    sim = DensityMatrixSimulator(n, 0, noise_mode="physical")
+    with sim.capture(ins) as g:
+        for rep in range(3):
+            g.launch()
+            assert np.array_equal(sim.density_matrix(), eager), f"replay {rep}"
+            sim.apply_gate(G.X(), 1)
+
+
+def test_graph_rejects_unfusable_ops():
+    n = 8
+    sim = StateVectorSimulator(n, 0)
+    _say("graph rejection")
+    with pytest.raises(BsimError, match="fusable"):
+        sim.capture([Instruction(G.H(), [0]),
+                     Instruction(G.Unitary(random_unitary(3, np.random.default_rng(1))), [0, 1, 2])])
+    dm = DensityMatrixSimulator(4, 0)  # reference-mode Kraus runs in its own kernel
+    with pytest.raises(BsimError, match="fusable"):
+        dm.capture([Instruction(Nz.Depolarizing(0.1), [0])])
+    # the handle still works after a refused capture
+    sim.evolve([Instruction(G.H(), [0])])
+    assert abs(sim.probabilities()[0] - 0.5) < 1e-14
+
+
+def test_autotune_picks_a_valid_config(keep_tile_config):
+    n = 16
+    ins = W.brickwall(n, 6, seed=3)
+    _say("autotune 16q")
+    sim = StateVectorSimulator(n, 0)
+    t, lo, ms = sim.autotune(ins)
+    assert 10 <= t <= 13 and 2 <= lo <= min(6, t - 2) and ms > 0
+    cur_t, cur_lo = ctypes.c_int(), ctypes.c_int()
+    L.call("bsim_get_tile_config", ctypes.byref(cur_t), ctypes.byref(cur_lo))
+    assert (cur_t.value, cur_lo.value) == (t, lo)
+    sim.reinit(n, 0)
+    got = sim.evolve(ins).state_vector()
+    np.testing.assert_allclose(got, O.evolve_sv(n, ins), atol=1e-12)
+
+
+def test_autotune_small_state_keeps_config(keep_tile_config):
+    n = 8  # below the smallest candidate tile: nothing to try
+    t0, lo0 = ctypes.c_int(), ctypes.c_int()
+    L.call("bsim_get_tile_config", ctypes.byref(t0), ctypes.byref(lo0))
+    t, lo, ms = StateVectorSimulator(n, 0).autotune(W.brickwall(n, 2, seed=1))
+    assert (t, lo, ms) == (t0.value, lo0.value, 0.0)
