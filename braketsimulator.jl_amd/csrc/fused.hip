@@ -276,12 +276,17 @@ static uint32_t op_regs(const EOp& e, Reg& reg, bool* outside) {
     return m;
 }
 
-// per-amplitude VALU cost of an op as a separate sweep op (for the fusion rule)
+// VALU instructions per thread of an op as a separate sweep op on a
+// 16-amplitude sub-cube (the fusion rule compares these): dense 1-bit 128
+// FP64 + dispatch, 64 selects more with controls; dense 2-bit 256 FP64; a
+// sign op ~80 bit operations; a general diagonal 64 FP64 + its index.
+constexpr int COST_DENSE1 = 138, COST_DENSE2 = 266;
 static int sweep_cost(const EOp& e) {
     uint64_t pm;
     int pd;
-    if (e.kind == E_DIAG) return sign_pair(e, pm, pd) ? 8 : 8;
-    return e.g.k == 1 ? (e.g.cmask ? 14 : 12) : 20;
+    if (e.kind == E_DIAG) return sign_pair(e, pm, pd) ? 90 : 130;
+    const int ctl = e.g.cmask ? 64 : 0;
+    return (e.g.k == 1 ? COST_DENSE1 : COST_DENSE2) + ctl;
 }
 
 // Embedding of op e into the block basis: block index bit p <-> register R[p].
@@ -335,7 +340,7 @@ static void fuse_sweep(const std::vector<EOp>& eops, const std::vector<int>& tak
         const int nb = popc(b.regs);
         int cost = 0;
         for (int i : b.ops) cost += sweep_cost(eops[i]);
-        const int fused = nb == 1 ? 12 : 20;
+        const int fused = nb == 1 ? COST_DENSE1 : COST_DENSE2;
         if (b.ops.size() < 2 || cost <= fused) {
             for (int i : b.ops) emit_one(i);
             return;

@@ -171,6 +171,11 @@ __device__ __forceinline__ void sw_diag(SubCube& s, const TOp* __restrict__ op) 
     });
 }
 
+__device__ __forceinline__ uint32_t xor_and(uint32_t a, uint32_t b, uint32_t c) {
+    // a ^ (b & c) in one v_bitop3 (truth table over a = 0xF0, b = 0xCC, c = 0xAA)
+    return __builtin_amdgcn_bitop3_b32(a, b, c, (unsigned char)(0xF0 ^ (0xCC & 0xAA)));
+}
+
 // +-1 diagonal (CZ layers): negate where
 // P(i) = sum_k parity(i & (i >> pd[k]) & pm[k]) is 1.  P is a quadratic form
 // over GF(2): for i = gbase ^ r_j,
@@ -179,19 +184,24 @@ __device__ __forceinline__ void sw_diag(SubCube& s, const TOp* __restrict__ op) 
 __device__ __forceinline__ void sw_sign(SubCube& s, const TOp* __restrict__ op) {
     const uint64_t a = s.gbase;
     uint32_t m = op->jmask;
-    int par = 0;
-    for (int k = 0; k < op->npairs; ++k) par ^= __popcll(a & (a >> op->pd[k]) & op->pm[k]);
-    if (par & 1) m ^= 0xFFFFu;
-    if (__popcll(a & op->sgm[0]) & 1) m ^= 0xAAAAu;
-    if (__popcll(a & op->sgm[1]) & 1) m ^= 0xCCCCu;
-    if (__popcll(a & op->sgm[2]) & 1) m ^= 0xF0F0u;
-    if (__popcll(a & op->sgm[3]) & 1) m ^= 0xFF00u;
+    uint32_t par = 0;
+    for (int k = 0; k < op->npairs; ++k) par += __popcll(a & (a >> op->pd[k]) & op->pm[k]);
+    // all-ones where the parity bit is set (sign-extended bit 0), masked into m
+    auto flip = [&](uint32_t p, uint32_t pattern) { m = xor_and(m, (uint32_t)__builtin_amdgcn_sbfe(p, 0, 1), pattern); };
+    flip(par, 0xFFFFu);
+    flip(__popcll(a & op->sgm[0]), 0xAAAAu);
+    flip(__popcll(a & op->sgm[1]), 0xCCCCu);
+    flip(__popcll(a & op->sgm[2]), 0xF0F0u);
+    flip(__popcll(a & op->sgm[3]), 0xFF00u);
     sfor<0, 16>([&](auto J) {
         constexpr int j = decltype(J)::value;
-        const uint64_t f = (uint64_t)((m << (31 - j)) & 0x80000000u) << 32;  // sign bit of register j
+        const uint32_t t = m << (31 - j);  // bit j of m at the sign position (garbage below, masked off)
         c2 v = getv<j>(s.v);
-        v.x = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, v.x) ^ f);
-        v.y = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, v.y) ^ f);
+        const uint64_t bx = __builtin_bit_cast(uint64_t, v.x), by = __builtin_bit_cast(uint64_t, v.y);
+        const uint64_t hx = xor_and((uint32_t)(bx >> 32), t, 0x80000000u);
+        const uint64_t hy = xor_and((uint32_t)(by >> 32), t, 0x80000000u);
+        v.x = __builtin_bit_cast(double, (bx & 0xFFFFFFFFull) | (hx << 32));
+        v.y = __builtin_bit_cast(double, (by & 0xFFFFFFFFull) | (hy << 32));
         setv<j>(s.v, v);
     });
 }
