@@ -88,7 +88,7 @@ struct Plan {
 
 struct TileConfig {
     int tile_bits = 12;
-    int low_bits = 4;
+    int low_bits = 3;  // 8 contiguous amplitudes (128 B) per run: fewer passes than 4, same coalescing
     // FP64 work budget of one pass, in operations per amplitude (dense 1q
     // = 8, dense 2q = 32, diagonal = 4, sign = 1); 0 = unlimited.  Bounds
     // the fusion width so that a pass stays HBM-bound.

@@ -71,9 +71,29 @@ struct SubCube {
 
 // Dense ops come in two instantiations: without controls (the common case)
 // and with the split control test; the choice is made once per op.
+// The fields every dense op needs, read with the op's kind before the
+// dispatch so that a 1-bit dense op costs one scalar-load round trip.
+struct OpHead {
+    int32_t kind, a, b;
+    uint64_t cmask;
+    c2 m0, m1, m2, m3;
+};
+__device__ __forceinline__ OpHead load_head(const TOp* __restrict__ op) {
+    OpHead h;
+    h.kind = op->kind;
+    h.a = op->a;
+    h.b = op->b;
+    h.cmask = op->cmask;
+    h.m0 = to_c2(op->m[0]);
+    h.m1 = to_c2(op->m[1]);
+    h.m2 = to_c2(op->m[2]);
+    h.m3 = to_c2(op->m[3]);
+    return h;
+}
+
 template <int A, bool CTL>
-__device__ __forceinline__ void sw_dense1_t(SubCube& s, const TOp* __restrict__ op) {
-    const c2 m00 = to_c2(op->m[0]), m10 = to_c2(op->m[1]), m01 = to_c2(op->m[2]), m11 = to_c2(op->m[3]);
+__device__ __forceinline__ void sw_dense1_t(SubCube& s, const TOp* __restrict__ op, const OpHead& hd) {
+    const c2 m00 = hd.m0, m10 = hd.m1, m01 = hd.m2, m11 = hd.m3;
     bool tok = true;
     uint32_t jm = 0xFFFFu;
     if constexpr (CTL) {
@@ -99,13 +119,13 @@ __device__ __forceinline__ void sw_dense1_t(SubCube& s, const TOp* __restrict__ 
 }
 
 template <int A>
-__device__ __forceinline__ void sw_dense1(SubCube& s, const TOp* __restrict__ op) {
-    if (op->cmask) sw_dense1_t<A, true>(s, op);
-    else sw_dense1_t<A, false>(s, op);
+__device__ __forceinline__ void sw_dense1(SubCube& s, const TOp* __restrict__ op, const OpHead& hd) {
+    if (hd.cmask) sw_dense1_t<A, true>(s, op, hd);
+    else sw_dense1_t<A, false>(s, op, hd);
 }
 
 template <int A, int B, bool CTL>
-__device__ __forceinline__ void sw_dense2_t(SubCube& s, const TOp* __restrict__ op) {
+__device__ __forceinline__ void sw_dense2_t(SubCube& s, const TOp* __restrict__ op, const OpHead& hd) {
     bool tok = true;
     uint32_t jm = 0xFFFFu;
     if constexpr (CTL) {
@@ -117,8 +137,9 @@ __device__ __forceinline__ void sw_dense2_t(SubCube& s, const TOp* __restrict__ 
         if constexpr ((j & ((1 << A) | (1 << B))) == 0) {
             constexpr int i1 = j | (1 << A), i2 = j | (1 << B), i3 = j | (1 << A) | (1 << B);
             const c2 a0 = getv<j>(s.v), a1 = getv<i1>(s.v), a2 = getv<i2>(s.v), a3 = getv<i3>(s.v);
+            const c2 col0[4] = { hd.m0, hd.m1, hd.m2, hd.m3 };
             auto row = [&](int r) {
-                c2 acc = cmul(to_c2(op->m[r]), a0);
+                c2 acc = cmul(col0[r], a0);
                 acc = cfma(to_c2(op->m[r + 4]), a1, acc);
                 acc = cfma(to_c2(op->m[r + 8]), a2, acc);
                 return cfma(to_c2(op->m[r + 12]), a3, acc);
@@ -140,9 +161,9 @@ __device__ __forceinline__ void sw_dense2_t(SubCube& s, const TOp* __restrict__ 
 }
 
 template <int A, int B>
-__device__ __forceinline__ void sw_dense2(SubCube& s, const TOp* __restrict__ op) {
-    if (op->cmask) sw_dense2_t<A, B, true>(s, op);
-    else sw_dense2_t<A, B, false>(s, op);
+__device__ __forceinline__ void sw_dense2(SubCube& s, const TOp* __restrict__ op, const OpHead& hd) {
+    if (hd.cmask) sw_dense2_t<A, B, true>(s, op, hd);
+    else sw_dense2_t<A, B, false>(s, op, hd);
 }
 
 // Diagonal index of register j = (index bits read from the thread's base)
@@ -207,18 +228,19 @@ __device__ __forceinline__ void sw_sign(SubCube& s, const TOp* __restrict__ op) 
 }
 
 __device__ __forceinline__ void run_op(SubCube& s, const TOp* __restrict__ op) {
-    switch (op->kind * 16 + op->a * 4 + op->b) {
+    const OpHead hd = load_head(op);
+    switch (hd.kind * 16 + hd.a * 4 + hd.b) {
         case 48: sw_sign(s, op); break;
-        case 0: sw_dense1<0>(s, op); break;
-        case 4: sw_dense1<1>(s, op); break;
-        case 8: sw_dense1<2>(s, op); break;
-        case 12: sw_dense1<3>(s, op); break;
-        case 16 + 1: sw_dense2<0, 1>(s, op); break;
-        case 16 + 2: sw_dense2<0, 2>(s, op); break;
-        case 16 + 3: sw_dense2<0, 3>(s, op); break;
-        case 16 + 6: sw_dense2<1, 2>(s, op); break;
-        case 16 + 7: sw_dense2<1, 3>(s, op); break;
-        case 16 + 11: sw_dense2<2, 3>(s, op); break;
+        case 0: sw_dense1<0>(s, op, hd); break;
+        case 4: sw_dense1<1>(s, op, hd); break;
+        case 8: sw_dense1<2>(s, op, hd); break;
+        case 12: sw_dense1<3>(s, op, hd); break;
+        case 16 + 1: sw_dense2<0, 1>(s, op, hd); break;
+        case 16 + 2: sw_dense2<0, 2>(s, op, hd); break;
+        case 16 + 3: sw_dense2<0, 3>(s, op, hd); break;
+        case 16 + 6: sw_dense2<1, 2>(s, op, hd); break;
+        case 16 + 7: sw_dense2<1, 3>(s, op, hd); break;
+        case 16 + 11: sw_dense2<2, 3>(s, op, hd); break;
         default: sw_diag(s, op); break;
     }
 }

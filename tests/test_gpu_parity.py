@@ -4,6 +4,7 @@ Bar: amplitudes, probabilities and expectations within 1e-10 absolute of
 the oracle; alias tables and shot indices bit-exact given the same
 probabilities, wsum and Philox draws.
 """
+import ctypes
 import math
 import sys
 
@@ -93,13 +94,15 @@ def test_tile_configs(tile, low):
     rng = np.random.default_rng(tile * 7 + low)
     ins = random_circuit(n, 150, rng)
     exp = O.evolve_sv(n, ins)
+    t0, lo0 = ctypes.c_int(), ctypes.c_int()
+    L.call("bsim_get_tile_config", ctypes.byref(t0), ctypes.byref(lo0))
     L.call("bsim_set_tile_config", tile, low)
     try:
         sim = StateVectorSimulator(n, 0)
         sim.evolve(ins)
         got = sim.state_vector()
     finally:
-        L.call("bsim_set_tile_config", 12, 4)
+        L.call("bsim_set_tile_config", t0.value, lo0.value)
     assert np.max(np.abs(got - exp)) < TOL
 
 
