@@ -271,7 +271,7 @@ __global__ __launch_bounds__(TB, 2) void k_batch(const BTask* __restrict__ tasks
             if (E.kind >= 0) {
                 const uint64_t first = (uint64_t)local * (uint64_t)E.chunk;
                 for (int c = 0; c < E.chunk; ++c) {
-                    tile_body(K.sv, K.n, passes + K.pass0 + E.kind, sweeps, tops, first + c, E.kind == 0, tile,
+                    tile_body<false>(K.sv, K.n, passes + K.pass0 + E.kind, sweeps, tops, first + c, E.kind == 0, tile,
                               scat_lo, scat_hi);
                     __syncthreads();  // the next tile rewrites the scatter tables
                 }

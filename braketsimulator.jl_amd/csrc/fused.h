@@ -20,12 +20,21 @@ struct EOp {
     int ktargets[8] = {0};
     int nkraus = 0;
     std::vector<double2> kraus;
+    // ... and, for kq <= 2, the same map as a predicated superoperator for the
+    // tile passes: g (dense on the 2 kq row/column bits) is applied where the
+    // stored row and column agree on every other bit,
+    // ((i ^ (i >> kshift)) & kmask) == 0.
+    uint64_t kmask = 0;
+    int kshift = 0;
+    // dense on 4 bits that may run as a tile op (the 16x16 superoperator of a
+    // 2-qubit physical channel); other 3+-bit dense ops launch on their own
+    bool tile4 = false;
     // differentiation (set by the adjoint path)
     int nparams = 0;
     int pidx[3] = { -1, -1, -1 };
     std::vector<double2> dm[3];
-    uint64_t touched() const;    // every vector bit the op reads or writes
-    bool fusable() const;        // fits a tile sweep (dense k <= 2, diag k <= 4)
+    uint64_t touched() const;    // every vector bit the op reads, writes or depends on
+    bool fusable() const;        // fits a tile sweep (dense k <= 2, diag k <= 4, reference Kraus kq <= 2)
 };
 
 // Translate bsim_op instructions into elementary ops on the vector.
@@ -36,9 +45,9 @@ int lower_ops(int nq, int kind, const bsim_op* ops, int n_ops, std::vector<EOp>&
 // Device-side tile program ------------------------------------------------
 
 struct TOp {
-    int32_t kind;      // 0 dense1, 1 dense2, 2 diag, 3 sign parity
+    int32_t kind;      // 0 dense1, 1 dense2, 2 diag, 3 sign parity, 4 predicated superoperator, 5 its matrix
     int32_t a, b;      // dense: register bits (a < b for dense2)
-    int32_t nd;        // diag: bit count
+    int32_t nd;        // diag: bit count; kind 4: TOps that follow with the matrix (0 or 15)
     int32_t dbit[4];   // diag: vector bit of diagonal index bit i
     uint64_t cmask, cval;
     // sign: amplitude i is negated when sum_k popcount(i & (i >> pd[k]) & pm[k]) is odd
@@ -50,7 +59,12 @@ struct TOp {
     // base and r_j the global bits of j's register bits.
     //   dense with controls: on_j = ((gbase & tmask) == tval) && bit j of jmask;
     //   sign: negate_j = bit j of (jmask ^ P(gbase) * 0xFFFF ^ sum_i parity(gbase & sgm[i]) * pattern_i),
-    //         P(x) = sum_k parity(x & (x >> pd[k]) & pm[k]), pattern_i = registers with bit i set.
+    //         P(x) = sum_k parity(x & (x >> pd[k]) & pm[k]), pattern_i = registers with bit i set;
+    //   kind 4: on_j = (Q(gbase) ^ sum over the register bits i of j of sgm[i]) == 0,
+    //         Q(x) = (x ^ (x >> pd[0])) & pm[0] (linear, so Q(gbase | r_j) splits).
+    //   Kind 4 on 2 registers (a < b) keeps its 4x4 matrix in m; on all 4
+    //   registers its 16x16 matrix (register order) is column c = m of the
+    //   c-th TOp from this one (the 15 that follow are kind 5).
     uint64_t tmask, tval;
     uint64_t sgm[4];
     uint32_t jmask;
@@ -84,6 +98,7 @@ struct Plan {
     std::vector<TOp> tops;
     std::vector<int> top_src;  // eop index each TOp came from
     int64_t fused_ops = 0;
+    bool has_kraus = false;    // holds kind-4 ops: launch k_tile_kraus
 };
 
 struct TileConfig {

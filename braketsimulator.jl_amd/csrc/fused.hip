@@ -31,8 +31,9 @@ uint64_t EOp::touched() const {
 }
 
 bool EOp::fusable() const {
-    if (kind == E_DENSE) return g.k <= 2;
+    if (kind == E_DENSE) return g.k <= 2 || (g.k == 4 && tile4);
     if (kind == E_DIAG) return g.k <= 4;
+    if (kind == E_KRAUS_REF) return kq <= 2 && kshift > 0;
     return false;
 }
 
@@ -129,6 +130,40 @@ int lower_ops(int nq, int kind, const bsim_op* ops, int n_ops, std::vector<EOp>&
                 size_t D = (size_t)1 << k, n = (size_t)op.n_kraus * D * D;
                 e.kraus.resize(n);
                 for (size_t x = 0; x < n; ++x) e.kraus[x] = make_double2(op.matrix[2 * x], op.matrix[2 * x + 1]);
+                if (k <= 2) {
+                    // Tile form.  The reference updates block B[r, c] = stored[ix(r), ix(c)]
+                    // (rows and columns sharing every non-target bit) to sum_m K B K^dagger,
+                    // ix: matrix bit 0 = target 0 (k = 1); bit 0 = target 1, bit 1 =
+                    // target 0 (k = 2, noise_kernels.jl:145-155).  On vec(B) (J = r + D c)
+                    // that is S[(r, c), (l, l')] = sum_m K[r, l] conj(K[c, l']); matrix bit
+                    // b is the stored-row bit of qubit q_b (vector qubit nq + q_b), bit
+                    // k + b its stored-column bit (vector qubit q_b).
+                    const int q[2] = { k == 1 ? op.targets[0] : op.targets[1], op.targets[0] };
+                    const size_t DD = D * D;
+                    std::vector<double> sup(2 * DD * DD, 0.0);
+                    for (int m = 0; m < op.n_kraus; ++m) {
+                        const double* K = op.matrix + 2 * (size_t)m * DD;
+                        for (size_t Jo = 0; Jo < DD; ++Jo)
+                            for (size_t Ji = 0; Ji < DD; ++Ji) {
+                                const size_t r = Jo % D, c = Jo / D, l = Ji % D, lp = Ji / D;
+                                const double ar = K[2 * (r + D * l)], ai = K[2 * (r + D * l) + 1];
+                                const double br = K[2 * (c + D * lp)], bi = -K[2 * (c + D * lp) + 1];
+                                sup[2 * (Jo + DD * Ji)] += ar * br - ai * bi;
+                                sup[2 * (Jo + DD * Ji) + 1] += ar * bi + ai * br;
+                            }
+                    }
+                    int t[4];
+                    uint64_t rowbits = 0;
+                    for (int b = 0; b < k; ++b) {
+                        t[b] = nq + q[b];
+                        t[k + b] = q[b];
+                        rowbits |= 1ull << (nq - 1 - q[b]);
+                    }
+                    if (int rc = make_spec(vq, 2 * k, t, 0, nullptr, nullptr, sup.data(), false, false, false, e.g))
+                        return rc;
+                    e.kmask = (((1ull << nq) - 1) & ~rowbits);
+                    e.kshift = nq;
+                }
                 out.push_back(std::move(e));
             } else {
                 // physical channel: superoperator sum_m K_m (column bits) (x) conj(K_m) (row bits)
@@ -167,6 +202,7 @@ int lower_ops(int nq, int kind, const bsim_op* ops, int n_ops, std::vector<EOp>&
                 e.kind = E_DENSE;
                 if (int rc = make_spec(vq, 2 * k, t, 0, nullptr, nullptr, sup.data(), false, false, false, e.g))
                     return rc;
+                e.tile4 = k == 2;
                 out.push_back(std::move(e));
             }
         } else if (op.type == BSIM_OP_SUPEROP) {
@@ -184,6 +220,7 @@ int lower_ops(int nq, int kind, const bsim_op* ops, int n_ops, std::vector<EOp>&
             EOp e;
             e.kind = E_DENSE;
             if (int rc = make_spec(vq, 2 * k, t, 0, nullptr, nullptr, op.matrix, false, false, false, e.g)) return rc;
+            e.tile4 = k == 2;
             out.push_back(std::move(e));
         } else {
             return fail(BSIM_EINVAL, "unknown op type");
@@ -238,7 +275,7 @@ static bool sign_pair(const EOp& e, uint64_t& pm, int& pd) {
 
 // FP64 operations per amplitude of one op inside a tile sweep
 static int op_cost(const EOp& e) {
-    if (e.kind == E_DENSE) return e.g.k == 1 ? 8 : 32;
+    if (e.kind == E_DENSE) return e.g.k == 1 ? 8 : (e.g.k == 2 ? 32 : 128);
     if (e.kind == E_DIAG) {
         uint64_t pm;
         int pd;
@@ -374,7 +411,8 @@ static void fuse_sweep(const std::vector<EOp>& eops, const std::vector<int>& tak
     for (int i : take) {
         const EOp& e = eops[i];
         bool outside = false;
-        const uint32_t R = op_regs(e, reg, &outside);
+        // a reference Kraus op depends on every bit: nothing moves across it
+        const uint32_t R = e.kind == E_KRAUS_REF ? 0xFu : op_regs(e, reg, &outside);
         const bool fusable = !outside && (e.kind == E_DENSE || e.kind == E_DIAG) && popc(R) <= 2 && R != 0;
         uint32_t U = R;
         for (auto& b : pend)
@@ -432,8 +470,12 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
             done[head] = 1;
             continue;
         }
-        // ---- pass: choose the tile bits and the ops it absorbs
+        // ---- pass: choose the tile bits and the ops it absorbs (the low bits
+        // give way when the first op needs more than T - L bits: a 2-qubit
+        // reference Kraus op takes 4)
         uint64_t S = (L > 0) ? ((1ull << L) - 1) : 0;
+        if ((eops[head].kind == E_DENSE || eops[head].kind == E_KRAUS_REF) && popc(S | target_mask(eops[head])) > T)
+            S = 0;
         uint64_t blocked = 0;
         int cost = 0;
         inc.clear();
@@ -447,7 +489,7 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
                 blocked |= t;
             } else if (!e.fusable() || over) {
                 blocked |= t;
-            } else if (e.kind == E_DENSE) {
+            } else if (e.kind == E_DENSE || e.kind == E_KRAUS_REF) {
                 uint64_t need = target_mask(e);
                 if ((need & ~S) == 0 || popc(S | need) <= T) {
                     S |= need;
@@ -462,7 +504,7 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
             }
             if ((blocked & all) == all) break;
         }
-        if (inc.empty()) {  // cannot happen with L <= T-2; never loop without progress
+        if (inc.empty()) {  // cannot happen (the head op always fits); never loop without progress
             plan.steps.push_back({ false, -1, (int)head });
             done[head] = 1;
             continue;
@@ -497,7 +539,7 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
                     next.push_back(i);
                     continue;
                 }
-                if (e.kind == E_DENSE) {
+                if (e.kind == E_DENSE || e.kind == E_KRAUS_REF) {
                     uint32_t need = 0;
                     for (int b = 0; b < e.g.k; ++b) need |= 1u << pos[e.g.tb[b]];
                     if (popc(W | need) <= 4) {
@@ -554,6 +596,49 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
                     op.pm[0] = pm;
                     op.pd[0] = pd;
                     op.npairs = 1;
+                } else if (e.kind == E_KRAUS_REF || (e.kind == E_DENSE && e.g.k == 4)) {
+                    // reference Kraus block update, or (mask 0: every block) a
+                    // 4-bit dense superoperator
+                    plan.has_kraus = true;
+                    op.kind = 4;
+                    op.pm[0] = e.kind == E_KRAUS_REF ? e.kmask : 0;
+                    op.pd[0] = e.kind == E_KRAUS_REF ? e.kshift : 0;
+                    if (e.g.k == 2) {
+                        int a = reg(e.g.tb[0]), b = reg(e.g.tb[1]);
+                        auto sw2 = [](int x) { return ((x & 1) << 1) | ((x >> 1) & 1); };
+                        op.a = a < b ? a : b;
+                        op.b = a < b ? b : a;
+                        for (int r = 0; r < 4; ++r)
+                            for (int c = 0; c < 4; ++c)
+                                op.m[r + 4 * c] = a < b ? e.g.m[r + 4 * c] : e.g.m[sw2(r) + 4 * sw2(c)];
+                    } else {
+                        // 16x16 in register order: register index j (bit r = register r)
+                        // <-> matrix index J (bit b = matrix bit b on register rb[b])
+                        int rb[4];
+                        for (int b = 0; b < 4; ++b) rb[b] = reg(e.g.tb[b]);
+                        auto J = [&](int j) {
+                            int x = 0;
+                            for (int b = 0; b < 4; ++b) x |= ((j >> rb[b]) & 1) << b;
+                            return x;
+                        };
+                        op.a = 0;
+                        op.b = 0;
+                        op.nd = 15;
+                        TOp col[16];
+                        for (int c = 0; c < 16; ++c) {
+                            memset(&col[c], 0, sizeof(TOp));
+                            col[c].kind = 5;
+                            for (int r = 0; r < 16; ++r) col[c].m[r] = e.g.m[J(r) + 16 * J(c)];
+                        }
+                        for (int r = 0; r < 16; ++r) op.m[r] = col[0].m[r];
+                        plan.tops.push_back(op);
+                        plan.top_src.push_back(i);
+                        for (int c = 1; c < 16; ++c) {
+                            plan.tops.push_back(col[c]);
+                            plan.top_src.push_back(i);
+                        }
+                        return;
+                    }
                 } else if (e.kind == E_DIAG) {
                     op.kind = 2;
                     op.nd = e.g.k;
@@ -599,7 +684,10 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
             };
             for (int o = sw.op_begin; o < sw.op_end; ++o) {
                 TOp& op = plan.tops[o];
-                if (op.kind == 3) {
+                if (op.kind == 4) {
+                    // Q is linear over GF(2): Q(gbase | r_j) = Q(gbase) ^ sum of Q(register bits)
+                    for (int r = 0; r < 4; ++r) op.sgm[r] = (gb[r] ^ (gb[r] >> op.pd[0])) & op.pm[0];
+                } else if (op.kind == 3) {
                     uint32_t q = 0;
                     for (int j = 0; j < 16; ++j) {
                         const uint64_t x = rj(j);
@@ -639,13 +727,23 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
 
 namespace bsim {
 
-// One launch = one pass: block b runs tile b.
-__global__ __launch_bounds__(TB) void k_tile(double2* __restrict__ sv, int vq, const Pass* __restrict__ pass,
-                                             const Sweep* __restrict__ sweeps, const TOp* __restrict__ tops) {
+// One launch = one pass: block b runs tile b.  k_tile_kraus also runs the
+// reference Kraus ops of density-matrix plans.
+template <bool KRAUS>
+__device__ __forceinline__ void tile_pass(double2* __restrict__ sv, int vq, const Pass* __restrict__ pass,
+                                          const Sweep* __restrict__ sweeps, const TOp* __restrict__ tops) {
     extern __shared__ double2 tile[];
     __shared__ uint64_t scat_lo[256];   // scatter of local bits 0..7
     __shared__ uint64_t scat_hi[64];    // scatter of local bits 8..13
-    tile_body(sv, vq, pass, sweeps, tops, blockIdx.x, false, tile, scat_lo, scat_hi);
+    tile_body<KRAUS>(sv, vq, pass, sweeps, tops, blockIdx.x, false, tile, scat_lo, scat_hi);
+}
+__global__ __launch_bounds__(TB) void k_tile(double2* __restrict__ sv, int vq, const Pass* __restrict__ pass,
+                                             const Sweep* __restrict__ sweeps, const TOp* __restrict__ tops) {
+    tile_pass<false>(sv, vq, pass, sweeps, tops);
+}
+__global__ __launch_bounds__(TB) void k_tile_kraus(double2* __restrict__ sv, int vq, const Pass* __restrict__ pass,
+                                                   const Sweep* __restrict__ sweeps, const TOp* __restrict__ tops) {
+    tile_pass<true>(sv, vq, pass, sweeps, tops);
 }
 
 }  // namespace bsim
@@ -731,7 +829,7 @@ int launch_plan(bsim_state* h, double2* d, const std::vector<EOp>& eops, const P
             hipEvent_t e0, e1;
             if (timed && prof().on)
                 if (int rc = prof_mark(h->stream, &e0)) return rc;
-            hipLaunchKernelGGL(k_tile, dim3(grid), dim3(TB), shmem, h->stream, d, h->vq,
+            hipLaunchKernelGGL(plan.has_kraus ? k_tile_kraus : k_tile, dim3(grid), dim3(TB), shmem, h->stream, d, h->vq,
                                (const Pass*)base + st.pass, (const Sweep*)(base + off_s), (const TOp*)(base + off_o));
             BSIM_HIP(hipGetLastError());
             if (timed && prof().on)
@@ -757,8 +855,9 @@ int prepare_tile_kernel() {
     if (g_shmem_set) return BSIM_OK;
     // allow > 64 KiB of dynamic LDS: a padded 13-bit tile (the kernel's
     // static scatter tables take another 2.5 KiB of the 160 KiB)
-    BSIM_HIP(hipFuncSetAttribute((const void*)k_tile, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)(sizeof(double2) * ((1 << 13) + (1 << 9)))));
+    for (const void* k : { (const void*)k_tile, (const void*)k_tile_kraus })
+        BSIM_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)(sizeof(double2) * ((1 << 13) + (1 << 9)))));
     g_shmem_set = true;
     return BSIM_OK;
 }

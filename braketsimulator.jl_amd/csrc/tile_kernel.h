@@ -74,7 +74,7 @@ struct SubCube {
 // The fields every dense op needs, read with the op's kind before the
 // dispatch so that a 1-bit dense op costs one scalar-load round trip.
 struct OpHead {
-    int32_t kind, a, b;
+    int32_t kind, a, b, nd;
     uint64_t cmask;
     c2 m0, m1, m2, m3;
 };
@@ -83,6 +83,7 @@ __device__ __forceinline__ OpHead load_head(const TOp* __restrict__ op) {
     h.kind = op->kind;
     h.a = op->a;
     h.b = op->b;
+    h.nd = op->nd;
     h.cmask = op->cmask;
     h.m0 = to_c2(op->m[0]);
     h.m1 = to_c2(op->m[1]);
@@ -227,8 +228,89 @@ __device__ __forceinline__ void sw_sign(SubCube& s, const TOp* __restrict__ op) 
     });
 }
 
-__device__ __forceinline__ void run_op(SubCube& s, const TOp* __restrict__ op) {
+// Reference-semantics Kraus block update as a predicated superoperator
+// (kind 4).  Q(gbase) is this thread's part of the block test; a register
+// group is updated when Q(gbase) equals the uniform part of its register
+// bits.  On two registers the 4x4 runs like a dense 2-bit op with a select
+// per group.
+__device__ __forceinline__ uint64_t kraus_q(const SubCube& s, const TOp* __restrict__ op) {
+    const uint64_t g = s.gbase;
+    return (g ^ (g >> op->pd[0])) & op->pm[0];
+}
+
+template <int A, int B>
+__device__ __forceinline__ void sw_kraus2(SubCube& s, const TOp* __restrict__ op, const OpHead& hd) {
+    const uint64_t q = kraus_q(s, op);
+    const uint64_t s0 = op->sgm[0], s1 = op->sgm[1], s2 = op->sgm[2], s3 = op->sgm[3];
+    const c2 col0[4] = { hd.m0, hd.m1, hd.m2, hd.m3 };
+    sfor<0, 16>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        if constexpr ((j & ((1 << A) | (1 << B))) == 0) {
+            constexpr int i1 = j | (1 << A), i2 = j | (1 << B), i3 = j | (1 << A) | (1 << B);
+            const uint64_t uj = ((j & 1) ? s0 : 0) ^ ((j & 2) ? s1 : 0) ^ ((j & 4) ? s2 : 0) ^ ((j & 8) ? s3 : 0);
+            const bool on = q == uj;
+            const c2 a0 = getv<j>(s.v), a1 = getv<i1>(s.v), a2 = getv<i2>(s.v), a3 = getv<i3>(s.v);
+            auto row = [&](int r) {
+                c2 acc = cmul(col0[r], a0);
+                acc = cfma(to_c2(op->m[r + 4]), a1, acc);
+                acc = cfma(to_c2(op->m[r + 8]), a2, acc);
+                return cfma(to_c2(op->m[r + 12]), a3, acc);
+            };
+            const c2 r0 = row(0), r1 = row(1), r2 = row(2), r3 = row(3);
+            setv<j>(s.v, csel(on, r0, a0));
+            setv<i1>(s.v, csel(on, r1, a1));
+            setv<i2>(s.v, csel(on, r2, a2));
+            setv<i3>(s.v, csel(on, r3, a3));
+        }
+    });
+}
+
+// On all four registers (a 2-qubit channel: both qubits' row and column
+// bits) the block is the whole sub-cube, updated only where Q(gbase) == 0 --
+// a few threads of a few tiles.  Those threads put the sub-cube back into
+// their own LDS slots and accumulate one output at a time from there, so
+// the 16x16 product needs no extra registers.
+__device__ __forceinline__ void sw_kraus4(SubCube& s, const TOp* __restrict__ op, double2* __restrict__ tile,
+                                          int pl, int o0, int o1, int o2, int o3) {
+    if (kraus_q(s, op) != 0) return;
+    auto slot = [&](int j) {
+        const int o = ((j & 1) ? o0 : 0) + ((j & 2) ? o1 : 0) + ((j & 4) ? o2 : 0) + ((j & 8) ? o3 : 0);
+        return pl + phys(o);
+    };
+    sfor<0, 16>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        tile[slot(j)] = to_d2(getv<j>(s.v));
+    });
+    sfor<0, 16>([&](auto J) {
+        constexpr int r = decltype(J)::value;
+        c2 acc = { 0.0, 0.0 };
+        for (int c = 0; c < 16; ++c) acc = cfma(to_c2(op[c].m[r]), to_c2(tile[slot(c)]), acc);
+        setv<r>(s.v, acc);
+    });
+}
+
+// Apply one tile op; returns the number of TOp entries it occupies.  KRAUS:
+// the plan may hold reference Kraus ops (density-matrix plans only; the
+// state-vector instantiation carries no code for them).
+template <bool KRAUS>
+__device__ __forceinline__ int run_op(SubCube& s, const TOp* __restrict__ op, double2* __restrict__ tile, int pl,
+                                      int o0, int o1, int o2, int o3) {
     const OpHead hd = load_head(op);
+    if (KRAUS && hd.kind == 4) {
+        if (hd.nd) {
+            sw_kraus4(s, op, tile, pl, o0, o1, o2, o3);
+            return 1 + hd.nd;
+        }
+        switch (hd.a * 4 + hd.b) {
+            case 1: sw_kraus2<0, 1>(s, op, hd); break;
+            case 2: sw_kraus2<0, 2>(s, op, hd); break;
+            case 3: sw_kraus2<0, 3>(s, op, hd); break;
+            case 6: sw_kraus2<1, 2>(s, op, hd); break;
+            case 7: sw_kraus2<1, 3>(s, op, hd); break;
+            default: sw_kraus2<2, 3>(s, op, hd); break;
+        }
+        return 1;
+    }
     switch (hd.kind * 16 + hd.a * 4 + hd.b) {
         case 48: sw_sign(s, op); break;
         case 0: sw_dense1<0>(s, op, hd); break;
@@ -243,11 +325,13 @@ __device__ __forceinline__ void run_op(SubCube& s, const TOp* __restrict__ op) {
         case 16 + 11: sw_dense2<2, 3>(s, op, hd); break;
         default: sw_diag(s, op); break;
     }
+    return 1;
 }
 
 // One tile of one pass.  `init`: the state is |0...0> and is not read (the
 // tile is built in LDS).  LDS: tile (2^T + 2^T/16 amplitudes), scat_lo[256],
 // scat_hi[64].
+template <bool KRAUS>
 __device__ __forceinline__ void tile_body(double2* __restrict__ sv, int vq, const Pass* __restrict__ pass,
                                           const Sweep* __restrict__ sweeps, const TOp* __restrict__ tops,
                                           uint64_t tile_index, bool init, double2* __restrict__ tile,
@@ -324,7 +408,7 @@ __device__ __forceinline__ void tile_body(double2* __restrict__ sv, int vq, cons
                 const int o = ((j & 1) ? o0 : 0) + ((j & 2) ? o1 : 0) + ((j & 4) ? o2 : 0) + ((j & 8) ? o3 : 0);
                 setv<j>(s.v, to_c2(tile[pl + phys(o)]));
             });
-            for (int o = ob; o < oe; ++o) run_op(s, tops + o);
+            for (int o = ob; o < oe;) o += run_op<KRAUS>(s, tops + o, tile, pl, o0, o1, o2, o3);
             sfor<0, 16>([&](auto J) {
                 constexpr int j = decltype(J)::value;
                 const int o = ((j & 1) ? o0 : 0) + ((j & 2) ? o1 : 0) + ((j & 4) ? o2 : 0) + ((j & 8) ? o3 : 0);

@@ -10,13 +10,14 @@
 #include "fused.h"
 using namespace bsim;
 typedef std::complex<double> cd;
-extern "C" int planexec(int nq, const bsim_op* ops, int n, double* state) {
+static int run(int nq, int kind, const bsim_op* ops, int n, double* state) {
     std::vector<EOp> e;
-    if (int rc = lower_ops(nq, BSIM_SV, ops, n, e, false)) return rc;
+    if (int rc = lower_ops(nq, kind, ops, n, e, false)) return rc;
+    const int vq = kind == BSIM_SV ? nq : 2 * nq;
     Plan p;
-    if (int rc = make_plan(nq, e, tile_config(), p)) return rc;
+    if (int rc = make_plan(vq, e, tile_config(), p)) return rc;
     cd* v = (cd*)state;
-    const uint64_t N = 1ull << nq;
+    const uint64_t N = 1ull << vq;
     for (auto& st : p.steps) {
         if (!st.tile) { printf("non-tile step\n"); return -99; }
         const Pass& P = p.passes[st.pass];
@@ -47,13 +48,48 @@ extern "C" int planexec(int nq, const bsim_op* ops, int n, double* state) {
                         const bool direct = (i & op.cmask) == op.cval;
                         const bool split = ((gbase & op.tmask) == op.tval) && ((op.jmask >> j) & 1);
                         if (direct != split) return -11;
+                    } else if (op.kind == 4) {
+                        const bool direct = ((i ^ (i >> op.pd[0])) & op.pm[0]) == 0;
+                        uint64_t u = 0;
+                        for (int r = 0; r < 4; ++r)
+                            if ((j >> r) & 1) u ^= op.sgm[r];
+                        const bool split = ((gbase ^ (gbase >> op.pd[0])) & op.pm[0]) == u;
+                        if (direct != split) return -12;
                     }
                 }
             }
             for (int o = S.op_begin; o < S.op_end; ++o) {
                 const TOp& op = p.tops[o];
                 auto M = [&](int j) { return cd(op.m[j].x, op.m[j].y); };
-                if (op.kind == 0) {
+                auto pred = [&](uint64_t i) { return ((i ^ (i >> op.pd[0])) & op.pm[0]) == 0; };
+                if (op.kind == 5) continue;  // matrix columns of the preceding kind-4 op
+                if (op.kind == 4 && op.nd == 0) {
+                    uint64_t ba = 1ull << vb[op.a], bb = 1ull << vb[op.b];
+                    for (uint64_t i = 0; i < N; ++i) if (!(i & ba) && !(i & bb) && pred(i)) {
+                        uint64_t id[4] = {i, i | ba, i | bb, i | ba | bb};
+                        cd x[4], y[4];
+                        for (int k = 0; k < 4; ++k) x[k] = v[id[k]];
+                        for (int r = 0; r < 4; ++r) { y[r] = 0; for (int c = 0; c < 4; ++c) y[r] += M(r + 4 * c) * x[c]; }
+                        for (int k = 0; k < 4; ++k) v[id[k]] = y[k];
+                    }
+                } else if (op.kind == 4) {
+                    for (uint64_t i = 0; i < N; ++i) {
+                        if (i & G) continue;
+                        if (!pred(i)) continue;
+                        uint64_t id[16];
+                        for (int j = 0; j < 16; ++j) {
+                            id[j] = i;
+                            for (int r = 0; r < 4; ++r) if ((j >> r) & 1) id[j] |= 1ull << vb[r];
+                        }
+                        cd x[16], y[16];
+                        for (int k = 0; k < 16; ++k) x[k] = v[id[k]];
+                        for (int r = 0; r < 16; ++r) {
+                            y[r] = 0;
+                            for (int c = 0; c < 16; ++c) y[r] += cd(p.tops[o + c].m[r].x, p.tops[o + c].m[r].y) * x[c];
+                        }
+                        for (int k = 0; k < 16; ++k) v[id[k]] = y[k];
+                    }
+                } else if (op.kind == 0) {
                     uint64_t bit = 1ull << vb[op.a];
                     for (uint64_t i = 0; i < N; ++i) if (!(i & bit)) {
                         if (op.cmask && ((i & op.cmask) != op.cval)) continue;
@@ -88,4 +124,18 @@ extern "C" int planexec(int nq, const bsim_op* ops, int n, double* state) {
         }
     }
     return 0;
+}
+
+extern "C" int planexec(int nq, const bsim_op* ops, int n, double* state) { return run(nq, BSIM_SV, ops, n, state); }
+// density matrix: state is the stored 4^n vector (column-major)
+extern "C" int planexec_dm(int nq, const bsim_op* ops, int n, double* state) { return run(nq, BSIM_DM, ops, n, state); }
+// number of plan steps that are not tile passes
+extern "C" int plan_singles(int nq, int kind, const bsim_op* ops, int n) {
+    std::vector<EOp> e;
+    if (int rc = lower_ops(nq, kind, ops, n, e, false)) return rc;
+    Plan p;
+    if (int rc = make_plan(kind == BSIM_SV ? nq : 2 * nq, e, tile_config(), p)) return rc;
+    int k = 0;
+    for (auto& st : p.steps) k += st.tile ? 0 : 1;
+    return k;
 }

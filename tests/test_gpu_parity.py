@@ -135,6 +135,26 @@ def test_dm_random_vs_oracle(noise_mode, n, seed, mode):
     np.testing.assert_allclose(sim.probabilities(), O.dm_probabilities(exp, n), atol=TOL)
 
 
+@pytest.mark.parametrize("n,depth", [(6, 4), (9, 3)])
+def test_dm_reference_kraus_fused_into_tiles(n, depth):
+    """Reference-semantics 1- and 2-qubit Kraus channels run inside the
+    fused tile passes (no separate Kraus launches) and still reproduce the
+    reference's block-diagonal update."""
+    _say(f"dm noisy brickwall n={n} depth={depth} reference Kraus in tiles")
+    ins = W.brickwall(n, depth, seed=n, noise=True)
+    ol = bsim.lower_instructions(ins, L.BSIM_DM, "reference")
+    st = (ctypes.c_int64 * 4)()
+    L.call("bsim_plan_stats", 2 * n, 1, ol.ptr, ol.n, st)
+    assert st[1] == 0
+    exp = O.evolve_dm(n, ins, mode="reference")
+    sim = DensityMatrixSimulator(n, 0)
+    sim.evolve(ins)
+    assert np.max(np.abs(sim.density_matrix().ravel(order="F") - exp)) < TOL
+    per_op = DensityMatrixSimulator(n, 0)
+    per_op.evolve(ins, mode=1)  # the separate block kernel
+    assert np.max(np.abs(per_op.density_matrix().ravel(order="F") - exp)) < TOL
+
+
 def test_dm_apply_gate_and_noise_api():
     n = 5
     rng = np.random.default_rng(77)

@@ -40,8 +40,8 @@ def keep_tile_config():
     L.call("bsim_set_tile_config", t.value, lo.value)
 
 
-def _plan_singles(n, kind, ins):
-    ol = lower_instructions(ins, kind, "physical")
+def _plan_singles(n, kind, ins, mode="physical"):
+    ol = lower_instructions(ins, kind, mode)
     st = (ctypes.c_int64 * 4)()
     vq = n if kind == L.BSIM_SV else 2 * n
     L.call("bsim_plan_stats", vq, int(kind == L.BSIM_DM), ol.ptr, ol.n, st)
@@ -109,6 +109,21 @@ def test_graph_density_matrix_with_noise():
             sim.apply_gate(G.X(), 1)
 
 
+def test_graph_density_matrix_reference_noise():
+    n = 6
+    ins = W.brickwall(n, 3, seed=8, noise=True)  # reference Kraus ops inside the tile passes
+    assert _plan_singles(n, L.BSIM_DM, ins, "reference") == 0
+    _say("graph dm noisy reference")
+    eager = DensityMatrixSimulator(n, 0).evolve(ins).density_matrix()
+    np.testing.assert_allclose(eager.ravel(order="F"), O.evolve_dm(n, ins, "reference"), atol=1e-12)
+    sim = DensityMatrixSimulator(n, 0)
+    with sim.capture(ins) as g:
+        for rep in range(3):
+            g.launch()
+            assert np.array_equal(sim.density_matrix(), eager), f"replay {rep}"
+            sim.apply_noise(Nz.Depolarizing(0.2), 3)
+
+
 def test_graph_rejects_unfusable_ops():
     n = 8
     sim = StateVectorSimulator(n, 0)
@@ -116,9 +131,9 @@ def test_graph_rejects_unfusable_ops():
     with pytest.raises(BsimError, match="fusable"):
         sim.capture([Instruction(G.H(), [0]),
                      Instruction(G.Unitary(random_unitary(3, np.random.default_rng(1))), [0, 1, 2])])
-    dm = DensityMatrixSimulator(4, 0)  # reference-mode Kraus runs in its own kernel
+    dm = DensityMatrixSimulator(4, 0)  # a 3-qubit reference-mode Kraus channel runs in its own kernel
     with pytest.raises(BsimError, match="fusable"):
-        dm.capture([Instruction(Nz.Depolarizing(0.1), [0])])
+        dm.capture([Instruction(Nz.Kraus([np.eye(8)]), [0, 1, 3])])
     # the handle still works after a refused capture
     sim.evolve([Instruction(G.H(), [0])])
     assert abs(sim.probabilities()[0] - 0.5) < 1e-14

@@ -10,6 +10,7 @@ from conftest import ROOT
 import bsim
 from bsim import _lib as L
 from bsim import gates as G
+from bsim import noises as Nz
 from bsim.gates import kernel_call
 from bsim.simulators import Instruction, lower_instructions
 from bsim import workloads as W
@@ -115,7 +116,13 @@ def test_plan_stats_dm_and_kq():
     ol = lower_instructions(ins, L.BSIM_DM)
     stats = (ctypes.c_int64 * 4)()
     L.call("bsim_plan_stats", 12, 1, ol.ptr, ol.n, stats)
-    assert stats[1] > 0  # reference-semantics Kraus ops run as their own launches
+    assert stats[1] == 0  # 1- and 2-qubit reference Kraus channels run inside the tile passes
+    assert stats[3] == ol.n + sum(1 for i in ins if isinstance(i.operator, G.Gate))  # DM gates lower to 2 ops
+    # a 3-qubit reference Kraus channel keeps its own kernel
+    k3 = Instruction(Nz.Kraus([np.eye(8)]), [0, 2, 4])
+    ol3 = lower_instructions(ins[:5] + [k3], L.BSIM_DM)
+    L.call("bsim_plan_stats", 12, 1, ol3.ptr, ol3.n, stats)
+    assert stats[1] == 1
 
 
 def test_bad_op_rejected_without_gpu():
