@@ -121,6 +121,7 @@ _SIGS = {
     "bsim_graph_launch": (C.c_int, [C.c_void_p]),
     "bsim_graph_destroy": (C.c_int, [C.c_void_p]),
     "bsim_autotune": (C.c_int, [C.c_void_p, C.POINTER(Op), C.c_int, _ip, _ip, _dp]),
+    "bsim_partial_trace": (C.c_int, [C.c_void_p, C.c_int, _ip, _dp]),
 }
 
 _lib = None

@@ -60,6 +60,23 @@ def permute_probability(probs: np.ndarray, n: int, targets) -> np.ndarray:
     return out
 
 
+def permute_density_matrix(rho: np.ndarray, n: int, targets) -> np.ndarray:
+    """src/result_types.jl:129-149: new bit q of an index = the bit of
+    qubit targets[q]."""
+    t = list(targets)
+    D = 2 ** len(t)
+
+    def perm(ix):
+        unperm = [(ix >> q) & 1 for q in range(n)]
+        return sum(unperm[n - tt - 1] << q for q, tt in enumerate(t))
+
+    p = [perm(ix) for ix in range(D)]
+    out = np.zeros((D, D), dtype=rho.dtype)
+    for ix in range(D):
+        out[p[ix], p] = rho[ix, :]
+    return out
+
+
 def calculate(rt, sim):
     """calculate(result_type, simulator)."""
     n = sim.qubit_count
@@ -84,11 +101,13 @@ def calculate(rt, sim):
         return [sim.expectation_op_squared(rt.observable, q) - sim.expectation(rt.observable, q) ** 2 for q in t]
     if isinstance(rt, StateVector):
         return sim.state_vector()
-    if isinstance(rt, DensityMatrix):
+    if isinstance(rt, DensityMatrix):  # src/result_types.jl:240-246
         t = list(rt.targets or [])
-        if t and t != list(range(n)):
-            raise NotImplementedError("partial traces are outside the GPU path")
-        return sim.density_matrix()
+        if not t or t == list(range(n)):
+            return sim.density_matrix()
+        if len(t) == n:
+            return permute_density_matrix(sim.density_matrix(), n, t)
+        return sim.partial_trace(t)
     if isinstance(rt, Amplitude):
         sv = sim.state_vector()
         return {s: sv[int(s, 2)] for s in rt.states}

@@ -182,6 +182,22 @@ def _kron_to_ascending(m: np.ndarray, targets) -> np.ndarray:
     return t.transpose(perm).reshape(2 ** k, 2 ** k)
 
 
+def _jl_shl(x: int, s: int) -> int:
+    """Julia's x << s (a negative s shifts right)."""
+    return x << s if s >= 0 else x >> -s
+
+
+def _swap_bits(ix: int, qubit_map: dict) -> int:
+    """swap_bits (src/dm_simulator.jl:264-278), Julia shift semantics kept."""
+    for in_q, out_q in qubit_map.items():
+        if in_q < out_q:
+            in_val = _jl_shl(_jl_shl(1, in_q) & ix, -in_q)
+            out_val = _jl_shl(_jl_shl(1, out_q) & ix, -out_q)
+            if in_val != out_val:
+                ix = ix ^ _jl_shl(1, in_q) ^ _jl_shl(1, out_q)
+    return ix
+
+
 def _seed() -> int:
     return int.from_bytes(os.urandom(8), "little")
 
@@ -248,6 +264,47 @@ class _Simulator:
         ol = lower_instructions(list(operations), self.kind, getattr(self, "noise_mode", "reference"))
         L.call("bsim_evolve", self._h, ol.ptr, ol.n, int(mode))
         return self
+
+    def partial_trace(self, output_qubits=None):
+        """partial_trace(sim, output_qubits) (src/sv_simulator.jl:272-316,
+        src/dm_simulator.jl:280-323), with the reference's special cases:
+        no outputs -> the norm (SV, real) or trace (DM); every qubit -> the
+        full matrix, unpermuted; unsorted outputs -> the sorted reduction
+        reindexed by the reference's swap_bits rule.  The reduction itself
+        runs on the device (bsim_partial_trace)."""
+        n = self.qubit_count
+        out_q = list(range(n)) if output_qubits is None else [int(q) for q in output_qubits]
+        if not out_q:
+            v = self._reduced([])[0, 0]
+            return float(v.real) if self.kind == L.BSIM_SV else complex(v)
+        if len(set(out_q)) == n:
+            return self.density_matrix()
+        k = len(out_q)
+        red = self._reduced(sorted(out_q))
+        if out_q == sorted(out_q):
+            return red
+        mapping = {k - q - 1: k - m - 1 for m, q in enumerate(out_q)}  # output_qubit_mapping
+        D = 2 ** k
+        out = np.zeros((D, D), dtype=np.complex128)
+        sw = [_swap_bits(i, mapping) for i in range(D)]
+        if self.kind == L.BSIM_SV:  # final_rho[out_ix, out_jx] += traced over ix <= jx
+            for ix in range(D):
+                for jx in range(ix, D):
+                    out[sw[ix], sw[jx]] += red[ix, jx]
+                    if sw[jx] != sw[ix]:
+                        out[sw[jx], sw[ix]] += np.conj(red[ix, jx])
+        else:  # final_rho[out_ix, out_jx] = sum(...) in raw_ix order
+            for raw in range(D * D):
+                ix, jx = raw // D, raw % D
+                out[sw[ix], sw[jx]] = red[ix, jx]
+        return out
+
+    def _reduced(self, targets):
+        k = len(targets)
+        out = np.empty((2 ** k) * (2 ** k), dtype=np.complex128)
+        t = np.array(targets, dtype=np.int32)
+        L.call("bsim_partial_trace", self._h, k, L.iptr(t), L.dptr(out))
+        return out.reshape((2 ** k, 2 ** k), order="F")
 
     def capture(self, operations, reset: bool = True) -> "CapturedEvolution":
         """Record evolve!(operations) -- preceded by the reset to |0...0>

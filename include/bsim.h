@@ -187,6 +187,15 @@ int bsim_probabilities(bsim_state* h, double* out);
  * the target qubits in ascending order, the smallest qubit number being
  * the most significant bit. */
 int bsim_marginal_probabilities(bsim_state* h, int nt, const int* targets, double* out);
+/* partial_trace (src/sv_simulator.jl:272-316, src/dm_simulator.jl:280-323):
+ * the reduced density matrix of nt <= 8 target qubits, out = 4^nt complex
+ * (column-major 2^nt x 2^nt, interleaved).  Output index bit nt-1-b holds
+ * targets[b] (the first target is the most significant).  SV: out[i, j] =
+ * sum_r psi[x(i, r)] conj(psi[x(j, r)]); DM: out[i, j] = sum_r
+ * stored[x(i, r), x(j, r)], the stored matrix traced as the reference does.
+ * nt = 0 gives the norm (SV) / trace (DM) in out[0].  The reference's own
+ * reindexing of unsorted targets is applied above this call. */
+int bsim_partial_trace(bsim_state* h, int nt, const int* targets, double* out);
 /* Expectation of Pauli words (X/Y/Z/I tensor products, src/result_types.jl:
  * 161-167 with src/sv_simulator.jl:240-248, src/dm_simulator.jl:253-260).
  * Word w acts with Pauli paulis[w*n + q] ('I','X','Y','Z') on qubit q.

@@ -256,6 +256,102 @@ def marginal(probs: np.ndarray, n: int, targets) -> np.ndarray:
 
 
 # ---------------------------------------------------------------------------
+# reduced density matrices (pure Python loops: small registers only)
+# ---------------------------------------------------------------------------
+
+def _jl_shl(x: int, s: int) -> int:
+    return x << s if s >= 0 else x >> -s
+
+
+def _pad_bits(ix: int, to_pad) -> int:
+    """pad_bit / pad_bits (src/gate_kernels.jl:34-38, :78-84)."""
+    for bit in to_pad:
+        left = (ix >> bit) << bit
+        ix = (left << 1) ^ (ix - left)
+    return ix
+
+
+def _swap_bits(ix: int, qubit_map) -> int:
+    """swap_bits (src/dm_simulator.jl:264-278); flip_bit is ix ^ (1 << bit)
+    (src/gate_kernels.jl:119-121) with Julia's shift semantics."""
+    for in_q, out_q in qubit_map:
+        if in_q < out_q:
+            in_val = _jl_shl(_jl_shl(1, in_q) & ix, -in_q)
+            out_val = _jl_shl(_jl_shl(1, out_q) & ix, -out_q)
+            if in_val != out_val:
+                ix = ix ^ _jl_shl(1, in_q) ^ _jl_shl(1, out_q)
+    return ix
+
+
+def partial_trace(state: np.ndarray, n: int, output_qubits, dm: bool):
+    """partial_trace for a state vector (src/sv_simulator.jl:272-316) or a
+    density matrix given as the reference stores it (src/dm_simulator.jl:
+    280-323), loop for loop.  ``state``: the 2^n vector, or the 2^n x 2^n
+    matrix."""
+    from itertools import combinations
+
+    out_q = list(output_qubits)
+    if not out_q:
+        return np.trace(state) if dm else float(np.sum(np.abs(state) ** 2))
+    if len(set(out_q)) == n:
+        return state if dm else np.outer(state, np.conj(state))
+    qubits = [q for q in range(n) if q not in out_q]
+    endian = sorted(n - q - 1 for q in qubits)
+    combos = [()] + [c for r in range(1, len(endian) + 1) for c in combinations(endian, r)]
+    D = 2 ** (n - len(qubits))
+    rho = np.zeros((D, D), dtype=np.complex128)
+    k = len(out_q)
+    needs_perm = out_q != sorted(out_q)
+    mapping = list(zip([k - q - 1 for q in out_q], [k - m - 1 for m in range(k)])) if needs_perm else []
+
+    def flips(ix):
+        p = _pad_bits(ix, endian)
+        out = []
+        for c in combos:
+            f = p
+            for b in c:
+                f ^= 1 << b
+            out.append(f)
+        return out
+
+    if not dm:
+        for ix in range(D):
+            for jx in range(ix, D):
+                fi, fj = flips(ix), flips(jx)
+                oi = _swap_bits(ix, mapping) if needs_perm else ix
+                oj = _swap_bits(jx, mapping) if needs_perm else jx
+                traced = complex(np.dot(np.conj(state[fj]), state[fi]))
+                rho[oi, oj] += traced
+                if oj != oi:
+                    rho[oj, oi] += np.conj(traced)
+    else:
+        for raw in range(D * D):
+            ix, jx = raw // D, raw % D
+            fi, fj = flips(ix), flips(jx)
+            oi = _swap_bits(ix, mapping) if needs_perm else ix
+            oj = _swap_bits(jx, mapping) if needs_perm else jx
+            rho[oi, oj] = sum(state[a, b] for a, b in zip(fi, fj))
+    return rho
+
+
+def permute_density_matrix(rho: np.ndarray, n: int, targets) -> np.ndarray:
+    """src/result_types.jl:129-149."""
+    t = list(targets)
+    D = 2 ** len(t)
+    out = np.zeros((D, D), dtype=rho.dtype)
+
+    def perm(ix):
+        bits = [(ix >> q) & 1 for q in range(n)]
+        permed = [bits[n - tt - 1] for tt in t]
+        return sum(b << q for q, b in enumerate(permed))
+
+    for ix in range(D):
+        for jx in range(D):
+            out[perm(ix), perm(jx)] = rho[ix, jx]
+    return out
+
+
+# ---------------------------------------------------------------------------
 # sampling checkers
 # ---------------------------------------------------------------------------
 

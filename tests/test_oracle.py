@@ -342,3 +342,51 @@ def test_cdf_shots_frequencies():
     f = np.bincount(s, minlength=16) / len(s)
     assert f[3] == 0 and f[9] == 0
     assert (np.abs(f - p) < 5 * np.sqrt(p * (1 - p) / len(s)) + 1e-12).all()
+
+
+# ---- reduced density matrices (partial_trace) --------------------------------
+
+def _golden_dm():
+    import json
+    d = json.loads((GOLDEN / "density_matrix_results.json").read_text())
+    cm = lambda rows: np.array([[complex(*x) for x in r] for r in rows])
+    k = np.arange(16)
+    psi = np.sqrt(k / 120.0) * np.where(k < 8, 1.0, 1j)
+    return d, psi, cm(d["density_matrix_3"]), cm(d["density_matrix_0_2_1"])
+
+
+def test_partial_trace_golden():
+    """DensityMatrix(3) and DensityMatrix([0, 2, 1]) of the reference's
+    test state (test/test_result_types.jl:44-60, rtol 1e-6); the second one
+    takes the unsorted-target reindexing."""
+    d, psi, m3, m021 = _golden_dm()
+    np.testing.assert_allclose(O.partial_trace(psi, 4, [3], dm=False), m3, rtol=d["rtol"], atol=1e-8)
+    np.testing.assert_allclose(O.partial_trace(psi, 4, [0, 2, 1], dm=False), m021, rtol=d["rtol"], atol=1e-8)
+    full = np.outer(psi, np.conj(psi))
+    # DensityMatrix(reverse(0:3)) == kron(adjoint(sv), sv) (test/test_result_types.jl:212-219)
+    np.testing.assert_allclose(O.permute_density_matrix(full, 4, [3, 2, 1, 0]), full, atol=1e-15)
+
+
+@pytest.mark.parametrize("n", [3, 4, 5])
+def test_partial_trace_sv_equals_dm(n):
+    """test/test_sv_simulator.jl:654-662: the state-vector and density-matrix
+    traces agree for every sorted output set; plus closed forms: the norm for
+    no outputs, the full projector for all of them, and a product state's
+    reduced matrix is its factor."""
+    from itertools import combinations
+
+    rng = np.random.default_rng(n)
+    psi = rng.normal(size=2 ** n) + 1j * rng.normal(size=2 ** n)
+    psi /= np.linalg.norm(psi)
+    full = np.outer(psi, np.conj(psi))
+    for r in range(1, n + 1):
+        for q in combinations(range(n), r):
+            np.testing.assert_allclose(O.partial_trace(psi, n, list(q), dm=False),
+                                       O.partial_trace(full, n, list(q), dm=True), atol=1e-13)
+    assert abs(O.partial_trace(psi, n, [], dm=False) - 1.0) < 1e-13
+    a = rng.normal(size=2) + 1j * rng.normal(size=2)
+    a /= np.linalg.norm(a)
+    b = rng.normal(size=2 ** (n - 1)) + 1j * rng.normal(size=2 ** (n - 1))
+    b /= np.linalg.norm(b)
+    prod = np.kron(a, b)  # qubit 0 is the factor a
+    np.testing.assert_allclose(O.partial_trace(prod, n, [0], dm=False), np.outer(a, np.conj(a)), atol=1e-13)
