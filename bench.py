@@ -82,6 +82,7 @@ def main():
     ap.add_argument("--batch", type=int, default=4096, help="circuits in the batch leg (0 skips it)")
     ap.add_argument("--dm-qubits", type=int, default=14, help="noisy density-matrix leg (0 skips it)")
     ap.add_argument("--graph-reps", type=int, default=500, help="captured-evolution replays (0 skips the leg)")
+    ap.add_argument("--per-gate", type=int, default=1, help="1: time the per-gate launch path on one layer")
     ap.add_argument("--sweep-sets", type=int, default=256, help="16-qubit ansatz parameter sets with gradients")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="profiling aid: one fused evolve of the workload and one 8 GiB device copy, no JSON")
@@ -195,6 +196,8 @@ def main():
         "z_checksum": float(np.sum(z)),
     }
     del sim
+    if a.per_gate:
+        rec["per_gate"] = per_gate_leg(n, ins)
     if a.adjoint_qubits > 0:
         rec["adjoint"] = adjoint_leg(a.adjoint_qubits, a.ps_sample)
     if a.batch > 0:
@@ -317,6 +320,37 @@ def dm_leg(nq: int):
                                   f"{len(hins)} ops, TFIM H, checkpoint every ceil(sqrt(ops))",
                       "s": round(t_adj, 4), "grads_per_s": round(1.0 / t_adj, 3), "expval": ev}
     log(f"density matrix adjoint {n12}q: {t_adj:.3f} s")
+    return out
+
+
+def per_gate_leg(n: int, ins):
+    """The per-gate launch path (bsim_evolve mode 1, the reference's loop
+    shape) on the workload's first layer: its 30 U gates (each one HBM pass,
+    32 B x 2^n) and its CZ gates (a diagonal over the control-set half,
+    16 B x 2^n), timed apart."""
+    from bsim import _lib as L
+    from bsim import gates as G
+    from bsim.simulators import StateVectorSimulator, lower_instructions
+
+    layer_u = [i for i in ins[: 2 * n] if isinstance(i.operator, G.U)][:n]
+    layer_cz = [i for i in ins[n: 2 * n] if isinstance(i.operator, G.CZ)]
+    sim = StateVectorSimulator(n, 0)
+    out = {"workload": f"first brick-wall layer at {n} qubits, one launch per gate"}
+    for name, part, unit_bytes in (("u", layer_u, 32.0 * 2 ** n), ("cz", layer_cz, 16.0 * 2 ** n)):
+        ol = lower_instructions(part, L.BSIM_SV)
+        L.call("bsim_evolve", sim.handle, ol.ptr, ol.n, 1)  # warm-up
+        L.call("bsim_synchronize", sim.handle)
+        t = time.perf_counter()
+        for _ in range(2):
+            L.call("bsim_evolve", sim.handle, ol.ptr, ol.n, 1)
+        L.call("bsim_synchronize", sim.handle)
+        per = (time.perf_counter() - t) / (2 * len(part))
+        gbs = unit_bytes / per / 1e9
+        out[name] = {"gates": len(part), "ms_per_gate": round(per * 1e3, 3), "pass_GBps": round(gbs, 1),
+                     "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4)}
+        log(f"per-gate {name}: {per * 1e3:.2f} ms per gate, {gbs:.0f} GB/s")
+    out["note"] = "pass_GBps: algorithmic bytes per gate (U: every amplitude read and written once; CZ: the control-set half)"
+    del sim
     return out
 
 

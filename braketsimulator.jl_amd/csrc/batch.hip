@@ -38,6 +38,9 @@
 
 namespace bsim {
 
+constexpr int TB = tile_threads<4>();  // batch workgroup size
+
+
 constexpr int CHUNK_BITS = 12;  // F1 chunk: 4096 amplitudes
 constexpr int SEG_BITS = 6;     // CDF segment: 64 amplitudes
 constexpr int ZP = 1 + CHUNK_BITS;           // per-chunk Z partials: total + one per chunk bit
@@ -271,7 +274,7 @@ __global__ __launch_bounds__(TB, 2) void k_batch(const BTask* __restrict__ tasks
             if (E.kind >= 0) {
                 const uint64_t first = (uint64_t)local * (uint64_t)E.chunk;
                 for (int c = 0; c < E.chunk; ++c) {
-                    tile_body<false>(K.sv, K.n, passes + K.pass0 + E.kind, sweeps, tops, first + c, E.kind == 0, tile,
+                    tile_body<4, false>(K.sv, K.n, passes + K.pass0 + E.kind, sweeps, tops, first + c, E.kind == 0, tile,
                               scat_lo, scat_hi);
                     __syncthreads();  // the next tile rewrites the scatter tables
                 }
@@ -308,7 +311,9 @@ static void plan_task(const bsim_task& t, TaskPlan& out) {
         out.msg = last_error_text();
         return;
     }
-    if ((out.rc = make_plan(t.qubit_count, eops, tile_config(), out.plan)) != BSIM_OK) {
+    TileConfig cfg = tile_config();
+    cfg.reg_bits = 4;  // the batch kernel runs 256-thread, 16-amplitude sweeps
+    if ((out.rc = make_plan(t.qubit_count, eops, cfg, out.plan)) != BSIM_OK) {
         out.msg = last_error_text();
         return;
     }
@@ -322,6 +327,7 @@ static void plan_task(const bsim_task& t, TaskPlan& out) {
         Pass P;
         memset(&P, 0, sizeof(P));
         P.T = std::min(tile_config().tile_bits, t.qubit_count);
+        P.R = 4;
         for (int b = 0; b < P.T; ++b) {
             P.lbits[b] = b;
             P.lmask |= 1ull << b;

@@ -73,7 +73,7 @@ struct TOp {
 };
 
 struct Sweep {
-    int32_t sb[4];     // local bit indices of the 4 register bits, ascending
+    int32_t sb[4];     // local bit indices of the pass's R register bits, ascending
     int32_t op_begin, op_end;
 };
 
@@ -81,6 +81,7 @@ struct Pass {
     int32_t T;                // tile bits
     int32_t nsweeps;
     int32_t sweep_begin;
+    int32_t R;                // register bits per sweep (3 or 4)
     int32_t lbits[16];        // vector bit of local bit i (ascending)
     uint64_t lmask;
 };
@@ -104,6 +105,11 @@ struct Plan {
 struct TileConfig {
     int tile_bits = 12;
     int low_bits = 3;  // 8 contiguous amplitudes (128 B) per run: fewer passes than 4, same coalescing
+    // register bits per sweep: 4 (16-amplitude sub-cubes, 256 threads, two
+    // waves per SIMD) or 3 (8 amplitudes, 512 threads, four waves: the same
+    // speed on the 30-qubit brick-wall, more sweeps); a pass with a 4-bit op
+    // or a reference Kraus op always uses 4
+    int reg_bits = 4;
     // FP64 work budget of one pass, in operations per amplitude (dense 1q
     // = 8, dense 2q = 32, diagonal = 4, sign = 1); 0 = unlimited.  Bounds
     // the fusion width so that a pass stays HBM-bound.

@@ -523,7 +523,15 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
                 pos[b] = l++;
             }
         P.sweep_begin = (int)plan.sweeps.size();
-        // ---- sweeps: groups of ops whose dense targets fit 4 register bits
+        // register bits per sweep: 4 when the pass holds a 4-bit op or a
+        // reference Kraus op (the Kraus tile ops exist for 4 register bits only)
+        int RB = cfg.reg_bits == 3 ? 3 : 4;
+        for (int i : inc)
+            if (eops[i].kind == E_KRAUS_REF || (eops[i].kind == E_DENSE && eops[i].g.k > 2)) RB = 4;
+        if (T < RB + 2) RB = 4;  // tiny tiles: the 4-bit layout covers them
+        P.R = RB;
+        const int NRJ = 1 << RB;
+        // ---- sweeps: groups of ops whose dense targets fit RB register bits
         std::vector<int> rem = inc;
         std::vector<int> next;
         while (!rem.empty()) {
@@ -542,7 +550,7 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
                 if (e.kind == E_DENSE || e.kind == E_KRAUS_REF) {
                     uint32_t need = 0;
                     for (int b = 0; b < e.g.k; ++b) need |= 1u << pos[e.g.tb[b]];
-                    if (popc(W | need) <= 4) {
+                    if (popc(W | need) <= RB) {
                         W |= need;
                         take.push_back(i);
                     } else {
@@ -553,8 +561,9 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
                     take.push_back(i);
                 }
             }
-            for (int b = 0; b < T && popc(W) < 4; ++b) W |= 1u << b;
+            for (int b = 0; b < T && popc(W) < RB; ++b) W |= 1u << b;
             Sweep sw;
+            memset(&sw, 0, sizeof(sw));
             int nb = 0;
             for (int b = 0; b < T; ++b)
                 if (W & (1u << b)) sw.sb[nb++] = b;
@@ -562,7 +571,7 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
             auto reg = [&](int vbit) {
                 const int l = pos[vbit];
                 if (l < 0) return -1;
-                for (int r = 0; r < 4; ++r)
+                for (int r = 0; r < RB; ++r)
                     if (sw.sb[r] == l) return r;
                 return -1;
             };
@@ -671,8 +680,8 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
             fuse_sweep(eops, take, reg, emit_one, plan);
             sw.op_end = (int)plan.tops.size();
             // index tests split into a per-thread part and a part fixed by the register
-            uint64_t gb[4], G = 0;
-            for (int r = 0; r < 4; ++r) {
+            uint64_t gb[4] = { 0, 0, 0, 0 }, G = 0;
+            for (int r = 0; r < RB; ++r) {
                 gb[r] = 1ull << P.lbits[sw.sb[r]];
                 G |= gb[r];
             }
@@ -689,7 +698,7 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
                     for (int r = 0; r < 4; ++r) op.sgm[r] = (gb[r] ^ (gb[r] >> op.pd[0])) & op.pm[0];
                 } else if (op.kind == 3) {
                     uint32_t q = 0;
-                    for (int j = 0; j < 16; ++j) {
+                    for (int j = 0; j < NRJ; ++j) {
                         const uint64_t x = rj(j);
                         int par = 0;
                         for (int k = 0; k < op.npairs; ++k) par ^= popc(x & (x >> op.pd[k]) & op.pm[k]) & 1;
@@ -705,7 +714,7 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
                     op.tmask = op.cmask & ~G;
                     op.tval = op.cval & ~G;
                     uint32_t q = 0;
-                    for (int j = 0; j < 16; ++j) q |= (uint32_t)((rj(j) & op.cmask) == (op.cval & G)) << j;
+                    for (int j = 0; j < NRJ; ++j) q |= (uint32_t)((rj(j) & op.cmask) == (op.cval & G)) << j;
                     op.jmask = q;
                 }
             }
@@ -727,23 +736,30 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
 
 namespace bsim {
 
-// One launch = one pass: block b runs tile b.  k_tile_kraus also runs the
-// reference Kraus ops of density-matrix plans.
-template <bool KRAUS>
+// One launch = one pass: block b runs tile b.  k_tile: 4 register bits per
+// sweep (256 threads); k_tile_r3: 3 (512 threads, at most 128 VGPRs so that
+// four waves fit on a SIMD); k_tile_kraus: 4, with the reference Kraus ops of
+// density-matrix plans.
+template <int R, bool KRAUS>
 __device__ __forceinline__ void tile_pass(double2* __restrict__ sv, int vq, const Pass* __restrict__ pass,
                                           const Sweep* __restrict__ sweeps, const TOp* __restrict__ tops) {
     extern __shared__ double2 tile[];
-    __shared__ uint64_t scat_lo[256];   // scatter of local bits 0..7
-    __shared__ uint64_t scat_hi[64];    // scatter of local bits 8..13
-    tile_body<KRAUS>(sv, vq, pass, sweeps, tops, blockIdx.x, false, tile, scat_lo, scat_hi);
+    __shared__ uint64_t scat_lo[tile_threads<R>()];  // scatter of the low log2(TB) local bits
+    __shared__ uint64_t scat_hi[64];                 // scatter of the others
+    tile_body<R, KRAUS>(sv, vq, pass, sweeps, tops, blockIdx.x, false, tile, scat_lo, scat_hi);
 }
-__global__ __launch_bounds__(TB) void k_tile(double2* __restrict__ sv, int vq, const Pass* __restrict__ pass,
-                                             const Sweep* __restrict__ sweeps, const TOp* __restrict__ tops) {
-    tile_pass<false>(sv, vq, pass, sweeps, tops);
+__global__ __launch_bounds__(256) void k_tile(double2* __restrict__ sv, int vq, const Pass* __restrict__ pass,
+                                              const Sweep* __restrict__ sweeps, const TOp* __restrict__ tops) {
+    tile_pass<4, false>(sv, vq, pass, sweeps, tops);
 }
-__global__ __launch_bounds__(TB) void k_tile_kraus(double2* __restrict__ sv, int vq, const Pass* __restrict__ pass,
-                                                   const Sweep* __restrict__ sweeps, const TOp* __restrict__ tops) {
-    tile_pass<true>(sv, vq, pass, sweeps, tops);
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_tile_r3(
+    double2* __restrict__ sv, int vq, const Pass* __restrict__ pass, const Sweep* __restrict__ sweeps,
+    const TOp* __restrict__ tops) {
+    tile_pass<3, false>(sv, vq, pass, sweeps, tops);
+}
+__global__ __launch_bounds__(256) void k_tile_kraus(double2* __restrict__ sv, int vq, const Pass* __restrict__ pass,
+                                                    const Sweep* __restrict__ sweeps, const TOp* __restrict__ tops) {
+    tile_pass<4, true>(sv, vq, pass, sweeps, tops);
 }
 
 }  // namespace bsim
@@ -829,8 +845,10 @@ int launch_plan(bsim_state* h, double2* d, const std::vector<EOp>& eops, const P
             hipEvent_t e0, e1;
             if (timed && prof().on)
                 if (int rc = prof_mark(h->stream, &e0)) return rc;
-            hipLaunchKernelGGL(plan.has_kraus ? k_tile_kraus : k_tile, dim3(grid), dim3(TB), shmem, h->stream, d, h->vq,
-                               (const Pass*)base + st.pass, (const Sweep*)(base + off_s), (const TOp*)(base + off_o));
+            const bool r3 = P.R == 3;
+            hipLaunchKernelGGL(r3 ? k_tile_r3 : (plan.has_kraus ? k_tile_kraus : k_tile), dim3(grid), dim3(r3 ? 512 : 256),
+                               shmem, h->stream, d, h->vq, (const Pass*)base + st.pass, (const Sweep*)(base + off_s),
+                               (const TOp*)(base + off_o));
             BSIM_HIP(hipGetLastError());
             if (timed && prof().on)
                 if (int rc = prof_mark(h->stream, &e1)) return rc;
@@ -855,7 +873,7 @@ int prepare_tile_kernel() {
     if (g_shmem_set) return BSIM_OK;
     // allow > 64 KiB of dynamic LDS: a padded 13-bit tile (the kernel's
     // static scatter tables take another 2.5 KiB of the 160 KiB)
-    for (const void* k : { (const void*)k_tile, (const void*)k_tile_kraus })
+    for (const void* k : { (const void*)k_tile, (const void*)k_tile_r3, (const void*)k_tile_kraus })
         BSIM_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)(sizeof(double2) * ((1 << 13) + (1 << 9)))));
     g_shmem_set = true;
@@ -885,6 +903,13 @@ int bsim_set_tile_config(int tile_bits, int low_bits) {
 int bsim_get_tile_config(int* tile_bits, int* low_bits) {
     if (tile_bits) *tile_bits = tile_config().tile_bits;
     if (low_bits) *low_bits = tile_config().low_bits;
+    return BSIM_OK;
+}
+
+int bsim_set_sweep_bits(int bits, int* old) {
+    if (bits != 0 && bits != 3 && bits != 4) return fail(BSIM_EINVAL, "sweep register bits must be 3 or 4");
+    if (old) *old = tile_config().reg_bits;
+    if (bits) tile_config().reg_bits = bits;
     return BSIM_OK;
 }
 

@@ -10,8 +10,9 @@
 // host copies.
 //
 // The autotuner times the fused evolution of one instruction list for every
-// candidate tile width (tile bits) and contiguous-low-bit count on the
-// handle's own state, and keeps the fastest configuration.
+// candidate tile width (tile bits), contiguous-low-bit count and sweep width
+// (register bits) on the handle's own state, and keeps the fastest
+// configuration.
 #include <float.h>
 
 #include <vector>
@@ -104,13 +105,15 @@ int bsim_autotune(bsim_state* h, const bsim_op* ops, int n_ops, int* tile_bits, 
             (void)hipEventDestroy(b);
         }
     } evg{ e0, e1 };
-    int bestT = keep.tile_bits, bestL = keep.low_bits;
+    int bestT = keep.tile_bits, bestL = keep.low_bits, bestR = keep.reg_bits;
     double best = DBL_MAX;
     const int Tmax = h->vq < 13 ? h->vq : 13;
+    for (int RB = 4; RB >= 3; --RB)
     for (int T = 10; T <= Tmax; ++T)
         for (int L = 2; L <= 6 && L <= T - 2; ++L) {
             tile_config().tile_bits = T;
             tile_config().low_bits = L;
+            tile_config().reg_bits = RB;
             Plan plan;
             if (int rc = make_plan(h->vq, eops, tile_config(), plan)) {
                 tile_config() = keep;
@@ -137,6 +140,7 @@ int bsim_autotune(bsim_state* h, const bsim_op* ops, int n_ops, int* tile_bits, 
                 best = t;
                 bestT = T;
                 bestL = L;
+                bestR = RB;
             }
         }
     if (best == DBL_MAX) {  // no candidate fits this state: keep the current configuration
@@ -146,6 +150,7 @@ int bsim_autotune(bsim_state* h, const bsim_op* ops, int n_ops, int* tile_bits, 
         tile_config() = keep;
         tile_config().tile_bits = bestT;
         tile_config().low_bits = bestL;
+        tile_config().reg_bits = bestR;
     }
     h->alias_ready = false;
     if (tile_bits) *tile_bits = tile_config().tile_bits;

@@ -1,16 +1,20 @@
 // The fused tile kernel (one HBM pass of a planned instruction block).
 //
-// Block = 256 threads, one tile of 2^T amplitudes in LDS (padded by one
-// element per 16 so that strided sub-cube reads spread over the banks).
-//   1. load: local index j = tid + 256 i; the global index is the block's
+// One workgroup, one tile of 2^T amplitudes in LDS (padded by one element
+// per 16 so that strided sub-cube reads spread over the banks).  A pass
+// works with R register bits per sweep (R = 3 or 4, the planner's choice):
+// the workgroup has TB(R) = 2^(12 - R) threads, so a 12-bit tile gives each
+// thread one sub-cube of 2^R amplitudes per sweep (R = 3: 512 threads with
+// 8 amplitudes, four waves per SIMD; R = 4: 256 threads with 16, two).
+//   1. load: local index j = tid + TB i; the global index is the block's
 //      base plus the scatter of j over the tile's vector bits (two small LDS
-//      tables: local bits 0..7 and 8..T-1).  Local bits 0..low_bits-1 are
-//      vector bits 0..low_bits-1, so each wavefront load covers contiguous
-//      runs of 2^low_bits amplitudes.
-//   2. sweeps: a sweep names 4 local bits; each thread gathers a
-//      16-amplitude sub-cube into registers, applies the sweep's gates there
-//      (dense 1- and 2-qubit on the 4 bits, diagonal/controls on any vector
-//      bit through the global index) and writes it back.
+//      tables: the low log2(TB) local bits and the rest).  Local bits
+//      0..low_bits-1 are vector bits 0..low_bits-1, so each wavefront load
+//      covers contiguous runs of 2^low_bits amplitudes.
+//   2. sweeps: a sweep names R local bits; each thread gathers its sub-cube
+//      into registers, applies the sweep's gates there (dense 1- and 2-qubit
+//      on the register bits, diagonal/controls on any vector bit through the
+//      global index) and writes it back.
 //   3. store: the load pattern in reverse.
 // Every register index is a compile-time constant (sfor below), so the
 // sub-cube never leaves the VGPR file.
@@ -22,7 +26,10 @@
 
 namespace bsim {
 
-constexpr int TB = 256;
+template <int R>
+constexpr int tile_threads() { return 4096 >> R; }
+template <int R>
+constexpr int tile_thread_bits() { return 12 - R; }
 
 template <int I, int N, typename F>
 __device__ __forceinline__ void sfor(F&& f) {
@@ -37,15 +44,16 @@ __device__ __forceinline__ int phys(int j) { return j + (j >> 4); }
 struct c2 {
     double x, y;
 };
-// The 16-amplitude sub-cube is one 32-wide vector value, so it stays in
-// VGPRs: every element access below uses a compile-time index.
-typedef double dv32 __attribute__((ext_vector_type(32)));
-template <int J>
-__device__ __forceinline__ c2 getv(const dv32& v) {
+// The sub-cube is one vector value of 2^(R+1) doubles, so it stays in VGPRs:
+// every element access below uses a compile-time index.
+template <int N>
+using dvec = double __attribute__((ext_vector_type(N)));
+template <int J, int N>
+__device__ __forceinline__ c2 getv(const dvec<N>& v) {
     return { v[2 * J], v[2 * J + 1] };
 }
-template <int J>
-__device__ __forceinline__ void setv(dv32& v, c2 a) {
+template <int J, int N>
+__device__ __forceinline__ void setv(dvec<N>& v, c2 a) {
     v[2 * J] = a.x;
     v[2 * J + 1] = a.y;
 }
@@ -59,18 +67,23 @@ __device__ __forceinline__ c2 cfma(c2 a, c2 b, c2 c) {
 // between two stack addresses, i.e. scratch memory.
 __device__ __forceinline__ c2 csel(bool on, c2 a, c2 b) { return { on ? a.x : b.x, on ? a.y : b.y }; }
 
+template <int R>
 struct SubCube {
-    dv32 v;
+    static constexpr int NR = 1 << R;
+    dvec<2 * NR> v;
     uint64_t gbase;            // global index of register 0
-    uint64_t g0, g1, g2, g3;   // global bit of register bits 0..3
-    template <int J>
-    __device__ __forceinline__ uint64_t gidx() const {
-        return gbase | ((J & 1) ? g0 : 0) | ((J & 2) ? g1 : 0) | ((J & 4) ? g2 : 0) | ((J & 8) ? g3 : 0);
-    }
+    uint64_t g0, g1, g2, g3;   // global bit of register bits 0..R-1 (g3 = 0 when R = 3)
 };
 
-// Dense ops come in two instantiations: without controls (the common case)
-// and with the split control test; the choice is made once per op.
+// registers of a sub-cube whose index has bit i set (the sign op's patterns)
+template <int R>
+constexpr uint32_t reg_pattern(int i) {
+    uint32_t p = 0;
+    for (int j = 0; j < (1 << R); ++j)
+        if ((j >> i) & 1) p |= 1u << j;
+    return p;
+}
+
 // The fields every dense op needs, read with the op's kind before the
 // dispatch so that a 1-bit dense op costs one scalar-load round trip.
 struct OpHead {
@@ -92,8 +105,10 @@ __device__ __forceinline__ OpHead load_head(const TOp* __restrict__ op) {
     return h;
 }
 
-template <int A, bool CTL>
-__device__ __forceinline__ void sw_dense1_t(SubCube& s, const TOp* __restrict__ op, const OpHead& hd) {
+// Dense ops come in two instantiations: without controls (the common case)
+// and with the split control test; the choice is made once per op.
+template <int R, int A, bool CTL>
+__device__ __forceinline__ void sw_dense1_t(SubCube<R>& s, const TOp* __restrict__ op, const OpHead& hd) {
     const c2 m00 = hd.m0, m10 = hd.m1, m01 = hd.m2, m11 = hd.m3;
     bool tok = true;
     uint32_t jm = 0xFFFFu;
@@ -101,7 +116,7 @@ __device__ __forceinline__ void sw_dense1_t(SubCube& s, const TOp* __restrict__ 
         tok = (s.gbase & op->tmask) == op->tval;  // this thread's part of the control test
         jm = op->jmask;                           // the registers' part
     }
-    sfor<0, 16>([&](auto J) {
+    sfor<0, (1 << R)>([&](auto J) {
         constexpr int j = decltype(J)::value;
         if constexpr ((j & (1 << A)) == 0) {
             constexpr int k = j | (1 << A);
@@ -119,21 +134,21 @@ __device__ __forceinline__ void sw_dense1_t(SubCube& s, const TOp* __restrict__ 
     });
 }
 
-template <int A>
-__device__ __forceinline__ void sw_dense1(SubCube& s, const TOp* __restrict__ op, const OpHead& hd) {
-    if (hd.cmask) sw_dense1_t<A, true>(s, op, hd);
-    else sw_dense1_t<A, false>(s, op, hd);
+template <int R, int A>
+__device__ __forceinline__ void sw_dense1(SubCube<R>& s, const TOp* __restrict__ op, const OpHead& hd) {
+    if (hd.cmask) sw_dense1_t<R, A, true>(s, op, hd);
+    else sw_dense1_t<R, A, false>(s, op, hd);
 }
 
-template <int A, int B, bool CTL>
-__device__ __forceinline__ void sw_dense2_t(SubCube& s, const TOp* __restrict__ op, const OpHead& hd) {
+template <int R, int A, int B, bool CTL>
+__device__ __forceinline__ void sw_dense2_t(SubCube<R>& s, const TOp* __restrict__ op, const OpHead& hd) {
     bool tok = true;
     uint32_t jm = 0xFFFFu;
     if constexpr (CTL) {
         tok = (s.gbase & op->tmask) == op->tval;
         jm = op->jmask;
     }
-    sfor<0, 16>([&](auto J) {
+    sfor<0, (1 << R)>([&](auto J) {
         constexpr int j = decltype(J)::value;
         if constexpr ((j & ((1 << A) | (1 << B))) == 0) {
             constexpr int i1 = j | (1 << A), i2 = j | (1 << B), i3 = j | (1 << A) | (1 << B);
@@ -161,16 +176,17 @@ __device__ __forceinline__ void sw_dense2_t(SubCube& s, const TOp* __restrict__ 
     });
 }
 
-template <int A, int B>
-__device__ __forceinline__ void sw_dense2(SubCube& s, const TOp* __restrict__ op, const OpHead& hd) {
-    if (hd.cmask) sw_dense2_t<A, B, true>(s, op, hd);
-    else sw_dense2_t<A, B, false>(s, op, hd);
+template <int R, int A, int B>
+__device__ __forceinline__ void sw_dense2(SubCube<R>& s, const TOp* __restrict__ op, const OpHead& hd) {
+    if (hd.cmask) sw_dense2_t<R, A, B, true>(s, op, hd);
+    else sw_dense2_t<R, A, B, false>(s, op, hd);
 }
 
 // Diagonal index of register j = (index bits read from the thread's base)
 // | (index bits carried by the register bits of j).  The second part is the
 // same for every thread, so it is built from uniform values.
-__device__ __forceinline__ void sw_diag(SubCube& s, const TOp* __restrict__ op) {
+template <int R>
+__device__ __forceinline__ void sw_diag(SubCube<R>& s, const TOp* __restrict__ op) {
     const int nd = op->nd;
     int base_idx = 0, r0 = 0, r1 = 0, r2 = 0, r3 = 0;
     for (int i = 0; i < nd; ++i) {
@@ -183,7 +199,7 @@ __device__ __forceinline__ void sw_diag(SubCube& s, const TOp* __restrict__ op) 
     }
     const uint64_t cm = op->cmask, cv = op->cval;
     const uint64_t tb = s.gbase & cm;
-    sfor<0, 16>([&](auto J) {
+    sfor<0, (1 << R)>([&](auto J) {
         constexpr int j = decltype(J)::value;
         const int idx = base_idx | ((j & 1) ? r0 : 0) | ((j & 2) ? r1 : 0) | ((j & 4) ? r2 : 0) | ((j & 8) ? r3 : 0);
         const c2 f = to_c2(op->m[idx]);
@@ -202,20 +218,22 @@ __device__ __forceinline__ uint32_t xor_and(uint32_t a, uint32_t b, uint32_t c) 
 // P(i) = sum_k parity(i & (i >> pd[k]) & pm[k]) is 1.  P is a quadratic form
 // over GF(2): for i = gbase ^ r_j,
 // P(i) = P(gbase) ^ P(r_j) ^ sum over the register bits of j of
-// parity(gbase & sgm[bit]); the planner stores P(r_j) for all 16 j in jmask.
-__device__ __forceinline__ void sw_sign(SubCube& s, const TOp* __restrict__ op) {
+// parity(gbase & sgm[bit]); the planner stores P(r_j) for every register j
+// in jmask.
+template <int R>
+__device__ __forceinline__ void sw_sign(SubCube<R>& s, const TOp* __restrict__ op) {
     const uint64_t a = s.gbase;
     uint32_t m = op->jmask;
     uint32_t par = 0;
     for (int k = 0; k < op->npairs; ++k) par += __popcll(a & (a >> op->pd[k]) & op->pm[k]);
     // all-ones where the parity bit is set (sign-extended bit 0), masked into m
     auto flip = [&](uint32_t p, uint32_t pattern) { m = xor_and(m, (uint32_t)__builtin_amdgcn_sbfe(p, 0, 1), pattern); };
-    flip(par, 0xFFFFu);
-    flip(__popcll(a & op->sgm[0]), 0xAAAAu);
-    flip(__popcll(a & op->sgm[1]), 0xCCCCu);
-    flip(__popcll(a & op->sgm[2]), 0xF0F0u);
-    flip(__popcll(a & op->sgm[3]), 0xFF00u);
-    sfor<0, 16>([&](auto J) {
+    flip(par, (1u << (1 << R)) - 1);
+    flip(__popcll(a & op->sgm[0]), reg_pattern<R>(0));
+    flip(__popcll(a & op->sgm[1]), reg_pattern<R>(1));
+    flip(__popcll(a & op->sgm[2]), reg_pattern<R>(2));
+    if constexpr (R == 4) flip(__popcll(a & op->sgm[3]), reg_pattern<R>(3));
+    sfor<0, (1 << R)>([&](auto J) {
         constexpr int j = decltype(J)::value;
         const uint32_t t = m << (31 - j);  // bit j of m at the sign position (garbage below, masked off)
         c2 v = getv<j>(s.v);
@@ -229,17 +247,17 @@ __device__ __forceinline__ void sw_sign(SubCube& s, const TOp* __restrict__ op) 
 }
 
 // Reference-semantics Kraus block update as a predicated superoperator
-// (kind 4).  Q(gbase) is this thread's part of the block test; a register
-// group is updated when Q(gbase) equals the uniform part of its register
-// bits.  On two registers the 4x4 runs like a dense 2-bit op with a select
-// per group.
-__device__ __forceinline__ uint64_t kraus_q(const SubCube& s, const TOp* __restrict__ op) {
+// (kind 4, four register bits only).  Q(gbase) is this thread's part of the
+// block test; a register group is updated when Q(gbase) equals the uniform
+// part of its register bits.  On two registers the 4x4 runs like a dense
+// 2-bit op with a select per group.
+__device__ __forceinline__ uint64_t kraus_q(const SubCube<4>& s, const TOp* __restrict__ op) {
     const uint64_t g = s.gbase;
     return (g ^ (g >> op->pd[0])) & op->pm[0];
 }
 
 template <int A, int B>
-__device__ __forceinline__ void sw_kraus2(SubCube& s, const TOp* __restrict__ op, const OpHead& hd) {
+__device__ __forceinline__ void sw_kraus2(SubCube<4>& s, const TOp* __restrict__ op, const OpHead& hd) {
     const uint64_t q = kraus_q(s, op);
     const uint64_t s0 = op->sgm[0], s1 = op->sgm[1], s2 = op->sgm[2], s3 = op->sgm[3];
     const c2 col0[4] = { hd.m0, hd.m1, hd.m2, hd.m3 };
@@ -270,7 +288,7 @@ __device__ __forceinline__ void sw_kraus2(SubCube& s, const TOp* __restrict__ op
 // a few threads of a few tiles.  Those threads put the sub-cube back into
 // their own LDS slots and accumulate one output at a time from there, so
 // the 16x16 product needs no extra registers.
-__device__ __forceinline__ void sw_kraus4(SubCube& s, const TOp* __restrict__ op, double2* __restrict__ tile,
+__device__ __forceinline__ void sw_kraus4(SubCube<4>& s, const TOp* __restrict__ op, double2* __restrict__ tile,
                                           int pl, int o0, int o1, int o2, int o3) {
     if (kraus_q(s, op) != 0) return;
     auto slot = [&](int j) {
@@ -290,65 +308,81 @@ __device__ __forceinline__ void sw_kraus4(SubCube& s, const TOp* __restrict__ op
 }
 
 // Apply one tile op; returns the number of TOp entries it occupies.  KRAUS:
-// the plan may hold reference Kraus ops (density-matrix plans only; the
-// state-vector instantiation carries no code for them).
-template <bool KRAUS>
-__device__ __forceinline__ int run_op(SubCube& s, const TOp* __restrict__ op, double2* __restrict__ tile, int pl,
+// the plan may hold reference Kraus ops (density-matrix plans only, R = 4;
+// other instantiations carry no code for them).
+template <int R, bool KRAUS>
+__device__ __forceinline__ int run_op(SubCube<R>& s, const TOp* __restrict__ op, double2* __restrict__ tile, int pl,
                                       int o0, int o1, int o2, int o3) {
     const OpHead hd = load_head(op);
-    if (KRAUS && hd.kind == 4) {
-        if (hd.nd) {
-            sw_kraus4(s, op, tile, pl, o0, o1, o2, o3);
-            return 1 + hd.nd;
+    if constexpr (KRAUS && R == 4) {
+        if (hd.kind == 4) {
+            if (hd.nd) {
+                sw_kraus4(s, op, tile, pl, o0, o1, o2, o3);
+                return 1 + hd.nd;
+            }
+            switch (hd.a * 4 + hd.b) {
+                case 1: sw_kraus2<0, 1>(s, op, hd); break;
+                case 2: sw_kraus2<0, 2>(s, op, hd); break;
+                case 3: sw_kraus2<0, 3>(s, op, hd); break;
+                case 6: sw_kraus2<1, 2>(s, op, hd); break;
+                case 7: sw_kraus2<1, 3>(s, op, hd); break;
+                default: sw_kraus2<2, 3>(s, op, hd); break;
+            }
+            return 1;
         }
-        switch (hd.a * 4 + hd.b) {
-            case 1: sw_kraus2<0, 1>(s, op, hd); break;
-            case 2: sw_kraus2<0, 2>(s, op, hd); break;
-            case 3: sw_kraus2<0, 3>(s, op, hd); break;
-            case 6: sw_kraus2<1, 2>(s, op, hd); break;
-            case 7: sw_kraus2<1, 3>(s, op, hd); break;
-            default: sw_kraus2<2, 3>(s, op, hd); break;
-        }
-        return 1;
     }
-    switch (hd.kind * 16 + hd.a * 4 + hd.b) {
-        case 48: sw_sign(s, op); break;
-        case 0: sw_dense1<0>(s, op, hd); break;
-        case 4: sw_dense1<1>(s, op, hd); break;
-        case 8: sw_dense1<2>(s, op, hd); break;
-        case 12: sw_dense1<3>(s, op, hd); break;
-        case 16 + 1: sw_dense2<0, 1>(s, op, hd); break;
-        case 16 + 2: sw_dense2<0, 2>(s, op, hd); break;
-        case 16 + 3: sw_dense2<0, 3>(s, op, hd); break;
-        case 16 + 6: sw_dense2<1, 2>(s, op, hd); break;
-        case 16 + 7: sw_dense2<1, 3>(s, op, hd); break;
-        case 16 + 11: sw_dense2<2, 3>(s, op, hd); break;
-        default: sw_diag(s, op); break;
+    if constexpr (R == 4) {
+        switch (hd.kind * 16 + hd.a * 4 + hd.b) {
+            case 48: sw_sign<R>(s, op); break;
+            case 0: sw_dense1<R, 0>(s, op, hd); break;
+            case 4: sw_dense1<R, 1>(s, op, hd); break;
+            case 8: sw_dense1<R, 2>(s, op, hd); break;
+            case 12: sw_dense1<R, 3>(s, op, hd); break;
+            case 16 + 1: sw_dense2<R, 0, 1>(s, op, hd); break;
+            case 16 + 2: sw_dense2<R, 0, 2>(s, op, hd); break;
+            case 16 + 3: sw_dense2<R, 0, 3>(s, op, hd); break;
+            case 16 + 6: sw_dense2<R, 1, 2>(s, op, hd); break;
+            case 16 + 7: sw_dense2<R, 1, 3>(s, op, hd); break;
+            case 16 + 11: sw_dense2<R, 2, 3>(s, op, hd); break;
+            default: sw_diag<R>(s, op); break;
+        }
+    } else {
+        switch (hd.kind * 16 + hd.a * 4 + hd.b) {
+            case 48: sw_sign<R>(s, op); break;
+            case 0: sw_dense1<R, 0>(s, op, hd); break;
+            case 4: sw_dense1<R, 1>(s, op, hd); break;
+            case 8: sw_dense1<R, 2>(s, op, hd); break;
+            case 16 + 1: sw_dense2<R, 0, 1>(s, op, hd); break;
+            case 16 + 2: sw_dense2<R, 0, 2>(s, op, hd); break;
+            case 16 + 6: sw_dense2<R, 1, 2>(s, op, hd); break;
+            default: sw_diag<R>(s, op); break;
+        }
     }
     return 1;
 }
 
 // One tile of one pass.  `init`: the state is |0...0> and is not read (the
-// tile is built in LDS).  LDS: tile (2^T + 2^T/16 amplitudes), scat_lo[256],
+// tile is built in LDS).  LDS: tile (2^T + 2^T/16 amplitudes), scat_lo[TB],
 // scat_hi[64].
-template <bool KRAUS>
+template <int R, bool KRAUS>
 __device__ __forceinline__ void tile_body(double2* __restrict__ sv, int vq, const Pass* __restrict__ pass,
                                           const Sweep* __restrict__ sweeps, const TOp* __restrict__ tops,
                                           uint64_t tile_index, bool init, double2* __restrict__ tile,
                                           uint64_t* __restrict__ scat_lo, uint64_t* __restrict__ scat_hi) {
+    constexpr int TB = tile_threads<R>(), TBB = tile_thread_bits<R>(), NR = 1 << R;
     const int T = pass->T;
     const int nloc = 1 << T;
     const uint64_t lmask = pass->lmask;
     {
         const int t = threadIdx.x;
         uint64_t s = 0;
-        for (int b = 0; b < 8 && b < T; ++b)
+        for (int b = 0; b < TBB && b < T; ++b)
             if ((t >> b) & 1) s |= 1ull << pass->lbits[b];
         scat_lo[t] = s;
         if (t < 64) {
             uint64_t h = 0;
-            for (int b = 8; b < T; ++b)
-                if ((t >> (b - 8)) & 1) h |= 1ull << pass->lbits[b];
+            for (int b = TBB; b < T; ++b)
+                if ((t >> (b - TBB)) & 1) h |= 1ull << pass->lbits[b];
             scat_hi[t] = h;
         }
     }
@@ -360,56 +394,57 @@ __device__ __forceinline__ void tile_body(double2* __restrict__ sv, int vq, cons
             ++k;
         }
     __syncthreads();
-    const uint64_t s_t = base | scat_lo[threadIdx.x & 255];
-    const int iters = nloc >> 8;  // rows of 256 amplitudes (T >= 8)
+    const uint64_t s_t = base | scat_lo[threadIdx.x & (TB - 1)];
+    const int iters = nloc >> TBB;  // rows of TB amplitudes (T >= TBB)
     if (init) {
         for (int j = threadIdx.x; j < nloc; j += TB)
             tile[phys(j)] = make_double2((base == 0 && j == 0) ? 1.0 : 0.0, 0.0);
     } else if (iters == 0) {
         if ((int)threadIdx.x < nloc) tile[phys(threadIdx.x)] = sv[s_t];
     }
-    // issue up to 16 loads per thread before the first LDS write
-    for (int i0 = 0; i0 < iters && !init; i0 += 16) {
-        dv32 buf;
-        sfor<0, 16>([&](auto J) {
+    // issue up to NR loads per thread before the first LDS write
+    for (int i0 = 0; i0 < iters && !init; i0 += NR) {
+        dvec<2 * NR> buf;
+        sfor<0, NR>([&](auto J) {
             constexpr int jj = decltype(J)::value;
             if (i0 + jj < iters) setv<jj>(buf, to_c2(sv[s_t | scat_hi[i0 + jj]]));
         });
-        sfor<0, 16>([&](auto J) {
+        sfor<0, NR>([&](auto J) {
             constexpr int jj = decltype(J)::value;
-            if (i0 + jj < iters) tile[phys((int)threadIdx.x + ((i0 + jj) << 8))] = to_d2(getv<jj>(buf));
+            if (i0 + jj < iters) tile[phys((int)threadIdx.x + ((i0 + jj) << TBB))] = to_d2(getv<jj>(buf));
         });
     }
     __syncthreads();
-    const int nsub = nloc >> 4;
+    const int nsub = nloc >> R;
     for (int sidx = 0; sidx < pass->nsweeps; ++sidx) {
         const Sweep* sw = sweeps + pass->sweep_begin + sidx;
-        const int sb0 = sw->sb[0], sb1 = sw->sb[1], sb2 = sw->sb[2], sb3 = sw->sb[3];
+        const int sb0 = sw->sb[0], sb1 = sw->sb[1], sb2 = sw->sb[2], sb3 = R == 4 ? sw->sb[3] : 0;
         const int ob = sw->op_begin, oe = sw->op_end;
         // global bits of the register bits (uniform)
-        auto gl = [&](int l) { return scat_lo[(1 << l) & 255] | scat_hi[(1 << l) >> 8]; };
-        const uint64_t g0 = gl(sb0), g1 = gl(sb1), g2 = gl(sb2), g3 = gl(sb3);
+        auto gl = [&](int l) { return scat_lo[(1 << l) & (TB - 1)] | scat_hi[(1 << l) >> TBB]; };
+        const uint64_t g0 = gl(sb0), g1 = gl(sb1), g2 = gl(sb2), g3 = R == 4 ? gl(sb3) : 0;
         // local offsets of the registers: the register bits are disjoint from the
         // sub-cube's base, so phys(lb | o) = phys(lb) + phys(o)
-        const int o0 = 1 << sb0, o1 = 1 << sb1, o2 = 1 << sb2, o3 = 1 << sb3;
+        const int o0 = 1 << sb0, o1 = 1 << sb1, o2 = 1 << sb2, o3 = R == 4 ? 1 << sb3 : 0;
         for (int c = threadIdx.x; c < nsub; c += TB) {
             // base of sub-cube c: zero bits inserted at the (ascending) register bits
             auto ins = [](int x, int b) { const int lo = x & ((1 << b) - 1); return ((x - lo) << 1) | lo; };
-            const int lb = ins(ins(ins(ins(c, sb0), sb1), sb2), sb3);
+            int lb = ins(ins(ins(c, sb0), sb1), sb2);
+            if constexpr (R == 4) lb = ins(lb, sb3);
             const int pl = phys(lb);
-            SubCube s;
-            s.gbase = base | scat_lo[lb & 255] | scat_hi[lb >> 8];
+            SubCube<R> s;
+            s.gbase = base | scat_lo[lb & (TB - 1)] | scat_hi[lb >> TBB];
             s.g0 = g0;
             s.g1 = g1;
             s.g2 = g2;
             s.g3 = g3;
-            sfor<0, 16>([&](auto J) {
+            sfor<0, NR>([&](auto J) {
                 constexpr int j = decltype(J)::value;
                 const int o = ((j & 1) ? o0 : 0) + ((j & 2) ? o1 : 0) + ((j & 4) ? o2 : 0) + ((j & 8) ? o3 : 0);
                 setv<j>(s.v, to_c2(tile[pl + phys(o)]));
             });
-            for (int o = ob; o < oe;) o += run_op<KRAUS>(s, tops + o, tile, pl, o0, o1, o2, o3);
-            sfor<0, 16>([&](auto J) {
+            for (int o = ob; o < oe;) o += run_op<R, KRAUS>(s, tops + o, tile, pl, o0, o1, o2, o3);
+            sfor<0, NR>([&](auto J) {
                 constexpr int j = decltype(J)::value;
                 const int o = ((j & 1) ? o0 : 0) + ((j & 2) ? o1 : 0) + ((j & 4) ? o2 : 0) + ((j & 8) ? o3 : 0);
                 tile[pl + phys(o)] = to_d2(getv<j>(s.v));
@@ -420,15 +455,15 @@ __device__ __forceinline__ void tile_body(double2* __restrict__ sv, int vq, cons
     if (iters == 0) {
         if ((int)threadIdx.x < nloc) sv[s_t] = tile[phys(threadIdx.x)];
     }
-    // read up to 16 rows from LDS, then stream them out (non-temporal: the
+    // read up to NR rows from LDS, then stream them out (non-temporal: the
     // pass never re-reads what it wrote)
-    for (int i0 = 0; i0 < iters; i0 += 16) {
-        dv32 buf;
-        sfor<0, 16>([&](auto J) {
+    for (int i0 = 0; i0 < iters; i0 += NR) {
+        dvec<2 * NR> buf;
+        sfor<0, NR>([&](auto J) {
             constexpr int jj = decltype(J)::value;
-            if (i0 + jj < iters) setv<jj>(buf, to_c2(tile[phys((int)threadIdx.x + ((i0 + jj) << 8))]));
+            if (i0 + jj < iters) setv<jj>(buf, to_c2(tile[phys((int)threadIdx.x + ((i0 + jj) << TBB))]));
         });
-        sfor<0, 16>([&](auto J) {
+        sfor<0, NR>([&](auto J) {
             constexpr int jj = decltype(J)::value;
             if (i0 + jj < iters) {
                 double* p = (double*)(sv + (s_t | scat_hi[i0 + jj]));

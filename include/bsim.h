@@ -145,6 +145,12 @@ int bsim_plan_stats(int vector_qubits, int is_dm, const bsim_op* ops, int n_ops,
  * bits 0..tile_bits-2); 0 keeps the current value. */
 int bsim_set_tile_config(int tile_bits, int low_bits);
 int bsim_get_tile_config(int* tile_bits, int* low_bits);
+/* Register bits per LDS sweep of the fused path: 4 (16-amplitude
+ * sub-cubes, 256-thread workgroups; the default) or 3 (8 amplitudes, 512
+ * threads, four waves per SIMD); 0 keeps the current value, *old (may be
+ * NULL) receives the previous one.  Passes holding 4-bit ops or reference
+ * Kraus ops (density matrices) always use 4. */
+int bsim_set_sweep_bits(int bits, int* old);
 /* Fusion width of the fused path: FP64 operations per amplitude one pass
  * may spend (1-qubit gate 8, 2-qubit 32, diagonal 4, CZ-type sign 1);
  * 0 = unlimited.  Returns the previous value in *old (may be NULL). */
@@ -293,10 +299,11 @@ int bsim_graph_destroy(bsim_graph* g);
 /* Tile autotuner (the reference has no tunables besides CHUNK_SIZE,
  * src/BraketSimulator.jl:85-86): times the fused evolution of ops on h's
  * state (reset to |0...0> before every run, left in an unspecified state
- * afterwards) for tile_bits 10..min(13, vector qubits) and contiguous low
- * bits 2..min(6, tile_bits-2), keeps the fastest as the tile configuration
- * and returns it (outputs may be NULL; *best_ms = 0 when no candidate fits
- * and the configuration is unchanged). */
+ * afterwards) for tile_bits 10..min(13, vector qubits), contiguous low bits
+ * 2..min(6, tile_bits-2) and 3 or 4 sweep register bits, keeps the fastest
+ * as the tile configuration and returns it (outputs may be NULL; read the
+ * sweep bits back with bsim_set_sweep_bits(0, &bits); *best_ms = 0 when no
+ * candidate fits and the configuration is unchanged). */
 int bsim_autotune(bsim_state* h, const bsim_op* ops, int n_ops, int* tile_bits, int* low_bits, double* best_ms);
 
 #ifdef __cplusplus

@@ -23,25 +23,29 @@ static int run(int nq, int kind, const bsim_op* ops, int n, double* state) {
         const Pass& P = p.passes[st.pass];
         for (int s = 0; s < P.nsweeps; ++s) {
             const Sweep& S = p.sweeps[P.sweep_begin + s];
-            int vb[4];
-            for (int r = 0; r < 4; ++r) vb[r] = P.lbits[S.sb[r]];
+            const int R = P.R;  // register bits of the pass's sweeps
+            if (R != 3 && R != 4) return -13;
+            int vb[4] = {0, 0, 0, 0};
+            for (int r = 0; r < R; ++r) vb[r] = P.lbits[S.sb[r]];
             uint64_t G = 0;
-            for (int r = 0; r < 4; ++r) G |= 1ull << vb[r];
+            for (int r = 0; r < R; ++r) G |= 1ull << vb[r];
             // the per-thread / per-register split of every index test must
             // agree with the direct test on every index
             for (int o = S.op_begin; o < S.op_end; ++o) {
                 const TOp& op = p.tops[o];
+                if ((op.kind == 0 || op.kind == 1 || (op.kind == 4 && op.nd == 0)) && (op.a >= R || op.b >= R))
+                    return -15;  // a register bit the pass's sweeps do not have
                 for (uint64_t i = 0; i < N; ++i) {
                     const uint64_t gbase = i & ~G;
                     int j = 0;
-                    for (int r = 0; r < 4; ++r) j |= (int)((i >> vb[r]) & 1) << r;
+                    for (int r = 0; r < R; ++r) j |= (int)((i >> vb[r]) & 1) << r;
                     if (op.kind == 3) {
                         int direct = 0, split = (int)((op.jmask >> j) & 1);
                         for (int k = 0; k < op.npairs; ++k) {
                             direct ^= __builtin_popcountll(i & (i >> op.pd[k]) & op.pm[k]) & 1;
                             split ^= __builtin_popcountll(gbase & (gbase >> op.pd[k]) & op.pm[k]) & 1;
                         }
-                        for (int r = 0; r < 4; ++r)
+                        for (int r = 0; r < R; ++r)
                             if ((j >> r) & 1) split ^= __builtin_popcountll(gbase & op.sgm[r]) & 1;
                         if (direct != split) return -10;
                     } else if (op.kind <= 1 && op.cmask) {
@@ -50,6 +54,7 @@ static int run(int nq, int kind, const bsim_op* ops, int n, double* state) {
                         if (direct != split) return -11;
                     } else if (op.kind == 4) {
                         const bool direct = ((i ^ (i >> op.pd[0])) & op.pm[0]) == 0;
+                        if (R != 4) return -14;  // kind 4 runs in 4-register passes only
                         uint64_t u = 0;
                         for (int r = 0; r < 4; ++r)
                             if ((j >> r) & 1) u ^= op.sgm[r];

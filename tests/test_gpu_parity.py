@@ -87,9 +87,12 @@ def test_sv_random_vs_oracle(n, seed, mode):
     assert np.max(np.abs(got - exp)) < TOL
 
 
+@pytest.mark.parametrize("bits", [3, 4])
 @pytest.mark.parametrize("tile,low", [(6, 0), (8, 2), (10, 3), (12, 4), (13, 5), (12, 10)])
-def test_tile_configs(tile, low):
-    _say(f"tile config {tile} {low}")
+def test_tile_configs(tile, low, bits):
+    _say(f"tile config {tile} {low} sweep bits {bits}")
+    old_bits = ctypes.c_int()
+    L.call("bsim_set_sweep_bits", bits, ctypes.byref(old_bits))
     n = 16
     rng = np.random.default_rng(tile * 7 + low)
     ins = random_circuit(n, 150, rng)
@@ -103,6 +106,7 @@ def test_tile_configs(tile, low):
         got = sim.state_vector()
     finally:
         L.call("bsim_set_tile_config", t0.value, lo0.value)
+        L.call("bsim_set_sweep_bits", old_bits.value, None)
     assert np.max(np.abs(got - exp)) < TOL
 
 

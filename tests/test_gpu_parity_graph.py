@@ -34,10 +34,12 @@ def _say(msg):
 
 @pytest.fixture
 def keep_tile_config():
-    t, lo = ctypes.c_int(), ctypes.c_int()
+    t, lo, rb = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     L.call("bsim_get_tile_config", ctypes.byref(t), ctypes.byref(lo))
+    L.call("bsim_set_sweep_bits", 0, ctypes.byref(rb))
     yield
     L.call("bsim_set_tile_config", t.value, lo.value)
+    L.call("bsim_set_sweep_bits", rb.value, None)
 
 
 def _plan_singles(n, kind, ins, mode="physical"):

@@ -64,11 +64,13 @@ def _run(h, n, ins):
 
 @pytest.fixture
 def tile_config():
-    t, lo = ctypes.c_int(), ctypes.c_int()
+    t, lo, rb = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     L.load()
     L.call("bsim_get_tile_config", ctypes.byref(t), ctypes.byref(lo))
+    L.call("bsim_set_sweep_bits", 0, ctypes.byref(rb))
     yield
     L.call("bsim_set_tile_config", t.value, lo.value)
+    L.call("bsim_set_sweep_bits", rb.value, None)
 
 
 KINDS = ("1q", "2q", "ctrl", "diag", "pow")
@@ -82,9 +84,11 @@ def test_planner_random_circuits(interp, seed):
     assert np.max(np.abs(_run(interp, n, ins) - O.evolve_sv(n, ins))) < 1e-12
 
 
+@pytest.mark.parametrize("bits", [3, 4])
 @pytest.mark.parametrize("tile,low", [(6, 0), (8, 3), (12, 4), (12, 10)])
-def test_planner_tile_configs(interp, tile_config, tile, low):
+def test_planner_tile_configs(interp, tile_config, tile, low, bits):
     L.call("bsim_set_tile_config", tile, low)
+    L.call("bsim_set_sweep_bits", bits, None)
     rng = np.random.default_rng(tile * 31 + low)
     n = 13
     ins = random_circuit(n, 70, rng, kinds=KINDS)
