@@ -232,6 +232,13 @@ def adjoint_leg(nq: int, ps_sample: int):
     for _ in range(reps):
         ev, grad = A.adjoint_gradient(sim, ins, pmap, words, coeffs)
     t_adj = (time.perf_counter() - t) / reps
+    # the same gradient with one backward kernel per op
+    A.adjoint_gradient(sim, ins, pmap, words, coeffs, fused=False)
+    t = time.perf_counter()
+    for _ in range(reps):
+        _, grad_op = A.adjoint_gradient(sim, ins, pmap, words, coeffs, fused=False)
+    t_op = (time.perf_counter() - t) / reps
+    log(f"adjoint: fused backward {t_adj * 1e3:.1f} ms, per-op backward {t_op * 1e3:.1f} ms")
     # parameter shift on the first `ps_sample` parameters (2 evolutions each)
     sample = list(range(min(ps_sample, len(th))))
     t = time.perf_counter()
@@ -248,6 +255,8 @@ def adjoint_leg(nq: int, ps_sample: int):
         "expval": ev,
         "adjoint_s": round(t_adj, 4),
         "grads_per_s": round(1.0 / t_adj, 3),
+        "per_op_backward_s": round(t_op, 4),
+        "max_abs_diff_fused_vs_per_op": float(np.max(np.abs(grad - grad_op))),
         "param_shift_s_extrapolated": round(t_ps, 3),
         "param_shift_grads_per_s": round(1.0 / t_ps, 4),
         "speedup_vs_param_shift": round(t_ps / t_adj, 1),
@@ -351,7 +360,31 @@ def per_gate_leg(n: int, ins):
         log(f"per-gate {name}: {per * 1e3:.2f} ms per gate, {gbs:.0f} GB/s")
     out["note"] = "pass_GBps: algorithmic bytes per gate (U: every amplitude read and written once; CZ: the control-set half)"
     del sim
+    out["device_copy_GBps"] = copy_bandwidth()
+    log(f"device-to-device copy: {out['device_copy_GBps']:.0f} GB/s")
     return out
+
+
+def copy_bandwidth() -> float:
+    """Reference point for the per-gate figures: a plain 4 GiB
+    device-to-device copy (read + write bytes / time), best of 3."""
+    import torch
+
+    src = torch.ones(2 ** 29, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    best = float("inf")
+    for _ in range(3):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        e1.synchronize()
+        best = min(best, e0.elapsed_time(e1) * 1e-3)
+    del src, dst
+    torch.cuda.empty_cache()
+    return round(2 * 8.0 * 2 ** 29 / best / 1e9, 1)
 
 
 def graph_leg(reps: int):

@@ -116,6 +116,7 @@ _SIGS = {
     ),
     "bsim_batch_run": (C.c_int, [C.POINTER(Task), C.c_int, _dp]),
     "bsim_set_checkpoint_interval": (C.c_int, [C.c_int, C.POINTER(C.c_int)]),
+    "bsim_set_adjoint_fusion": (C.c_int, [C.c_int, C.POINTER(C.c_int)]),
     "bsim_dm_apply_superop": (C.c_int, [C.c_void_p, C.c_int, _ip, _dp]),
     "bsim_graph_create": (C.c_int, [C.c_void_p, C.POINTER(Op), C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     "bsim_graph_launch": (C.c_int, [C.c_void_p]),

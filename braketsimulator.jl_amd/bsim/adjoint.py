@@ -53,18 +53,22 @@ def with_params(instructions, param_map, theta):
     return out
 
 
-def adjoint_gradient(sim, instructions, param_map, words, coeffs, n_params=None, checkpoint_every=0):
+def adjoint_gradient(sim, instructions, param_map, words, coeffs, n_params=None, checkpoint_every=0, fused=True):
     """(<H>, dH/dtheta) for the circuit from |0...0>.  ``param_map`` maps
     (instruction index, angle index) -> parameter index.  On a
     ``DensityMatrixSimulator`` the circuit may hold noise; the forward pass is
     checkpointed every ``checkpoint_every`` ops (0: ceil(sqrt(ops))) and
-    recomputed segment by segment.  The simulator keeps the final state."""
+    recomputed segment by segment.  State vectors: ``fused`` runs the
+    backward sweep as tile passes (both vectors in LDS), otherwise one
+    kernel per op.  The simulator keeps the final state."""
     n_params = (max(param_map.values()) + 1 if param_map else 0) if n_params is None else n_params
     kind = getattr(sim, "kind", L.BSIM_SV)
     ol = lower_instructions(list(instructions), kind, noise_mode=getattr(sim, "noise_mode", "reference"),
                             param_map=param_map)
     old = C.c_int()
     L.call("bsim_set_checkpoint_interval", int(checkpoint_every), C.byref(old))
+    old_f = C.c_int()
+    L.call("bsim_set_adjoint_fusion", 1 if fused else 0, C.byref(old_f))
     grad = np.zeros(n_params, dtype=np.float64)
     c = np.ascontiguousarray(coeffs, dtype=np.float64)
     ev = C.c_double()
@@ -72,6 +76,7 @@ def adjoint_gradient(sim, instructions, param_map, words, coeffs, n_params=None,
            c.ctypes.data_as(C.POINTER(C.c_double)), n_params, grad.ctypes.data_as(C.POINTER(C.c_double)),
            C.byref(ev))
     L.call("bsim_set_checkpoint_interval", old.value, None)
+    L.call("bsim_set_adjoint_fusion", old_f.value, None)
     return ev.value, grad
 
 

@@ -44,7 +44,7 @@ FLAGS = [
 # Structurizing those uniform branches makes the compiler keep two copies of
 # the sub-cube and copy it back on every op; leaving uniform regions as plain
 # scalar branches keeps it in place (about 10% faster passes).
-TILE_FILES = {"fused.hip", "batch.hip"}
+TILE_FILES = {"fused.hip", "batch.hip", "adj_tile.hip"}
 TILE_FLAGS = ["-mllvm", "-structurizecfg-skip-uniform-regions=true"]
 
 

@@ -313,6 +313,7 @@ __global__ __launch_bounds__(AB) void k_dm_pauli_op(double2* __restrict__ lam, i
 }
 
 static int g_checkpoint_every = 0;  // 0: ceil(sqrt(ops))
+static int g_adj_fusion = 1;        // state vectors: fused tile backward sweep
 
 struct DmParam {
     int pidx;
@@ -564,15 +565,32 @@ int bsim_adjoint_gradient(bsim_state* h, const bsim_op* ops, int n_ops, int n_wo
     BSIM_HIP(hipMemcpyAsync(work, h->d, sizeof(double2) * (size_t)len, hipMemcpyDeviceToDevice, h->stream));
     // backward
     int slot = nslots;
-    for (int k = (int)eops.size() - 1; k >= 0; --k) {
-        const EOp& e = eops[k];
-        double* r = partials;  // unused by parameter-free ops
-        if (e.nparams > 0) r = partials + (size_t)(--slot) * grid * 3;
-        if (int rc = adjoint_step(h, work, lam, e, r, grid)) return rc;
-    }
-    if (nslots > 0) {
-        hipLaunchKernelGGL(k_adj_finish, dim3(nslots), dim3(256), 0, h->stream, (const double*)partials, grid, res);
-        BSIM_HIP(hipGetLastError());
+    if (g_adj_fusion && h->vq >= 5) {
+        // tile passes over the reversed list of U^dagger ops (both vectors in
+        // LDS, parameter products reduced inside the pass)
+        const int L = (int)eops.size();
+        std::vector<EOp> back;
+        Plan bplan;
+        int ns = 0, pm = 0;  // ns == nslots
+        if (int rc = adjoint_back_plan(h->vq, eops, back, bplan, ns, pm)) return rc;
+        auto single = [&](int i) -> int {
+            const EOp& e = eops[L - 1 - i];
+            if (e.nparams > 0)
+                return fail(BSIM_ENOTSUP, "adjoint: parametrized gates act on at most 2 dense or 4 diagonal targets");
+            return adjoint_step(h, work, lam, e, nullptr, grid);  // parameter-free: no partials
+        };
+        if (int rc = adjoint_tile_run(h, work, lam, bplan, nslots, pm, res, single)) return rc;
+    } else {
+        for (int k = (int)eops.size() - 1; k >= 0; --k) {
+            const EOp& e = eops[k];
+            double* r = partials;  // unused by parameter-free ops
+            if (e.nparams > 0) r = partials + (size_t)(--slot) * grid * 3;
+            if (int rc = adjoint_step(h, work, lam, e, r, grid)) return rc;
+        }
+        if (nslots > 0) {
+            hipLaunchKernelGGL(k_adj_finish, dim3(nslots), dim3(256), 0, h->stream, (const double*)partials, grid, res);
+            BSIM_HIP(hipGetLastError());
+        }
     }
     std::vector<double> hres(3 * (size_t)nslots + 2);
     BSIM_HIP(hipMemcpyAsync(hres.data(), res, sizeof(double) * hres.size(), hipMemcpyDeviceToHost, h->stream));
@@ -586,6 +604,12 @@ int bsim_adjoint_gradient(bsim_state* h, const bsim_op* ops, int n_ops, int n_wo
     }
     if (expval) *expval = hres[3 * (size_t)nslots];
     h->alias_ready = false;
+    return BSIM_OK;
+}
+
+int bsim_set_adjoint_fusion(int on, int* old) {
+    if (old) *old = g_adj_fusion;
+    g_adj_fusion = on ? 1 : 0;
     return BSIM_OK;
 }
 

@@ -2,6 +2,7 @@
 // HBM passes (LDS tiles) -> register sweeps.  Shared with the adjoint sweep.
 #pragma once
 
+#include <functional>
 #include <vector>
 
 #include "common.h"
@@ -33,6 +34,7 @@ struct EOp {
     int nparams = 0;
     int pidx[3] = { -1, -1, -1 };
     std::vector<double2> dm[3];
+    int aslot = -1;              // fused adjoint: partial-sum slot of the op's parameters
     uint64_t touched() const;    // every vector bit the op reads, writes or depends on
     bool fusable() const;        // fits a tile sweep (dense k <= 2, diag k <= 4, reference Kraus kq <= 2)
 };
@@ -68,7 +70,9 @@ struct TOp {
     uint64_t tmask, tval;
     uint64_t sgm[4];
     uint32_t jmask;
-    int32_t pad2_;
+    // fused adjoint: the op's derivative matrices follow as `nparam` kind-5
+    // TOps (m = dU in this op's layout, a = partial-sum slot, b = parameter)
+    int32_t nparam;
     double2 m[16];
 };
 
@@ -137,5 +141,20 @@ int launch_eop(bsim_state* h, double2* d, const EOp& e);
 
 // Kraus channel with reference (block-diagonal) semantics.
 int launch_kraus_ref(bsim_state* h, double2* d, const EOp& e);
+
+// Backward list of the state-vector adjoint: back[i] = U^dagger of
+// eops[L - 1 - i], parametrized ops numbered (aslot) in forward order, and
+// its plan (11 tile bits, 3 register bits).  nslots: parametrized ops; pm:
+// their largest parameter count.
+int adjoint_back_plan(int vq, const std::vector<EOp>& eops, std::vector<EOp>& back, Plan& plan, int& nslots,
+                      int& pm);
+
+// Fused backward sweep of the state-vector adjoint: runs `plan` (a plan of
+// the reversed U^dagger ops, 11 tile bits, 3 register bits; parametrized ops
+// carry aslot and their derivative payloads) on (psi, lam); a non-tile step
+// calls single(op index).  Writes out3[slot * 3 + p] = Re <lam|dU_p|psi> of
+// every slot (pm: the largest parameter count of an op).
+int adjoint_tile_run(bsim_state* h, double2* psi, double2* lam, const Plan& plan, int nslots, int pm, double* out3,
+                     const std::function<int(int)>& single);
 
 }  // namespace bsim

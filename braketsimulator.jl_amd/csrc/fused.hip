@@ -67,7 +67,13 @@ int lower_ops(int nq, int kind, const bsim_op* ops, int n_ops, std::vector<EOp>&
         if (op.type == BSIM_OP_GATE || op.type == BSIM_OP_DIAG) {
             bool diag = op.type == BSIM_OP_DIAG;
             const double* m = op.matrix;
-            if (!diag && k <= 4 && is_diagonal(m, k)) {
+            // a gate that is diagonal at this angle runs as a diagonal op only
+            // when its derivatives are diagonal too (Rx(0) = I, dRx/dtheta is not)
+            bool d_diag = true;
+            if (keep_params)
+                for (int j = 0; j < op.n_params && j < 3; ++j)
+                    if (op.dmatrix[j] && !is_diagonal(op.dmatrix[j], k)) d_diag = false;
+            if (!diag && k <= 4 && d_diag && is_diagonal(m, k)) {
                 int D = 1 << k;
                 tmp.resize(2 * (size_t)D);
                 for (int j = 0; j < D; ++j) {
@@ -413,7 +419,8 @@ static void fuse_sweep(const std::vector<EOp>& eops, const std::vector<int>& tak
         bool outside = false;
         // a reference Kraus op depends on every bit: nothing moves across it
         const uint32_t R = e.kind == E_KRAUS_REF ? 0xFu : op_regs(e, reg, &outside);
-        const bool fusable = !outside && (e.kind == E_DENSE || e.kind == E_DIAG) && popc(R) <= 2 && R != 0;
+        const bool fusable =
+            !outside && (e.kind == E_DENSE || e.kind == E_DIAG) && popc(R) <= 2 && R != 0 && e.aslot < 0;
         uint32_t U = R;
         for (auto& b : pend)
             if (b.regs & R) U |= b.regs;
@@ -583,7 +590,7 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
                 op.cval = e.g.cval;
                 uint64_t pm = 0;
                 int pd = 0;
-                if (e.kind == E_DIAG && sign_pair(e, pm, pd)) {
+                if (e.kind == E_DIAG && e.aslot < 0 && sign_pair(e, pm, pd)) {
                     // consecutive sign ops multiply: their parities add modulo 2
                     // (pair masks of the same bit distance add, others are kept
                     // as separate terms, up to 4)
@@ -673,6 +680,34 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
                                 for (int c = 0; c < 4; ++c) op.m[r + 4 * c] = e.g.m[sw2(r) + 4 * sw2(c)];
                         }
                     }
+                }
+                if (e.aslot >= 0 && e.nparams > 0) {
+                    // fused adjoint: derivative matrices, laid out like the op's own matrix
+                    op.nparam = e.nparams;
+                    plan.tops.push_back(op);
+                    plan.top_src.push_back(i);
+                    for (int p = 0; p < e.nparams; ++p) {
+                        TOp d;
+                        memset(&d, 0, sizeof(d));
+                        d.kind = 5;
+                        d.a = e.aslot;
+                        d.b = p;
+                        const int D = 1 << e.g.k;
+                        if (op.kind == 2) {
+                            for (int j = 0; j < D; ++j) d.m[j] = e.dm[p][j + D * j];
+                        } else if (op.kind == 0) {
+                            for (int j = 0; j < 4; ++j) d.m[j] = e.dm[p][j];
+                        } else {
+                            const int a0 = reg(e.g.tb[0]), b0 = reg(e.g.tb[1]);
+                            auto sw2 = [](int x) { return ((x & 1) << 1) | ((x >> 1) & 1); };
+                            for (int r = 0; r < 4; ++r)
+                                for (int c = 0; c < 4; ++c)
+                                    d.m[r + 4 * c] = a0 < b0 ? e.dm[p][r + 4 * c] : e.dm[p][sw2(r) + 4 * sw2(c)];
+                        }
+                        plan.tops.push_back(d);
+                        plan.top_src.push_back(i);
+                    }
+                    return;
                 }
                 plan.tops.push_back(op);
                 plan.top_src.push_back(i);

@@ -10,6 +10,110 @@
 #include "fused.h"
 using namespace bsim;
 typedef std::complex<double> cd;
+// Apply tile op o of plan p (register bits vb[0..R), their union G) to the
+// full vector v.
+static void apply_top(const Plan& p, int o, const int* vb, uint64_t G, uint64_t N, cd* v) {
+    const TOp& op = p.tops[o];
+    auto M = [&](int j) { return cd(op.m[j].x, op.m[j].y); };
+    auto pred = [&](uint64_t i) { return ((i ^ (i >> op.pd[0])) & op.pm[0]) == 0; };
+    if (op.kind == 5) return;  // matrix columns of the preceding kind-4 op
+    if (op.kind == 4 && op.nd == 0) {
+        uint64_t ba = 1ull << vb[op.a], bb = 1ull << vb[op.b];
+        for (uint64_t i = 0; i < N; ++i) if (!(i & ba) && !(i & bb) && pred(i)) {
+            uint64_t id[4] = {i, i | ba, i | bb, i | ba | bb};
+            cd x[4], y[4];
+            for (int k = 0; k < 4; ++k) x[k] = v[id[k]];
+            for (int r = 0; r < 4; ++r) { y[r] = 0; for (int c = 0; c < 4; ++c) y[r] += M(r + 4 * c) * x[c]; }
+            for (int k = 0; k < 4; ++k) v[id[k]] = y[k];
+        }
+    } else if (op.kind == 4) {
+        for (uint64_t i = 0; i < N; ++i) {
+            if (i & G) continue;
+            if (!pred(i)) continue;
+            uint64_t id[16];
+            for (int j = 0; j < 16; ++j) {
+                id[j] = i;
+                for (int r = 0; r < 4; ++r) if ((j >> r) & 1) id[j] |= 1ull << vb[r];
+            }
+            cd x[16], y[16];
+            for (int k = 0; k < 16; ++k) x[k] = v[id[k]];
+            for (int r = 0; r < 16; ++r) {
+                y[r] = 0;
+                for (int c = 0; c < 16; ++c) y[r] += cd(p.tops[o + c].m[r].x, p.tops[o + c].m[r].y) * x[c];
+            }
+            for (int k = 0; k < 16; ++k) v[id[k]] = y[k];
+        }
+    } else if (op.kind == 0) {
+        uint64_t bit = 1ull << vb[op.a];
+        for (uint64_t i = 0; i < N; ++i) if (!(i & bit)) {
+            if (op.cmask && ((i & op.cmask) != op.cval)) continue;
+            cd a = v[i], b = v[i | bit];
+            v[i] = M(0) * a + M(2) * b; v[i | bit] = M(1) * a + M(3) * b;
+        }
+    } else if (op.kind == 1) {
+        uint64_t ba = 1ull << vb[op.a], bb = 1ull << vb[op.b];
+        for (uint64_t i = 0; i < N; ++i) if (!(i & ba) && !(i & bb)) {
+            if (op.cmask && ((i & op.cmask) != op.cval)) continue;
+            uint64_t id[4] = {i, i | ba, i | bb, i | ba | bb};
+            cd x[4], y[4];
+            for (int k = 0; k < 4; ++k) x[k] = v[id[k]];
+            for (int r = 0; r < 4; ++r) { y[r] = 0; for (int c = 0; c < 4; ++c) y[r] += M(r + 4 * c) * x[c]; }
+            for (int k = 0; k < 4; ++k) v[id[k]] = y[k];
+        }
+    } else if (op.kind == 3) {
+        for (uint64_t i = 0; i < N; ++i) {
+            int par = 0;
+            for (int k = 0; k < op.npairs; ++k) par ^= __builtin_popcountll(i & (i >> op.pd[k]) & op.pm[k]) & 1;
+            if (par) v[i] = -v[i];
+        }
+    } else {
+        for (uint64_t i = 0; i < N; ++i) {
+            if (op.cmask && ((i & op.cmask) != op.cval)) continue;
+            int idx = 0;
+            for (int b = 0; b < op.nd; ++b) idx |= (int)((i >> op.dbit[b]) & 1) << b;
+            v[i] *= M(idx);
+        }
+    }
+    }
+
+// The per-thread / per-register split of every index test of sweep S must
+// agree with the direct test on every index.
+static int check_sweep(const Plan& p, const Sweep& S, int R, const int* vb, uint64_t G, uint64_t N) {
+    for (int o = S.op_begin; o < S.op_end; ++o) {
+        const TOp& op = p.tops[o];
+        if ((op.kind == 0 || op.kind == 1 || (op.kind == 4 && op.nd == 0)) && (op.a >= R || op.b >= R))
+            return -15;  // a register bit the pass's sweeps do not have
+        for (uint64_t i = 0; i < N; ++i) {
+            const uint64_t gbase = i & ~G;
+            int j = 0;
+            for (int r = 0; r < R; ++r) j |= (int)((i >> vb[r]) & 1) << r;
+            if (op.kind == 3) {
+                int direct = 0, split = (int)((op.jmask >> j) & 1);
+                for (int k = 0; k < op.npairs; ++k) {
+                    direct ^= __builtin_popcountll(i & (i >> op.pd[k]) & op.pm[k]) & 1;
+                    split ^= __builtin_popcountll(gbase & (gbase >> op.pd[k]) & op.pm[k]) & 1;
+                }
+                for (int r = 0; r < R; ++r)
+                    if ((j >> r) & 1) split ^= __builtin_popcountll(gbase & op.sgm[r]) & 1;
+                if (direct != split) return -10;
+            } else if (op.kind <= 1 && op.cmask) {
+                const bool direct = (i & op.cmask) == op.cval;
+                const bool split = ((gbase & op.tmask) == op.tval) && ((op.jmask >> j) & 1);
+                if (direct != split) return -11;
+            } else if (op.kind == 4) {
+                const bool direct = ((i ^ (i >> op.pd[0])) & op.pm[0]) == 0;
+                if (R != 4) return -14;  // kind 4 runs in 4-register passes only
+                uint64_t u = 0;
+                for (int r = 0; r < 4; ++r)
+                    if ((j >> r) & 1) u ^= op.sgm[r];
+                const bool split = ((gbase ^ (gbase >> op.pd[0])) & op.pm[0]) == u;
+                if (direct != split) return -12;
+            }
+        }
+    }
+    return 0;
+}
+
 static int run(int nq, int kind, const bsim_op* ops, int n, double* state) {
     std::vector<EOp> e;
     if (int rc = lower_ops(nq, kind, ops, n, e, false)) return rc;
@@ -29,103 +133,8 @@ static int run(int nq, int kind, const bsim_op* ops, int n, double* state) {
             for (int r = 0; r < R; ++r) vb[r] = P.lbits[S.sb[r]];
             uint64_t G = 0;
             for (int r = 0; r < R; ++r) G |= 1ull << vb[r];
-            // the per-thread / per-register split of every index test must
-            // agree with the direct test on every index
-            for (int o = S.op_begin; o < S.op_end; ++o) {
-                const TOp& op = p.tops[o];
-                if ((op.kind == 0 || op.kind == 1 || (op.kind == 4 && op.nd == 0)) && (op.a >= R || op.b >= R))
-                    return -15;  // a register bit the pass's sweeps do not have
-                for (uint64_t i = 0; i < N; ++i) {
-                    const uint64_t gbase = i & ~G;
-                    int j = 0;
-                    for (int r = 0; r < R; ++r) j |= (int)((i >> vb[r]) & 1) << r;
-                    if (op.kind == 3) {
-                        int direct = 0, split = (int)((op.jmask >> j) & 1);
-                        for (int k = 0; k < op.npairs; ++k) {
-                            direct ^= __builtin_popcountll(i & (i >> op.pd[k]) & op.pm[k]) & 1;
-                            split ^= __builtin_popcountll(gbase & (gbase >> op.pd[k]) & op.pm[k]) & 1;
-                        }
-                        for (int r = 0; r < R; ++r)
-                            if ((j >> r) & 1) split ^= __builtin_popcountll(gbase & op.sgm[r]) & 1;
-                        if (direct != split) return -10;
-                    } else if (op.kind <= 1 && op.cmask) {
-                        const bool direct = (i & op.cmask) == op.cval;
-                        const bool split = ((gbase & op.tmask) == op.tval) && ((op.jmask >> j) & 1);
-                        if (direct != split) return -11;
-                    } else if (op.kind == 4) {
-                        const bool direct = ((i ^ (i >> op.pd[0])) & op.pm[0]) == 0;
-                        if (R != 4) return -14;  // kind 4 runs in 4-register passes only
-                        uint64_t u = 0;
-                        for (int r = 0; r < 4; ++r)
-                            if ((j >> r) & 1) u ^= op.sgm[r];
-                        const bool split = ((gbase ^ (gbase >> op.pd[0])) & op.pm[0]) == u;
-                        if (direct != split) return -12;
-                    }
-                }
-            }
-            for (int o = S.op_begin; o < S.op_end; ++o) {
-                const TOp& op = p.tops[o];
-                auto M = [&](int j) { return cd(op.m[j].x, op.m[j].y); };
-                auto pred = [&](uint64_t i) { return ((i ^ (i >> op.pd[0])) & op.pm[0]) == 0; };
-                if (op.kind == 5) continue;  // matrix columns of the preceding kind-4 op
-                if (op.kind == 4 && op.nd == 0) {
-                    uint64_t ba = 1ull << vb[op.a], bb = 1ull << vb[op.b];
-                    for (uint64_t i = 0; i < N; ++i) if (!(i & ba) && !(i & bb) && pred(i)) {
-                        uint64_t id[4] = {i, i | ba, i | bb, i | ba | bb};
-                        cd x[4], y[4];
-                        for (int k = 0; k < 4; ++k) x[k] = v[id[k]];
-                        for (int r = 0; r < 4; ++r) { y[r] = 0; for (int c = 0; c < 4; ++c) y[r] += M(r + 4 * c) * x[c]; }
-                        for (int k = 0; k < 4; ++k) v[id[k]] = y[k];
-                    }
-                } else if (op.kind == 4) {
-                    for (uint64_t i = 0; i < N; ++i) {
-                        if (i & G) continue;
-                        if (!pred(i)) continue;
-                        uint64_t id[16];
-                        for (int j = 0; j < 16; ++j) {
-                            id[j] = i;
-                            for (int r = 0; r < 4; ++r) if ((j >> r) & 1) id[j] |= 1ull << vb[r];
-                        }
-                        cd x[16], y[16];
-                        for (int k = 0; k < 16; ++k) x[k] = v[id[k]];
-                        for (int r = 0; r < 16; ++r) {
-                            y[r] = 0;
-                            for (int c = 0; c < 16; ++c) y[r] += cd(p.tops[o + c].m[r].x, p.tops[o + c].m[r].y) * x[c];
-                        }
-                        for (int k = 0; k < 16; ++k) v[id[k]] = y[k];
-                    }
-                } else if (op.kind == 0) {
-                    uint64_t bit = 1ull << vb[op.a];
-                    for (uint64_t i = 0; i < N; ++i) if (!(i & bit)) {
-                        if (op.cmask && ((i & op.cmask) != op.cval)) continue;
-                        cd a = v[i], b = v[i | bit];
-                        v[i] = M(0) * a + M(2) * b; v[i | bit] = M(1) * a + M(3) * b;
-                    }
-                } else if (op.kind == 1) {
-                    uint64_t ba = 1ull << vb[op.a], bb = 1ull << vb[op.b];
-                    for (uint64_t i = 0; i < N; ++i) if (!(i & ba) && !(i & bb)) {
-                        if (op.cmask && ((i & op.cmask) != op.cval)) continue;
-                        uint64_t id[4] = {i, i | ba, i | bb, i | ba | bb};
-                        cd x[4], y[4];
-                        for (int k = 0; k < 4; ++k) x[k] = v[id[k]];
-                        for (int r = 0; r < 4; ++r) { y[r] = 0; for (int c = 0; c < 4; ++c) y[r] += M(r + 4 * c) * x[c]; }
-                        for (int k = 0; k < 4; ++k) v[id[k]] = y[k];
-                    }
-                } else if (op.kind == 3) {
-                    for (uint64_t i = 0; i < N; ++i) {
-                        int par = 0;
-                        for (int k = 0; k < op.npairs; ++k) par ^= __builtin_popcountll(i & (i >> op.pd[k]) & op.pm[k]) & 1;
-                        if (par) v[i] = -v[i];
-                    }
-                } else {
-                    for (uint64_t i = 0; i < N; ++i) {
-                        if (op.cmask && ((i & op.cmask) != op.cval)) continue;
-                        int idx = 0;
-                        for (int b = 0; b < op.nd; ++b) idx |= (int)((i >> op.dbit[b]) & 1) << b;
-                        v[i] *= M(idx);
-                    }
-                }
-            }
+            if (int rc = check_sweep(p, S, R, vb, G, N)) return rc;
+            for (int o = S.op_begin; o < S.op_end; ++o) apply_top(p, o, vb, G, N, v);
         }
     }
     return 0;
@@ -143,4 +152,129 @@ extern "C" int plan_singles(int nq, int kind, const bsim_op* ops, int n) {
     int k = 0;
     for (auto& st : p.steps) k += st.tile ? 0 : 1;
     return k;
+}
+
+// ---- fused adjoint backward plan (state vectors) ----------------------------
+
+// Re sum_i conj(lam_i) (dU psi)_i for tile op `op` with derivative payload d
+// (d.m in the op's own layout), over the indices where the op's controls hold.
+static double dot_top(const TOp& op, const TOp& d, const int* vb, uint64_t N, const cd* psi, const cd* lam) {
+    auto D = [&](int j) { return cd(d.m[j].x, d.m[j].y); };
+    auto ctl = [&](uint64_t i) { return op.cmask == 0 || (i & op.cmask) == op.cval; };
+    double acc = 0.0;
+    if (op.kind == 0) {
+        const uint64_t bit = 1ull << vb[op.a];
+        for (uint64_t i = 0; i < N; ++i)
+            if (!(i & bit) && ctl(i)) {
+                const cd a = psi[i], b = psi[i | bit];
+                acc += (std::conj(lam[i]) * (D(0) * a + D(2) * b) + std::conj(lam[i | bit]) * (D(1) * a + D(3) * b)).real();
+            }
+    } else if (op.kind == 1) {
+        const uint64_t ba = 1ull << vb[op.a], bb = 1ull << vb[op.b];
+        for (uint64_t i = 0; i < N; ++i)
+            if (!(i & ba) && !(i & bb) && ctl(i)) {
+                const uint64_t id[4] = { i, i | ba, i | bb, i | ba | bb };
+                for (int r = 0; r < 4; ++r) {
+                    cd y = 0;
+                    for (int c = 0; c < 4; ++c) y += D(r + 4 * c) * psi[id[c]];
+                    acc += (std::conj(lam[id[r]]) * y).real();
+                }
+            }
+    } else if (op.kind == 2) {
+        for (uint64_t i = 0; i < N; ++i)
+            if (ctl(i)) {
+                int idx = 0;
+                for (int b = 0; b < op.nd; ++b) idx |= (int)((i >> op.dbit[b]) & 1) << b;
+                acc += (std::conj(lam[i]) * D(idx) * psi[i]).real();
+            }
+    }
+    return acc;
+}
+
+// A non-tile step: the elementary op itself (dense or diagonal, any width).
+static void apply_eop_cpu(const EOp& e, uint64_t N, cd* v) {
+    const GateSpec& g = e.g;
+    const int D = 1 << g.k;
+    uint64_t tm = 0;
+    for (int b = 0; b < g.k; ++b) tm |= 1ull << g.tb[b];
+    std::vector<uint64_t> id(D);
+    std::vector<cd> x(D);
+    for (uint64_t i = 0; i < N; ++i) {
+        if ((i & tm) || (g.cmask && (i & g.cmask) != g.cval)) continue;
+        for (int j = 0; j < D; ++j) {
+            id[j] = i;
+            for (int b = 0; b < g.k; ++b)
+                if ((j >> b) & 1) id[j] |= 1ull << g.tb[b];
+            x[j] = v[id[j]];
+        }
+        for (int r = 0; r < D; ++r) {
+            cd y = 0;
+            if (g.diag) y = cd(g.m[r].x, g.m[r].y) * x[r];
+            else
+                for (int c = 0; c < D; ++c) y += cd(g.m[r + (size_t)D * c].x, g.m[r + (size_t)D * c].y) * x[c];
+            v[id[r]] = y;
+        }
+    }
+}
+
+// The backward sweep of the fused adjoint, interpreted on full vectors:
+// psi = the final state, lam = H psi on entry; grad[param] as
+// bsim_adjoint_gradient returns it.  Negative returns name a plan defect.
+extern "C" int adjexec(int nq, const bsim_op* ops, int n, int n_params, double* psi_, double* lam_, double* grad) {
+    std::vector<EOp> e;
+    if (int rc = lower_ops(nq, BSIM_SV, ops, n, e, true)) return rc;
+    std::vector<EOp> back;
+    Plan p;
+    int ns = 0, pm = 0;
+    if (int rc = adjoint_back_plan(nq, e, back, p, ns, pm)) return rc;
+    cd* psi = (cd*)psi_;
+    cd* lam = (cd*)lam_;
+    const uint64_t N = 1ull << nq;
+    std::vector<double> out(3 * (size_t)ns, 0.0);
+    std::vector<int> seen(3 * (size_t)ns, 0);
+    for (auto& st : p.steps) {
+        if (!st.tile) {
+            if (back[st.op].nparams) return -20;  // parametrized ops belong in tile passes
+            apply_eop_cpu(back[st.op], N, psi);
+            apply_eop_cpu(back[st.op], N, lam);
+            continue;
+        }
+        const Pass& P = p.passes[st.pass];
+        if (P.R != 3 || P.T > 11) return -21;  // the shape the backward tile kernel takes
+        for (int s = 0; s < P.nsweeps; ++s) {
+            const Sweep& S = p.sweeps[P.sweep_begin + s];
+            int vb[4] = { 0, 0, 0, 0 };
+            uint64_t G = 0;
+            for (int r = 0; r < 3; ++r) {
+                vb[r] = P.lbits[S.sb[r]];
+                G |= 1ull << vb[r];
+            }
+            if (int rc = check_sweep(p, S, 3, vb, G, N)) return rc;
+            for (int o = S.op_begin; o < S.op_end;) {
+                const TOp& op = p.tops[o];
+                if (op.kind == 5 || op.kind == 4) return -22;
+                if (o + 1 + op.nparam > S.op_end) return -23;
+                apply_top(p, o, vb, G, N, psi);
+                for (int q = 0; q < op.nparam; ++q) {
+                    const TOp& d = p.tops[o + 1 + q];
+                    if (d.kind != 5 || d.a < 0 || d.a >= ns || d.b != q) return -24;
+                    out[3 * (size_t)d.a + q] += dot_top(op, d, vb, N, psi, lam);
+                    ++seen[3 * (size_t)d.a + q];
+                }
+                apply_top(p, o, vb, G, N, lam);
+                o += 1 + op.nparam;
+            }
+        }
+    }
+    for (int q = 0; q < n_params; ++q) grad[q] = 0.0;
+    int slot = 0;
+    for (const EOp& x : e) {
+        if (x.nparams == 0) continue;
+        for (int q = 0; q < x.nparams; ++q) {
+            if (seen[3 * (size_t)slot + q] != 1) return -25;  // every product exactly once
+            grad[x.pidx[q]] += 2.0 * out[3 * (size_t)slot + q];
+        }
+        ++slot;
+    }
+    return 0;
 }

@@ -50,6 +50,9 @@ def interp():
         f.argtypes = [ctypes.c_int, ctypes.POINTER(L.Op), ctypes.c_int, ctypes.c_void_p]
     h.plan_singles.restype = ctypes.c_int
     h.plan_singles.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(L.Op), ctypes.c_int]
+    h.adjexec.restype = ctypes.c_int
+    h.adjexec.argtypes = [ctypes.c_int, ctypes.POINTER(L.Op), ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                          ctypes.c_void_p, ctypes.c_void_p]
     return h
 
 
@@ -172,3 +175,107 @@ def test_planner_dm_noisy_brickwall(interp, tile_config, tile, low, mode):
     L.call("bsim_set_tile_config", tile, low)
     ins = W.brickwall(6, 3, seed=2, noise=True)
     np.testing.assert_allclose(_run_dm(interp, 6, ins, mode), O.evolve_dm(6, ins, mode), atol=1e-12)
+
+
+# ---- fused adjoint: the backward plan ---------------------------------------
+
+_PAULI = {"X": np.array([[0, 1], [1, 0]], dtype=complex), "Y": np.array([[0, -1j], [1j, 0]]),
+          "Z": np.array([[1, 0], [0, -1]], dtype=complex)}
+
+
+def _h_apply(psi, n, words, coeffs):
+    out = np.zeros_like(psi)
+    for w, c in zip(words, coeffs):
+        v = psi.copy()
+        for q, ch in enumerate(w):
+            if ch != "I":
+                v = O.np_apply(v, n, _PAULI[ch], [q])
+        out += c * v
+    return out
+
+
+def _energy(n, ins, words, coeffs):
+    psi = O.evolve_sv(n, ins)
+    return sum(c * O.sv_expect_pauli(psi, n, w) for w, c in zip(words, coeffs))
+
+
+def _adj_interp(h, n, ins, pmap, words, coeffs, n_params):
+    """Gradient from the interpreted backward plan (libbsim's own
+    adjoint_back_plan: reversed U^dagger ops, payloads of dU)."""
+    psi = O.evolve_sv(n, ins).astype(np.complex128)
+    lam = _h_apply(psi, n, words, coeffs)
+    e_h = float(np.vdot(psi, lam).real)
+    assert abs(e_h - _energy(n, ins, words, coeffs)) < 1e-12  # pins _h_apply against the oracle's <P>
+    ol = lower_instructions(list(ins), L.BSIM_SV, param_map=pmap)
+    grad = np.zeros(n_params)
+    rc = h.adjexec(n, ol.ptr, ol.n, n_params, psi.ctypes.data, lam.ctypes.data, grad.ctypes.data)
+    assert rc == 0, f"adjexec returned {rc}"
+    return grad
+
+
+@pytest.mark.parametrize("n,layers", [(6, 2), (13, 2)])
+def test_adjoint_back_plan_hea(interp, n, layers):
+    """The backward plan of the hardware-efficient ansatz, interpreted on the
+    CPU, vs the oracle's parameter shift (exact for RY/RZ).  13 qubits: two
+    tiles of 11 bits per pass, parameters on bits outside the first tile."""
+    from bsim import adjoint as A
+
+    ins, pmap, th = W.hea(n, layers, seed=n)
+    words, coeffs = W.tfim(n)
+    grad = _adj_interp(interp, n, ins, pmap, words, coeffs, len(th))
+    ps = np.zeros(len(th))
+    for p in range(len(th)):
+        tp = th.copy(); tp[p] += np.pi / 2
+        tm = th.copy(); tm[p] -= np.pi / 2
+        ps[p] = 0.5 * (_energy(n, A.with_params(ins, pmap, tp), words, coeffs)
+                       - _energy(n, A.with_params(ins, pmap, tm), words, coeffs))
+    np.testing.assert_allclose(grad, ps, atol=1e-10)
+
+
+@pytest.mark.parametrize("n", [7, 12])
+def test_adjoint_back_plan_general_gates(interp, n):
+    """U(theta, phi, lambda) (three products per op), ZZ / CPhaseShift
+    (diagonal), controlled Rx with either control value, XX (dense 2-qubit),
+    Rx(0) (the identity, whose derivative is not diagonal) and a
+    parameter-free 3-qubit unitary (a non-tile step), vs central differences
+    of the oracle."""
+    from bsim import adjoint as A
+    from circuits import random_unitary
+
+    rng = np.random.default_rng(40 + n)
+    ins, pmap = [], {}
+    p = 0
+    for layer in range(3):
+        for q in range(n):
+            ins.append(Instruction(G.U(*rng.uniform(-3, 3, 3)), [q]))
+            for j in range(3):
+                pmap[(len(ins) - 1, j)] = p
+                p += 1
+        ins.append(Instruction(G.Rx(0.0), [layer % n]))
+        pmap[(len(ins) - 1, 0)] = p
+        p += 1
+        for q in range(n - 1):
+            a = float(rng.uniform(-3, 3))
+            g = [G.ZZ(a), G.CPhaseShift(a), G.Control(G.Rx(a), (1,)), G.XX(a), G.Control(G.Ry(a), (0,))][q % 5]
+            t = [q, q + 1] if q % 2 == 0 else [q + 1, (q + 5) % n]
+            if t[0] == t[1]:
+                t = [q, q + 1]
+            ins.append(Instruction(g, t))
+            pmap[(len(ins) - 1, 0)] = p
+            p += 1
+        ins.append(Instruction(G.Unitary(random_unitary(3, rng)), [0, n // 2, n - 1]))
+    th = np.zeros(p)
+    for (i, j), k in pmap.items():
+        th[k] = ins[i].operator.angles[j]
+    words, coeffs = W.tfim(n)
+    words = list(words) + ["Y" + "I" * (n - 2) + "X"]
+    coeffs = list(coeffs) + [0.3]
+    grad = _adj_interp(interp, n, ins, pmap, words, coeffs, p)
+    hstep = 1e-5
+    fd = np.zeros(p)
+    for k in range(p):
+        tp = th.copy(); tp[k] += hstep
+        tm = th.copy(); tm[k] -= hstep
+        fd[k] = (_energy(n, A.with_params(ins, pmap, tp), words, coeffs)
+                 - _energy(n, A.with_params(ins, pmap, tm), words, coeffs)) / (2 * hstep)
+    np.testing.assert_allclose(grad, fd, atol=1e-8)
