@@ -600,23 +600,62 @@ def kernel_call(g: Gate, targets) -> KernelCall:
     return _dense_call(g.matrix_rep(), targets)
 
 
-def derivative_matrices(g: Gate, targets) -> list:
-    """d(matrix)/d(angle_i) of the kernel matrix of ``g`` for every angle,
-    by an 8th-order central difference (error ~1e-14 for the trigonometric
-    entries of these gates)."""
-    out = []
+# Gates exp(-i theta/2 P) with P^2 = 1: the kernel matrix obeys
+# M(theta + pi) = -i P M(theta), so dM/dtheta = M(theta + pi) / 2 exactly.
+_HALF_ANGLE = ("Rx", "Ry", "Rz", "XX", "YY", "ZZ", "MultiRZ")
+# Two-shift rule: with D(s) = (M(theta + s) - M(theta - s)) / 2, a matrix whose
+# entries are trigonometric polynomials of theta with frequencies 0, 1/2 and
+# 1 has dM/dtheta = A D(pi) + D(pi/2), A = 1/2 - sin(pi/4): on frequency w the
+# right side is i (A sin(w pi) + sin(w pi/2)) = i w for w in {0, 1/2, 1}.
+_SHIFT_A = 0.5 - math.sqrt(0.5)
+
+
+def _pow_one(g: Gate) -> bool:
+    while True:
+        if g.pow_exponent != 1.0:
+            return False
+        if not isinstance(g, Control):
+            return True
+        g = g.g
+
+
+def _fd_derivative(g: Gate, targets, i: int) -> np.ndarray:
+    """8th-order central difference in angle i (error ~1e-14 for these
+    trigonometric entries)."""
     h = 1e-2
-    w = [4 / 5, -1 / 5, 4 / 105, -1 / 280]
-    for i in range(len(g.angles)):
-        acc = None
-        for k, wk in enumerate(w, start=1):
-            ap = list(g.angles)
-            am = list(g.angles)
-            ap[i] += k * h
-            am[i] -= k * h
-            mp = kernel_call(g.with_angles(ap), targets).matrix
-            mm = kernel_call(g.with_angles(am), targets).matrix
-            term = wk * (mp - mm)
-            acc = term if acc is None else acc + term
-        out.append(acc / h)
+    acc = None
+    for k, wk in enumerate((4 / 5, -1 / 5, 4 / 105, -1 / 280), start=1):
+        ap = list(g.angles)
+        am = list(g.angles)
+        ap[i] += k * h
+        am[i] -= k * h
+        term = wk * (kernel_call(g.with_angles(ap), targets).matrix - kernel_call(g.with_angles(am), targets).matrix)
+        acc = term if acc is None else acc + term
+    return acc / h
+
+
+def derivative_matrices(g: Gate, targets, which=None) -> list:
+    """d(kernel matrix)/d(angle i) of ``g`` for every angle i in ``which``
+    (default: all).  At pow_exponent 1 every angle of the library's gates
+    enters with frequencies 0, 1/2 and 1 (half-angle rotations, full-angle
+    phases), so the derivative is exact: one shifted matrix for the
+    exp(-i theta/2 P) family, the two-shift rule otherwise.  Other exponents
+    scale the frequencies: an 8th-order central difference."""
+    which = range(len(g.angles)) if which is None else which
+    out = []
+    exact = _pow_one(g)
+    for i in which:
+        if not exact:
+            out.append(_fd_derivative(g, targets, i))
+            continue
+
+        def at(shift):
+            a = list(g.angles)
+            a[i] += shift
+            return kernel_call(g.with_angles(a), targets).matrix
+
+        if type(g).__name__ in _HALF_ANGLE:
+            out.append(0.5 * at(math.pi))
+        else:
+            out.append(0.5 * (_SHIFT_A * (at(math.pi) - at(-math.pi)) + (at(0.5 * math.pi) - at(-0.5 * math.pi))))
     return out

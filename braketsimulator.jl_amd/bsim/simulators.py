@@ -149,9 +149,8 @@ def lower_instructions(instructions, kind: int, noise_mode: str = "reference", p
             if param_map is not None and op.parametrized:
                 idx = [(j, param_map[(i, j)]) for j in range(len(op.angles)) if (i, j) in param_map]
                 if idx:
-                    allm = derivative_matrices(op, ins.target)
                     params = [p for _, p in idx]
-                    dmats = [allm[j] for j, _ in idx]
+                    dmats = derivative_matrices(op, ins.target, [j for j, _ in idx])
             ol.add_call(call, params, dmats)
         elif isinstance(op, Superoperator):
             if kind != L.BSIM_DM:

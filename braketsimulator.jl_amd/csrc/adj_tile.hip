@@ -28,9 +28,27 @@ constexpr int AT_TB = 256;   // threads per workgroup (2^(11 - 3) sub-cubes)
 constexpr int AT_BITS = 8;   // log2(AT_TB)
 constexpr int AT_W = AT_TB / 64;
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+// DPP move of a double (two 32-bit moves): lanes whose source lies outside
+// their row, or whose row is masked off, receive 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_d(double v) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, ROW_MASK, 0xF, true);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, ROW_MASK, 0xF, true);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// Sum over the wavefront, complete in lane 63: an inclusive scan inside each
+// row of 16 lanes (row_shr 1, 2, 4, 8), then the row totals carried by
+// row_bcast:15 (rows 1, 3) and row_bcast:31 (rows 2, 3).  VALU moves only,
+// no LDS round trips.
+__device__ __forceinline__ double wave_sum63(double v) {
+    v += dpp_d<0x111, 0xF>(v);
+    v += dpp_d<0x112, 0xF>(v);
+    v += dpp_d<0x114, 0xF>(v);
+    v += dpp_d<0x118, 0xF>(v);
+    v += dpp_d<0x142, 0xA>(v);
+    v += dpp_d<0x143, 0xC>(v);
     return v;
 }
 
@@ -109,6 +127,20 @@ __device__ __forceinline__ double dot_diag(const SubCube<3>& sp, const SubCube<3
     return acc;
 }
 
+__device__ __forceinline__ double dot_rdiag(const SubCube<3>& sp, const SubCube<3>& sl, const TOp* __restrict__ op,
+                                            const TOp* __restrict__ d) {
+    const bool tok = op->cmask == 0 || (sp.gbase & op->tmask) == op->tval;
+    const uint32_t jm = op->cmask ? op->jmask : 0xFFu;
+    double acc = 0.0;
+    sfor<0, 8>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        const c2 mu = cmul(to_c2(d->m[j]), getv<j>(sp.v));
+        const c2 l = getv<j>(sl.v);
+        acc += (tok && ((jm >> j) & 1u)) ? (l.x * mu.x + l.y * mu.y) : 0.0;
+    });
+    return acc;
+}
+
 __device__ __forceinline__ double adj_dot(const SubCube<3>& sp, const SubCube<3>& sl, const TOp* __restrict__ op,
                                           const TOp* __restrict__ d) {
     switch (op->kind * 16 + op->a * 4 + op->b) {
@@ -118,6 +150,7 @@ __device__ __forceinline__ double adj_dot(const SubCube<3>& sp, const SubCube<3>
         case 16 + 1: return dot_dense2<0, 1>(sp, sl, op, d);
         case 16 + 2: return dot_dense2<0, 2>(sp, sl, op, d);
         case 16 + 6: return dot_dense2<1, 2>(sp, sl, op, d);
+        case 96: return dot_rdiag(sp, sl, op, d);
         default: return dot_diag(sp, sl, op, d);
     }
 }
@@ -204,9 +237,8 @@ __global__ __launch_bounds__(AT_TB) void k_adj_tile(double2* __restrict__ psi, d
             run_op<3, false>(sp, op, tp, pl, o0, o1, o2, 0);
             for (int p = 0; p < np; ++p) {
                 const TOp* d = op + 1 + p;
-                double v = adj_dot(sp, sl, op, d);
-                v = wave_sum(v);
-                if (lane == 0)
+                const double v = wave_sum63(adj_dot(sp, sl, op, d));
+                if (lane == 63)
                     partials[((size_t)(d->a * pm + d->b) * ntiles + tile_index) * AT_W + wave] = v;
             }
             run_op<3, false>(sl, op, tl, pl, o0, o1, o2, 0);

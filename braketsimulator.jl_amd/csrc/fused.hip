@@ -584,6 +584,12 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
             };
             auto emit_one = [&](int i) {
                 const EOp& e = eops[i];
+                // diagonal index of register j (kind 6)
+                auto diag_index = [&](int j) {
+                    int idx = 0;
+                    for (int b = 0; b < e.g.k; ++b) idx |= ((j >> reg(e.g.tb[b])) & 1) << b;
+                    return idx;
+                };
                 TOp op;
                 memset(&op, 0, sizeof(op));
                 op.cmask = e.g.cmask;
@@ -656,10 +662,19 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
                         return;
                     }
                 } else if (e.kind == E_DIAG) {
-                    op.kind = 2;
-                    op.nd = e.g.k;
-                    for (int b = 0; b < e.g.k; ++b) op.dbit[b] = e.g.tb[b];
-                    for (int j = 0; j < (1 << e.g.k); ++j) op.m[j] = e.g.m[j];
+                    bool inreg = true;
+                    for (int b = 0; b < e.g.k; ++b) inreg = inreg && reg(e.g.tb[b]) >= 0;
+                    if (inreg) {
+                        // every diagonal bit is a register bit: the factor of each
+                        // register is fixed for the sweep
+                        op.kind = 6;
+                        for (int j = 0; j < NRJ; ++j) op.m[j] = e.g.m[diag_index(j)];
+                    } else {
+                        op.kind = 2;
+                        op.nd = e.g.k;
+                        for (int b = 0; b < e.g.k; ++b) op.dbit[b] = e.g.tb[b];
+                        for (int j = 0; j < (1 << e.g.k); ++j) op.m[j] = e.g.m[j];
+                    }
                 } else {
                     if (e.g.k == 1) {
                         op.kind = 0;
@@ -695,6 +710,8 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
                         const int D = 1 << e.g.k;
                         if (op.kind == 2) {
                             for (int j = 0; j < D; ++j) d.m[j] = e.dm[p][j + D * j];
+                        } else if (op.kind == 6) {
+                            for (int j = 0; j < NRJ; ++j) d.m[j] = e.dm[p][diag_index(j) * (D + 1)];
                         } else if (op.kind == 0) {
                             for (int j = 0; j < 4; ++j) d.m[j] = e.dm[p][j];
                         } else {

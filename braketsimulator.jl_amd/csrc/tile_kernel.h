@@ -209,6 +209,24 @@ __device__ __forceinline__ void sw_diag(SubCube<R>& s, const TOp* __restrict__ o
     });
 }
 
+// Diagonal on register bits only (kind 6): register j is scaled by m[j], the
+// same entries for every thread (scalar loads).
+template <int R>
+__device__ __forceinline__ void sw_rdiag(SubCube<R>& s, const TOp* __restrict__ op, const OpHead& hd) {
+    bool tok = true;
+    uint32_t jm = 0xFFFFu;
+    if (hd.cmask) {
+        tok = (s.gbase & op->tmask) == op->tval;
+        jm = op->jmask;
+    }
+    sfor<0, (1 << R)>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        const c2 a = getv<j>(s.v);
+        const c2 x = cmul(to_c2(op->m[j]), a);
+        setv<j>(s.v, csel(tok && ((jm >> j) & 1u), x, a));
+    });
+}
+
 __device__ __forceinline__ uint32_t xor_and(uint32_t a, uint32_t b, uint32_t c) {
     // a ^ (b & c) in one v_bitop3 (truth table over a = 0xF0, b = 0xCC, c = 0xAA)
     return __builtin_amdgcn_bitop3_b32(a, b, c, (unsigned char)(0xF0 ^ (0xCC & 0xAA)));
@@ -344,7 +362,10 @@ __device__ __forceinline__ int run_op(SubCube<R>& s, const TOp* __restrict__ op,
             case 16 + 6: sw_dense2<R, 1, 2>(s, op, hd); break;
             case 16 + 7: sw_dense2<R, 1, 3>(s, op, hd); break;
             case 16 + 11: sw_dense2<R, 2, 3>(s, op, hd); break;
-            default: sw_diag<R>(s, op); break;
+            default:  // kinds 2 and 6 (kept out of the jump table)
+                if (hd.kind == 6) sw_rdiag<R>(s, op, hd);
+                else sw_diag<R>(s, op);
+                break;
         }
     } else {
         switch (hd.kind * 16 + hd.a * 4 + hd.b) {
@@ -355,7 +376,10 @@ __device__ __forceinline__ int run_op(SubCube<R>& s, const TOp* __restrict__ op,
             case 16 + 1: sw_dense2<R, 0, 1>(s, op, hd); break;
             case 16 + 2: sw_dense2<R, 0, 2>(s, op, hd); break;
             case 16 + 6: sw_dense2<R, 1, 2>(s, op, hd); break;
-            default: sw_diag<R>(s, op); break;
+            default:  // kinds 2 and 6 (kept out of the jump table)
+                if (hd.kind == 6) sw_rdiag<R>(s, op, hd);
+                else sw_diag<R>(s, op);
+                break;
         }
     }
     return 1;

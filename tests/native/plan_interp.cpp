@@ -10,9 +10,9 @@
 #include "fused.h"
 using namespace bsim;
 typedef std::complex<double> cd;
-// Apply tile op o of plan p (register bits vb[0..R), their union G) to the
+// Apply tile op o of plan p (R register bits vb[0..R), their union G) to the
 // full vector v.
-static void apply_top(const Plan& p, int o, const int* vb, uint64_t G, uint64_t N, cd* v) {
+static void apply_top(const Plan& p, int o, int R, const int* vb, uint64_t G, uint64_t N, cd* v) {
     const TOp& op = p.tops[o];
     auto M = [&](int j) { return cd(op.m[j].x, op.m[j].y); };
     auto pred = [&](uint64_t i) { return ((i ^ (i >> op.pd[0])) & op.pm[0]) == 0; };
@@ -60,6 +60,13 @@ static void apply_top(const Plan& p, int o, const int* vb, uint64_t G, uint64_t 
             for (int r = 0; r < 4; ++r) { y[r] = 0; for (int c = 0; c < 4; ++c) y[r] += M(r + 4 * c) * x[c]; }
             for (int k = 0; k < 4; ++k) v[id[k]] = y[k];
         }
+    } else if (op.kind == 6) {
+        for (uint64_t i = 0; i < N; ++i) {
+            if (op.cmask && ((i & op.cmask) != op.cval)) continue;
+            int j = 0;
+            for (int r = 0; r < R; ++r) j |= (int)((i >> vb[r]) & 1) << r;
+            v[i] *= M(j);
+        }
     } else if (op.kind == 3) {
         for (uint64_t i = 0; i < N; ++i) {
             int par = 0;
@@ -96,7 +103,7 @@ static int check_sweep(const Plan& p, const Sweep& S, int R, const int* vb, uint
                 for (int r = 0; r < R; ++r)
                     if ((j >> r) & 1) split ^= __builtin_popcountll(gbase & op.sgm[r]) & 1;
                 if (direct != split) return -10;
-            } else if (op.kind <= 1 && op.cmask) {
+            } else if ((op.kind <= 1 || op.kind == 6) && op.cmask) {
                 const bool direct = (i & op.cmask) == op.cval;
                 const bool split = ((gbase & op.tmask) == op.tval) && ((op.jmask >> j) & 1);
                 if (direct != split) return -11;
@@ -134,7 +141,7 @@ static int run(int nq, int kind, const bsim_op* ops, int n, double* state) {
             uint64_t G = 0;
             for (int r = 0; r < R; ++r) G |= 1ull << vb[r];
             if (int rc = check_sweep(p, S, R, vb, G, N)) return rc;
-            for (int o = S.op_begin; o < S.op_end; ++o) apply_top(p, o, vb, G, N, v);
+            for (int o = S.op_begin; o < S.op_end; ++o) apply_top(p, o, R, vb, G, N, v);
         }
     }
     return 0;
@@ -179,6 +186,13 @@ static double dot_top(const TOp& op, const TOp& d, const int* vb, uint64_t N, co
                     for (int c = 0; c < 4; ++c) y += D(r + 4 * c) * psi[id[c]];
                     acc += (std::conj(lam[id[r]]) * y).real();
                 }
+            }
+    } else if (op.kind == 6) {
+        for (uint64_t i = 0; i < N; ++i)
+            if (ctl(i)) {
+                int j = 0;
+                for (int r = 0; r < 3; ++r) j |= (int)((i >> vb[r]) & 1) << r;
+                acc += (std::conj(lam[i]) * D(j) * psi[i]).real();
             }
     } else if (op.kind == 2) {
         for (uint64_t i = 0; i < N; ++i)
@@ -254,14 +268,14 @@ extern "C" int adjexec(int nq, const bsim_op* ops, int n, int n_params, double* 
                 const TOp& op = p.tops[o];
                 if (op.kind == 5 || op.kind == 4) return -22;
                 if (o + 1 + op.nparam > S.op_end) return -23;
-                apply_top(p, o, vb, G, N, psi);
+                apply_top(p, o, 3, vb, G, N, psi);
                 for (int q = 0; q < op.nparam; ++q) {
                     const TOp& d = p.tops[o + 1 + q];
                     if (d.kind != 5 || d.a < 0 || d.a >= ns || d.b != q) return -24;
                     out[3 * (size_t)d.a + q] += dot_top(op, d, vb, N, psi, lam);
                     ++seen[3 * (size_t)d.a + q];
                 }
-                apply_top(p, o, vb, G, N, lam);
+                apply_top(p, o, 3, vb, G, N, lam);
                 o += 1 + op.nparam;
             }
         }
@@ -277,4 +291,15 @@ extern "C" int adjexec(int nq, const bsim_op* ops, int n, int n_params, double* 
         ++slot;
     }
     return 0;
+}
+
+// number of tile ops of kind `tkind` in the plan (with controls only, if `ctl`)
+extern "C" int plan_kind_count(int nq, int kind, const bsim_op* ops, int n, int tkind, int ctl) {
+    std::vector<EOp> e;
+    if (int rc = lower_ops(nq, kind, ops, n, e, false)) return rc;
+    Plan p;
+    if (int rc = make_plan(kind == BSIM_SV ? nq : 2 * nq, e, tile_config(), p)) return rc;
+    int k = 0;
+    for (auto& t : p.tops) k += (t.kind == tkind && (!ctl || t.cmask)) ? 1 : 0;
+    return k;
 }

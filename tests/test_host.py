@@ -142,6 +142,47 @@ def test_derivative_matrices_accuracy():
     assert len(du) == 3
 
 
+def _angled_gates(rng):
+    a = lambda: float(rng.uniform(-3, 3))  # noqa: E731
+    one = [G.Rx(a()), G.Ry(a()), G.Rz(a()), G.PhaseShift(a()), G.U(a(), a(), a()), G.GPi(a()), G.GPi2(a()),
+           G.PRx(a(), a())]
+    two = [G.MS(a(), a(), a()), G.PSwap(a()), G.XX(a()), G.YY(a()), G.XY(a()), G.ZZ(a()), G.CPhaseShift(a()),
+           G.CPhaseShift00(a()), G.CPhaseShift01(a()), G.CPhaseShift10(a()), G.SingleExcitation(a()),
+           G.SingleExcitationPlus(a()), G.SingleExcitationMinus(a()), G.FermionicSWAP(a())]
+    out = [(g, [0]) for g in one] + [(g, [1, 0]) for g in two]
+    out += [(G.MultiRZ(a()), [2, 0, 1]), (G.GPhase(2, a()), [0, 1]), (G.DoubleExcitation(a()), [0, 1, 2, 3])]
+    out += [(G.Control(G.Ry(a()), (1,)), [1, 0]), (G.Control(G.U(a(), a(), a()), (0,)), [0, 2]),
+            (G.Control(G.XY(a()), (1, 0)), [3, 0, 2, 1])]
+    out += [(G.Rx(a(), pow_exponent=0.37), [0]), (G.CPhaseShift(a(), pow_exponent=-1.5), [0, 1])]
+    return out
+
+
+def test_derivative_matrices_every_angled_gate():
+    """The exact derivative rules (shifted matrices) agree with an
+    independent 8th-order central difference for every angle of every gate
+    family, controlled forms and non-unit exponents included."""
+    rng = np.random.default_rng(5)
+    for g, t in _angled_gates(rng):
+        d = G.derivative_matrices(g, t)
+        assert len(d) == len(g.angles)
+        for i in range(len(g.angles)):
+            np.testing.assert_allclose(d[i], G._fd_derivative(g, t, i), atol=1e-12, err_msg=repr(g))
+    sub = G.derivative_matrices(G.U(0.3, 0.2, 0.1), [0], [2])
+    np.testing.assert_allclose(sub[0], G.derivative_matrices(G.U(0.3, 0.2, 0.1), [0])[2])
+
+
+def test_derivative_matrices_closed_forms():
+    """PhaseShift, U(theta, phi, lambda) in phi and a controlled Rx in
+    closed form."""
+    p = 0.61
+    np.testing.assert_allclose(G.derivative_matrices(G.PhaseShift(p), [0])[0],
+                               np.diag([0, 1j * np.exp(1j * p)]), atol=1e-15)
+    th, ph, la = 0.4, -1.1, 2.2
+    du = G.derivative_matrices(G.U(th, ph, la), [0])[1]
+    exact = np.array([[0, 0], [1j * np.exp(1j * ph) * math.sin(th / 2), 1j * np.exp(1j * (ph + la)) * math.cos(th / 2)]])
+    np.testing.assert_allclose(du, exact, atol=1e-15)
+
+
 def test_workload_sizes():
     ins, th = W.ry_qft(20)
     assert len(ins) == 20 + 20 + 190 + 10

@@ -50,6 +50,9 @@ def interp():
         f.argtypes = [ctypes.c_int, ctypes.POINTER(L.Op), ctypes.c_int, ctypes.c_void_p]
     h.plan_singles.restype = ctypes.c_int
     h.plan_singles.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(L.Op), ctypes.c_int]
+    h.plan_kind_count.restype = ctypes.c_int
+    h.plan_kind_count.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(L.Op), ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int]
     h.adjexec.restype = ctypes.c_int
     h.adjexec.argtypes = [ctypes.c_int, ctypes.POINTER(L.Op), ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                           ctypes.c_void_p, ctypes.c_void_p]
@@ -124,6 +127,31 @@ def test_planner_sign_terms(interp):
         ins += [Instruction(G.CZ(), [q, q + d]) for q in range(0, n - d, 2)]
     ins += [Instruction(G.Rx(0.3 * q), [q]) for q in range(n)]
     assert np.max(np.abs(_run(interp, n, ins) - O.evolve_sv(n, ins))) < 1e-12
+
+
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("tile,low,bits", [(12, 3, 4), (6, 0, 3), (8, 2, 4)])
+def test_planner_register_diagonals(interp, tile_config, seed, tile, low, bits):
+    """Diagonal gates whose bits are all register bits of their sweep run as
+    register-diagonal tile ops (one factor per register); controls on
+    register and on thread bits split like those of dense ops."""
+    L.call("bsim_set_tile_config", tile, low)
+    L.call("bsim_set_sweep_bits", bits, None)
+    rng = np.random.default_rng(500 + seed)
+    n = 8
+    ins = [Instruction(G.H(), [q]) for q in range(n)]
+    for q in range(n - 1):
+        a = rng.uniform(-3, 3, 4)
+        ins.append(Instruction(G.Rx(a[0]), [q]))
+        ins.append(Instruction(G.Control(G.Rz(a[1]), (int(rng.integers(2)),)), [q + 1, q]))
+        ins.append(Instruction(G.Control(G.PhaseShift(a[2]), (int(rng.integers(2)),)), [(q + 4) % n, q]))
+        ins.append(Instruction(G.Rz(a[3]), [q]))
+        ins.append(Instruction(G.CPhaseShift(a[0] - a[1]), [q, q + 1]))
+        ins.append(Instruction(G.Ry(a[2]), [q + 1]))
+    ol = lower_instructions(ins, L.BSIM_SV)
+    assert interp.plan_kind_count(n, L.BSIM_SV, ol.ptr, ol.n, 6, 0) > 0
+    assert interp.plan_kind_count(n, L.BSIM_SV, ol.ptr, ol.n, 6, 1) > 0
+    np.testing.assert_allclose(_run(interp, n, ins), O.evolve_sv(n, ins), atol=1e-12)
 
 
 def _run_dm(h, n, ins, mode="reference"):
