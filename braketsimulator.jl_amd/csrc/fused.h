@@ -49,7 +49,8 @@ int lower_ops(int nq, int kind, const bsim_op* ops, int n_ops, std::vector<EOp>&
 struct TOp {
     // 0 dense1, 1 dense2, 2 diag, 3 sign parity, 4 predicated superoperator,
     // 5 its matrix, 6 diagonal on register bits only (m[j] = the factor of
-    // register j, so every thread reads the same entries)
+    // register j, so every thread reads the same entries), 7 X on register a
+    // (swaps, 3-register sweeps only; m holds the X matrix)
     int32_t kind;
     int32_t a, b;      // dense: register bits (a < b for dense2)
     int32_t nd;        // diag: bit count; kind 4: TOps that follow with the matrix (0 or 15)
@@ -62,7 +63,7 @@ struct TOp {
     // Split of the index tests over a sweep (computed by the planner): a
     // register j of a sub-cube has index gbase | r_j, gbase the sub-cube's
     // base and r_j the global bits of j's register bits.
-    //   dense and kind 6 with controls: on_j = ((gbase & tmask) == tval) && bit j of jmask;
+    //   dense and kinds 6, 7 with controls: on_j = ((gbase & tmask) == tval) && bit j of jmask;
     //   sign: negate_j = bit j of (jmask ^ P(gbase) * 0xFFFF ^ sum_i parity(gbase & sgm[i]) * pattern_i),
     //         P(x) = sum_k parity(x & (x >> pd[k]) & pm[k]), pattern_i = registers with bit i set;
     //   kind 4: on_j = (Q(gbase) ^ sum over the register bits i of j of sgm[i]) == 0,

@@ -319,6 +319,15 @@ static uint32_t op_regs(const EOp& e, Reg& reg, bool* outside) {
     return m;
 }
 
+// Dense 1-bit op whose matrix is exactly X (NOT, CNOT, Toffoli targets): in
+// 3-register sweeps a tile op that swaps registers instead of multiplying.
+static bool is_x(const EOp& e) {
+    if (e.kind != E_DENSE || e.g.k != 1) return false;
+    const double2* m = e.g.m.data();
+    return m[0].x == 0.0 && m[0].y == 0.0 && m[1].x == 1.0 && m[1].y == 0.0 && m[2].x == 1.0 && m[2].y == 0.0 &&
+           m[3].x == 0.0 && m[3].y == 0.0;
+}
+
 // VALU instructions per thread of an op as a separate sweep op on a
 // 16-amplitude sub-cube (the fusion rule compares these): dense 1-bit 128
 // FP64 + dispatch, 64 selects more with controls; dense 2-bit 256 FP64; a
@@ -677,7 +686,9 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
                     }
                 } else {
                     if (e.g.k == 1) {
-                        op.kind = 0;
+                        // X as swaps in 3-register sweeps (the 4-register kernel keeps
+                        // its op dispatch lean: it runs X as a dense op)
+                        op.kind = RB == 3 && is_x(e) && !(e.aslot >= 0 && e.nparams > 0) ? 7 : 0;
                         op.a = reg(e.g.tb[0]);
                         for (int j = 0; j < 4; ++j) op.m[j] = e.g.m[j];
                     } else {

@@ -209,6 +209,39 @@ __device__ __forceinline__ void sw_diag(SubCube<R>& s, const TOp* __restrict__ o
     });
 }
 
+// X on register A (kind 7, 3-register sweeps): the register pairs swap
+// where the controls hold.
+template <int R, int A, bool CTL>
+__device__ __forceinline__ void sw_x_t(SubCube<R>& s, const TOp* __restrict__ op) {
+    bool tok = true;
+    uint32_t jm = 0xFFFFu;
+    if constexpr (CTL) {
+        tok = (s.gbase & op->tmask) == op->tval;
+        jm = op->jmask;
+    }
+    sfor<0, (1 << R)>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        if constexpr ((j & (1 << A)) == 0) {
+            constexpr int k = j | (1 << A);
+            const c2 a = getv<j>(s.v), b = getv<k>(s.v);
+            if constexpr (CTL) {
+                const bool on = tok && ((jm >> j) & 1u);
+                setv<j>(s.v, csel(on, b, a));
+                setv<k>(s.v, csel(on, a, b));
+            } else {
+                setv<j>(s.v, b);
+                setv<k>(s.v, a);
+            }
+        }
+    });
+}
+
+template <int R, int A>
+__device__ __forceinline__ void sw_x(SubCube<R>& s, const TOp* __restrict__ op, const OpHead& hd) {
+    if (hd.cmask) sw_x_t<R, A, true>(s, op);
+    else sw_x_t<R, A, false>(s, op);
+}
+
 // Diagonal on register bits only (kind 6): register j is scaled by m[j], the
 // same entries for every thread (scalar loads).
 template <int R>
@@ -376,9 +409,18 @@ __device__ __forceinline__ int run_op(SubCube<R>& s, const TOp* __restrict__ op,
             case 16 + 1: sw_dense2<R, 0, 1>(s, op, hd); break;
             case 16 + 2: sw_dense2<R, 0, 2>(s, op, hd); break;
             case 16 + 6: sw_dense2<R, 1, 2>(s, op, hd); break;
-            default:  // kinds 2 and 6 (kept out of the jump table)
-                if (hd.kind == 6) sw_rdiag<R>(s, op, hd);
-                else sw_diag<R>(s, op);
+            default:  // kinds 2, 6 and 7 (kept out of the jump table)
+                if (hd.kind == 6) {
+                    sw_rdiag<R>(s, op, hd);
+                } else if (hd.kind == 7) {
+                    switch (hd.a) {
+                        case 0: sw_x<R, 0>(s, op, hd); break;
+                        case 1: sw_x<R, 1>(s, op, hd); break;
+                        default: sw_x<R, 2>(s, op, hd); break;
+                    }
+                } else {
+                    sw_diag<R>(s, op);
+                }
                 break;
         }
     }

@@ -43,7 +43,7 @@ static void apply_top(const Plan& p, int o, int R, const int* vb, uint64_t G, ui
             }
             for (int k = 0; k < 16; ++k) v[id[k]] = y[k];
         }
-    } else if (op.kind == 0) {
+    } else if (op.kind == 0 || op.kind == 7) {  // 7: m holds X
         uint64_t bit = 1ull << vb[op.a];
         for (uint64_t i = 0; i < N; ++i) if (!(i & bit)) {
             if (op.cmask && ((i & op.cmask) != op.cval)) continue;
@@ -88,7 +88,7 @@ static void apply_top(const Plan& p, int o, int R, const int* vb, uint64_t G, ui
 static int check_sweep(const Plan& p, const Sweep& S, int R, const int* vb, uint64_t G, uint64_t N) {
     for (int o = S.op_begin; o < S.op_end; ++o) {
         const TOp& op = p.tops[o];
-        if ((op.kind == 0 || op.kind == 1 || (op.kind == 4 && op.nd == 0)) && (op.a >= R || op.b >= R))
+        if ((op.kind == 0 || op.kind == 1 || op.kind == 7 || (op.kind == 4 && op.nd == 0)) && (op.a >= R || op.b >= R))
             return -15;  // a register bit the pass's sweeps do not have
         for (uint64_t i = 0; i < N; ++i) {
             const uint64_t gbase = i & ~G;
@@ -103,7 +103,7 @@ static int check_sweep(const Plan& p, const Sweep& S, int R, const int* vb, uint
                 for (int r = 0; r < R; ++r)
                     if ((j >> r) & 1) split ^= __builtin_popcountll(gbase & op.sgm[r]) & 1;
                 if (direct != split) return -10;
-            } else if ((op.kind <= 1 || op.kind == 6) && op.cmask) {
+            } else if ((op.kind <= 1 || op.kind == 6 || op.kind == 7) && op.cmask) {
                 const bool direct = (i & op.cmask) == op.cval;
                 const bool split = ((gbase & op.tmask) == op.tval) && ((op.jmask >> j) & 1);
                 if (direct != split) return -11;

@@ -154,6 +154,37 @@ def test_planner_register_diagonals(interp, tile_config, seed, tile, low, bits):
     np.testing.assert_allclose(_run(interp, n, ins), O.evolve_sv(n, ins), atol=1e-12)
 
 
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("tile,low", [(12, 3), (6, 0), (8, 2)])
+def test_planner_x_swaps(interp, tile_config, seed, tile, low):
+    """In 3-register sweeps X, CNOT, Toffoli and 0-controlled X run as
+    register swaps (with the control split of dense ops) between rotations."""
+    bits = 3
+    L.call("bsim_set_tile_config", tile, low)
+    L.call("bsim_set_sweep_bits", bits, None)
+    rng = np.random.default_rng(700 + seed)
+    n = 9
+    ins = [Instruction(G.Ry(float(rng.uniform(-3, 3))), [q]) for q in range(n)]
+    for _ in range(30):
+        q = [int(x) for x in rng.choice(n, 3, replace=False)]
+        r = rng.random()
+        if r < 0.25:
+            ins.append(Instruction(G.X(), [q[0]]))
+        elif r < 0.55:
+            ins.append(Instruction(G.CNot(), q[:2]))
+        elif r < 0.7:
+            ins.append(Instruction(G.CCNot(), q))
+        elif r < 0.8:
+            ins.append(Instruction(G.Control(G.X(), (0,)), q[:2]))
+        else:
+            ins.append(Instruction(G.Rx(float(rng.uniform(-3, 3))), [q[0]]))
+        ins.append(Instruction(G.Rz(float(rng.uniform(-3, 3))), [q[1]]))
+    ol = lower_instructions(ins, L.BSIM_SV)
+    assert interp.plan_kind_count(n, L.BSIM_SV, ol.ptr, ol.n, 7, 0) > 0
+    assert interp.plan_kind_count(n, L.BSIM_SV, ol.ptr, ol.n, 7, 1) > 0
+    np.testing.assert_allclose(_run(interp, n, ins), O.evolve_sv(n, ins), atol=1e-12)
+
+
 def _run_dm(h, n, ins, mode="reference"):
     ol = lower_instructions(ins, L.BSIM_DM, mode)
     assert h.plan_singles(n, L.BSIM_DM, ol.ptr, ol.n) == 0, "every op should be planned into tile passes"
