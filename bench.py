@@ -365,26 +365,55 @@ def per_gate_leg(n: int, ins):
     return out
 
 
+class _DeviceCopy:
+    """Two device buffers of ``nbytes`` and timed device-to-device copies
+    between them, through the HIP runtime libbsim itself links (ctypes: no
+    second GPU runtime in the process)."""
+
+    def __init__(self, nbytes: int):
+        import ctypes as C
+
+        self.C = C
+        self.hip = C.CDLL("libamdhip64.so")
+        self.n = nbytes
+        self.src, self.dst = C.c_void_p(), C.c_void_p()
+        self._ok(self.hip.hipMalloc(C.byref(self.src), C.c_size_t(nbytes)), "hipMalloc")
+        self._ok(self.hip.hipMalloc(C.byref(self.dst), C.c_size_t(nbytes)), "hipMalloc")
+        self._ok(self.hip.hipMemset(self.src, 0, C.c_size_t(nbytes)), "hipMemset")
+        self._ok(self.hip.hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+    def _ok(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed with HIP error {rc}")
+
+    def copy_seconds(self) -> float:
+        C = self.C
+        e0, e1 = C.c_void_p(), C.c_void_p()
+        self._ok(self.hip.hipEventCreate(C.byref(e0)), "hipEventCreate")
+        self._ok(self.hip.hipEventCreate(C.byref(e1)), "hipEventCreate")
+        self._ok(self.hip.hipEventRecord(e0, None), "hipEventRecord")
+        self._ok(self.hip.hipMemcpyAsync(self.dst, self.src, C.c_size_t(self.n), 3, None), "hipMemcpyAsync")
+        self._ok(self.hip.hipEventRecord(e1, None), "hipEventRecord")
+        self._ok(self.hip.hipEventSynchronize(e1), "hipEventSynchronize")
+        ms = C.c_float()
+        self._ok(self.hip.hipEventElapsedTime(C.byref(ms), e0, e1), "hipEventElapsedTime")
+        self.hip.hipEventDestroy(e0)
+        self.hip.hipEventDestroy(e1)
+        return ms.value * 1e-3
+
+    def close(self):
+        self.hip.hipFree(self.src)
+        self.hip.hipFree(self.dst)
+
+
 def copy_bandwidth() -> float:
     """Reference point for the per-gate figures: a plain 4 GiB
     device-to-device copy (read + write bytes / time), best of 3."""
-    import torch
-
-    src = torch.ones(2 ** 29, dtype=torch.float64, device="cuda")
-    dst = torch.empty_like(src)
-    dst.copy_(src)
-    torch.cuda.synchronize()
-    best = float("inf")
-    for _ in range(3):
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        dst.copy_(src)
-        e1.record()
-        e1.synchronize()
-        best = min(best, e0.elapsed_time(e1) * 1e-3)
-    del src, dst
-    torch.cuda.empty_cache()
-    return round(2 * 8.0 * 2 ** 29 / best / 1e9, 1)
+    cp = _DeviceCopy(2 ** 32)
+    cp.copy_seconds()
+    best = min(cp.copy_seconds() for _ in range(3))
+    cp.close()
+    return round(2 * 2 ** 32 / best / 1e9, 1)
 
 
 def graph_leg(reps: int):
@@ -434,8 +463,6 @@ def pmc_probe(n: int, depth: int):
     """Kernels for hardware-counter runs: the fused passes of one evolve
     (k_tile) and a plain device copy of 2^29 doubles (4 GiB read + 4 GiB
     written), whose FETCH_SIZE / WRITE_SIZE calibrate the counters."""
-    import torch
-
     from bsim import _lib as L
     from bsim import workloads as W
     from bsim.simulators import StateVectorSimulator, lower_instructions
@@ -446,10 +473,9 @@ def pmc_probe(n: int, depth: int):
     L.call("bsim_synchronize", sim.handle)
     log(f"pmc probe: evolve {n}q done")
     del sim
-    src = torch.ones(2 ** 29, dtype=torch.float64, device="cuda")
-    dst = torch.empty_like(src)
-    dst.copy_(src)
-    torch.cuda.synchronize()
+    cp = _DeviceCopy(2 ** 32)
+    cp.copy_seconds()
+    cp.close()
     log("pmc probe: copy of 4 GiB done")
 
 
