@@ -3,7 +3,7 @@
 Host-side mirror of the reference's simulator API over the libbsim C ABI
 (include/bsim.h).  Every compute call runs in libbsim's HIP kernels.
 """
-from . import gates, noises, observables
+from . import gates, noises, observables, wire
 from ._lib import BsimError, load
 from .gates import *  # noqa: F401,F403 (gate types, as the reference exports them)
 from .noises import *  # noqa: F401,F403

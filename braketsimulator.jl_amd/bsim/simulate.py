@@ -17,6 +17,10 @@ class Program:
     instructions: list
     results: list = field(default_factory=list)
     basis_rotation_instructions: list = field(default_factory=list)
+    # the OpenQASM text and inputs the instructions were parsed from, echoed
+    # into a task result's action (src/BraketSimulator.jl:430-433)
+    source: str | None = None
+    inputs: dict | None = None
 
     def qubit_count(self) -> int:
         qs = set()

@@ -208,7 +208,10 @@ class _Simulator:
         self._h = C.c_void_p()
         self.qubit_count = int(qubit_count)
         self.shots = int(shots)
-        L.call("bsim_state_create", self.qubit_count, self.kind, C.byref(self._h))
+        # 0 qubits (the reference's create_sim, src/BraketSimulator.jl:402-408):
+        # no device state until reinit gives the program's qubit count
+        if self.qubit_count != 0:
+            L.call("bsim_state_create", self.qubit_count, self.kind, C.byref(self._h))
         self._observed = None
 
     def __del__(self):
@@ -226,8 +229,9 @@ class _Simulator:
     # -- lifecycle ----------------------------------------------------------
     def reinit(self, qubit_count: int, shots: int):
         """reinit! (src/sv_simulator.jl:96-127, src/dm_simulator.jl:96-128)."""
-        if qubit_count != self.qubit_count:
-            L.load().bsim_state_destroy(self._h)
+        if qubit_count != self.qubit_count or not self._h.value:
+            if self._h.value:
+                L.load().bsim_state_destroy(self._h)
             self._h = C.c_void_p()
             self.qubit_count = int(qubit_count)
             L.call("bsim_state_create", self.qubit_count, self.kind, C.byref(self._h))
