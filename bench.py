@@ -23,6 +23,7 @@ for p in (ROOT, ROOT / "braketsimulator.jl_amd", ROOT / "oracle"):
         sys.path.insert(0, str(p))
 
 METRIC = "30q SV HBM GB/s + gates/s; 24q adjoint grads/s vs param-shift at 1e-10"
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector
 HBM_PEAK_GBS = 8000.0
 
 
@@ -78,6 +79,8 @@ def main():
     ap.add_argument("--shots", type=int, default=1000000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--adjoint-qubits", type=int, default=24, help="0 skips the adjoint leg")
+    ap.add_argument("--adjoint-large", type=int, default=28,
+                    help="qubits of the second adjoint workload (the same ansatz; 0 skips it)")
     ap.add_argument("--ps-sample", type=int, default=24, help="parameters timed by parameter shift")
     ap.add_argument("--batch", type=int, default=4096, help="circuits in the batch leg (0 skips it)")
     ap.add_argument("--dm-qubits", type=int, default=14, help="noisy density-matrix leg (0 skips it)")
@@ -154,6 +157,7 @@ def main():
     lib.bsim_set_profiling(0)
 
     pass_bytes = 32.0 * 2 ** n
+    flop_per_launch = algorithmic_flops(ins, n) / max(passes, 1)
     evolve_s = phases["evolve"] / a.steps
     value = passes * pass_bytes / evolve_s / 1e9
     avg_kernel_ms = kms.value / max(launches.value, 1)
@@ -192,6 +196,15 @@ def main():
             "traffic_source": "profiles/k_tile_brickwall_30q.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
             "avg_kernel_ms": round(avg_kernel_ms, 4),
             "launches_timed": int(launches.value),
+            # the pass does many gates per HBM round trip and is bound by FP64
+            # issue, not by HBM: its arithmetic against the FP64 vector peak
+            "fp64": {
+                "flop_per_launch": flop_per_launch,
+                "achieved_tflops": round(flop_per_launch / (avg_kernel_ms * 1e-3) / 1e12, 2),
+                "peak_tflops": FP64_PEAK_TFLOPS,
+                "frac": round(flop_per_launch / (avg_kernel_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
+                "flop_model": "complex FP64 per amplitude: dense 1q 14 (2 cmul + 1 cadd), CZ 0 (sign flip)",
+            },
         },
         "z_checksum": float(np.sum(z)),
     }
@@ -200,6 +213,8 @@ def main():
         rec["per_gate"] = per_gate_leg(n, ins)
     if a.adjoint_qubits > 0:
         rec["adjoint"] = adjoint_leg(a.adjoint_qubits, a.ps_sample)
+    if a.adjoint_large > 0:
+        rec["adjoint_large"] = adjoint_large_leg(a.adjoint_large)
     if a.batch > 0:
         rec["batch"] = batch_leg(a.batch, a.sweep_sets)
     if a.dm_qubits > 0:
@@ -262,6 +277,64 @@ def adjoint_leg(nq: int, ps_sample: int):
         "speedup_vs_param_shift": round(t_ps / t_adj, 1),
         "max_abs_diff_vs_param_shift": float(np.max(np.abs(grad[sample] - ps))),
         "param_shift_sample": f"first {len(sample)} of {len(th)} parameters, 2 fused evolutions + <H> each",
+    }
+
+
+def algorithmic_flops(ins, n: int) -> float:
+    """FP64 flops an evolve of ``ins`` needs: a dense k-qubit gate costs
+    2^k complex multiplies and 2^k - 1 complex adds per amplitude (6 and 2
+    flops), a diagonal one multiply unless its entries are +-1, controls
+    scale by the fraction of amplitudes they select."""
+    import numpy as np
+
+    from bsim.gates import kernel_call
+
+    total = 0.0
+    for i in ins:
+        call = kernel_call(i.operator, i.target)
+        m = np.asarray(call.matrix)
+        frac = 0.5 ** len(getattr(call, "controls", []) or [])
+        if m.ndim == 2 and np.count_nonzero(m - np.diag(np.diagonal(m))) == 0:
+            m = np.diagonal(m)
+        if getattr(call, "diagonal", False) or m.ndim == 1:
+            d = m.reshape(-1)
+            per = 0.0 if np.all(np.isin(d, (1, -1))) else 6.0
+        else:
+            k2 = m.shape[0]
+            per = 6.0 * k2 + 2.0 * (k2 - 1)
+        total += per * frac * 2 ** n
+    return total
+
+
+def adjoint_large_leg(nq: int):
+    """The 8-layer hardware-efficient ansatz at ``nq`` qubits (28: 4 GiB per
+    vector): fused adjoint gradients (best of 2 after a warm-up) and one
+    gradient with one backward kernel per op, for the difference."""
+    import numpy as np
+
+    from bsim import adjoint as A
+    from bsim import workloads as W
+    from bsim.simulators import StateVectorSimulator
+
+    ins, pmap, th = W.hea(nq, 8, seed=0)
+    words, coeffs = W.tfim(nq)
+    sim = StateVectorSimulator(nq, 0)
+    log(f"adjoint: {nq} qubits, {len(th)} parameters")
+    _, g = A.adjoint_gradient(sim, ins, pmap, words, coeffs)
+    best = float("inf")
+    for _ in range(2):
+        t = time.perf_counter()
+        ev, g = A.adjoint_gradient(sim, ins, pmap, words, coeffs)
+        best = min(best, time.perf_counter() - t)
+    t = time.perf_counter()
+    _, g_op = A.adjoint_gradient(sim, ins, pmap, words, coeffs, fused=False)
+    t_op = time.perf_counter() - t
+    log(f"adjoint {nq}q: fused {best * 1e3:.1f} ms, per-op {t_op * 1e3:.1f} ms")
+    del sim
+    return {
+        "workload": f"HEA {nq} qubits x 8 layers, {len(th)} params, TFIM H",
+        "expval": ev, "adjoint_s": round(best, 4), "grads_per_s": round(1.0 / best, 3),
+        "per_op_backward_s": round(t_op, 4), "max_abs_diff_fused_vs_per_op": float(np.max(np.abs(g - g_op))),
     }
 
 
