@@ -3,11 +3,14 @@
 // The backward sweep visits the ops in reverse; for op k (matrix U, and for
 // each angle p its derivative dU_p):
 //     psi <- U^dagger psi;   g_p += 2 Re <lambda| dU_p |psi>;   lambda <- U^dagger lambda.
+// With both vectors updated first the product is the same number,
+// <lambda|dU|psi> = <U^dagger lambda| U^dagger dU |psi>, so each op runs on
+// the two vectors in one dispatch and its payloads hold U^dagger dU.
 // The reversed list of U^dagger ops is planned like a forward evolution
 // (tile passes of 11 bits, sweeps of 3 register bits); a pass loads the tiles
 // of both vectors into LDS, and every thread carries two 8-amplitude
 // sub-cubes (psi and lambda) through the sweep's ops, computing the
-// <lambda|dU|psi> terms of its sub-cube between the two applications.  Each
+// products of its sub-cube after each parametrized op.  Each
 // wave's sum goes to partials[((slot * PM + p) * ntiles + tile) * W + wave]
 // (PM: the largest parameter count of an op, W waves per workgroup), summed
 // later in a fixed order.  Reordering ops on
@@ -155,6 +158,85 @@ __device__ __forceinline__ double adj_dot(const SubCube<3>& sp, const SubCube<3>
     }
 }
 
+// Both sub-cubes through one op, one dispatch: the op's index tests (sign
+// parities, diagonal indices) are computed once for the two.
+__device__ __forceinline__ void sign_both(SubCube<3>& sp, SubCube<3>& sl, const TOp* __restrict__ op) {
+    const uint64_t a = sp.gbase;
+    uint32_t m = op->jmask;
+    uint32_t par = 0;
+    for (int k = 0; k < op->npairs; ++k) par += __popcll(a & (a >> op->pd[k]) & op->pm[k]);
+    auto flip = [&](uint32_t p, uint32_t pattern) { m = xor_and(m, (uint32_t)__builtin_amdgcn_sbfe(p, 0, 1), pattern); };
+    flip(par, 0xFFu);
+    flip(__popcll(a & op->sgm[0]), reg_pattern<3>(0));
+    flip(__popcll(a & op->sgm[1]), reg_pattern<3>(1));
+    flip(__popcll(a & op->sgm[2]), reg_pattern<3>(2));
+    auto neg = [&](SubCube<3>& s) {
+        sfor<0, 8>([&](auto J) {
+            constexpr int j = decltype(J)::value;
+            const uint32_t t = m << (31 - j);
+            c2 v = getv<j>(s.v);
+            const uint64_t bx = __builtin_bit_cast(uint64_t, v.x), by = __builtin_bit_cast(uint64_t, v.y);
+            const uint64_t hx = xor_and((uint32_t)(bx >> 32), t, 0x80000000u);
+            const uint64_t hy = xor_and((uint32_t)(by >> 32), t, 0x80000000u);
+            v.x = __builtin_bit_cast(double, (bx & 0xFFFFFFFFull) | (hx << 32));
+            v.y = __builtin_bit_cast(double, (by & 0xFFFFFFFFull) | (hy << 32));
+            setv<j>(s.v, v);
+        });
+    };
+    neg(sp);
+    neg(sl);
+}
+
+__device__ __forceinline__ void diag_both(SubCube<3>& sp, SubCube<3>& sl, const TOp* __restrict__ op) {
+    const int nd = op->nd;
+    int base_idx = 0, r0 = 0, r1 = 0, r2 = 0;
+    for (int i = 0; i < nd; ++i) {
+        const uint64_t bit = 1ull << op->dbit[i];
+        base_idx |= (sp.gbase & bit) ? (1 << i) : 0;
+        r0 |= (sp.g0 == bit) ? (1 << i) : 0;
+        r1 |= (sp.g1 == bit) ? (1 << i) : 0;
+        r2 |= (sp.g2 == bit) ? (1 << i) : 0;
+    }
+    const uint64_t cm = op->cmask, cv = op->cval;
+    const uint64_t tb = sp.gbase & cm;
+    sfor<0, 8>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        const int idx = base_idx | ((j & 1) ? r0 : 0) | ((j & 2) ? r1 : 0) | ((j & 4) ? r2 : 0);
+        const c2 f = to_c2(op->m[idx]);
+        const uint64_t uj = (((j & 1) ? sp.g0 : 0) | ((j & 2) ? sp.g1 : 0) | ((j & 4) ? sp.g2 : 0)) & cm;
+        const bool on = cm == 0 || (tb | uj) == cv;
+        setv<j>(sp.v, csel(on, cmul(f, getv<j>(sp.v)), getv<j>(sp.v)));
+        setv<j>(sl.v, csel(on, cmul(f, getv<j>(sl.v)), getv<j>(sl.v)));
+    });
+}
+
+__device__ __forceinline__ void run_op_both(SubCube<3>& sp, SubCube<3>& sl, const TOp* __restrict__ op) {
+    const OpHead hd = load_head(op);
+    switch (hd.kind * 16 + hd.a * 4 + hd.b) {
+        case 48: sign_both(sp, sl, op); break;
+        case 0: sw_dense1<3, 0>(sp, op, hd); sw_dense1<3, 0>(sl, op, hd); break;
+        case 4: sw_dense1<3, 1>(sp, op, hd); sw_dense1<3, 1>(sl, op, hd); break;
+        case 8: sw_dense1<3, 2>(sp, op, hd); sw_dense1<3, 2>(sl, op, hd); break;
+        case 16 + 1: sw_dense2<3, 0, 1>(sp, op, hd); sw_dense2<3, 0, 1>(sl, op, hd); break;
+        case 16 + 2: sw_dense2<3, 0, 2>(sp, op, hd); sw_dense2<3, 0, 2>(sl, op, hd); break;
+        case 16 + 6: sw_dense2<3, 1, 2>(sp, op, hd); sw_dense2<3, 1, 2>(sl, op, hd); break;
+        default:
+            if (hd.kind == 6) {
+                sw_rdiag<3>(sp, op, hd);
+                sw_rdiag<3>(sl, op, hd);
+            } else if (hd.kind == 7) {
+                switch (hd.a) {
+                    case 0: sw_x<3, 0>(sp, op, hd); sw_x<3, 0>(sl, op, hd); break;
+                    case 1: sw_x<3, 1>(sp, op, hd); sw_x<3, 1>(sl, op, hd); break;
+                    default: sw_x<3, 2>(sp, op, hd); sw_x<3, 2>(sl, op, hd); break;
+                }
+            } else {
+                diag_both(sp, sl, op);
+            }
+            break;
+    }
+}
+
 // One tile of one backward pass (T <= 11 bits, 3 register bits per sweep).
 __global__ __launch_bounds__(AT_TB) void k_adj_tile(double2* __restrict__ psi, double2* __restrict__ lam, int vq,
                                                     const Pass* __restrict__ pass, const Sweep* __restrict__ sweeps,
@@ -234,14 +316,13 @@ __global__ __launch_bounds__(AT_TB) void k_adj_tile(double2* __restrict__ psi, d
         for (int o = ob; o < oe;) {
             const TOp* op = tops + o;
             const int np = op->nparam;
-            run_op<3, false>(sp, op, tp, pl, o0, o1, o2, 0);
+            run_op_both(sp, sl, op);
             for (int p = 0; p < np; ++p) {
                 const TOp* d = op + 1 + p;
                 const double v = wave_sum63(adj_dot(sp, sl, op, d));
                 if (lane == 63)
                     partials[((size_t)(d->a * pm + d->b) * ntiles + tile_index) * AT_W + wave] = v;
             }
-            run_op<3, false>(sl, op, tl, pl, o0, o1, o2, 0);
             o += 1 + np;
         }
         if (valid) {
@@ -315,6 +396,25 @@ int adjoint_back_plan(int vq, const std::vector<EOp>& eops, std::vector<EOp>& ba
         if (b.nparams > 0) {
             b.aslot = --slot;
             pm = std::max(pm, b.nparams);
+            // payload U^dagger dU: the products are taken after both vectors
+            // are updated, <lam|dU|psi'> = <U^dagger lam| U^dagger dU |psi'>
+            const int D = 1 << b.g.k;
+            for (int p = 0; p < b.nparams; ++p) {
+                std::vector<double2> t((size_t)D * D, make_double2(0.0, 0.0));
+                for (int r = 0; r < D; ++r)
+                    for (int c = 0; c < D; ++c) {
+                        double2 acc = make_double2(0.0, 0.0);
+                        for (int k2 = 0; k2 < D; ++k2) {
+                            const double2 u = b.g.diag ? (k2 == r ? b.g.m[r] : make_double2(0.0, 0.0))
+                                                       : b.g.m[r + (size_t)D * k2];
+                            const double2 d = b.dm[p][k2 + (size_t)D * c];
+                            acc.x += u.x * d.x - u.y * d.y;
+                            acc.y += u.x * d.y + u.y * d.x;
+                        }
+                        t[r + (size_t)D * c] = acc;
+                    }
+                b.dm[p].swap(t);
+            }
         }
         back.push_back(std::move(b));
     }

@@ -74,8 +74,9 @@ struct TOp {
     uint64_t tmask, tval;
     uint64_t sgm[4];
     uint32_t jmask;
-    // fused adjoint: the op's derivative matrices follow as `nparam` kind-5
-    // TOps (m = dU in this op's layout, a = partial-sum slot, b = parameter)
+    // fused adjoint: the op's product matrices follow as `nparam` kind-5
+    // TOps (m = the EOp's dm in this op's layout, a = partial-sum slot,
+    // b = parameter)
     int32_t nparam;
     double2 m[16];
 };
@@ -147,9 +148,10 @@ int launch_eop(bsim_state* h, double2* d, const EOp& e);
 int launch_kraus_ref(bsim_state* h, double2* d, const EOp& e);
 
 // Backward list of the state-vector adjoint: back[i] = U^dagger of
-// eops[L - 1 - i], parametrized ops numbered (aslot) in forward order, and
-// its plan (11 tile bits, 3 register bits).  nslots: parametrized ops; pm:
-// their largest parameter count.
+// eops[L - 1 - i] with dm[p] = U^dagger dU_p (the product matrix against
+// the updated vectors), parametrized ops numbered (aslot) in forward order,
+// and its plan (11 tile bits, 3 register bits).  nslots: parametrized ops;
+// pm: their largest parameter count.
 int adjoint_back_plan(int vq, const std::vector<EOp>& eops, std::vector<EOp>& back, Plan& plan, int& nslots,
                       int& pm);
 

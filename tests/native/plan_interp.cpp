@@ -163,8 +163,9 @@ extern "C" int plan_singles(int nq, int kind, const bsim_op* ops, int n) {
 
 // ---- fused adjoint backward plan (state vectors) ----------------------------
 
-// Re sum_i conj(lam_i) (dU psi)_i for tile op `op` with derivative payload d
-// (d.m in the op's own layout), over the indices where the op's controls hold.
+// Re sum_i conj(lam_i) (D psi)_i for tile op `op` with product payload d
+// (D = d.m in the op's own layout), over the indices where the op's
+// controls hold.
 static double dot_top(const TOp& op, const TOp& d, const int* vb, uint64_t N, const cd* psi, const cd* lam) {
     auto D = [&](int j) { return cd(d.m[j].x, d.m[j].y); };
     auto ctl = [&](uint64_t i) { return op.cmask == 0 || (i & op.cmask) == op.cval; };
@@ -268,14 +269,15 @@ extern "C" int adjexec(int nq, const bsim_op* ops, int n, int n_params, double* 
                 const TOp& op = p.tops[o];
                 if (op.kind == 5 || op.kind == 4) return -22;
                 if (o + 1 + op.nparam > S.op_end) return -23;
+                // both vectors first, then the products with the payloads
                 apply_top(p, o, 3, vb, G, N, psi);
+                apply_top(p, o, 3, vb, G, N, lam);
                 for (int q = 0; q < op.nparam; ++q) {
                     const TOp& d = p.tops[o + 1 + q];
                     if (d.kind != 5 || d.a < 0 || d.a >= ns || d.b != q) return -24;
                     out[3 * (size_t)d.a + q] += dot_top(op, d, vb, N, psi, lam);
                     ++seen[3 * (size_t)d.a + q];
                 }
-                apply_top(p, o, 3, vb, G, N, lam);
                 o += 1 + op.nparam;
             }
         }
