@@ -87,37 +87,87 @@ class Superoperator:
 # ---- lowering to bsim_op arrays ----------------------------------------
 
 class OpList:
-    """A ctypes array of bsim_op plus the numpy buffers it points into."""
+    """An array of bsim_op plus the buffer its matrices live in.  Ops are
+    recorded as plain Python values and written into a numpy array laid out
+    like the C struct in one go when the array is first used; all matrices
+    (column-major complex128) share one buffer, so the pointer fields are
+    filled by one vector operation."""
+
+    _DTYPE = None
+
+    @classmethod
+    def _dtype(cls):
+        if cls._DTYPE is None:
+            f = L.Op
+            names = ["type", "n_targets", "n_controls", "n_kraus", "targets", "controls", "control_values", "matrix",
+                     "n_params", "param_index", "dmatrix"]
+            formats = ["<i4", "<i4", "<i4", "<i4", ("<i4", L.MAX_TARGETS), ("<i4", L.MAX_CONTROLS),
+                       ("<i4", L.MAX_CONTROLS), "<u8", "<i4", ("<i4", 3), ("<u8", 3)]
+            cls._DTYPE = np.dtype({"names": names, "formats": formats,
+                                   "offsets": [getattr(f, n).offset for n in names], "itemsize": C.sizeof(f)})
+        return cls._DTYPE
 
     def __init__(self, n: int):
-        self.arr = (L.Op * max(n, 1))()
         self.n = 0
-        self._keep = []
+        self._ops = []    # (type, targets, controls, cvals, n_kraus, params)
+        self._mats = []   # flattened complex128 arrays
+        self._where = []  # (op index, slot): slot 0 = matrix, 1 + p = dmatrix[p]
+        self._rec = None
+        self._arr = None
 
-    def _mat(self, m) -> C.POINTER(C.c_double):
-        a = L.cmat(m) if np.ndim(m) == 2 else np.ascontiguousarray(m, dtype=np.complex128)
-        self._keep.append(a)
-        return a.ctypes.data_as(C.POINTER(C.c_double))
+    def _mat(self, m, op: int, slot: int):
+        a = np.asarray(m, dtype=np.complex128)
+        self._mats.append(a.ravel(order="F") if a.ndim == 2 else a.ravel())
+        self._where.append((op, slot))
 
     def add(self, typ, targets, matrix, controls=(), cvals=(), n_kraus=0, params=None, dmats=None):
-        op = self.arr[self.n]
-        op.type = typ
-        op.n_targets = len(targets)
-        op.n_controls = len(controls)
-        op.n_kraus = n_kraus
-        for i, t in enumerate(targets):
-            op.targets[i] = int(t)
-        for i, c in enumerate(controls):
-            op.controls[i] = int(c)
-            op.control_values[i] = int(cvals[i]) if i < len(cvals) else 1
-        op.matrix = self._mat(matrix)
-        op.n_params = 0
+        i = self.n
+        self._ops.append((typ, targets, controls, cvals, n_kraus, params))
+        self._mat(matrix, i, 0)
         if params:
-            op.n_params = len(params)
-            for i, (p, dm) in enumerate(zip(params, dmats)):
-                op.param_index[i] = int(p)
-                op.dmatrix[i] = self._mat(dm)
+            for p, dm in enumerate(dmats):
+                self._mat(dm, i, 1 + p)
         self.n += 1
+        self._rec = None
+
+    def _finish(self):
+        if self._rec is not None:
+            return
+        rec = np.zeros(max(self.n, 1), dtype=self._dtype())
+        if self.n:
+            ops = self._ops
+            zt, zc = [0] * L.MAX_TARGETS, [0] * L.MAX_CONTROLS
+            rec["type"][:self.n] = [o[0] for o in ops]
+            rec["n_targets"][:self.n] = [len(o[1]) for o in ops]
+            rec["n_controls"][:self.n] = [len(o[2]) for o in ops]
+            rec["n_kraus"][:self.n] = [o[4] for o in ops]
+            rec["targets"][:self.n] = [list(o[1]) + zt[len(o[1]):] for o in ops]
+            if any(o[2] for o in ops):
+                rec["controls"][:self.n] = [list(o[2]) + zc[len(o[2]):] for o in ops]
+                rec["control_values"][:self.n] = [
+                    [int(o[3][k]) if k < len(o[3]) else 1 for k in range(len(o[2]))] + zc[len(o[2]):] for o in ops]
+            if any(o[5] for o in ops):
+                rec["n_params"][:self.n] = [len(o[5]) if o[5] else 0 for o in ops]
+                rec["param_index"][:self.n] = [(list(o[5]) + [0, 0, 0])[:3] if o[5] else [0, 0, 0] for o in ops]
+        if self._mats:
+            buf = np.concatenate(self._mats)
+            sizes = np.fromiter((m.size for m in self._mats), dtype=np.int64, count=len(self._mats))
+            addr = buf.ctypes.data + 16 * (np.cumsum(sizes) - sizes)
+            where = np.asarray(self._where, dtype=np.int64)
+            main = where[:, 1] == 0
+            rec["matrix"][where[main, 0]] = addr[main]
+            for p in range(3):
+                sel = where[:, 1] == 1 + p
+                if np.any(sel):
+                    rec["dmatrix"][where[sel, 0], p] = addr[sel]
+            self._buf = buf
+        self._rec = rec
+        self._arr = (L.Op * len(rec)).from_buffer(rec)
+
+    @property
+    def arr(self):
+        self._finish()
+        return self._arr
 
     def add_call(self, call: KernelCall, params=None, dmats=None):
         if call.diagonal:
@@ -132,7 +182,8 @@ class OpList:
 
     @property
     def ptr(self):
-        return C.cast(self.arr, C.POINTER(L.Op))
+        self._finish()
+        return C.cast(self._arr, C.POINTER(L.Op))
 
 
 def lower_instructions(instructions, kind: int, noise_mode: str = "reference", param_map=None) -> OpList:
