@@ -16,6 +16,7 @@
 // later in a fixed order.  Reordering ops on
 // disjoint bits inside a pass does not change these terms: both vectors see
 // the same unitaries.
+#include <atomic>
 #include <string.h>
 
 #include <algorithm>
@@ -424,7 +425,7 @@ int adjoint_back_plan(int vq, const std::vector<EOp>& eops, std::vector<EOp>& ba
     return make_plan(vq, back, cfg, plan);
 }
 
-static bool g_adj_shmem = false;
+static std::atomic<bool> g_adj_shmem{ false };
 
 static int adjoint_tile_prepare() {
     if (g_adj_shmem) return BSIM_OK;

@@ -12,6 +12,7 @@
 // reduces the <lambda|dU|psi> products of the op's parameters in the same
 // pass (wavefront shuffle tree -> LDS -> per-block partials -> one ordered
 // sum), so the gradient is deterministic.
+#include <atomic>
 #include <string.h>
 
 #include <algorithm>
@@ -312,8 +313,8 @@ __global__ __launch_bounds__(AB) void k_dm_pauli_op(double2* __restrict__ lam, i
     }
 }
 
-static int g_checkpoint_every = 0;  // 0: ceil(sqrt(ops))
-static int g_adj_fusion = 1;        // state vectors: fused tile backward sweep
+static std::atomic<int> g_checkpoint_every{ 0 };  // 0: ceil(sqrt(ops))
+static std::atomic<int> g_adj_fusion{ 1 };        // state vectors: fused tile backward sweep
 
 struct DmParam {
     int pidx;
@@ -425,8 +426,9 @@ static int dm_adjoint(bsim_state* h, const bsim_op* ops, int n_ops, const std::v
     const int64_t len = h->len;
     const size_t bytes = sizeof(double2) * (size_t)len;
     const int D = n_ops;
-    int s = g_checkpoint_every > 0 ? g_checkpoint_every : 1;
-    if (g_checkpoint_every <= 0)
+    const int every = g_checkpoint_every.load();
+    int s = every > 0 ? every : 1;
+    if (every <= 0)
         while ((int64_t)s * s < D) ++s;
     const int nseg = D > 0 ? (D + s - 1) / s : 0;
     int nslots = 0;

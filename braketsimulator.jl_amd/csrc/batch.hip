@@ -27,6 +27,7 @@
 // (shot, 0, 0, 0), key seed), and the shot is the first i whose inclusive
 // prefix W_0 + ... + W_i exceeds t -- found through the chunk prefix, the
 // segment prefix and a scan of one segment.
+#include <atomic>
 #include <string.h>
 
 #include <algorithm>
@@ -425,7 +426,7 @@ int bsim_batch_run(const bsim_task* tasks, int n_tasks, double* latency_us) {
     BSIM_HIP(hipGetDevice(&dev_id));
     BSIM_HIP(hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, dev_id));
     if (clock_khz <= 0) clock_khz = 100000;
-    static bool attr = false;
+    static std::atomic<bool> attr{ false };
     const int T = tile_config().tile_bits;
     const size_t shmem = std::max(sizeof(double2) * (size_t)((1 << T) + ((1 << T) >> 4)), sizeof(double) * ZR * TB);
     if (!attr) {

@@ -8,6 +8,7 @@
 // register bits: each thread pulls a 16-amplitude sub-cube from LDS into
 // registers, applies all gates of the sweep there, and pushes it back.
 // Diagonal gates and controls read the global index and need no tile bit.
+#include <atomic>
 #include <string.h>
 
 #include <algorithm>
@@ -930,7 +931,7 @@ int run_plan(bsim_state* h, double2* d, const std::vector<EOp>& eops, const Plan
     return launch_plan(h, d, eops, plan, (const char*)h->ops.ptr, true);
 }
 
-static bool g_shmem_set = false;
+static std::atomic<bool> g_shmem_set{ false };
 
 int prepare_tile_kernel() {
     if (g_shmem_set) return BSIM_OK;

@@ -15,6 +15,7 @@
 //                 LDS, one output amplitude per thread.
 //  - k_diag:      diagonal gates, one coalesced pass over the control
 //                 subspace.
+#include <atomic>
 #include <string.h>
 
 #include <algorithm>
@@ -286,7 +287,7 @@ int launch_gate(double2* d, int vq, const GateSpec& g, Workspace& scratch, hipSt
         int64_t gpb = 256 / (int64_t)D;
         int64_t nblk = grid_for((a.ngroups + gpb - 1) / gpb, 1, cap_blocks());
         size_t shmem = sizeof(double2) * (D * D + 256);
-        static bool attr_set = false;
+        static std::atomic<bool> attr_set{ false };
         if (!attr_set && shmem > 64 * 1024) {
             BSIM_HIP(hipFuncSetAttribute((const void*)k_dense_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
                                          160 * 1024));
