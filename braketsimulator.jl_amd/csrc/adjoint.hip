@@ -320,6 +320,10 @@ struct DmParam {
     int pidx;
     GateSpec dcol, urow, proj;  // dU on the column bits, conj(U) on the row bits, control projector
     bool ctl;
+    // 1-qubit gate without controls: dU (column bit) x conj(U) (row bit) as
+    // one 4x4 map on the two bits, dotted with Lambda in one pass
+    bool one_pass = false;
+    GateSpec sop;
 };
 struct DmStep {
     std::vector<EOp> fwd, adj;
@@ -397,6 +401,24 @@ static int dm_steps(int nq, const bsim_op* ops, int n_ops, int n_params, std::ve
             if (int rc = make_spec(vq, k, tr, op.n_controls, cr, op.control_values, op.matrix, false, true, false, P.urow))
                 return rc;
             P.ctl = op.n_controls > 0;
+            if (k == 1 && !P.ctl) {
+                // S[(a_c + 2 a_r), (b_c + 2 b_r)] = dU[a_c, b_c] conj(U[a_r, b_r]); matrix bit 0 is
+                // the column bit (vector qubit q), bit 1 the row bit (vector qubit nq + q)
+                double S[32];
+                const double* dU = op.dmatrix[j];
+                const double* U = op.matrix;
+                for (int r = 0; r < 4; ++r)
+                    for (int c = 0; c < 4; ++c) {
+                        const int ac = r & 1, ar = r >> 1, bc = c & 1, br = c >> 1;
+                        const double dx = dU[2 * (ac + 2 * bc)], dy = dU[2 * (ac + 2 * bc) + 1];
+                        const double ux = U[2 * (ar + 2 * br)], uy = -U[2 * (ar + 2 * br) + 1];
+                        S[2 * (r + 4 * c)] = dx * ux - dy * uy;
+                        S[2 * (r + 4 * c) + 1] = dx * uy + dy * ux;
+                    }
+                const int t2[2] = { op.targets[0], op.targets[0] + nq };
+                if (int rc = make_spec(vq, 2, t2, 0, nullptr, nullptr, S, false, false, false, P.sop)) return rc;
+                P.one_pass = true;
+            }
             if (P.ctl) {  // dU of a controlled gate vanishes off the control subspace
                 const int nc = op.n_controls, DC = 1 << nc;
                 std::vector<double> d(2 * (size_t)DC, 0.0);
@@ -411,6 +433,54 @@ static int dm_steps(int nq, const bsim_op* ops, int n_ops, int n_params, std::ve
         }
     }
     return BSIM_OK;
+}
+
+// Re sum conj(Lambda) (S rho) over the 4-element groups of S's two bits:
+// 2 Re tr(Lambda dU rho U^dagger) / 2 for a 1-qubit gate in one pass.
+constexpr int DM_GRID = 1024;
+__global__ __launch_bounds__(AB) void k_dm_grad(const double2* __restrict__ rho, const double2* __restrict__ lam,
+                                                AdjArgs a, AdjMats<2> M, double* __restrict__ partials) {
+    __shared__ double lds[3 * 8];
+    double acc[1] = { 0.0 };
+    const uint64_t b0 = 1ull << a.t0, b1 = 1ull << a.t1;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < a.ngroups; g += stride) {
+        uint64_t base = (uint64_t)g;
+        for (int i = 0; i < a.nins; ++i) base = pad_bit(base, a.ins[i]);
+        base |= a.cval;
+        double2 x[4], l[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint64_t off = base | ((j & 1) ? b0 : 0) | ((j & 2) ? b1 : 0);
+            x[j] = rho[off];
+            l[j] = lam[off];
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            double2 mu = cmul(M.udag[r], x[0]);
+#pragma unroll
+            for (int c = 1; c < 4; ++c) mu = cfma(M.udag[r + 4 * c], x[c], mu);
+            acc[0] += l[r].x * mu.x + l[r].y * mu.y;
+        }
+    }
+    block_sum_adj<1>(acc, lds, partials + (size_t)blockIdx.x * 3);
+}
+
+__global__ __launch_bounds__(256) void k_dm_grad_finish(const double* __restrict__ partials, int nblocks,
+                                                        double* __restrict__ out) {
+    __shared__ double sh[256];
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) v += partials[(size_t)b * 3];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = sh[0];
+        out[1] = 0.0;
+    }
 }
 
 static int apply_eops(bsim_state* h, double2* d, const std::vector<EOp>& e) {
@@ -466,7 +536,10 @@ static int dm_adjoint(bsim_state* h, const bsim_op* ops, int n_ops, const std::v
                        (int)words.size());
     BSIM_HIP(hipGetLastError());
     if (int rc = dot_device(lam, h->d, len, h->red, h->stream, res + 2 * (size_t)nslots)) return rc;
-    // backward, segment by segment
+    // backward, segment by segment (one-pass products use their own partials)
+    // (sized for dot_device too, so that neither reallocates the other's buffer)
+    if (int rc = ensure_ws(h->red, sizeof(double) * std::max<size_t>(3 * (size_t)DM_GRID, 4096 * 2 + 64))) return rc;
+    double* gpart = (double*)h->red.ptr;
     int slot = nslots;
     std::vector<int> slot_pidx(nslots);
     for (int c = nseg - 1; c >= 0; --c) {
@@ -480,6 +553,21 @@ static int dm_adjoint(bsim_state* h, const bsim_op* ops, int n_ops, const std::v
         for (int k = b - 1; k >= a; --k) {
             for (int j = (int)st[k].params.size() - 1; j >= 0; --j) {
                 const DmParam& P = st[k].params[j];
+                if (P.one_pass) {
+                    AdjArgs aa = adj_args(h->vq, P.sop, true);
+                    AdjMats<2> M;
+                    memset(&M, 0, sizeof(M));
+                    for (int x = 0; x < 16; ++x) M.udag[x] = P.sop.m[x];
+                    hipLaunchKernelGGL(k_dm_grad, dim3(DM_GRID), dim3(AB), 0, h->stream, (const double2*)buf(nseg + (k - a)),
+                                       (const double2*)lam, aa, M, gpart);
+                    BSIM_HIP(hipGetLastError());
+                    --slot;
+                    slot_pidx[slot] = P.pidx;
+                    hipLaunchKernelGGL(k_dm_grad_finish, dim3(1), dim3(256), 0, h->stream, (const double*)gpart, DM_GRID,
+                                       res + 2 * (size_t)slot);
+                    BSIM_HIP(hipGetLastError());
+                    continue;
+                }
                 BSIM_HIP(hipMemcpyAsync(sig, buf(nseg + (k - a)), bytes, hipMemcpyDeviceToDevice, h->stream));
                 if (int rc = launch_gate(sig, h->vq, P.dcol, h->scratch, h->stream)) return rc;
                 if (P.ctl)
