@@ -542,15 +542,35 @@ static int dm_adjoint(bsim_state* h, const bsim_op* ops, int n_ops, const std::v
     double* gpart = (double*)h->red.ptr;
     int slot = nslots;
     std::vector<int> slot_pidx(nslots);
+    auto run_fused = [&](double2* d, std::vector<EOp>& list) -> int {
+        if (list.empty()) return BSIM_OK;
+        Plan plan;
+        if (int rc = make_plan(h->vq, list, tile_config(), plan)) return rc;
+        if (int rc = run_plan(h, d, list, plan)) return rc;
+        list.clear();
+        return BSIM_OK;
+    };
+    std::vector<EOp> pend;
     for (int c = nseg - 1; c >= 0; --c) {
         const int a = c * s, b = std::min(D, (c + 1) * s);
-        BSIM_HIP(hipMemcpyAsync(buf(nseg), buf(c), bytes, hipMemcpyDeviceToDevice, h->stream));
-        for (int k = a; k + 1 < b; ++k) {  // states before ops a+1 .. b-1
-            double2* nx = buf(nseg + (k - a) + 1);
-            BSIM_HIP(hipMemcpyAsync(nx, buf(nseg + (k - a)), bytes, hipMemcpyDeviceToDevice, h->stream));
-            if (int rc = apply_eops(h, nx, st[k].fwd)) return rc;
+        // states before the segment's parametrized ops: the ops between two of
+        // them run as fused passes on a working copy (sig), each state kept by a copy
+        std::vector<double2*> before((size_t)(b - a), nullptr);
+        BSIM_HIP(hipMemcpyAsync(sig, buf(c), bytes, hipMemcpyDeviceToDevice, h->stream));
+        int used = 0;
+        for (int k = a; k < b; ++k) {
+            if (!st[k].params.empty()) {
+                if (int rc = run_fused(sig, pend)) return rc;
+                before[k - a] = buf(nseg + used++);
+                BSIM_HIP(hipMemcpyAsync(before[k - a], sig, bytes, hipMemcpyDeviceToDevice, h->stream));
+            }
+            pend.insert(pend.end(), st[k].fwd.begin(), st[k].fwd.end());
         }
+        pend.clear();  // the ops after the last parametrized one are not needed
+        // Lambda backwards through the segment, fused between parametrized ops
         for (int k = b - 1; k >= a; --k) {
+            if (!st[k].params.empty())
+                if (int rc = run_fused(lam, pend)) return rc;
             for (int j = (int)st[k].params.size() - 1; j >= 0; --j) {
                 const DmParam& P = st[k].params[j];
                 if (P.one_pass) {
@@ -558,7 +578,7 @@ static int dm_adjoint(bsim_state* h, const bsim_op* ops, int n_ops, const std::v
                     AdjMats<2> M;
                     memset(&M, 0, sizeof(M));
                     for (int x = 0; x < 16; ++x) M.udag[x] = P.sop.m[x];
-                    hipLaunchKernelGGL(k_dm_grad, dim3(DM_GRID), dim3(AB), 0, h->stream, (const double2*)buf(nseg + (k - a)),
+                    hipLaunchKernelGGL(k_dm_grad, dim3(DM_GRID), dim3(AB), 0, h->stream, (const double2*)before[k - a],
                                        (const double2*)lam, aa, M, gpart);
                     BSIM_HIP(hipGetLastError());
                     --slot;
@@ -568,7 +588,7 @@ static int dm_adjoint(bsim_state* h, const bsim_op* ops, int n_ops, const std::v
                     BSIM_HIP(hipGetLastError());
                     continue;
                 }
-                BSIM_HIP(hipMemcpyAsync(sig, buf(nseg + (k - a)), bytes, hipMemcpyDeviceToDevice, h->stream));
+                BSIM_HIP(hipMemcpyAsync(sig, before[k - a], bytes, hipMemcpyDeviceToDevice, h->stream));
                 if (int rc = launch_gate(sig, h->vq, P.dcol, h->scratch, h->stream)) return rc;
                 if (P.ctl)
                     if (int rc = launch_gate(sig, h->vq, P.proj, h->scratch, h->stream)) return rc;
@@ -577,8 +597,9 @@ static int dm_adjoint(bsim_state* h, const bsim_op* ops, int n_ops, const std::v
                 slot_pidx[slot] = P.pidx;
                 if (int rc = dot_device(lam, sig, len, h->red, h->stream, res + 2 * (size_t)slot)) return rc;
             }
-            if (int rc = apply_eops(h, lam, st[k].adj)) return rc;
+            pend.insert(pend.end(), st[k].adj.begin(), st[k].adj.end());
         }
+        if (int rc = run_fused(lam, pend)) return rc;
     }
     std::vector<double> hres(2 * (size_t)nslots + 2);
     BSIM_HIP(hipMemcpyAsync(hres.data(), res, sizeof(double) * hres.size(), hipMemcpyDeviceToHost, h->stream));
