@@ -435,6 +435,12 @@ static int adjoint_tile_prepare() {
     return BSIM_OK;
 }
 
+size_t adjoint_tile_partial_bytes(const Plan& plan, int vq, int nslots, int pm) {
+    int64_t nt = 1;
+    for (const Pass& P : plan.passes) nt = std::max<int64_t>(nt, (int64_t)1 << (vq - P.T));
+    return sizeof(double) * (size_t)nslots * (size_t)std::max(pm, 1) * (size_t)nt * AT_W;
+}
+
 int adjoint_tile_run(bsim_state* h, double2* psi, double2* lam, const Plan& plan, int nslots, int pm, double* out3,
                      const std::function<int(int)>& single) {
     if (int rc = adjoint_tile_prepare()) return rc;

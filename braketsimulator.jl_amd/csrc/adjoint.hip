@@ -676,14 +676,21 @@ int bsim_adjoint_gradient(bsim_state* h, const bsim_op* ops, int n_ops, int n_wo
     BSIM_HIP(hipMemcpyAsync(work, h->d, sizeof(double2) * (size_t)len, hipMemcpyDeviceToDevice, h->stream));
     // backward
     int slot = nslots;
-    if (g_adj_fusion && h->vq >= 5) {
-        // tile passes over the reversed list of U^dagger ops (both vectors in
-        // LDS, parameter products reduced inside the pass)
-        const int L = (int)eops.size();
-        std::vector<EOp> back;
-        Plan bplan;
-        int ns = 0, pm = 0;  // ns == nslots
+    // tile passes over the reversed list of U^dagger ops (both vectors in LDS,
+    // parameter products reduced inside the pass), unless their per-tile
+    // partial sums would outgrow max(4 GiB, the state's size): then one
+    // kernel per op
+    const int L = (int)eops.size();
+    std::vector<EOp> back;
+    Plan bplan;
+    int ns = 0, pm = 0;  // ns == nslots
+    bool fused = g_adj_fusion && h->vq >= 5;
+    if (fused) {
         if (int rc = adjoint_back_plan(h->vq, eops, back, bplan, ns, pm)) return rc;
+        const size_t limit = std::max<size_t>((size_t)4 << 30, sizeof(double2) * (size_t)len);
+        fused = adjoint_tile_partial_bytes(bplan, h->vq, nslots, pm) <= limit;
+    }
+    if (fused) {
         auto single = [&](int i) -> int {
             const EOp& e = eops[L - 1 - i];
             if (e.nparams > 0)

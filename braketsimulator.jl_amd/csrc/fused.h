@@ -155,6 +155,9 @@ int launch_kraus_ref(bsim_state* h, double2* d, const EOp& e);
 int adjoint_back_plan(int vq, const std::vector<EOp>& eops, std::vector<EOp>& back, Plan& plan, int& nslots,
                       int& pm);
 
+// Bytes of the per-tile partial sums adjoint_tile_run needs for `plan`.
+size_t adjoint_tile_partial_bytes(const Plan& plan, int vq, int nslots, int pm);
+
 // Fused backward sweep of the state-vector adjoint: runs `plan` (a plan of
 // the reversed U^dagger ops, 11 tile bits, 3 register bits; parametrized ops
 // carry aslot and their derivative payloads) on (psi, lam); a non-tile step

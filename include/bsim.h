@@ -262,8 +262,10 @@ int bsim_set_checkpoint_interval(int ops, int* old);
 /* State-vector adjoint backward sweep: 1 (default) = tile passes over the
  * reversed U^dagger ops with psi and lambda both in LDS and the parameter
  * products reduced inside the pass; 0 = one kernel per op.  Both give the
- * same gradient up to summation order.  Returns the previous value in *old
- * (may be NULL).  No reference counterpart. */
+ * same gradient up to summation order.  With 1, states of fewer than 5
+ * qubits, and sweeps whose per-tile partial sums would exceed max(4 GiB,
+ * the state's bytes), still run one kernel per op.  Returns the previous
+ * value in *old (may be NULL).  No reference counterpart. */
 int bsim_set_adjoint_fusion(int on, int* old);
 /* <H> for a Pauli sum on the current state. */
 int bsim_expect_pauli_sum(bsim_state* h, int n_words, const char* paulis, const double* coeffs, double* out);
