@@ -28,6 +28,7 @@
 // prefix W_0 + ... + W_i exceeds t -- found through the chunk prefix, the
 // segment prefix and a scan of one segment.
 #include <atomic>
+#include <memory>
 #include <string.h>
 
 #include <algorithm>
@@ -384,7 +385,10 @@ int bsim_batch_run(const bsim_task* tasks, int n_tasks, double* latency_us) {
     }
     std::vector<Pass> passes(pass0[n_tasks]);
     std::vector<Sweep> sweeps(std::max<size_t>(sw0[n_tasks], 1));
-    std::vector<TOp> tops(std::max<size_t>(op0[n_tasks], 1));
+    // every entry is written below: left uninitialized (hundreds of MB for large batches)
+    const size_t n_tops = std::max<size_t>(op0[n_tasks], 1);
+    std::unique_ptr<TOp[]> tops_buf(new TOp[n_tops]);
+    TOp* tops = tops_buf.get();
     parallel_for(n_tasks, [&](int i) {
         const Plan& p = tp[i].plan;
         for (size_t k = 0; k < p.passes.size(); ++k) {
@@ -494,7 +498,7 @@ int bsim_batch_run(const bsim_task* tasks, int n_tasks, double* latency_us) {
     size_t off_passes = off_entries + al(sizeof(BEntry) * entries.size());
     size_t off_sweeps = off_passes + al(sizeof(Pass) * passes.size());
     size_t off_tops = off_sweeps + al(sizeof(Sweep) * sweeps.size());
-    size_t off_ctl = off_tops + al(sizeof(TOp) * tops.size());
+    size_t off_ctl = off_tops + al(sizeof(TOp) * n_tops);
     size_t off_done = off_ctl + al(sizeof(BCtl));
     size_t off_fin = off_done + al(sizeof(unsigned long long) * entries.size());
     size_t off_z = off_fin + al(sizeof(unsigned long long) * n_tasks);
@@ -536,7 +540,7 @@ int bsim_batch_run(const bsim_task* tasks, int n_tasks, double* latency_us) {
     BSIM_HIP(hipMemcpyAsync(dev + off_entries, entries.data(), sizeof(BEntry) * entries.size(), hipMemcpyHostToDevice, s));
     BSIM_HIP(hipMemcpyAsync(dev + off_passes, passes.data(), sizeof(Pass) * passes.size(), hipMemcpyHostToDevice, s));
     BSIM_HIP(hipMemcpyAsync(dev + off_sweeps, sweeps.data(), sizeof(Sweep) * sweeps.size(), hipMemcpyHostToDevice, s));
-    BSIM_HIP(hipMemcpyAsync(dev + off_tops, tops.data(), sizeof(TOp) * tops.size(), hipMemcpyHostToDevice, s));
+    BSIM_HIP(hipMemcpyAsync(dev + off_tops, tops, sizeof(TOp) * n_tops, hipMemcpyHostToDevice, s));
     std::vector<unsigned long long> zeros(std::max(entries.size(), (size_t)n_tasks) + 8, 0ull);
     BSIM_HIP(hipMemcpyAsync(dev + off_ctl, zeros.data(), sizeof(BCtl), hipMemcpyHostToDevice, s));
     BSIM_HIP(hipMemcpyAsync(dev + off_done, zeros.data(), sizeof(unsigned long long) * entries.size(),
