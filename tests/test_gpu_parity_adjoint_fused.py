@@ -127,3 +127,16 @@ def test_fused_adjoint_shared_parameters_and_small_state():
         ev_f, g_f, ev_p, g_p = _both(n, ins, pm, words, coeffs)
         assert ev_f == ev_p
         np.testing.assert_allclose(g_f, g_p, atol=TOL)
+
+
+def test_fused_adjoint_without_parameters():
+    """No parametrized gate: the backward sweep carries no products, the
+    gradient is empty and <H> matches the oracle."""
+    n = 12
+    _say("fused adjoint without parameters n=12")
+    ins, _, _ = W.hea(n, 2, seed=1)
+    words, coeffs = W.tfim(n)
+    ev, grad = A.adjoint_gradient(StateVectorSimulator(n, 0), ins, {}, words, coeffs, n_params=0, fused=True)
+    psi = O.evolve_sv(n, ins)
+    assert grad.shape == (0,)
+    assert abs(ev - sum(c * O.sv_expect_pauli(psi, n, w) for w, c in zip(words, coeffs))) < TOL
