@@ -167,6 +167,24 @@ class TwoQubitPauliChannel(Noise):
         return ks
 
 
+class MultiQubitPauliChannel(TwoQubitPauliChannel):
+    """MultiQubitPauliChannel{N} (src/noises.jl:71-90): N is the length of
+    the Pauli words in ``probabilities``.  The reference defines Kraus
+    matrices for N = 2 only (TwoQubitPauliChannel, src/noise_kernels.jl:66);
+    other N raise, as there."""
+
+    name = "MultiQubitPauliChannel"
+
+    def __init__(self, probabilities: dict):
+        self.probabilities = dict(probabilities)
+        self.n_qubits = len(next(iter(self.probabilities))) if self.probabilities else 0
+
+    def kraus_rep(self):
+        if self.n_qubits != 2:
+            raise NotImplementedError("Kraus matrices of a Pauli channel are defined for 2 qubits only")
+        return TwoQubitPauliChannel.kraus_rep(self)
+
+
 NOISES = {c.name: c for c in (Kraus, BitFlip, PhaseFlip, Depolarizing, PauliChannel, AmplitudeDamping,
                               GeneralizedAmplitudeDamping, PhaseDamping, TwoQubitDepolarizing,
-                              TwoQubitDephasing, TwoQubitPauliChannel)}
+                              TwoQubitDephasing, TwoQubitPauliChannel, MultiQubitPauliChannel)}

@@ -192,3 +192,19 @@ def test_workload_sizes():
     assert len(pm) == 384 and len(h) == 8 * (48 + 23)
     words, c = W.tfim(24)
     assert len(words) == 23 + 24
+
+
+def test_multi_qubit_pauli_channel():
+    """MultiQubitPauliChannel{N} (src/noises.jl:71-90): for N = 2 the Kraus
+    matrices of TwoQubitPauliChannel, trace preserving; other N raise."""
+    from bsim import noises as Nz
+
+    probs = {"XY": 0.1, "ZZ": 0.05, "IX": 0.02}
+    a, b = Nz.MultiQubitPauliChannel(probs), Nz.TwoQubitPauliChannel(probs)
+    assert a.n_qubits == 2
+    ka = a.kraus_rep()
+    for x, y in zip(ka, b.kraus_rep()):
+        np.testing.assert_array_equal(x, y)
+    np.testing.assert_allclose(sum(k.conj().T @ k for k in ka), np.eye(4), atol=1e-15)
+    with pytest.raises(NotImplementedError):
+        Nz.MultiQubitPauliChannel({"XYZ": 0.1}).kraus_rep()
