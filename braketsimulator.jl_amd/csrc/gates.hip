@@ -11,7 +11,9 @@
 //  - k_dense<K>:  K = 1, 2 targets with the matrix in kernel arguments;
 //                 threads own one amplitude group (strided pairs/quads whose
 //                 low halves are consecutive across lanes).
-//  - k_dense_lds: K = 3..6 targets; matrix and amplitude groups staged in
+//  - k_dense_mfma: K = 4, 5 targets on the FP64 matrix cores (the real
+//                 form of the gate as a GEMM over amplitude groups).
+//  - k_dense_lds: K = 3, 6 targets; matrix and amplitude groups staged in
 //                 LDS, one output amplitude per thread.
 //  - k_diag:      diagonal gates, one coalesced pass over the control
 //                 subspace.
@@ -19,6 +21,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "gates.h"
 
@@ -182,6 +185,138 @@ __global__ __launch_bounds__(256) void k_dense_lds(double2* __restrict__ sv, Ins
     }
 }
 
+// K = 4, 5 on the matrix cores.  A dense K-qubit gate is the real GEMM
+//   [Re y; Im y] = [[Re M, -Im M], [Im M, Re M]] [Re x; Im x]
+// (2D x 2D times 2D x groups, D = 2^K) done with v_mfma_f64_16x16x4f64:
+// A = 16 x 4 blocks of the real matrix (lane l: row l % 16, column l / 16),
+// B = 4 x 16 blocks of the amplitudes (lane l: row l / 16, group l % 16),
+// D = 16 x 16 accumulator tiles (lane l, register r: row 4 r + l / 16,
+// group l % 16; measured on gfx950 with a one-hot probe).
+// A workgroup stages 64 groups (2^(6+K) amplitudes) in LDS; element e of the
+// block takes its bits, in ascending order, at the block's local positions
+// (the 6 lowest non-target bits and the K target bits, sorted), so that
+// consecutive threads load and store consecutive addresses wherever the
+// targets are.  Each wave multiplies 16 groups with the matrix blocks
+// (`areal`, prepared by the host in lane order) in registers (K = 4) or in
+// LDS (K = 5).
+struct MfmaLayout {
+    int nloc;          // 6 + K local bits
+    int16_t code[11];  // local bit i: target b (code b) or group bit t (code 16 + t)
+};
+
+template <int K>
+__global__ __launch_bounds__(256) void k_dense_mfma(double2* __restrict__ sv, InsArgs a,
+                                                    const double* __restrict__ areal, int4 tlo, int thi,
+                                                    MfmaLayout lay) {
+    constexpr int D = 1 << K, RT = 2 * D / 16, KS = 2 * D / 4, GB = 64, PAD = D + 1, NA = RT * KS;
+    constexpr bool A_IN_LDS = K >= 5;
+    __shared__ double2 X[GB * PAD];
+    __shared__ double AL[A_IN_LDS ? NA * 64 : 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double av[A_IN_LDS ? 1 : NA];
+    if constexpr (A_IN_LDS) {
+        for (int i = threadIdx.x; i < NA * 64; i += 256) AL[i] = areal[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < NA; ++i) av[i] = areal[(size_t)i * 64 + lane];
+    }
+    const int tb[5] = { tlo.x, tlo.y, tlo.z, tlo.w, thi };
+    // this thread's elements e = threadIdx.x + 256 i: LDS slot (group, amplitude)
+    // and address offset from the block's base (the deposit of the group's 6
+    // low bits at the 6 lowest free positions, the amplitude's at the targets)
+    int slot[D / 4];
+    uint64_t loc[D / 4];
+#pragma unroll
+    for (int i = 0; i < D / 4; ++i) {
+        const int e = threadIdx.x + 256 * i;
+        int gc = 0, j = 0;
+        for (int b = 0; b < lay.nloc; ++b) {
+            const int bit = (e >> b) & 1, cd = lay.code[b];
+            if (cd >= 16) gc |= bit << (cd - 16);
+            else j |= bit << cd;
+        }
+        slot[i] = gc * PAD + j;
+        uint64_t off = group_base((uint64_t)gc, a) & ~a.cval;
+#pragma unroll
+        for (int b = 0; b < K; ++b)
+            if ((j >> b) & 1) off |= 1ull << tb[b];
+        loc[i] = off | ((uint64_t)gc << 58);  // group index (6 bits) kept in the top bits for the range test
+    }
+    const int64_t nblk = (a.ngroups + GB - 1) / GB;
+    const int c = 16 * wave + (lane & 15), kq = lane >> 4;
+    // software pipeline: the next block's amplitudes are loaded into registers
+    // while this block's products run on the matrix cores
+    double2 pre[D / 4];
+    auto fetch = [&](int64_t blk) {
+        const int64_t g0 = blk * GB;
+        const uint64_t base = group_base((uint64_t)g0, a);  // g0 is a multiple of 64
+#pragma unroll
+        for (int i = 0; i < D / 4; ++i) {
+            const int64_t g = g0 + (int64_t)(loc[i] >> 58);
+            pre[i] = g < a.ngroups ? sv[base | (loc[i] & ((1ull << 58) - 1))] : make_double2(0.0, 0.0);
+        }
+    };
+    if (blockIdx.x < nblk) fetch(blockIdx.x);
+    for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+        const int64_t g0 = blk * GB;
+        const uint64_t base = group_base((uint64_t)g0, a);
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < D / 4; ++i) X[slot[i]] = pre[i];
+        __syncthreads();
+        if (blk + gridDim.x < nblk) fetch(blk + gridDim.x);
+        double bv[KS];
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            const int q = 4 * ks + kq;
+            bv[ks] = q < D ? X[c * PAD + q].x : X[c * PAD + q - D].y;
+        }
+        // the RT accumulator tiles are independent: interleave them so that
+        // consecutive MFMAs do not wait for each other
+        typedef double d4 __attribute__((ext_vector_type(4)));
+        d4 acc[RT];
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) acc[rt] = d4{ 0.0, 0.0, 0.0, 0.0 };
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) {
+                const double am = A_IN_LDS ? AL[(rt * KS + ks) * 64 + lane] : av[A_IN_LDS ? 0 : rt * KS + ks];
+                acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(am, bv[ks], acc[rt], 0, 0, 0);
+            }
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * rt + 4 * r + kq;
+                if (row < D) X[c * PAD + row].x = acc[rt][r];
+                else X[c * PAD + row - D].y = acc[rt][r];
+            }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < D / 4; ++i) {
+            const int64_t g = g0 + (int64_t)(loc[i] >> 58);
+            if (g < a.ngroups) sv[base | (loc[i] & ((1ull << 58) - 1))] = X[slot[i]];
+        }
+    }
+}
+
+// A blocks of the real form of the D x D complex matrix m (column-major), in
+// the order k_dense_mfma reads them: block (rt, ks), lane l holds
+// R[16 rt + l % 16][4 ks + l / 16].
+static void mfma_operands(const std::vector<double2>& m, int K, std::vector<double>& out) {
+    const int D = 1 << K, RT = 2 * D / 16, KS = 2 * D / 4;
+    out.assign((size_t)RT * KS * 64, 0.0);
+    auto R = [&](int r, int k) {
+        const double2 z = m[(size_t)(r % D) + (size_t)D * (k % D)];
+        if (r < D) return k < D ? z.x : -z.y;
+        return k < D ? z.y : z.x;
+    };
+    for (int rt = 0; rt < RT; ++rt)
+        for (int ks = 0; ks < KS; ++ks)
+            for (int l = 0; l < 64; ++l) out[((size_t)rt * KS + ks) * 64 + l] = R(16 * rt + l % 16, 4 * ks + l / 16);
+}
+
 // Diagonal: amplitude idx (in the control subspace) times d[j(idx)].
 __global__ __launch_bounds__(256) void k_diag(double2* __restrict__ sv, InsArgs a, const double2* __restrict__ dg,
                                               int K, int4 tlo, int4 thi) {
@@ -278,6 +413,44 @@ int launch_gate(double2* d, int vq, const GateSpec& g, Workspace& scratch, hipSt
         SmallMat<2> M;
         for (int i = 0; i < 16; ++i) M.m[i] = g.m[i];
         hipLaunchKernelGGL(k_dense<2>, dim3((unsigned)grid), dim3(256), 0, s, d, a, M, g.tb[0], g.tb[1]);
+    } else if (g.k == 4 || g.k == 5) {
+        std::vector<double> av;
+        mfma_operands(g.m, g.k, av);
+        if (int rc = ensure_ws(scratch, sizeof(double) * av.size())) return rc;
+        BSIM_HIP(hipMemcpyAsync(scratch.ptr, av.data(), sizeof(double) * av.size(), hipMemcpyHostToDevice, s));
+        const int64_t nblk = grid_for((a.ngroups + 63) / 64, 1, cap_blocks());
+        const int4 tlo = make_int4(g.tb[0], g.tb[1], g.tb[2], g.tb[3]);
+        const int thi = g.k > 4 ? g.tb[4] : 0;
+        // local bits of a block of 64 groups: the 6 lowest bits that are neither
+        // targets nor controls, and the targets, in ascending position order
+        MfmaLayout lay;
+        memset(&lay, 0, sizeof(lay));
+        {
+            uint64_t tmask = 0, used = 0;
+            for (int b = 0; b < g.k; ++b) tmask |= 1ull << g.tb[b];
+            for (int i = 0; i < a.nins; ++i) used |= 1ull << a.ins[i];
+            int ng = 0, n = 0;
+            for (int bit = 0; bit < vq && n < 6 + g.k; ++bit) {
+                if (tmask & (1ull << bit)) {
+                    int b = 0;
+                    while (g.tb[b] != bit) ++b;
+                    lay.code[n++] = (int16_t)b;
+                } else if (!(used & (1ull << bit)) && ng < 6) {
+                    lay.code[n++] = (int16_t)(16 + ng++);
+                }
+            }
+            // fewer than 6 free bits (tiny states): the remaining group bits on top
+            while (ng < 6) lay.code[n++] = (int16_t)(16 + ng++);
+            lay.nloc = n;
+        }
+        if (g.k == 4)
+            hipLaunchKernelGGL(k_dense_mfma<4>, dim3((unsigned)nblk), dim3(256), 0, s, d, a,
+                               (const double*)scratch.ptr, tlo, thi, lay);
+        else
+            hipLaunchKernelGGL(k_dense_mfma<5>, dim3((unsigned)nblk), dim3(256), 0, s, d, a,
+                               (const double*)scratch.ptr, tlo, thi, lay);
+        // the operand staging buffer is reused by the next launch: wait for this one
+        BSIM_HIP(hipStreamSynchronize(s));
     } else if (g.k <= 6) {
         size_t D = (size_t)1 << g.k;
         if (int rc = ensure_ws(scratch, sizeof(double2) * D * D)) return rc;
