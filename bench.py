@@ -81,6 +81,7 @@ def main():
     ap.add_argument("--adjoint-qubits", type=int, default=24, help="0 skips the adjoint leg")
     ap.add_argument("--adjoint-large", type=int, default=28,
                     help="qubits of the second adjoint workload (the same ansatz; 0 skips it)")
+    ap.add_argument("--dense-k", type=int, default=28, help="qubits of the dense 3..6-target gate leg (0 skips it)")
     ap.add_argument("--ps-sample", type=int, default=24, help="parameters timed by parameter shift")
     ap.add_argument("--batch", type=int, default=4096, help="circuits in the batch leg (0 skips it)")
     ap.add_argument("--dm-qubits", type=int, default=14, help="noisy density-matrix leg (0 skips it)")
@@ -215,6 +216,8 @@ def main():
         rec["adjoint"] = adjoint_leg(a.adjoint_qubits, a.ps_sample)
     if a.adjoint_large > 0:
         rec["adjoint_large"] = adjoint_large_leg(a.adjoint_large)
+    if a.dense_k > 0:
+        rec["dense_k"] = dense_k_leg(a.dense_k)
     if a.batch > 0:
         rec["batch"] = batch_leg(a.batch, a.sweep_sets)
     if a.dm_qubits > 0:
@@ -336,6 +339,40 @@ def adjoint_large_leg(nq: int):
         "expval": ev, "adjoint_s": round(best, 4), "grads_per_s": round(1.0 / best, 3),
         "per_op_backward_s": round(t_op, 4), "max_abs_diff_fused_vs_per_op": float(np.max(np.abs(g - g_op))),
     }
+
+
+def dense_k_leg(nq: int):
+    """Single dense gates of k = 3..6 targets (random unitaries) at ``nq``
+    qubits through the per-gate path: k = 4..6 run on the FP64 matrix cores,
+    k = 3 through LDS.  Low, high and scattered targets; 12 timed gates each
+    after one warm-up evolve of 4.  Bytes per gate: 32 B x 2^n (read + write)."""
+    import numpy as np
+
+    from bsim import _lib as L
+    from bsim import gates as G
+    from bsim.simulators import Instruction, StateVectorSimulator
+
+    rng = np.random.default_rng(0)
+    sim = StateVectorSimulator(nq, 0)
+    out = {"workload": f"{nq} qubits, one dense k-target gate per launch", "ms_per_gate": {}}
+    for k in (3, 4, 5, 6):
+        for name, tg in (("low", list(range(k))), ("high", [nq - 1 - i for i in range(k)]),
+                         ("scattered", [0, 9, 15, 22, 27, 5][:k])):
+            ins = []
+            for _ in range(4):
+                z = rng.normal(size=(2 ** k, 2 ** k)) + 1j * rng.normal(size=(2 ** k, 2 ** k))
+                ins.append(Instruction(G.Unitary(np.linalg.qr(z)[0]), tg))
+            sim.evolve(ins, mode=1)
+            L.call("bsim_synchronize", sim._h)
+            t = time.perf_counter()
+            for _ in range(3):
+                sim.evolve(ins, mode=1)
+            L.call("bsim_synchronize", sim._h)
+            dt = (time.perf_counter() - t) / 12
+            out["ms_per_gate"][f"k{k}_{name}"] = round(dt * 1e3, 3)
+            log(f"dense k={k} {name} {tg}: {dt * 1e3:.2f} ms per gate, {32 * 2 ** nq / dt / 1e9:.0f} GB/s")
+    del sim
+    return out
 
 
 def measured_traffic(n: int, depth: int):

@@ -12,8 +12,9 @@
 //                 threads own one amplitude group (strided pairs/quads whose
 //                 low halves are consecutive across lanes).
 //  - k_dense_mfma: K = 4, 5 targets on the FP64 matrix cores (the real
-//                 form of the gate as a GEMM over amplitude groups).
-//  - k_dense_lds: K = 3, 6 targets; matrix and amplitude groups staged in
+//                 form of the gate as a GEMM over amplitude groups);
+//                 k_dense_mfma6 the same for K = 6, rows split over waves.
+//  - k_dense_lds: K = 3 targets; matrix and amplitude groups staged in
 //                 LDS, one output amplitude per thread.
 //  - k_diag:      diagonal gates, one coalesced pass over the control
 //                 subspace.
@@ -75,7 +76,7 @@ struct InsArgs {
     int64_t ngroups;
 };
 
-__device__ __forceinline__ uint64_t group_base(uint64_t g, const InsArgs& a) {
+__host__ __device__ __forceinline__ uint64_t group_base(uint64_t g, const InsArgs& a) {
     for (int i = 0; i < a.nins; ++i) g = pad_bit(g, a.ins[i]);
     return g | a.cval;
 }
@@ -201,7 +202,7 @@ __global__ __launch_bounds__(256) void k_dense_lds(double2* __restrict__ sv, Ins
 // LDS (K = 5).
 struct MfmaLayout {
     int nloc;          // 6 + K local bits
-    int16_t code[11];  // local bit i: target b (code b) or group bit t (code 16 + t)
+    int16_t code[12];  // local bit i: target b (code b) or group bit t (code 16 + t)
 };
 
 template <int K>
@@ -297,6 +298,107 @@ __global__ __launch_bounds__(256) void k_dense_mfma(double2* __restrict__ sv, In
         for (int i = 0; i < D / 4; ++i) {
             const int64_t g = g0 + (int64_t)(loc[i] >> 58);
             if (g < a.ngroups) sv[base | (loc[i] & ((1ull << 58) - 1))] = X[slot[i]];
+        }
+    }
+}
+
+// K = 6: the real form is 128 x 128 (128 KiB), too large for LDS beside the
+// groups, so the rows are split over the waves instead: wave w computes row
+// tiles 2w and 2w + 1 for all 64 groups and keeps those 32 rows of the matrix
+// (64 operands per lane) in registers; every B operand read from LDS feeds two
+// MFMAs.
+struct Mfma6Hi {  // the parts of elements e = t + 256 i from i's bits (host-computed)
+    int gc[16], sl[16];
+    uint64_t off[16];
+};
+
+__global__ __launch_bounds__(256) void k_dense_mfma6(double2* __restrict__ sv, InsArgs a,
+                                                      const double* __restrict__ areal, int4 tlo, int2 thi,
+                                                      MfmaLayout lay, Mfma6Hi hi) {
+    constexpr int K = 6, D = 64, KS = 2 * D / 4, GB = 64, PAD = D + 1, NE = D / 4;
+    __shared__ double2 X[GB * PAD];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double av[2 * KS];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) av[t * KS + ks] = areal[((size_t)(2 * wave + t) * KS + ks) * 64 + lane];
+    const int tb[6] = { tlo.x, tlo.y, tlo.z, tlo.w, thi.x, thi.y };
+    // element e = threadIdx.x + 256 i: its group in the block, LDS slot and
+    // address offset from the block's base.  Each is the OR (sum) of a part
+    // from e's low 8 bits (the thread's, computed here) and a part from i's
+    // bits (uniform, in the kernel arguments).
+    int gct = 0, slt;
+    uint64_t offt;
+    {
+        int j = 0;
+        for (int b = 0; b < 8; ++b) {
+            const int bit = (threadIdx.x >> b) & 1, cd = lay.code[b];
+            if (cd >= 16) gct |= bit << (cd - 16);
+            else j |= bit << cd;
+        }
+        slt = gct * PAD + j;
+        offt = group_base((uint64_t)gct, a) & ~a.cval;
+#pragma unroll
+        for (int b = 0; b < K; ++b)
+            if ((j >> b) & 1) offt |= 1ull << tb[b];
+    }
+    const int64_t nblk = (a.ngroups + GB - 1) / GB;
+    const int cl = lane & 15, kq = lane >> 4;
+    for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+        const int64_t g0 = blk * GB;
+        const uint64_t base = group_base((uint64_t)g0, a);  // g0 is a multiple of 64
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NE; ++i) {
+            X[slt + hi.sl[i]] = g0 + (gct | hi.gc[i]) < a.ngroups ? sv[base | offt | hi.off[i]] : make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+        typedef double d4 __attribute__((ext_vector_type(4)));
+        d4 acc[2][4];
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) acc[t][ct] = d4{ 0.0, 0.0, 0.0, 0.0 };
+        // B operands one k-step ahead; the scheduling barrier keeps the
+        // compiler from hoisting every LDS read of the unrolled loop (which
+        // costs a register per read)
+        auto bload = [&](int ks, double* b) {
+            const int q = 4 * ks + kq;
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) {
+                const double2& z = X[(16 * ct + cl) * PAD + (q & (D - 1))];
+                b[ct] = q < D ? z.x : z.y;
+            }
+        };
+        double bn[4];
+        bload(0, bn);
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            double bc[4] = { bn[0], bn[1], bn[2], bn[3] };
+            if (ks + 1 < KS) bload(ks + 1, bn);
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+                for (int t = 0; t < 2; ++t)
+                    acc[t][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t * KS + ks], bc[ct], acc[t][ct], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        __syncthreads();  // every wave has read all of X
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = 16 * (2 * wave + t) + 4 * r + kq, c = 16 * ct + cl;
+                    if (row < D) X[c * PAD + row].x = acc[t][ct][r];
+                    else X[c * PAD + row - D].y = acc[t][ct][r];
+                }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NE; ++i) {
+            if (g0 + (gct | hi.gc[i]) < a.ngroups) sv[base | offt | hi.off[i]] = X[slt + hi.sl[i]];
         }
     }
 }
@@ -413,7 +515,7 @@ int launch_gate(double2* d, int vq, const GateSpec& g, Workspace& scratch, hipSt
         SmallMat<2> M;
         for (int i = 0; i < 16; ++i) M.m[i] = g.m[i];
         hipLaunchKernelGGL(k_dense<2>, dim3((unsigned)grid), dim3(256), 0, s, d, a, M, g.tb[0], g.tb[1]);
-    } else if (g.k == 4 || g.k == 5) {
+    } else if (g.k >= 4 && g.k <= 6) {
         std::vector<double> av;
         mfma_operands(g.m, g.k, av);
         if (int rc = ensure_ws(scratch, sizeof(double) * av.size())) return rc;
@@ -421,6 +523,7 @@ int launch_gate(double2* d, int vq, const GateSpec& g, Workspace& scratch, hipSt
         const int64_t nblk = grid_for((a.ngroups + 63) / 64, 1, cap_blocks());
         const int4 tlo = make_int4(g.tb[0], g.tb[1], g.tb[2], g.tb[3]);
         const int thi = g.k > 4 ? g.tb[4] : 0;
+        const int2 thi6 = make_int2(thi, g.k > 5 ? g.tb[5] : 0);
         // local bits of a block of 64 groups: the 6 lowest bits that are neither
         // targets nor controls, and the targets, in ascending position order
         MfmaLayout lay;
@@ -446,9 +549,28 @@ int launch_gate(double2* d, int vq, const GateSpec& g, Workspace& scratch, hipSt
         if (g.k == 4)
             hipLaunchKernelGGL(k_dense_mfma<4>, dim3((unsigned)nblk), dim3(256), 0, s, d, a,
                                (const double*)scratch.ptr, tlo, thi, lay);
-        else
+        else if (g.k == 5)
             hipLaunchKernelGGL(k_dense_mfma<5>, dim3((unsigned)nblk), dim3(256), 0, s, d, a,
                                (const double*)scratch.ptr, tlo, thi, lay);
+        else {
+            Mfma6Hi hi;
+            for (int i = 0; i < 16; ++i) {
+                int gc = 0, j = 0;
+                for (int b = 8; b < lay.nloc; ++b) {
+                    const int bit = ((256 * i) >> b) & 1, cd = lay.code[b];
+                    if (cd >= 16) gc |= bit << (cd - 16);
+                    else j |= bit << cd;
+                }
+                uint64_t off = group_base((uint64_t)gc, a) & ~a.cval;
+                for (int b = 0; b < 6; ++b)
+                    if ((j >> b) & 1) off |= 1ull << g.tb[b];
+                hi.gc[i] = gc;
+                hi.sl[i] = gc * 65 + j;
+                hi.off[i] = off;
+            }
+            hipLaunchKernelGGL(k_dense_mfma6, dim3((unsigned)nblk), dim3(256), 0, s, d, a,
+                               (const double*)scratch.ptr, tlo, thi6, lay, hi);
+        }
         // the operand staging buffer is reused by the next launch: wait for this one
         BSIM_HIP(hipStreamSynchronize(s));
     } else if (g.k <= 6) {

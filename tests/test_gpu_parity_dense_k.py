@@ -1,6 +1,6 @@
-"""Dense k-qubit gates (k = 3..6): 4 and 5 targets run on the FP64 matrix
-cores (k_dense_mfma: the gate's real form as a GEMM over amplitude groups),
-3 and 6 through LDS.  Against the oracle at 1e-12 on random unitaries with
+"""Dense k-qubit gates (k = 3..6): 4 to 6 targets run on the FP64 matrix
+cores (k_dense_mfma, k_dense_mfma6: the gate's real form as a GEMM over
+amplitude groups), 3 through LDS.  Against the oracle at 1e-12 on random unitaries with
 random (unsorted) targets, with and without controls, in the per-gate path,
 as non-tile steps of a fused evolve, and on density matrices; sizes from one
 partial block of groups to many."""
@@ -67,7 +67,7 @@ def test_dense_k_vs_oracle(k, n, controls, mode):
     np.testing.assert_allclose(sim.state_vector(), want, atol=TOL)
 
 
-@pytest.mark.parametrize("k", [4, 5])
+@pytest.mark.parametrize("k", [4, 5, 6])
 def test_dense_k_apply_gate_and_dm(k):
     rng = np.random.default_rng(77 + k)
     n = 7
@@ -79,9 +79,9 @@ def test_dense_k_apply_gate_and_dm(k):
     sim.evolve(pre)
     sim.apply_gate(G.Unitary(u), *t)
     np.testing.assert_allclose(sim.state_vector(), O.evolve_sv(n, pre + [Instruction(G.Unitary(u), t)]), atol=TOL)
-    # density matrix (5 qubits): the gate on the column bits, its conjugate on
-    # the row bits, each a k-bit dense pass over 2^(10 - k) groups
-    m = 5
+    # density matrix (5 or 6 qubits): the gate on the column bits, its
+    # conjugate on the row bits, each a k-bit dense pass over 2^(2m - k) groups
+    m = max(5, k)
     t2 = [int(x) for x in rng.choice(m, k, replace=False)]
     u2 = random_unitary(k, rng)
     pre2 = [Instruction(G.H(), [q]) for q in range(m)] + [Instruction(G.Rz(0.4 * q + 0.1), [q]) for q in range(m)]
