@@ -303,36 +303,33 @@ __global__ __launch_bounds__(256) void k_dense_mfma(double2* __restrict__ sv, In
 }
 
 // K = 6: the real form is 128 x 128 (128 KiB), too large for LDS beside the
-// groups, so the rows are split over the waves instead: wave w computes row
-// tiles 2w and 2w + 1 for all 64 groups and keeps those 32 rows of the matrix
-// (64 operands per lane) in registers; every B operand read from LDS feeds two
-// MFMAs.
-struct Mfma6Hi {  // the parts of elements e = t + 256 i from i's bits (host-computed)
-    int gc[16], sl[16];
-    uint64_t off[16];
+// groups, so the rows are split over the waves instead: each of 8 waves
+// computes one 16-row tile for all 64 groups and keeps those 16 rows of the
+// matrix (32 operands per lane) in registers.
+struct Mfma6Hi {  // the parts of elements e = t + 512 i from i's bits (host-computed)
+    int gc[8], sl[8];
+    uint64_t off[8];
 };
 
-__global__ __launch_bounds__(256) void k_dense_mfma6(double2* __restrict__ sv, InsArgs a,
+__global__ __launch_bounds__(512) void k_dense_mfma6(double2* __restrict__ sv, InsArgs a,
                                                       const double* __restrict__ areal, int4 tlo, int2 thi,
                                                       MfmaLayout lay, Mfma6Hi hi) {
-    constexpr int K = 6, D = 64, KS = 2 * D / 4, GB = 64, PAD = D + 1, NE = D / 4;
+    constexpr int K = 6, D = 64, KS = 2 * D / 4, GB = 64, PAD = D + 1, NE = D / 8;
     __shared__ double2 X[GB * PAD];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    double av[2 * KS];
+    double av[KS];
 #pragma unroll
-    for (int t = 0; t < 2; ++t)
-#pragma unroll
-        for (int ks = 0; ks < KS; ++ks) av[t * KS + ks] = areal[((size_t)(2 * wave + t) * KS + ks) * 64 + lane];
+    for (int ks = 0; ks < KS; ++ks) av[ks] = areal[((size_t)wave * KS + ks) * 64 + lane];
     const int tb[6] = { tlo.x, tlo.y, tlo.z, tlo.w, thi.x, thi.y };
-    // element e = threadIdx.x + 256 i: its group in the block, LDS slot and
+    // element e = threadIdx.x + 512 i: its group in the block, LDS slot and
     // address offset from the block's base.  Each is the OR (sum) of a part
-    // from e's low 8 bits (the thread's, computed here) and a part from i's
+    // from e's low 9 bits (the thread's, computed here) and a part from i's
     // bits (uniform, in the kernel arguments).
     int gct = 0, slt;
     uint64_t offt;
     {
         int j = 0;
-        for (int b = 0; b < 8; ++b) {
+        for (int b = 0; b < 9; ++b) {
             const int bit = (threadIdx.x >> b) & 1, cd = lay.code[b];
             if (cd >= 16) gct |= bit << (cd - 16);
             else j |= bit << cd;
@@ -355,11 +352,9 @@ __global__ __launch_bounds__(256) void k_dense_mfma6(double2* __restrict__ sv, I
         }
         __syncthreads();
         typedef double d4 __attribute__((ext_vector_type(4)));
-        d4 acc[2][4];
+        d4 acc[4];
 #pragma unroll
-        for (int t = 0; t < 2; ++t)
-#pragma unroll
-            for (int ct = 0; ct < 4; ++ct) acc[t][ct] = d4{ 0.0, 0.0, 0.0, 0.0 };
+        for (int ct = 0; ct < 4; ++ct) acc[ct] = d4{ 0.0, 0.0, 0.0, 0.0 };
         // B operands one k-step ahead; the scheduling barrier keeps the
         // compiler from hoisting every LDS read of the unrolled loop (which
         // costs a register per read)
@@ -379,22 +374,18 @@ __global__ __launch_bounds__(256) void k_dense_mfma6(double2* __restrict__ sv, I
             if (ks + 1 < KS) bload(ks + 1, bn);
 #pragma unroll
             for (int ct = 0; ct < 4; ++ct)
-#pragma unroll
-                for (int t = 0; t < 2; ++t)
-                    acc[t][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t * KS + ks], bc[ct], acc[t][ct], 0, 0, 0);
+                acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ks], bc[ct], acc[ct], 0, 0, 0);
             __builtin_amdgcn_sched_barrier(0);
         }
         __syncthreads();  // every wave has read all of X
 #pragma unroll
-        for (int t = 0; t < 2; ++t)
+        for (int ct = 0; ct < 4; ++ct)
 #pragma unroll
-            for (int ct = 0; ct < 4; ++ct)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int row = 16 * (2 * wave + t) + 4 * r + kq, c = 16 * ct + cl;
-                    if (row < D) X[c * PAD + row].x = acc[t][ct][r];
-                    else X[c * PAD + row - D].y = acc[t][ct][r];
-                }
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * wave + 4 * r + kq, c = 16 * ct + cl;
+                if (row < D) X[c * PAD + row].x = acc[ct][r];
+                else X[c * PAD + row - D].y = acc[ct][r];
+            }
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < NE; ++i) {
@@ -554,10 +545,10 @@ int launch_gate(double2* d, int vq, const GateSpec& g, Workspace& scratch, hipSt
                                (const double*)scratch.ptr, tlo, thi, lay);
         else {
             Mfma6Hi hi;
-            for (int i = 0; i < 16; ++i) {
+            for (int i = 0; i < 8; ++i) {
                 int gc = 0, j = 0;
-                for (int b = 8; b < lay.nloc; ++b) {
-                    const int bit = ((256 * i) >> b) & 1, cd = lay.code[b];
+                for (int b = 9; b < lay.nloc; ++b) {
+                    const int bit = ((512 * i) >> b) & 1, cd = lay.code[b];
                     if (cd >= 16) gc |= bit << (cd - 16);
                     else j |= bit << cd;
                 }
@@ -568,7 +559,7 @@ int launch_gate(double2* d, int vq, const GateSpec& g, Workspace& scratch, hipSt
                 hi.sl[i] = gc * 65 + j;
                 hi.off[i] = off;
             }
-            hipLaunchKernelGGL(k_dense_mfma6, dim3((unsigned)nblk), dim3(256), 0, s, d, a,
+            hipLaunchKernelGGL(k_dense_mfma6, dim3((unsigned)nblk), dim3(512), 0, s, d, a,
                                (const double*)scratch.ptr, tlo, thi6, lay, hi);
         }
         // the operand staging buffer is reused by the next launch: wait for this one
