@@ -562,7 +562,8 @@ int launch_gate(double2* d, int vq, const GateSpec& g, Workspace& scratch, hipSt
             hipLaunchKernelGGL(k_dense_mfma6, dim3((unsigned)nblk), dim3(512), 0, s, d, a,
                                (const double*)scratch.ptr, tlo, thi6, lay, hi);
         }
-        // the operand staging buffer is reused by the next launch: wait for this one
+        // the operands were copied from a host vector that ends with this call:
+        // wait until the copy (and the kernel) has completed
         BSIM_HIP(hipStreamSynchronize(s));
     } else if (g.k <= 6) {
         size_t D = (size_t)1 << g.k;
