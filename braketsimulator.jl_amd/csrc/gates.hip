@@ -342,15 +342,26 @@ __global__ __launch_bounds__(512) void k_dense_mfma6(double2* __restrict__ sv, I
     }
     const int64_t nblk = (a.ngroups + GB - 1) / GB;
     const int cl = lane & 15, kq = lane >> 4;
-    for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    // software pipeline: the next block's amplitudes are loaded into registers
+    // while this block's products run (one 8-wave workgroup fits a CU either
+    // way, so the registers cost no occupancy)
+    double2 pre[NE];
+    auto fetch = [&](int64_t blk) {
         const int64_t g0 = blk * GB;
         const uint64_t base = group_base((uint64_t)g0, a);  // g0 is a multiple of 64
+#pragma unroll
+        for (int i = 0; i < NE; ++i)
+            pre[i] = g0 + (gct | hi.gc[i]) < a.ngroups ? sv[base | offt | hi.off[i]] : make_double2(0.0, 0.0);
+    };
+    if (blockIdx.x < nblk) fetch(blockIdx.x);
+    for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+        const int64_t g0 = blk * GB;
+        const uint64_t base = group_base((uint64_t)g0, a);
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < NE; ++i) {
-            X[slt + hi.sl[i]] = g0 + (gct | hi.gc[i]) < a.ngroups ? sv[base | offt | hi.off[i]] : make_double2(0.0, 0.0);
-        }
+        for (int i = 0; i < NE; ++i) X[slt + hi.sl[i]] = pre[i];
         __syncthreads();
+        if (blk + gridDim.x < nblk) fetch(blk + gridDim.x);
         typedef double d4 __attribute__((ext_vector_type(4)));
         d4 acc[4];
 #pragma unroll
