@@ -18,7 +18,6 @@
 //                 LDS, one output amplitude per thread.
 //  - k_diag:      diagonal gates, one coalesced pass over the control
 //                 subspace.
-#include <atomic>
 #include <string.h>
 
 #include <algorithm>
@@ -576,7 +575,7 @@ int launch_gate(double2* d, int vq, const GateSpec& g, Workspace& scratch, hipSt
         // the operands were copied from a host vector that ends with this call:
         // wait until the copy (and the kernel) has completed
         BSIM_HIP(hipStreamSynchronize(s));
-    } else if (g.k <= 6) {
+    } else if (g.k == 3) {
         size_t D = (size_t)1 << g.k;
         if (int rc = ensure_ws(scratch, sizeof(double2) * D * D)) return rc;
         BSIM_HIP(hipMemcpyAsync(scratch.ptr, g.m.data(), sizeof(double2) * D * D, hipMemcpyHostToDevice, s));
@@ -584,13 +583,7 @@ int launch_gate(double2* d, int vq, const GateSpec& g, Workspace& scratch, hipSt
         for (int b = 0; b < g.k; ++b) tb[b] = g.tb[b];
         int64_t gpb = 256 / (int64_t)D;
         int64_t nblk = grid_for((a.ngroups + gpb - 1) / gpb, 1, cap_blocks());
-        size_t shmem = sizeof(double2) * (D * D + 256);
-        static std::atomic<bool> attr_set{ false };
-        if (!attr_set && shmem > 64 * 1024) {
-            BSIM_HIP(hipFuncSetAttribute((const void*)k_dense_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         160 * 1024));
-            attr_set = true;
-        }
+        size_t shmem = sizeof(double2) * (D * D + 256);  // 5 KiB
         hipLaunchKernelGGL(k_dense_lds, dim3((unsigned)nblk), dim3(256), shmem, s, d, a, (const double2*)scratch.ptr,
                            g.k, make_int4(tb[0], tb[1], tb[2], tb[3]), make_int2(tb[4], tb[5]));
     } else {
