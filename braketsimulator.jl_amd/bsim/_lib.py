@@ -17,6 +17,8 @@ PKG_ROOT = Path(__file__).resolve().parent.parent
 LIB_PATH = Path(os.environ.get("BSIM_LIB", PKG_ROOT / "lib" / "libbsim.so"))
 
 BSIM_OK = 0
+BSIM_EINVAL = -1
+BSIM_ENOTSUP = -4
 BSIM_SV = 0
 BSIM_DM = 1
 
@@ -121,7 +123,8 @@ _SIGS = {
     "bsim_graph_create": (C.c_int, [C.c_void_p, C.POINTER(Op), C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     "bsim_graph_launch": (C.c_int, [C.c_void_p]),
     "bsim_graph_destroy": (C.c_int, [C.c_void_p]),
-    "bsim_autotune": (C.c_int, [C.c_void_p, C.POINTER(Op), C.c_int, _ip, _ip, _dp]),
+    "bsim_autotune": (C.c_int, [C.c_void_p, C.POINTER(Op), C.c_int, _ip, _ip, _ip, _ip, _dp]),
+    "bsim_set_max_blocks": (C.c_int, [C.c_int, _ip]),
     "bsim_partial_trace": (C.c_int, [C.c_void_p, C.c_int, _ip, _dp]),
     "bsim_set_sweep_bits": (C.c_int, [C.c_int, _ip]),
 }

@@ -501,6 +501,14 @@ class Control(Gate):
         return g
 
     def matrix_rep(self):  # :431-444
+        if isinstance(self.g, GPhase):
+            # :445-457: a 2^N diagonal (the controls are the phase's own
+            # qubits) with the phase at the control index, then the power
+            d = np.ones(2 ** self.g.n_qubits, dtype=np.complex128)
+            ctrl = sum(b << (b_ix - 1) for b_ix, b in enumerate(self.bitvals, start=1))
+            d[ctrl] = np.exp(1j * self.g.angles[0] * self.g.pow_exponent)
+            m = np.diag(d)
+            return m if self.pow_exponent == 1.0 else _mpow(m, self.pow_exponent)
         inner = self.inner().matrix_rep()
         iq = self.g.n_qubits
         tot = iq + len(self.bitvals)

@@ -232,6 +232,15 @@ def _kron_to_ascending(m: np.ndarray, targets) -> np.ndarray:
     return t.transpose(perm).reshape(2 ** k, 2 ** k)
 
 
+def _reorder_kron(m: np.ndarray, src, dst) -> np.ndarray:
+    """Re-index a matrix in kron order of qubits ``src`` (first = MSB) to
+    kron order of ``dst`` (the same qubits in another order)."""
+    k = len(src)
+    perm = [list(src).index(q) for q in dst]
+    t = m.reshape([2] * (2 * k))
+    return t.transpose(perm + [k + p for p in perm]).reshape(2 ** k, 2 ** k)
+
+
 def _jl_shl(x: int, s: int) -> int:
     """Julia's x << s (a negative s shifts right)."""
     return x << s if s >= 0 else x >> -s
@@ -263,6 +272,7 @@ class _Simulator:
         # no device state until reinit gives the program's qubit count
         if self.qubit_count != 0:
             L.call("bsim_state_create", self.qubit_count, self.kind, C.byref(self._h))
+        self._gen = 0  # bumped whenever the device state is re-created
         self._observed = None
 
     def __del__(self):
@@ -284,6 +294,7 @@ class _Simulator:
             if self._h.value:
                 L.load().bsim_state_destroy(self._h)
             self._h = C.c_void_p()
+            self._gen += 1
             self.qubit_count = int(qubit_count)
             L.call("bsim_state_create", self.qubit_count, self.kind, C.byref(self._h))
         else:
@@ -365,14 +376,17 @@ class _Simulator:
         when ``reset`` -- as one hipGraph on this state (bsim_graph_create)."""
         return CapturedEvolution(self, operations, reset)
 
-    def autotune(self, operations):
+    def autotune(self, operations) -> dict:
         """Time the fused evolution of ``operations`` for every tile
-        configuration and keep the fastest (bsim_autotune).  Returns
-        (tile_bits, low_bits, ms); the state is left unspecified."""
+        configuration, then every fusion width at the best one, and keep the
+        fastest (bsim_autotune).  Returns {tile_bits, low_bits, sweep_bits,
+        fusion_limit, ms}; the state is left unspecified."""
         ol = lower_instructions(list(operations), self.kind, getattr(self, "noise_mode", "reference"))
-        t, lo, ms = C.c_int(), C.c_int(), C.c_double()
-        L.call("bsim_autotune", self._h, ol.ptr, ol.n, C.byref(t), C.byref(lo), C.byref(ms))
-        return t.value, lo.value, ms.value
+        t, lo, rb, fl, ms = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        L.call("bsim_autotune", self._h, ol.ptr, ol.n, C.byref(t), C.byref(lo), C.byref(rb), C.byref(fl),
+               C.byref(ms))
+        return {"tile_bits": t.value, "low_bits": lo.value, "sweep_bits": rb.value, "fusion_limit": fl.value,
+                "ms": ms.value}
 
     def _apply_call(self, call: KernelCall, vq_targets, conj: int):
         lib = L.load()
@@ -441,6 +455,19 @@ class _Simulator:
             if squared:
                 fs = [np.eye(f.shape[0], dtype=np.complex128) if o.pauli is not None or isinstance(o, HObs) else f @ f
                       for f, o in zip(fs, obs.factors)]
+            # The reference applies each factor on its own targets
+            # (src/result_types.jl:203-220); a Hermitian factor's matrix is in
+            # kron order of its targets as given only for 2 qubits on a state
+            # vector (src/sv_simulator.jl:179-198), otherwise of its targets
+            # sorted (:200-230, src/dm_simulator.jl:190-224).  Bring every
+            # factor to the kron order of its given targets first.
+            ix = 0
+            for j, (f, o) in enumerate(zip(fs, obs.factors)):
+                kf = o.n_qubits
+                tf = targets[ix:ix + kf]
+                ix += kf
+                if isinstance(o, HermitianObservable) and (kf >= 3 or (kf == 2 and self.kind != L.BSIM_SV)):
+                    fs[j] = _reorder_kron(f, sorted(tf), tf)
             m = np.ones((1, 1), dtype=np.complex128)
             for f in fs:
                 m = np.kron(m, f)
@@ -645,15 +672,17 @@ class CapturedEvolution:
     def __init__(self, sim: _Simulator, operations, reset: bool = True):
         self._g = C.c_void_p()
         self._sim = sim
-        self._hval = sim.handle.value
+        self._gen = sim._gen
         ol = lower_instructions(list(operations), sim.kind, getattr(sim, "noise_mode", "reference"))
         L.call("bsim_graph_create", sim.handle, ol.ptr, ol.n, int(bool(reset)), C.byref(self._g))
 
     def launch(self) -> _Simulator:
         if not self._g.value:
             raise L.BsimError("captured evolution is closed")
-        if self._sim.handle is None or self._sim.handle.value != self._hval:
+        if self._sim.handle is None or self._sim._gen != self._gen:
             raise L.BsimError("the simulator's state was re-created after the capture")
+        # libbsim checks again: a graph whose state was destroyed or re-created
+        # (even at the same address) is refused
         L.call("bsim_graph_launch", self._g)
         return self._sim
 

@@ -307,13 +307,13 @@ struct TaskPlan {
     std::string msg;
 };
 
-static void plan_task(const bsim_task& t, TaskPlan& out) {
+static void plan_task(const bsim_task& t, const TileConfig& base, TaskPlan& out) {
     std::vector<EOp> eops;
     if ((out.rc = lower_ops(t.qubit_count, BSIM_SV, t.ops, t.n_ops, eops, false)) != BSIM_OK) {
         out.msg = last_error_text();
         return;
     }
-    TileConfig cfg = tile_config();
+    TileConfig cfg = base;
     cfg.reg_bits = 4;  // the batch kernel runs 256-thread, 16-amplitude sweeps
     if ((out.rc = make_plan(t.qubit_count, eops, cfg, out.plan)) != BSIM_OK) {
         out.msg = last_error_text();
@@ -328,7 +328,7 @@ static void plan_task(const bsim_task& t, TaskPlan& out) {
     if (out.plan.passes.empty()) {  // no gates: one empty pass still builds |0>
         Pass P;
         memset(&P, 0, sizeof(P));
-        P.T = std::min(tile_config().tile_bits, t.qubit_count);
+        P.T = std::min(base.tile_bits, t.qubit_count);
         P.R = 4;
         for (int b = 0; b < P.T; ++b) {
             P.lbits[b] = b;
@@ -372,7 +372,8 @@ int bsim_batch_run(const bsim_task* tasks, int n_tasks, double* latency_us) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(BSIM_ENODEV, "no HIP device visible");
     // ---- plans, in parallel on the host
     std::vector<TaskPlan> tp(n_tasks);
-    parallel_for(n_tasks, [&](int i) { plan_task(tasks[i], tp[i]); });
+    const TileConfig cfg = tile_config();  // one snapshot for planning and the LDS size
+    parallel_for(n_tasks, [&](int i) { plan_task(tasks[i], cfg, tp[i]); });
     for (int i = 0; i < n_tasks; ++i)
         if (tp[i].rc != BSIM_OK) return fail(tp[i].rc, tp[i].msg);
     // ---- one table of passes / sweeps / ops for the whole batch
@@ -431,7 +432,7 @@ int bsim_batch_run(const bsim_task* tasks, int n_tasks, double* latency_us) {
     BSIM_HIP(hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, dev_id));
     if (clock_khz <= 0) clock_khz = 100000;
     static std::atomic<bool> attr{ false };
-    const int T = tile_config().tile_bits;
+    const int T = cfg.tile_bits;
     const size_t shmem = std::max(sizeof(double2) * (size_t)((1 << T) + ((1 << T) >> 4)), sizeof(double) * ZR * TB);
     if (!attr) {
         BSIM_HIP(hipFuncSetAttribute((const void*)k_batch, hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "bsim.h"
 
@@ -44,6 +45,8 @@ struct bsim_state {
     // reduction partials, gate scratch copy, alias table, planner buffers
     bsim::Workspace red, res, scratch, ap, alias, alias_work, ops, lam;
     bool alias_ready = false;
+    uint64_t gen = 0;                   // unique per created state (process-wide counter)
+    std::vector<bsim_graph*> graphs;    // captured evolutions on this state
 };
 
 namespace bsim {
@@ -51,6 +54,7 @@ namespace bsim {
 int ensure_ws(Workspace& w, size_t bytes);
 void free_ws(Workspace& w);
 int check_state(const bsim_state* h);
+void detach_graphs(bsim_state* h);  // graph.hip
 int device_cu_count();
 
 // ---- device helpers ----------------------------------------------------

@@ -124,7 +124,10 @@ struct TileConfig {
     // the fusion width so that a pass stays HBM-bound.
     int max_cost = 0;
 };
-TileConfig& tile_config();
+// A snapshot of the library-wide configuration (taken under a lock), and
+// its setter; every entry point plans with one snapshot.
+TileConfig tile_config();
+void set_tile_config(const TileConfig& c);
 
 int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan& plan);
 

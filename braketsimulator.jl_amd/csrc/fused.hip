@@ -13,15 +13,27 @@
 
 #include <algorithm>
 #include <complex>
+#include <mutex>
 #include <vector>
 
 #include "fused.h"
 
 namespace bsim {
 
-TileConfig& tile_config() {
-    static TileConfig cfg;
-    return cfg;
+// The library-wide tile configuration.  Entry points take one snapshot per
+// call (tile_config()), so a call never plans with a half-updated value and
+// setters on one thread do not race with planning on another.
+static std::mutex g_cfg_mu;
+static TileConfig g_cfg;
+
+TileConfig tile_config() {
+    std::lock_guard<std::mutex> lock(g_cfg_mu);
+    return g_cfg;
+}
+
+void set_tile_config(const TileConfig& c) {
+    std::lock_guard<std::mutex> lock(g_cfg_mu);
+    g_cfg = c;
 }
 
 uint64_t EOp::touched() const {
@@ -951,7 +963,8 @@ using namespace bsim;
 extern "C" {
 
 int bsim_set_tile_config(int tile_bits, int low_bits) {
-    TileConfig& c = tile_config();
+    std::lock_guard<std::mutex> lock(g_cfg_mu);
+    TileConfig c = g_cfg;
     if (tile_bits) {
         if (tile_bits < 6 || tile_bits > 13) return fail(BSIM_EINVAL, "tile_bits must be 6..13");
         c.tile_bits = tile_bits;
@@ -961,26 +974,30 @@ int bsim_set_tile_config(int tile_bits, int low_bits) {
         c.low_bits = low_bits;
     }
     if (c.low_bits > c.tile_bits - 2) c.low_bits = c.tile_bits - 2;
+    g_cfg = c;
     return BSIM_OK;
 }
 
 int bsim_get_tile_config(int* tile_bits, int* low_bits) {
-    if (tile_bits) *tile_bits = tile_config().tile_bits;
-    if (low_bits) *low_bits = tile_config().low_bits;
+    const TileConfig c = tile_config();
+    if (tile_bits) *tile_bits = c.tile_bits;
+    if (low_bits) *low_bits = c.low_bits;
     return BSIM_OK;
 }
 
 int bsim_set_sweep_bits(int bits, int* old) {
     if (bits != 0 && bits != 3 && bits != 4) return fail(BSIM_EINVAL, "sweep register bits must be 3 or 4");
-    if (old) *old = tile_config().reg_bits;
-    if (bits) tile_config().reg_bits = bits;
+    std::lock_guard<std::mutex> lock(g_cfg_mu);
+    if (old) *old = g_cfg.reg_bits;
+    if (bits) g_cfg.reg_bits = bits;
     return BSIM_OK;
 }
 
 int bsim_set_fusion_limit(int max_cost, int* old) {
     if (max_cost < 0) return fail(BSIM_EINVAL, "max_cost must be >= 0");
-    if (old) *old = tile_config().max_cost;
-    tile_config().max_cost = max_cost;
+    std::lock_guard<std::mutex> lock(g_cfg_mu);
+    if (old) *old = g_cfg.max_cost;
+    g_cfg.max_cost = max_cost;
     return BSIM_OK;
 }
 

@@ -21,6 +21,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <vector>
 
 #include "gates.h"
@@ -466,7 +467,12 @@ static InsArgs ins_args(int vq, const GateSpec& g, bool include_targets) {
     return a;
 }
 
-static int64_t cap_blocks() { return (int64_t)device_cu_count() * 64; }
+// Grid cap of the per-gate kernels (bsim_set_max_blocks; 0 = 64 per CU).
+static std::atomic<int> g_max_blocks{ 0 };
+static int64_t cap_blocks() {
+    const int m = g_max_blocks.load(std::memory_order_relaxed);
+    return m > 0 ? (int64_t)m : (int64_t)device_cu_count() * 64;
+}
 
 int launch_gate(double2* d, int vq, const GateSpec& g, Workspace& scratch, hipStream_t s) {
     if (g.diag) {
@@ -609,6 +615,13 @@ int launch_double_excitation(double2* d, int vq, const int* e, double c, double 
 using namespace bsim;
 
 extern "C" {
+
+int bsim_set_max_blocks(int blocks, int* old) {
+    if (blocks < 0) return fail(BSIM_EINVAL, "blocks must be >= 0");
+    if (old) *old = g_max_blocks.load();
+    g_max_blocks.store(blocks);
+    return BSIM_OK;
+}
 
 int bsim_apply_1q(bsim_state* h, int q, const double* m, int conj) {
     if (int rc = check_state(h)) return rc;

@@ -11,22 +11,37 @@
 //
 // The autotuner times the fused evolution of one instruction list for every
 // candidate tile width (tile bits), contiguous-low-bit count and sweep width
-// (register bits) on the handle's own state, and keeps the fastest
+// (register bits) on the handle's own state, then the fusion width (the
+// FP64 budget of one pass) at the best of those, and keeps the fastest
 // configuration.
 #include <float.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "fused.h"
 
 struct bsim_graph {
-    bsim_state* h = nullptr;
+    bsim_state* h = nullptr;   // null once the state has been destroyed
+    uint64_t gen = 0;          // generation of h at capture
+    double2* d = nullptr;      // the amplitudes the captured kernels write
     char* tables = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
 
 using namespace bsim;
+
+namespace bsim {
+
+// Called by bsim_state_destroy: the state's graphs lose their handle, so a
+// later launch fails cleanly instead of replaying kernels on freed memory.
+void detach_graphs(bsim_state* h) {
+    for (bsim_graph* g : h->graphs) g->h = nullptr;
+    h->graphs.clear();
+}
+
+}  // namespace bsim
 
 extern "C" {
 
@@ -43,6 +58,8 @@ int bsim_graph_create(bsim_state* h, const bsim_op* ops, int n_ops, int reset, b
     if (int rc = prepare_tile_kernel()) return rc;
     bsim_graph* g = new bsim_graph();
     g->h = h;
+    g->gen = h->gen;
+    g->d = h->d;
     auto cleanup = [&](int rc) {
         if (g->exec) (void)hipGraphExecDestroy(g->exec);
         if (g->graph) (void)hipGraphDestroy(g->graph);
@@ -66,13 +83,16 @@ int bsim_graph_create(bsim_state* h, const bsim_op* ops, int n_ops, int reset, b
     if (e != hipSuccess) return cleanup(hip_fail(e, "hipStreamEndCapture"));
     if ((e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0)) != hipSuccess)
         return cleanup(hip_fail(e, "hipGraphInstantiate"));
+    h->graphs.push_back(g);
     *out = g;
     return BSIM_OK;
 }
 
 int bsim_graph_launch(bsim_graph* g) {
     if (!g || !g->exec) return fail(BSIM_EINVAL, "null graph");
+    if (!g->h) return fail(BSIM_EINVAL, "the state this graph was captured on has been destroyed");
     if (int rc = check_state(g->h)) return rc;
+    if (g->h->gen != g->gen || g->h->d != g->d) return fail(BSIM_EINVAL, "the graph's state has been re-created");
     BSIM_HIP(hipGraphLaunch(g->exec, g->h->stream));
     g->h->alias_ready = false;
     return BSIM_OK;
@@ -80,7 +100,11 @@ int bsim_graph_launch(bsim_graph* g) {
 
 int bsim_graph_destroy(bsim_graph* g) {
     if (!g) return BSIM_OK;
-    if (g->h && g->h->stream) (void)hipStreamSynchronize(g->h->stream);
+    if (g->h) {
+        std::vector<bsim_graph*>& v = g->h->graphs;
+        v.erase(std::remove(v.begin(), v.end(), g), v.end());
+        if (g->h->stream) (void)hipStreamSynchronize(g->h->stream);
+    }
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
     if (g->tables) (void)hipFree(g->tables);
@@ -88,13 +112,33 @@ int bsim_graph_destroy(bsim_graph* g) {
     return BSIM_OK;
 }
 
-int bsim_autotune(bsim_state* h, const bsim_op* ops, int n_ops, int* tile_bits, int* low_bits, double* best_ms) {
+// Time one configuration: the faster of two runs (reset + fused evolve)
+// after a warm-up run.
+static int time_config(bsim_state* h, const std::vector<EOp>& eops, const TileConfig& cfg, hipEvent_t e0,
+                       hipEvent_t e1, double& best) {
+    best = DBL_MAX;
+    Plan plan;
+    if (int rc = make_plan(h->vq, eops, cfg, plan)) return rc;
+    for (int rep = 0; rep < 3; ++rep) {
+        if (int rc = bsim_state_reset(h)) return rc;
+        BSIM_HIP(hipEventRecord(e0, h->stream));
+        if (int rc = run_plan(h, h->d, eops, plan)) return rc;
+        BSIM_HIP(hipEventRecord(e1, h->stream));
+        BSIM_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        BSIM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (rep > 0 && ms < best) best = ms;
+    }
+    return BSIM_OK;
+}
+
+int bsim_autotune(bsim_state* h, const bsim_op* ops, int n_ops, int* tile_bits, int* low_bits, int* sweep_bits,
+                  int* fusion_limit, double* best_ms) {
     if (int rc = check_state(h)) return rc;
     if (n_ops < 0 || (n_ops > 0 && !ops)) return fail(BSIM_EINVAL, "bad autotune arguments");
     std::vector<EOp> eops;
     if (int rc = lower_ops(h->nq, h->kind, ops, n_ops, eops, false)) return rc;
     if (int rc = prepare_tile_kernel()) return rc;
-    const TileConfig keep = tile_config();
     hipEvent_t e0, e1;
     BSIM_HIP(hipEventCreate(&e0));
     BSIM_HIP(hipEventCreate(&e1));
@@ -105,56 +149,55 @@ int bsim_autotune(bsim_state* h, const bsim_op* ops, int n_ops, int* tile_bits, 
             (void)hipEventDestroy(b);
         }
     } evg{ e0, e1 };
-    int bestT = keep.tile_bits, bestL = keep.low_bits, bestR = keep.reg_bits;
+    // Candidates are planned with configurations of their own; the library
+    // configuration changes once, to the winner, at the end.
+    const TileConfig keep = tile_config();
+    TileConfig best_cfg = keep;
     double best = DBL_MAX;
     const int Tmax = h->vq < 13 ? h->vq : 13;
+    // stage 1: tile width, contiguous low bits and sweep width at the
+    // current fusion limit
     for (int RB = 4; RB >= 3; --RB)
-    for (int T = 10; T <= Tmax; ++T)
-        for (int L = 2; L <= 6 && L <= T - 2; ++L) {
-            tile_config().tile_bits = T;
-            tile_config().low_bits = L;
-            tile_config().reg_bits = RB;
-            Plan plan;
-            if (int rc = make_plan(h->vq, eops, tile_config(), plan)) {
-                tile_config() = keep;
-                return rc;
+        for (int T = 10; T <= Tmax; ++T)
+            for (int L = 2; L <= 6 && L <= T - 2; ++L) {
+                TileConfig c = keep;
+                c.tile_bits = T;
+                c.low_bits = L;
+                c.reg_bits = RB;
+                double t = DBL_MAX;
+                if (int rc = time_config(h, eops, c, e0, e1, t)) return rc;
+                if (t < best) {
+                    best = t;
+                    best_cfg = c;
+                }
             }
+    // stage 2: the fusion width (FP64 operations per amplitude one pass may
+    // spend; 0 = unlimited) at the best tile shape
+    if (best < DBL_MAX) {
+        static const int limits[] = { 0, 48, 96, 192, 384 };
+        for (int lim : limits) {
+            if (lim == best_cfg.max_cost) continue;
+            TileConfig c = best_cfg;
+            c.max_cost = lim;
             double t = DBL_MAX;
-            for (int rep = 0; rep < 3; ++rep) {  // the first run warms up
-                if (int rc = bsim_state_reset(h)) {
-                    tile_config() = keep;
-                    return rc;
-                }
-                BSIM_HIP(hipEventRecord(e0, h->stream));
-                if (int rc = run_plan(h, h->d, eops, plan)) {
-                    tile_config() = keep;
-                    return rc;
-                }
-                BSIM_HIP(hipEventRecord(e1, h->stream));
-                BSIM_HIP(hipEventSynchronize(e1));
-                float ms = 0;
-                BSIM_HIP(hipEventElapsedTime(&ms, e0, e1));
-                if (rep > 0 && ms < t) t = ms;
-            }
+            if (int rc = time_config(h, eops, c, e0, e1, t)) return rc;
             if (t < best) {
                 best = t;
-                bestT = T;
-                bestL = L;
-                bestR = RB;
+                best_cfg = c;
             }
         }
+    }
     if (best == DBL_MAX) {  // no candidate fits this state: keep the current configuration
-        tile_config() = keep;
         best = 0;
+        best_cfg = keep;
     } else {
-        tile_config() = keep;
-        tile_config().tile_bits = bestT;
-        tile_config().low_bits = bestL;
-        tile_config().reg_bits = bestR;
+        set_tile_config(best_cfg);
     }
     h->alias_ready = false;
-    if (tile_bits) *tile_bits = tile_config().tile_bits;
-    if (low_bits) *low_bits = tile_config().low_bits;
+    if (tile_bits) *tile_bits = best_cfg.tile_bits;
+    if (low_bits) *low_bits = best_cfg.low_bits;
+    if (sweep_bits) *sweep_bits = best_cfg.reg_bits;
+    if (fusion_limit) *fusion_limit = best_cfg.max_cost;
     if (best_ms) *best_ms = best;
     return BSIM_OK;
 }

@@ -4,6 +4,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <atomic>
 #include <mutex>
 #include <string>
 
@@ -12,6 +13,7 @@
 namespace bsim {
 
 static thread_local std::string g_last_error;
+static std::atomic<uint64_t> g_state_gen{ 0 };
 
 void set_error(const std::string& msg) { g_last_error = msg; }
 const std::string& last_error_text() { return g_last_error; }
@@ -80,7 +82,7 @@ using namespace bsim;
 
 extern "C" {
 
-int bsim_version(void) { return 10000; }
+int bsim_version(void) { return 10100; }
 
 int bsim_last_error(char* buf, int64_t len) {
     const std::string& e = g_last_error;
@@ -117,6 +119,7 @@ int bsim_state_create(int qubit_count, int kind, bsim_state** out) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(BSIM_ENODEV, "no HIP device visible");
     bsim_state* h = new bsim_state();
+    h->gen = ++g_state_gen;
     h->nq = qubit_count;
     h->kind = kind;
     h->vq = vq;
@@ -145,6 +148,7 @@ int bsim_state_create(int qubit_count, int kind, bsim_state** out) {
 int bsim_state_destroy(bsim_state* h) {
     if (!h) return BSIM_OK;
     if (h->stream) hipStreamSynchronize(h->stream);
+    detach_graphs(h);
     if (h->d) hipFree(h->d);
     free_ws(h->red);
     free_ws(h->res);

@@ -45,7 +45,7 @@ typedef struct bsim_state bsim_state;
 
 /* ---- library ------------------------------------------------------ */
 
-/* Version of the ABI (major*10000 + minor*100 + patch). */
+/* Version of the ABI (major*10000 + minor*100 + patch): 10100. */
 int bsim_version(void);
 /* Copy the last error message of this thread into buf (NUL-terminated);
  * returns its full length. */
@@ -308,11 +308,23 @@ int bsim_graph_destroy(bsim_graph* g);
  * src/BraketSimulator.jl:85-86): times the fused evolution of ops on h's
  * state (reset to |0...0> before every run, left in an unspecified state
  * afterwards) for tile_bits 10..min(13, vector qubits), contiguous low bits
- * 2..min(6, tile_bits-2) and 3 or 4 sweep register bits, keeps the fastest
- * as the tile configuration and returns it (outputs may be NULL; read the
- * sweep bits back with bsim_set_sweep_bits(0, &bits); *best_ms = 0 when no
- * candidate fits and the configuration is unchanged). */
-int bsim_autotune(bsim_state* h, const bsim_op* ops, int n_ops, int* tile_bits, int* low_bits, double* best_ms);
+ * 2..min(6, tile_bits-2) and 3 or 4 sweep register bits, then -- at the
+ * best of those -- the fusion width (bsim_set_fusion_limit: 0 = unlimited,
+ * 48, 96, 192, 384 FP64 operations per amplitude and pass), keeps the
+ * fastest as the library configuration and returns it (outputs may be
+ * NULL; *best_ms = 0 when no candidate fits and the configuration is
+ * unchanged).  Candidates are planned with private configurations: other
+ * threads keep planning with the configuration in force until the winner
+ * is published. */
+int bsim_autotune(bsim_state* h, const bsim_op* ops, int n_ops, int* tile_bits, int* low_bits, int* sweep_bits,
+                  int* fusion_limit, double* best_ms);
+
+/* Grid cap of the per-gate kernels (their grid-stride loops): at most
+ * `blocks` workgroups per launch, 0 = the default (64 per compute unit).
+ * *old (may be NULL) receives the previous value.  A tuning and testing
+ * knob with no reference counterpart: small caps make small states take
+ * the multi-block-per-workgroup paths that large states take. */
+int bsim_set_max_blocks(int blocks, int* old);
 
 #ifdef __cplusplus
 }

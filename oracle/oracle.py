@@ -248,6 +248,87 @@ def dm_expect_matrix(rho: np.ndarray, n: int, m: np.ndarray, targets) -> float:
     return float(np.real(np.trace(k.reshape((N, N), order="F"))))
 
 
+_STD = {"X": _PAULI["X"], "Y": _PAULI["Y"], "Z": _PAULI["Z"],
+        "H": np.array([[1, 1], [1, -1]], dtype=np.complex128) / math.sqrt(2.0), "I": np.eye(2, dtype=np.complex128)}
+
+
+def _apply_factor(k: np.ndarray, n: int, factor, targets, dm: bool):
+    """apply_observable! of one factor (a letter of a standard observable, or
+    a Hermitian matrix) on a copy: standard observables are their gates
+    (src/result_types.jl:196-201; on a density matrix the gate acts on the
+    column bits of vec(rho), src/dm_simulator.jl:181-189); Hermitian
+    matrices go to src/sv_simulator.jl:162-230 (1q; 2q t1 = MSB; k > 2
+    sorted) or src/dm_simulator.jl:190-224 (transpose, sorted, every k)."""
+    L = lib()
+    t = list(targets)
+    if isinstance(factor, str):
+        L.ref_apply_1q(_d(k), 2 * n if dm else n, t[0], _d(_cm(_STD[factor])))
+        return
+    mm = _cm(factor)
+    if dm:
+        ta = np.array(t, dtype=np.int32)
+        L.ref_dm_apply_hermitian(_d(k), n, len(ta), ta.ctypes.data_as(_ip), _d(mm))
+    elif len(t) == 1:
+        L.ref_apply_1q(_d(k), n, t[0], _d(mm))
+    elif len(t) == 2:
+        L.ref_apply_hermitian_2q(_d(k), n, t[0], t[1], _d(mm))
+    else:
+        ta = np.array(t, dtype=np.int32)
+        L.ref_apply_kq(_d(k), n, len(t), ta.ctypes.data_as(_ip), _d(mm))
+
+
+def _factor_qubits(f) -> int:
+    return 1 if isinstance(f, str) else int(round(math.log2(np.asarray(f).shape[0])))
+
+
+def expect_factors(state: np.ndarray, n: int, factors, targets, dm: bool) -> float:
+    """expectation(sim, TensorProduct(factors) or the single factor, targets...):
+    the factors applied in turn to a copy of the state, each on its own
+    slice of the targets (src/result_types.jl:203-220), then real(dot(psi,
+    copy)) (src/sv_simulator.jl:240-248) or real(tr) (src/dm_simulator.jl:
+    253-260).  ``state`` is the state vector or the stored 4^n vector."""
+    k = np.ascontiguousarray(state, dtype=np.complex128).copy()
+    ix = 0
+    for f in factors:
+        q = _factor_qubits(f)
+        _apply_factor(k, n, f, targets[ix:ix + q], dm)
+        ix += q
+    if dm:
+        N = 2 ** n
+        return float(np.real(np.trace(k.reshape((N, N), order="F"))))
+    return dot(np.ascontiguousarray(state, dtype=np.complex128), k).real
+
+
+def expect_op_squared(state: np.ndarray, n: int, factors, targets, dm: bool) -> float:
+    """expectation_op_squared (src/result_types.jl:168-194): 1 when every
+    factor is a standard observable, else standard factors become I and
+    Hermitian ones M*M."""
+    if all(isinstance(f, str) for f in factors):
+        return 1.0
+    sq = ["I" if isinstance(f, str) else np.asarray(f) @ np.asarray(f) for f in factors]
+    return expect_factors(state, n, sq, targets, dm)
+
+
+def calculate_expectation(state, n, factors, targets, dm):
+    """calculate(::Expectation) (src/result_types.jl:160-166): one value, or
+    one per target when the target count is not the observable's."""
+    t = list(targets) if targets else list(range(n))
+    nq = sum(_factor_qubits(f) for f in factors)
+    if len(t) == nq:
+        return expect_factors(state, n, factors, t, dm)
+    return [expect_factors(state, n, factors, [q], dm) for q in t]
+
+
+def calculate_variance(state, n, factors, targets, dm):
+    """calculate(::Variance) (src/result_types.jl:223-238)."""
+    t = list(targets) if targets else list(range(n))
+    nq = sum(_factor_qubits(f) for f in factors)
+    if len(t) == nq:
+        return expect_op_squared(state, n, factors, t, dm) - expect_factors(state, n, factors, t, dm) ** 2
+    return [expect_op_squared(state, n, factors, [q], dm) - expect_factors(state, n, factors, [q], dm) ** 2
+            for q in t]
+
+
 def marginal(probs: np.ndarray, n: int, targets) -> np.ndarray:
     t = np.array(list(targets), dtype=np.int32)
     out = np.empty(2 ** len(t))

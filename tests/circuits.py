@@ -30,10 +30,31 @@ def _cmatrix(rows):
     return np.array([[complex(x[0], x[1]) for x in r] for r in rows], dtype=np.complex128)
 
 
+def gate_from_spec(spec):
+    """[name, args] -> Gate; "Control" takes [inner spec, bitvals],
+    "GPhaseN" takes [n, angle]."""
+    name, args = spec
+    if name == "Control":
+        return G.Control(gate_from_spec(args[0]), args[1])
+    if name == "GPhaseN":
+        return G.GPhase(int(args[0]), args[1])
+    if name == "MultiRZ":
+        return G.MultiRZ(*args)
+    return G.GATES[name](*args)
+
+
 def fixture_instruction(op):
     name, args, targets = op
     if name == "GPhase1":
         return Instruction(G.GPhase(1, args[0]), targets)
+    if name == "GPhaseN":
+        return Instruction(G.GPhase(int(args[0]), args[1]), targets)
+    if name == "MultiRZ":
+        return Instruction(G.MultiRZ(*args), targets)
+    if name == "ControlOf":
+        return Instruction(G.Control(gate_from_spec(args[0]), args[1]), targets)
+    if name == "UnitaryOf":  # Unitary(Matrix(matrix_rep(gate)))
+        return Instruction(G.Unitary(gate_from_spec(args).matrix_rep()), targets)
     if name == "Unitary":
         m = _kron_spec(args)[0] if isinstance(args, str) else _cmatrix(args)
         return Instruction(G.Unitary(m), targets)
@@ -51,8 +72,21 @@ def fixture_instruction(op):
     raise KeyError(name)
 
 
-def fixture_ops(case):
-    return [fixture_instruction(o) for o in case["ops"]]
+def fixture_ops(case, ops=None):
+    return [fixture_instruction(o) for o in (case["ops"] if ops is None else ops)]
+
+
+def fixture_observable(factors):
+    """Observable of a closed-form fixture: one factor or a TensorProduct."""
+    from bsim import observables as Ob
+
+    def one(f):
+        if isinstance(f, str):
+            return {"X": Ob.X, "Y": Ob.Y, "Z": Ob.Z, "H": Ob.H, "I": Ob.I}[f]()
+        return Ob.HermitianObservable(_cmatrix(f[1]))
+
+    fs = [one(f) for f in factors]
+    return fs[0] if len(fs) == 1 else Ob.TensorProduct(fs)
 
 
 def grcs16(path):

@@ -90,3 +90,72 @@ def test_dense_k_apply_gate_and_dm(k):
         dm.evolve(pre2 + [Instruction(G.Unitary(u2), t2)], mode=mode)
         ref = O.evolve_dm(m, pre2 + [Instruction(G.Unitary(u2), t2)], "reference")
         np.testing.assert_allclose(dm.density_matrix().ravel(order="F"), ref, atol=TOL)
+
+
+@pytest.fixture
+def grid_cap():
+    """Cap the per-gate kernels' grids (bsim_set_max_blocks) so that small
+    states walk the grid-stride loops -- and the next-block register
+    prefetch of the matrix-core kernels -- that states above about 20
+    qubits take with the default grid; restored afterwards."""
+    import ctypes
+
+    from bsim import _lib as L
+
+    old = ctypes.c_int()
+    L.call("bsim_set_max_blocks", 0, ctypes.byref(old))
+
+    def set_cap(blocks):
+        L.call("bsim_set_max_blocks", blocks, None)
+
+    yield set_cap
+    L.call("bsim_set_max_blocks", old.value, None)
+
+
+@pytest.mark.parametrize("k", [3, 4, 5, 6])
+@pytest.mark.parametrize("blocks", [1, 3, 7])
+@pytest.mark.parametrize("n,controls", [(12, 0), (14, 1), (16, 0), (16, 2)])
+def test_dense_k_grid_stride(k, blocks, n, controls, grid_cap):
+    """Dense k-target gates with at most `blocks` workgroups per launch: each
+    workgroup takes many blocks of 64 groups in turn (odd caps leave a
+    partial last round), low / high / scattered targets via the random
+    target choice, with and without controls, against the oracle."""
+    rng = np.random.default_rng(31 * k + 7 * blocks + n + controls)
+    ins, want = _circuit(n, k, rng, controls)
+    _say(f"dense k={k} n={n} controls={controls} grid cap {blocks}")
+    grid_cap(blocks)
+    sim = StateVectorSimulator(n, 0)
+    sim.evolve(ins, mode=1)
+    got = sim.state_vector()
+    grid_cap(0)
+    np.testing.assert_allclose(got, want, atol=TOL)
+
+
+@pytest.mark.parametrize("tg", [[0, 1, 2, 3, 4, 5], [26, 25, 24, 23, 22, 21], [0, 9, 15, 22, 26, 5]])
+def test_dense_k_large_state_roundtrip(tg):
+    """At 27 qubits (the default grid strides): U then U^dagger for k = 4, 5,
+    6 on low, high and scattered targets, with and without a control, gives
+    back the state within 1e-12, and U alone moves it (the norm is kept)."""
+    n = 27
+    rng = np.random.default_rng(len(tg) + tg[1])
+    sim = StateVectorSimulator(n, 0)
+    prep = [Instruction(G.Ry(float(a)), [q]) for q, a in enumerate(rng.uniform(0.1, 3.0, n))]
+    sim.evolve(prep)
+    psi0 = sim.state_vector()
+    for k in (4, 5, 6):
+        for controls in (0, 1):
+            q = tg[:k]
+            u = random_unitary(k, rng)
+            if controls:
+                c = [x for x in (13, 17, 19) if x not in q][0]
+                fwd = Instruction(G.Control(G.Unitary(u), (1,)), [c] + q)
+                bwd = Instruction(G.Control(G.Unitary(u.conj().T), (1,)), [c] + q)
+            else:
+                fwd, bwd = Instruction(G.Unitary(u), q), Instruction(G.Unitary(u.conj().T), q)
+            _say(f"27q round trip k={k} controls={controls} targets={q}")
+            sim.evolve([fwd], mode=1)
+            mid = sim.state_vector()
+            assert abs(np.vdot(mid, mid).real - 1.0) < 1e-11
+            assert np.max(np.abs(mid - psi0)) > 1e-3
+            sim.evolve([bwd], mode=1)
+            np.testing.assert_allclose(sim.state_vector(), psi0, atol=1e-12)

@@ -34,12 +34,15 @@ def _say(msg):
 
 @pytest.fixture
 def keep_tile_config():
-    t, lo, rb = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    t, lo, rb, fl = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     L.call("bsim_get_tile_config", ctypes.byref(t), ctypes.byref(lo))
     L.call("bsim_set_sweep_bits", 0, ctypes.byref(rb))
+    L.call("bsim_set_fusion_limit", 0, ctypes.byref(fl))
+    L.call("bsim_set_fusion_limit", fl.value, None)
     yield
     L.call("bsim_set_tile_config", t.value, lo.value)
     L.call("bsim_set_sweep_bits", rb.value, None)
+    L.call("bsim_set_fusion_limit", fl.value, None)
 
 
 def _plan_singles(n, kind, ins, mode="physical"):
@@ -146,11 +149,15 @@ def test_autotune_picks_a_valid_config(keep_tile_config):
     ins = W.brickwall(n, 6, seed=3)
     _say("autotune 16q")
     sim = StateVectorSimulator(n, 0)
-    t, lo, ms = sim.autotune(ins)
-    assert 10 <= t <= 13 and 2 <= lo <= min(6, t - 2) and ms > 0
-    cur_t, cur_lo = ctypes.c_int(), ctypes.c_int()
+    r = sim.autotune(ins)
+    t, lo = r["tile_bits"], r["low_bits"]
+    assert 10 <= t <= 13 and 2 <= lo <= min(6, t - 2) and r["ms"] > 0
+    assert r["sweep_bits"] in (3, 4) and r["fusion_limit"] in (0, 48, 96, 192, 384)
+    cur_t, cur_lo, cur_rb, cur_fl = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     L.call("bsim_get_tile_config", ctypes.byref(cur_t), ctypes.byref(cur_lo))
-    assert (cur_t.value, cur_lo.value) == (t, lo)
+    L.call("bsim_set_sweep_bits", 0, ctypes.byref(cur_rb))
+    L.call("bsim_set_fusion_limit", r["fusion_limit"], ctypes.byref(cur_fl))
+    assert (cur_t.value, cur_lo.value, cur_rb.value, cur_fl.value) == (t, lo, r["sweep_bits"], r["fusion_limit"])
     sim.reinit(n, 0)
     got = sim.evolve(ins).state_vector()
     np.testing.assert_allclose(got, O.evolve_sv(n, ins), atol=1e-12)
@@ -160,5 +167,49 @@ def test_autotune_small_state_keeps_config(keep_tile_config):
     n = 8  # below the smallest candidate tile: nothing to try
     t0, lo0 = ctypes.c_int(), ctypes.c_int()
     L.call("bsim_get_tile_config", ctypes.byref(t0), ctypes.byref(lo0))
-    t, lo, ms = StateVectorSimulator(n, 0).autotune(W.brickwall(n, 2, seed=1))
-    assert (t, lo, ms) == (t0.value, lo0.value, 0.0)
+    r = StateVectorSimulator(n, 0).autotune(W.brickwall(n, 2, seed=1))
+    assert (r["tile_bits"], r["low_bits"], r["ms"]) == (t0.value, lo0.value, 0.0)
+
+
+def test_fusion_limits_match_oracle(keep_tile_config):
+    """Every fusion width the autotuner may pick gives the oracle's state."""
+    n = 14
+    ins = W.brickwall(n, 8, seed=4)
+    exp = O.evolve_sv(n, ins)
+    for lim in (48, 96, 192, 384, 0):
+        _say(f"fusion limit {lim}")
+        L.call("bsim_set_fusion_limit", lim, None)
+        sim = StateVectorSimulator(n, 0)
+        np.testing.assert_allclose(sim.evolve(ins).state_vector(), exp, atol=1e-12)
+
+
+def test_graph_after_reinit_is_refused():
+    """A graph outlives a re-created state: launching it fails cleanly (the
+    state was destroyed, even if the new one sits at the same address) and
+    closing it still works; the new state is untouched."""
+    n = 10
+    ins = W.brickwall(n, 3, seed=2)
+    sim = StateVectorSimulator(n, 0)
+    g = sim.capture(ins)
+    g.launch()
+    ref = sim.state_vector()
+    np.testing.assert_allclose(ref, O.evolve_sv(n, ins), atol=1e-12)
+    _say("graph: reinit with two more qubits, then launch")
+    sim.reinit(n + 2, 0)
+    with pytest.raises(BsimError):
+        g.launch()
+    g.close()
+    p = sim.probabilities()
+    assert p[0] == 1.0 and p.sum() == 1.0
+    # the same through the C ABI: destroy the state, keep the graph
+    lib = L.load()
+    h = ctypes.c_void_p()
+    L.call("bsim_state_create", n, L.BSIM_SV, ctypes.byref(h))
+    ol = lower_instructions(ins, L.BSIM_SV)
+    graph = ctypes.c_void_p()
+    L.call("bsim_graph_create", h, ol.ptr, ol.n, 1, ctypes.byref(graph))
+    L.call("bsim_graph_launch", graph)
+    L.call("bsim_state_destroy", h)
+    assert lib.bsim_graph_launch(graph) == L.BSIM_EINVAL
+    assert "destroyed" in L.last_error()
+    assert lib.bsim_graph_destroy(graph) == 0

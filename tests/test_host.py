@@ -34,7 +34,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_version_and_error_string_without_gpu():
     lib = L.load()
-    assert lib.bsim_version() == 10000
+    assert lib.bsim_version() == 10100
     buf = ctypes.create_string_buffer(64)
     lib.bsim_last_error(buf, 64)
 
