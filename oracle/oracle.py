@@ -38,6 +38,7 @@ _SIGS = {
     "ref_apply_ctrl_1c2t": [_dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp],
     "ref_apply_ctrl_2c1t": [_dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp],
     "ref_apply_kq": [_dp, C.c_int, C.c_int, _ip, _dp],
+    "ref_apply_kq_sparse": [_dp, C.c_int, C.c_int, _ip, _dp],
     "ref_apply_double_excitation": [_dp, C.c_int, _ip, C.c_double, C.c_double],
     "ref_apply_hermitian_2q": [_dp, C.c_int, C.c_int, C.c_int, _dp],
     "ref_dm_phaseflip": [_dp, C.c_int, C.c_int, C.c_double],
@@ -145,9 +146,42 @@ def evolve_sv(n: int, instructions, psi: np.ndarray | None = None) -> np.ndarray
     return psi
 
 
-def evolve_dm(n: int, instructions, mode: str = "reference", rho: np.ndarray | None = None) -> np.ndarray:
+def _kraus_kron_qubits(targets):
+    """Qubits of a Kraus matrix's kron order (first = MSB) as the reference's
+    kernels index it: 1 and 2 qubits as given (2q: t1 is the MSB,
+    src/noise_kernels.jl:144-155), k > 2 ascending (sorted endian targets,
+    :175-186)."""
+    t = [int(x) for x in targets]
+    return t if len(t) <= 2 else sorted(t)
+
+
+def physical_channel_superop(ks, targets, n: int):
+    """The physical channel S <- sum_m conj(K_m) S K_m^T on the stored matrix
+    S = rho^T as one operator on the 4^n vector: S'[r, c] = sum
+    conj(K)[r, r'] K[c, c'] S[r', c'], i.e. kron(K, conj(K)) over the
+    channel's column bits (vector qubits t) then row bits (vector qubits
+    n + t).  Returns (matrix in kron order of the ascending vector qubits,
+    those vector qubits): the convention of ref_apply_kq."""
+    kq = _kraus_kron_qubits(targets)
+    k = len(kq)
+    sup = sum(np.kron(np.asarray(K, dtype=np.complex128), np.conj(np.asarray(K, dtype=np.complex128))) for K in ks)
+    src = list(kq) + [n + q for q in kq]
+    dst = sorted(src)
+    perm = [src.index(q) for q in dst]
+    t = sup.reshape([2] * (4 * k))
+    m = t.transpose(perm + [2 * k + p for p in perm]).reshape(4 ** k, 4 ** k)
+    return m, dst
+
+
+def evolve_dm(n: int, instructions, mode: str = "reference", rho: np.ndarray | None = None,
+              physical_as_superop: bool = False) -> np.ndarray:
     """evolve! (src/dm_simulator.jl:170-179) on the oracle.  Returns the
-    column-major stored matrix flattened (length 4^n)."""
+    column-major stored matrix flattened (length 4^n).  ``physical_as_superop``
+    (physical mode only) applies each Kraus channel as its superoperator
+    on its 2k vector qubits
+    instead of block by block, skipping the superoperator's zero entries
+    (ref_apply_kq_sparse): the same map, pinned against the block form on
+    small registers, and fast enough for 14-qubit matrices."""
     from bsim.gates import Gate, kernel_call
     from bsim.noises import Noise, PhaseFlip
 
@@ -168,6 +202,11 @@ def evolve_dm(n: int, instructions, mode: str = "reference", rho: np.ndarray | N
                 L.ref_dm_phaseflip(_d(rho), n, int(t[0]), float(op.probability))
                 continue
             ks = op.kraus_rep()
+            if mode == "physical" and physical_as_superop:
+                m, vq = physical_channel_superop(ks, ins.target, n)
+                ta = np.array(vq, dtype=np.int32)
+                L.ref_apply_kq_sparse(_d(rho), 2 * n, len(vq), ta.ctypes.data_as(_ip), _d(_cm(m)))
+                continue
             flat = np.ascontiguousarray(np.concatenate([_cm(k).ravel(order="F") for k in ks]))
             f = L.ref_dm_kraus_physical if mode == "physical" else L.ref_dm_kraus_ref
             f(_d(rho), n, len(t), t.ctypes.data_as(_ip), len(ks), _d(flat))

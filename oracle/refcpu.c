@@ -234,6 +234,56 @@ void ref_apply_double_excitation(double* sv_, int nq, const int* t, double c, do
     }
 }
 
+/* Not a reference loop: a dense operator on k vector bits with the
+ * convention of ref_apply_kq (bit j of the matrix index is the j-th
+ * smallest endian target), skipping the operator's zero entries.  Used for
+ * the superoperators of physical noise channels on large density matrices,
+ * where most entries are zero; pinned against the block form of
+ * ref_dm_kraus_physical on small registers. */
+void ref_apply_kq_sparse(double* sv_, int nq, int k, const int* targets, const double* m) {
+    cplx* sv = (cplx*)sv_;
+    int ord[16];
+    for (int i = 0; i < k; ++i) ord[i] = nq - 1 - targets[i];
+    sort_ints(ord, k);
+    int N = 1 << k;
+    int64_t n = ((int64_t)1 << nq) >> k;
+    int* nzc = (int*)malloc(sizeof(int) * N * N);   /* per row: nonzero columns */
+    cplx* nzv = (cplx*)malloc(sizeof(cplx) * N * N);
+    int* cnt = (int*)calloc(N, sizeof(int));
+    int64_t* off = (int64_t*)malloc(sizeof(int64_t) * N);
+    for (int row = 0; row < N; ++row)
+        for (int col = 0; col < N; ++col) {
+            const double* e = m + 2 * (row + N * col);
+            if (e[0] != 0.0 || e[1] != 0.0) {
+                nzc[row * N + cnt[row]] = col;
+                nzv[row * N + cnt[row]].re = e[0];
+                nzv[row * N + cnt[row]].im = e[1];
+                ++cnt[row];
+            }
+        }
+    for (int j = 0; j < N; ++j) {
+        int64_t f = 0;
+        for (int b = 0; b < k; ++b) if ((j >> b) & 1) f |= (int64_t)1 << ord[b];
+        off[j] = f;
+    }
+#pragma omp parallel
+    {
+        cplx* a = (cplx*)malloc(sizeof(cplx) * N);
+#pragma omp for schedule(static)
+        for (int64_t ix = 0; ix < n; ++ix) {
+            int64_t p = pad_bits(ix, ord, k);
+            for (int j = 0; j < N; ++j) a[j] = sv[p | off[j]];
+            for (int row = 0; row < N; ++row) {
+                cplx acc = { 0.0, 0.0 };
+                for (int t = 0; t < cnt[row]; ++t) acc = cadd(acc, cmul(nzv[row * N + t], a[nzc[row * N + t]]));
+                sv[p | off[row]] = acc;
+            }
+        }
+        free(a);
+    }
+    free(nzc); free(nzv); free(cnt); free(off);
+}
+
 /* sv_simulator.jl:179-198: 2-qubit Hermitian observable; order
  * (00, flip t2, flip t1, 11), i.e. t1 is the MSB of the 4x4 index. */
 void ref_apply_hermitian_2q(double* sv_, int nq, int t1, int t2, const double* m) {

@@ -158,7 +158,7 @@ def test_double_excitation_kernel(n):
     the oracle's kernel on low, high and scattered targets, with pow."""
     rng = np.random.default_rng(n)
     base = random_circuit(n, 30, rng)
-    for tg in ([0, 1, 2, 3], [n - 1, n - 2, n - 3, n - 4], [n - 2, 0, n // 2, 3], [2, n - 1, 1, n // 2 + 1]):
+    for tg in ([0, 1, 2, 3], [n - 1, n - 2, n - 3, n - 4], [n - 2, 0, n // 2, 1], [2, n - 1, 1, n // 2 + 1]):
         ins = base + [Instruction(G.DoubleExcitation(0.61), tg), Instruction(G.H(), [tg[0]]),
                       Instruction(G.DoubleExcitation(-1.37, pow_exponent=2.5), list(reversed(tg)))]
         exp = O.evolve_sv(n, ins)

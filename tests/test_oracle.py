@@ -390,3 +390,22 @@ def test_partial_trace_sv_equals_dm(n):
     b /= np.linalg.norm(b)
     prod = np.kron(a, b)  # qubit 0 is the factor a
     np.testing.assert_allclose(O.partial_trace(prod, n, [0], dm=False), np.outer(a, np.conj(a)), atol=1e-13)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_physical_superop_form_equals_block_form(seed):
+    """The superoperator form of a physical Kraus channel (used for the
+    14-qubit checks) equals the block-by-block form, channels of 1, 2 and 3
+    qubits on unsorted targets."""
+    rng = np.random.default_rng(300 + seed)
+    n = 4
+    ops = []
+    for _ in range(6):
+        ops += random_circuit(n, 2, rng, kinds=("1q", "2q"))
+        ops.append(random_noise(n, rng))
+    ops.append(Instruction(Nz.Kraus([random_unitary(3, rng) * math.sqrt(0.5), random_unitary(3, rng) * math.sqrt(0.5)]),
+                           [3, 0, 2]))
+    ops.append(Instruction(Nz.TwoQubitDepolarizing(0.1), [2, 0]))
+    a = O.evolve_dm(n, ops, mode="physical")
+    b = O.evolve_dm(n, ops, mode="physical", physical_as_superop=True)
+    np.testing.assert_allclose(b, a, atol=1e-13)
