@@ -69,6 +69,12 @@ class Task(C.Structure):
         ("seed", C.c_uint64),
         ("expect_z", _dp),
         ("samples", _i64p),
+        ("n_words", C.c_int32),
+        ("n_params", C.c_int32),
+        ("paulis", C.c_char_p),
+        ("coeffs", _dp),
+        ("grad", _dp),
+        ("expval", _dp),
     ]
 
 

@@ -13,6 +13,10 @@ W_i = floor(p_i 2^n 2^32) with Philox-4x32-10(counter = shot index, key =
 seed), r = (x1:x0), t = floor(r * sum(W) / 2^64): the first i with
 W_0 + ... + W_i > t.  The distribution is the alias method's (both sample
 the same probability vector), the individual draws are not.
+
+Parameter sweeps ride in the same launch: a ``GradientTask`` runs the
+circuit forward, forms lambda = H psi and sweeps backward with the fused
+adjoint tile body, returning the gradient of bsim_adjoint_gradient.
 """
 from __future__ import annotations
 
@@ -26,16 +30,34 @@ from .simulators import lower_instructions
 
 @dataclass
 class BatchResult:
-    expect_z: np.ndarray      # <Z_q>, qubit q = index q
-    samples: np.ndarray       # basis-state indices (qubit 0 = MSB)
+    expect_z: np.ndarray      # <Z_q>, qubit q = index q (circuits)
+    samples: np.ndarray       # basis-state indices (qubit 0 = MSB) (circuits)
     latency_us: float         # completion time from the launch start
+    grad: np.ndarray | None = None   # gradient tasks: d<H>/dtheta
+    expval: float | None = None      # gradient tasks: <H>
+
+
+@dataclass
+class GradientTask:
+    """One parameter set of a sweep: the adjoint gradient of <H> =
+    sum_w coeffs[w] <P_w> for ``instructions`` from |0...0>, with
+    ``param_map`` as for adjoint.adjoint_gradient ((instruction index,
+    angle index) -> parameter index)."""
+
+    qubit_count: int
+    instructions: list
+    param_map: dict
+    words: list
+    coeffs: np.ndarray
+    n_params: int | None = None
 
 
 def run_batch(circuits, shots=1000, seeds=None):
     """circuits: list of (qubit_count, instructions) -- or (qubit_count, ops)
     with ops already a bsim_op array (``n`` and ``ptr``, as OpList and
-    workloads.PackedOps have).  ``shots`` and ``seeds`` are scalars or
-    per-circuit sequences.  Returns a list of BatchResult."""
+    workloads.PackedOps have) -- and GradientTask items, all run in the same
+    persistent launch.  ``shots`` and ``seeds`` are scalars or per-item
+    sequences (gradient tasks take no shots).  Returns a list of BatchResult."""
     n = len(circuits)
     if n == 0:
         return []
@@ -44,20 +66,53 @@ def run_batch(circuits, shots=1000, seeds=None):
         (0 if seeds is None else int(seeds)) + i for i in range(n)]
     tasks = (L.Task * n)()
     keep, outs = [], []
-    for i, (nq, ins) in enumerate(circuits):
+    for i, item in enumerate(circuits):
+        t = tasks[i]
+        t.seed = int(seeds_l[i]) & 0xFFFFFFFFFFFFFFFF
+        if isinstance(item, GradientTask):
+            nq = item.qubit_count
+            pm = item.param_map
+            npar = (max(pm.values()) + 1 if pm else 0) if item.n_params is None else int(item.n_params)
+            ol = lower_instructions(list(item.instructions), L.BSIM_SV, param_map=pm)
+            for w in item.words:
+                if len(w) != nq:
+                    raise ValueError(f"Pauli word {w!r} must have {nq} letters")
+            buf = "".join(item.words).encode()
+            c = np.ascontiguousarray(item.coeffs, dtype=np.float64)
+            g = np.zeros(max(npar, 1), dtype=np.float64)
+            ev = np.zeros(1, dtype=np.float64)
+            keep.append((ol, buf, c))
+            outs.append(("grad", g[:npar], ev))
+            t.qubit_count = int(nq)
+            t.n_ops = ol.n
+            t.ops = ol.ptr
+            t.shots = 0
+            t.n_words = len(item.words)
+            t.n_params = npar
+            t.paulis = buf
+            t.coeffs = L.dptr(c)
+            t.grad = L.dptr(g)
+            t.expval = L.dptr(ev)
+            continue
+        nq, ins = item
         ol = ins if hasattr(ins, "ptr") else lower_instructions(ins, L.BSIM_SV)
         keep.append(ol)
         z = np.zeros(nq, dtype=np.float64)
         s = np.zeros(max(int(shots_l[i]), 1), dtype=np.int64)
-        outs.append((z, s))
-        t = tasks[i]
+        outs.append(("circuit", z, s[: int(shots_l[i])]))
         t.qubit_count = int(nq)
         t.n_ops = ol.n
         t.ops = ol.ptr
         t.shots = int(shots_l[i])
-        t.seed = int(seeds_l[i]) & 0xFFFFFFFFFFFFFFFF
         t.expect_z = L.dptr(z)
         t.samples = L.i64ptr(s)
     lat = np.zeros(n, dtype=np.float64)
     L.call("bsim_batch_run", tasks, n, L.dptr(lat))
-    return [BatchResult(z, s[: int(shots_l[i])], float(lat[i])) for i, (z, s) in enumerate(outs)]
+    res = []
+    for i, o in enumerate(outs):
+        if o[0] == "grad":
+            res.append(BatchResult(np.zeros(0), np.zeros(0, dtype=np.int64), float(lat[i]), o[1].copy(),
+                                   float(o[2][0])))
+        else:
+            res.append(BatchResult(o[1], o[2], float(lat[i])))
+    return res

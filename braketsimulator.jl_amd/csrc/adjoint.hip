@@ -147,24 +147,6 @@ __global__ __launch_bounds__(AB) void k_adj_diag(double2* __restrict__ psi, doub
     if (P > 0) block_sum_adj<P>(acc, lds, partials + (size_t)blockIdx.x * 3);
 }
 
-// lambda_i = sum_w c_w phase_w(i) psi_{i ^ f_w}
-struct PWord {
-    uint64_t f, y, z;
-    int32_t ny, pad_;
-    double c;
-};
-
-__device__ __forceinline__ double2 pphase(uint64_t i, const PWord& w) {
-    int neg = (__popcll(~i & w.y) + __popcll(i & w.z)) & 1;
-    double s = neg ? -w.c : w.c;
-    switch (w.ny & 3) {
-        case 0: return make_double2(s, 0);
-        case 1: return make_double2(0, s);
-        case 2: return make_double2(-s, 0);
-        default: return make_double2(0, -s);
-    }
-}
-
 __global__ __launch_bounds__(AB) void k_pauli_sum(const double2* __restrict__ psi, double2* __restrict__ lam,
                                                   int64_t len, const PWord* __restrict__ words, int nw) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;

@@ -21,6 +21,20 @@
 // when a wave holds many tiles: they cut the agent-scope release/acquire
 // pairs, whose L2 write-back and invalidate dominate short tickets.
 //
+// Gradient tasks (the parameter sweeps: a circuit with parametrized gates
+// and a Pauli-sum observable H) share the launch.  Their stages after the
+// forward passes are
+//   LAM      one ticket per 4096-amplitude chunk: lambda = H psi on the
+//            chunk and the chunk's part of <psi|H|psi>;
+//   back b   one ticket per run of tiles of backward pass b: the fused
+//            adjoint tile body (psi and lambda in LDS, U^dagger applied to
+//            both, the <lambda|dU|psi> products of every parametrized op
+//            summed per wave);
+//   GFIN     one ticket: the fixed-order sums of the products and of <H>.
+// The backward plans and the per-wave sums are those of the single-state
+// fused adjoint (bsim_adjoint_gradient), so a batched gradient is the
+// serial one up to the summation order of <H>.
+//
 // Shots are drawn by inverse CDF over the exact integer weights
 // W_i = floor(p_i * 2^n * 2^32) (p_i = re^2 + im^2 rounded as written):
 // t = floor(r * sum(W) / 2^64) for r = (x1:x0) of Philox-4x32-10(counter
@@ -35,7 +49,9 @@
 #include <thread>
 #include <vector>
 
+#include "adj_tile_body.h"
 #include "fused.h"
+#include "reduce.h"
 #include "tile_kernel.h"
 
 namespace bsim {
@@ -49,7 +65,7 @@ constexpr int ZP = 1 + CHUNK_BITS;           // per-chunk Z partials: total + on
 constexpr int ZR = ZP + (26 - CHUNK_BITS);   // F2 rows: + one per bit above the chunk
 constexpr int MAX_CHUNK = 8;                 // tiles per ticket
 
-enum : int32_t { K_F1 = -1, K_F2 = -2, K_F3 = -3 };
+enum : int32_t { K_F1 = -1, K_F2 = -2, K_F3 = -3, K_LAM = -4, K_BWD = -5, K_GFIN = -6 };
 
 struct BTask {
     double2* sv;
@@ -63,6 +79,14 @@ struct BTask {
     int32_t n, pass0;
     int32_t cbits;    // log2 amplitudes per chunk = min(12, n)
     int32_t sbits;    // log2 amplitudes per segment = min(6, cbits)
+    // gradient tasks (nwords > 0)
+    double2* lam;
+    const PWord* words;
+    double* partials;  // [nslots * pm][ntiles][AT_W] per-wave products
+    double* hdot;      // [nchunks]: chunk parts of <psi|H|psi>
+    double* gres;      // [nslots * pm + 1]: summed products, then <H>
+    int64_t ntiles;    // largest tile count of a backward pass
+    int32_t nwords, bpass0, rows, pm;
 };
 
 struct BEntry {
@@ -70,7 +94,8 @@ struct BEntry {
     int32_t dep;         // entry whose tickets must all be done first, -1: none
     int32_t chunk;       // tiles per ticket (tile passes)
     int64_t tickets, prefix;
-    int32_t last, pad_;  // last entry of its circuit
+    int32_t last;        // last entry of its circuit
+    int32_t sub;         // K_BWD: backward pass index
 };
 
 struct BCtl {
@@ -222,6 +247,41 @@ __device__ __noinline__ void f3_shots(const BTask& K, int64_t s0) {
     K.sout[s] = idx;
 }
 
+// LAM: lambda = H psi on chunk c, and the chunk's Re <psi|lambda> into
+// hdot[c] (a fixed tree over the workgroup).  `red`: LDS of TB doubles.
+__device__ __noinline__ void lam_chunk(const BTask& K, int64_t c, double* red) {
+    const int L = 1 << K.cbits;
+    const uint64_t base = (uint64_t)c << K.cbits;
+    double acc = 0.0;
+    for (int j = threadIdx.x; j < L; j += TB) {
+        const uint64_t i = base + (uint64_t)j;
+        double2 a = make_double2(0.0, 0.0);
+        for (int w = 0; w < K.nwords; ++w) a = cfma(pphase(i, K.words[w]), K.sv[i ^ K.words[w].f], a);
+        K.lam[i] = a;
+        const double2 p = K.sv[i];
+        acc += p.x * a.x + p.y * a.y;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = TB / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) K.hdot[c] = red[0];
+}
+
+// GFIN: every product row summed in the fixed order of the single-state
+// finish, then <H> over the chunks.  `sh`: LDS of TB doubles.
+__device__ __noinline__ void grad_finish(const BTask& K, double* sh) {
+    const int64_t per = K.ntiles * AT_W;
+    for (int r = 0; r < K.rows; ++r) {
+        const double v = adj_row_sum(K.partials + (size_t)r * per, per, sh);
+        if (threadIdx.x == 0) K.gres[r] = v;
+    }
+    const double e = adj_row_sum(K.hdot, (int64_t)1 << (K.n - K.cbits), sh);
+    if (threadIdx.x == 0) K.gres[K.rows] = e;
+}
+
 __global__ __launch_bounds__(TB, 2) void k_batch(const BTask* __restrict__ tasks, const BEntry* __restrict__ entries,
                                                  int nentries, int64_t total, const Pass* __restrict__ passes,
                                                  const Sweep* __restrict__ sweeps, const TOp* __restrict__ tops,
@@ -280,12 +340,23 @@ __global__ __launch_bounds__(TB, 2) void k_batch(const BTask* __restrict__ tasks
                               scat_lo, scat_hi);
                     __syncthreads();  // the next tile rewrites the scatter tables
                 }
+            } else if (E.kind == K_BWD) {
+                const uint64_t first = (uint64_t)local * (uint64_t)E.chunk;
+                for (int c = 0; c < E.chunk; ++c) {
+                    adj_tile_body(K.sv, K.lam, K.n, passes + K.bpass0 + E.sub, sweeps, tops, K.partials, K.pm,
+                                  K.ntiles, first + c, tile, scat_lo, scat_hi);
+                    __syncthreads();  // the next tile rewrites the scatter tables
+                }
             } else if (E.kind == K_F1) {
                 f1_chunk(K, local, (double*)tile, scat_lo);
             } else if (E.kind == K_F2) {
                 f2_reduce(K, (double*)tile, scat_lo);
-            } else {
+            } else if (E.kind == K_F3) {
                 f3_shots(K, local * TB);
+            } else if (E.kind == K_LAM) {
+                lam_chunk(K, local, (double*)tile);
+            } else {
+                grad_finish(K, (double*)tile);
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -302,14 +373,20 @@ __global__ __launch_bounds__(TB, 2) void k_batch(const BTask* __restrict__ tasks
 // Host side -------------------------------------------------------------------
 
 struct TaskPlan {
-    Plan plan;
+    Plan plan;        // forward passes
+    bool grad = false;
+    Plan bplan;       // gradient tasks: backward passes
+    int nslots = 0, pm = 0;
+    std::vector<std::vector<int>> slot_params;  // parameter index of each (slot, p), slots in forward order
+    std::vector<PWord> words;
     int rc = BSIM_OK;
     std::string msg;
 };
 
 static void plan_task(const bsim_task& t, const TileConfig& base, TaskPlan& out) {
+    out.grad = t.n_words > 0;
     std::vector<EOp> eops;
-    if ((out.rc = lower_ops(t.qubit_count, BSIM_SV, t.ops, t.n_ops, eops, false)) != BSIM_OK) {
+    if ((out.rc = lower_ops(t.qubit_count, BSIM_SV, t.ops, t.n_ops, eops, out.grad)) != BSIM_OK) {
         out.msg = last_error_text();
         return;
     }
@@ -335,6 +412,47 @@ static void plan_task(const bsim_task& t, const TileConfig& base, TaskPlan& out)
             P.lmask |= 1ull << b;
         }
         out.plan.passes.push_back(P);
+    }
+    if (!out.grad) return;
+    for (const EOp& e : eops) {
+        if (e.nparams == 0) continue;
+        std::vector<int> ps;
+        for (int p = 0; p < e.nparams; ++p) {
+            if (e.pidx[p] < 0 || e.pidx[p] >= t.n_params) {
+                out.rc = BSIM_EINVAL;
+                out.msg = "batch gradient: param_index out of range";
+                return;
+            }
+            ps.push_back(e.pidx[p]);
+        }
+        out.slot_params.push_back(ps);
+    }
+    std::vector<EOp> back;
+    if ((out.rc = adjoint_back_plan(t.qubit_count, eops, back, out.bplan, out.nslots, out.pm, &base)) != BSIM_OK) {
+        out.msg = last_error_text();
+        return;
+    }
+    for (auto& st : out.bplan.steps)
+        if (!st.tile) {
+            out.rc = BSIM_ENOTSUP;
+            out.msg = "batch gradients use fusable gates only (dense <= 2 targets, diagonal <= 4)";
+            return;
+        }
+    for (const Pass& P : out.bplan.passes)
+        if (P.R != 3 || P.T > 11) {
+            out.rc = BSIM_ENOTSUP;
+            out.msg = "batch gradient: backward pass shape";
+            return;
+        }
+    out.pm = std::max(out.pm, 1);
+    out.words.resize(t.n_words);
+    for (int w = 0; w < t.n_words; ++w) {
+        PauliMask pmk;
+        if ((out.rc = parse_pauli(t.qubit_count, t.paulis + (size_t)w * t.qubit_count, pmk)) != BSIM_OK) {
+            out.msg = last_error_text();
+            return;
+        }
+        out.words[w] = { pmk.x | pmk.y, pmk.y, pmk.z, pmk.ny, 0, t.coeffs[w] };
     }
 }
 
@@ -364,9 +482,15 @@ int bsim_batch_run(const bsim_task* tasks, int n_tasks, double* latency_us) {
     if (n_tasks < 1 || !tasks) return fail(BSIM_EINVAL, "no tasks");
     for (int i = 0; i < n_tasks; ++i) {
         const bsim_task& t = tasks[i];
-        if (t.qubit_count < 4 || t.qubit_count > 26) return fail(BSIM_ENOTSUP, "batch circuits take 4..26 qubits");
-        if (t.n_ops < 0 || (t.n_ops > 0 && !t.ops) || t.shots < 0 || !t.expect_z || (t.shots > 0 && !t.samples))
-            return fail(BSIM_EINVAL, "bad batch task");
+        if (t.n_ops < 0 || (t.n_ops > 0 && !t.ops) || t.shots < 0) return fail(BSIM_EINVAL, "bad batch task");
+        if (t.n_words > 0) {
+            if (t.qubit_count < 5 || t.qubit_count > 26) return fail(BSIM_ENOTSUP, "batch gradients take 5..26 qubits");
+            if (t.shots != 0 || !t.paulis || !t.coeffs || t.n_params < 0 || (t.n_params > 0 && !t.grad))
+                return fail(BSIM_EINVAL, "bad batch gradient task");
+        } else {
+            if (t.qubit_count < 4 || t.qubit_count > 26) return fail(BSIM_ENOTSUP, "batch circuits take 4..26 qubits");
+            if (t.n_words < 0 || !t.expect_z || (t.shots > 0 && !t.samples)) return fail(BSIM_EINVAL, "bad batch task");
+        }
     }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(BSIM_ENODEV, "no HIP device visible");
@@ -376,43 +500,56 @@ int bsim_batch_run(const bsim_task* tasks, int n_tasks, double* latency_us) {
     parallel_for(n_tasks, [&](int i) { plan_task(tasks[i], cfg, tp[i]); });
     for (int i = 0; i < n_tasks; ++i)
         if (tp[i].rc != BSIM_OK) return fail(tp[i].rc, tp[i].msg);
-    // ---- one table of passes / sweeps / ops for the whole batch
+    // ---- one table of passes / sweeps / ops for the whole batch: per task
+    // its forward plan, then (gradient tasks) its backward plan
     std::vector<BTask> bt(n_tasks);
     std::vector<size_t> pass0(n_tasks + 1, 0), sw0(n_tasks + 1, 0), op0(n_tasks + 1, 0);
     for (int i = 0; i < n_tasks; ++i) {
-        pass0[i + 1] = pass0[i] + tp[i].plan.passes.size();
-        sw0[i + 1] = sw0[i] + tp[i].plan.sweeps.size();
-        op0[i + 1] = op0[i] + tp[i].plan.tops.size();
+        const Plan& f = tp[i].plan;
+        const Plan& b = tp[i].bplan;
+        pass0[i + 1] = pass0[i] + f.passes.size() + b.passes.size();
+        sw0[i + 1] = sw0[i] + f.sweeps.size() + b.sweeps.size();
+        op0[i + 1] = op0[i] + f.tops.size() + b.tops.size();
     }
-    std::vector<Pass> passes(pass0[n_tasks]);
+    std::vector<Pass> passes(std::max<size_t>(pass0[n_tasks], 1));
     std::vector<Sweep> sweeps(std::max<size_t>(sw0[n_tasks], 1));
     // every entry is written below: left uninitialized (hundreds of MB for large batches)
     const size_t n_tops = std::max<size_t>(op0[n_tasks], 1);
     std::unique_ptr<TOp[]> tops_buf(new TOp[n_tops]);
     TOp* tops = tops_buf.get();
     parallel_for(n_tasks, [&](int i) {
-        const Plan& p = tp[i].plan;
-        for (size_t k = 0; k < p.passes.size(); ++k) {
-            Pass P = p.passes[k];
-            P.sweep_begin += (int32_t)sw0[i];
-            passes[pass0[i] + k] = P;
+        size_t pbase = pass0[i], sbase = sw0[i], obase = op0[i];
+        for (const Plan* p : { &tp[i].plan, &tp[i].bplan }) {
+            for (size_t k = 0; k < p->passes.size(); ++k) {
+                Pass P = p->passes[k];
+                P.sweep_begin += (int32_t)sbase;
+                passes[pbase + k] = P;
+            }
+            for (size_t k = 0; k < p->sweeps.size(); ++k) {
+                Sweep sw = p->sweeps[k];
+                sw.op_begin += (int32_t)obase;
+                sw.op_end += (int32_t)obase;
+                sweeps[sbase + k] = sw;
+            }
+            if (!p->tops.empty()) memcpy(&tops[obase], p->tops.data(), sizeof(TOp) * p->tops.size());
+            pbase += p->passes.size();
+            sbase += p->sweeps.size();
+            obase += p->tops.size();
         }
-        for (size_t k = 0; k < p.sweeps.size(); ++k) {
-            Sweep s = p.sweeps[k];
-            s.op_begin += (int32_t)op0[i];
-            s.op_end += (int32_t)op0[i];
-            sweeps[sw0[i] + k] = s;
-        }
-        if (!p.tops.empty()) memcpy(&tops[op0[i]], p.tops.data(), sizeof(TOp) * p.tops.size());
     });
-    // ---- device arena layout per circuit: state, Z partials, chunk totals, segment prefix
+    // ---- device arena layout per task.  Circuits: state, Z partials, chunk
+    // totals, segment prefix.  Gradients: psi, lambda, chunk <H> parts,
+    // results, Pauli words (their per-wave products live in one region of
+    // their own, cleared before the launch).
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    std::vector<size_t> off_sv(n_tasks);
-    size_t arena = 0, zlen = 0, slen = 0;
+    std::vector<size_t> off_sv(n_tasks), off_part(n_tasks, 0);
+    size_t arena = 0, zlen = 0, slen = 0, plen = 0;
+    size_t bwd_lds = 0;
     for (int i = 0; i < n_tasks; ++i) {
         const int n = tasks[i].qubit_count;
         const int cbits = std::min(CHUNK_BITS, n), sbits = std::min(SEG_BITS, cbits);
         const size_t nch = (size_t)1 << (n - cbits), nseg = (size_t)1 << (cbits - sbits);
+        const size_t amps = al(((size_t)1 << n) * sizeof(double2));
         bt[i].n = n;
         bt[i].pass0 = (int32_t)pass0[i];
         bt[i].cbits = cbits;
@@ -420,10 +557,27 @@ int bsim_batch_run(const bsim_task* tasks, int n_tasks, double* latency_us) {
         bt[i].shots = tasks[i].shots;
         bt[i].seed = tasks[i].seed;
         off_sv[i] = arena;
-        arena += al(((size_t)1 << n) * sizeof(double2)) + al(nch * ZP * sizeof(double)) +
-                 al(nch * sizeof(uint64_t)) + al(nch * nseg * sizeof(uint64_t));
-        zlen += (size_t)n;
-        slen += (size_t)tasks[i].shots;
+        if (tp[i].grad) {
+            const TaskPlan& g = tp[i];
+            int64_t nt = 1;
+            for (const Pass& P : g.bplan.passes) {
+                nt = std::max<int64_t>(nt, (int64_t)1 << (n - P.T));
+                bwd_lds = std::max(bwd_lds, 2 * sizeof(double2) * (size_t)((1 << P.T) + ((1 << P.T) >> 4)));
+            }
+            bt[i].nwords = (int32_t)g.words.size();
+            bt[i].bpass0 = (int32_t)(pass0[i] + g.plan.passes.size());
+            bt[i].rows = g.nslots * g.pm;
+            bt[i].pm = g.pm;
+            bt[i].ntiles = nt;
+            arena += 2 * amps + al(nch * sizeof(double)) + al(((size_t)bt[i].rows + 1) * sizeof(double)) +
+                     al(g.words.size() * sizeof(PWord));
+            off_part[i] = plen;
+            plen += (size_t)bt[i].rows * (size_t)nt * AT_W;
+        } else {
+            arena += amps + al(nch * ZP * sizeof(double)) + al(nch * sizeof(uint64_t)) + al(nch * nseg * sizeof(uint64_t));
+            zlen += (size_t)n;
+            slen += (size_t)tasks[i].shots;
+        }
     }
     // ---- launch shape (needed for the ticket runs)
     if (int rc = prepare_tile_kernel()) return rc;
@@ -433,7 +587,8 @@ int bsim_batch_run(const bsim_task* tasks, int n_tasks, double* latency_us) {
     if (clock_khz <= 0) clock_khz = 100000;
     static std::atomic<bool> attr{ false };
     const int T = cfg.tile_bits;
-    const size_t shmem = std::max(sizeof(double2) * (size_t)((1 << T) + ((1 << T) >> 4)), sizeof(double) * ZR * TB);
+    const size_t shmem = std::max({ sizeof(double2) * (size_t)((1 << T) + ((1 << T) >> 4)), sizeof(double) * ZR * TB,
+                                    bwd_lds });
     if (!attr) {
         BSIM_HIP(hipFuncSetAttribute((const void*)k_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)(sizeof(double2) * ((1 << 13) + (1 << 9)))));
@@ -443,23 +598,39 @@ int bsim_batch_run(const bsim_task* tasks, int n_tasks, double* latency_us) {
     BSIM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_batch, TB, shmem));
     if (per_cu < 1) per_cu = 1;
     const int grid = device_cu_count() * per_cu;
-    // ---- tickets: stage-wave by stage-wave, circuits with most passes first
-    std::vector<int> order(n_tasks);
-    for (int i = 0; i < n_tasks; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int a, int b) { return tp[a].plan.passes.size() > tp[b].plan.passes.size(); });
+    // ---- tickets: stage-wave by stage-wave, tasks with most stages first.
+    // Circuit stages: forward passes, F1, F2, F3 (with shots).  Gradient
+    // stages: forward passes, LAM, backward passes, GFIN.
     std::vector<int> stages(n_tasks), last_entry(n_tasks, -1);
     size_t maxs = 0;
     for (int i = 0; i < n_tasks; ++i) {
-        stages[i] = (int)tp[i].plan.passes.size() + 2 + (tasks[i].shots > 0 ? 1 : 0);
+        const int np = (int)tp[i].plan.passes.size();
+        stages[i] = tp[i].grad ? np + 2 + (int)tp[i].bplan.passes.size() : np + 2 + (tasks[i].shots > 0 ? 1 : 0);
         maxs = std::max(maxs, (size_t)stages[i]);
     }
+    std::vector<int> order(n_tasks);
+    for (int i = 0; i < n_tasks; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return stages[a] > stages[b]; });
+    // the tile pass of stage w of task i (nullptr: not a tile stage) and, for
+    // backward passes, its index
+    auto stage_pass = [&](int i, int w, int& bwd) -> const Pass* {
+        const int np = (int)tp[i].plan.passes.size();
+        bwd = -1;
+        if (w < np) return &tp[i].plan.passes[w];
+        if (tp[i].grad && w > np && w <= np + (int)tp[i].bplan.passes.size()) {
+            bwd = w - np - 1;
+            return &tp[i].bplan.passes[bwd];
+        }
+        return nullptr;
+    };
     std::vector<BEntry> entries;
     int64_t total = 0;
     for (size_t w = 0; w < maxs; ++w) {
         int64_t wave_tiles = 0;
-        for (int i : order)
-            if (w < tp[i].plan.passes.size()) wave_tiles += (int64_t)1 << (bt[i].n - tp[i].plan.passes[w].T);
+        for (int i : order) {
+            int bw;
+            if (const Pass* P = stage_pass(i, (int)w, bw)) wave_tiles += (int64_t)1 << (bt[i].n - P->T);
+        }
         int64_t run = 1;  // tiles per ticket: keep >= 4 tickets per resident workgroup in the wave
         while (run < MAX_CHUNK && wave_tiles / (run * 2) >= 4 * (int64_t)grid) run *= 2;
         for (int i : order) {
@@ -471,11 +642,16 @@ int bsim_batch_run(const bsim_task* tasks, int n_tasks, double* latency_us) {
             e.dep = last_entry[i];
             e.chunk = 1;
             const int64_t nch = (int64_t)1 << (bt[i].n - bt[i].cbits);
-            if ((int)w < np) {
-                const int64_t ntiles = (int64_t)1 << (bt[i].n - tp[i].plan.passes[w].T);
-                e.kind = (int32_t)w;
+            int bw;
+            if (const Pass* P = stage_pass(i, (int)w, bw)) {
+                const int64_t ntiles = (int64_t)1 << (bt[i].n - P->T);
+                e.kind = bw < 0 ? (int32_t)w : K_BWD;
+                e.sub = bw < 0 ? 0 : bw;
                 e.chunk = (int32_t)std::min(run, ntiles);
                 e.tickets = ntiles / e.chunk;
+            } else if (tp[i].grad) {
+                e.kind = (int)w == np ? K_LAM : K_GFIN;
+                e.tickets = (int)w == np ? nch : 1;
             } else if ((int)w == np) {
                 e.kind = K_F1;
                 e.tickets = nch;
@@ -503,8 +679,9 @@ int bsim_batch_run(const bsim_task* tasks, int n_tasks, double* latency_us) {
     size_t off_done = off_ctl + al(sizeof(BCtl));
     size_t off_fin = off_done + al(sizeof(unsigned long long) * entries.size());
     size_t off_z = off_fin + al(sizeof(unsigned long long) * n_tasks);
-    size_t off_s = off_z + al(sizeof(double) * zlen);
-    size_t off_arena = off_s + al(sizeof(int64_t) * std::max<size_t>(slen, 1));
+    size_t off_s = off_z + al(sizeof(double) * std::max<size_t>(zlen, 1));
+    size_t off_p = off_s + al(sizeof(int64_t) * std::max<size_t>(slen, 1));
+    size_t off_arena = off_p + al(sizeof(double) * std::max<size_t>(plen, 1));
     size_t bytes = off_arena + arena;
     char* dev = nullptr;
     hipError_t e = hipMalloc((void**)&dev, bytes);
@@ -515,12 +692,29 @@ int bsim_batch_run(const bsim_task* tasks, int n_tasks, double* latency_us) {
             if (p) (void)hipFree(p);
         }
     } guard{ dev };
+    std::vector<std::vector<PWord>*> word_src(n_tasks, nullptr);
+    std::vector<size_t> off_words(n_tasks, 0), off_gres(n_tasks, 0);
     size_t zo = 0, so = 0;
     for (int i = 0; i < n_tasks; ++i) {
         const size_t nch = (size_t)1 << (bt[i].n - bt[i].cbits);
+        const size_t amps = al(((size_t)1 << bt[i].n) * sizeof(double2));
         char* p = dev + off_arena + off_sv[i];
         bt[i].sv = (double2*)p;
-        p += al(((size_t)1 << bt[i].n) * sizeof(double2));
+        p += amps;
+        if (tp[i].grad) {
+            bt[i].lam = (double2*)p;
+            p += amps;
+            bt[i].hdot = (double*)p;
+            p += al(nch * sizeof(double));
+            bt[i].gres = (double*)p;
+            off_gres[i] = (size_t)(p - dev);
+            p += al(((size_t)bt[i].rows + 1) * sizeof(double));
+            bt[i].words = (const PWord*)p;
+            off_words[i] = (size_t)(p - dev);
+            word_src[i] = &tp[i].words;
+            bt[i].partials = (double*)(dev + off_p) + off_part[i];
+            continue;
+        }
         bt[i].zpart = (double*)p;
         p += al(nch * ZP * sizeof(double));
         bt[i].ctot = (uint64_t*)p;
@@ -542,6 +736,11 @@ int bsim_batch_run(const bsim_task* tasks, int n_tasks, double* latency_us) {
     BSIM_HIP(hipMemcpyAsync(dev + off_passes, passes.data(), sizeof(Pass) * passes.size(), hipMemcpyHostToDevice, s));
     BSIM_HIP(hipMemcpyAsync(dev + off_sweeps, sweeps.data(), sizeof(Sweep) * sweeps.size(), hipMemcpyHostToDevice, s));
     BSIM_HIP(hipMemcpyAsync(dev + off_tops, tops, sizeof(TOp) * n_tops, hipMemcpyHostToDevice, s));
+    for (int i = 0; i < n_tasks; ++i)
+        if (word_src[i] && !word_src[i]->empty())
+            BSIM_HIP(hipMemcpyAsync(dev + off_words[i], word_src[i]->data(), sizeof(PWord) * word_src[i]->size(),
+                                    hipMemcpyHostToDevice, s));
+    if (plen) BSIM_HIP(hipMemsetAsync(dev + off_p, 0, sizeof(double) * plen, s));
     std::vector<unsigned long long> zeros(std::max(entries.size(), (size_t)n_tasks) + 8, 0ull);
     BSIM_HIP(hipMemcpyAsync(dev + off_ctl, zeros.data(), sizeof(BCtl), hipMemcpyHostToDevice, s));
     BSIM_HIP(hipMemcpyAsync(dev + off_done, zeros.data(), sizeof(unsigned long long) * entries.size(),
@@ -558,19 +757,32 @@ int bsim_batch_run(const bsim_task* tasks, int n_tasks, double* latency_us) {
     BCtl ctl;
     BSIM_HIP(hipMemcpy(&ctl, dev + off_ctl, sizeof(ctl), hipMemcpyDeviceToHost));
     if (ctl.err) return fail(BSIM_EHIP, "batch: a dependency wait exceeded its time limit");
-    std::vector<double> z(zlen);
-    BSIM_HIP(hipMemcpy(z.data(), dev + off_z, sizeof(double) * zlen, hipMemcpyDeviceToHost));
+    std::vector<double> z(std::max<size_t>(zlen, 1));
+    if (zlen) BSIM_HIP(hipMemcpy(z.data(), dev + off_z, sizeof(double) * zlen, hipMemcpyDeviceToHost));
     std::vector<int64_t> shots(std::max<size_t>(slen, 1));
     if (slen) BSIM_HIP(hipMemcpy(shots.data(), dev + off_s, sizeof(int64_t) * slen, hipMemcpyDeviceToHost));
     std::vector<unsigned long long> fin(n_tasks);
     BSIM_HIP(hipMemcpy(fin.data(), dev + off_fin, sizeof(unsigned long long) * n_tasks, hipMemcpyDeviceToHost));
     zo = 0;
     so = 0;
+    std::vector<double> gres;
     for (int i = 0; i < n_tasks; ++i) {
-        memcpy(tasks[i].expect_z, z.data() + zo, sizeof(double) * bt[i].n);
-        if (tasks[i].shots) memcpy(tasks[i].samples, shots.data() + so, sizeof(int64_t) * tasks[i].shots);
-        zo += (size_t)bt[i].n;
-        so += (size_t)tasks[i].shots;
+        const bsim_task& t = tasks[i];
+        if (tp[i].grad) {
+            // grad[p] = 2 Re <lambda|dU|psi> summed over the ops carrying p
+            gres.resize((size_t)bt[i].rows + 1);
+            BSIM_HIP(hipMemcpy(gres.data(), dev + off_gres[i], sizeof(double) * gres.size(), hipMemcpyDeviceToHost));
+            for (int p = 0; p < t.n_params; ++p) t.grad[p] = 0.0;
+            for (size_t slot = 0; slot < tp[i].slot_params.size(); ++slot)
+                for (size_t p = 0; p < tp[i].slot_params[slot].size(); ++p)
+                    t.grad[tp[i].slot_params[slot][p]] += 2.0 * gres[slot * (size_t)bt[i].pm + p];
+            if (t.expval) *t.expval = gres[(size_t)bt[i].rows];
+        } else {
+            memcpy(t.expect_z, z.data() + zo, sizeof(double) * bt[i].n);
+            if (t.shots) memcpy(t.samples, shots.data() + so, sizeof(int64_t) * t.shots);
+            zo += (size_t)bt[i].n;
+            so += (size_t)t.shots;
+        }
         if (latency_us)
             latency_us[i] = fin[i] > ctl.t0 ? (double)(fin[i] - ctl.t0) * 1000.0 / (double)clock_khz : 0.0;
     }

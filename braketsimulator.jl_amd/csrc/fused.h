@@ -155,8 +155,10 @@ int launch_kraus_ref(bsim_state* h, double2* d, const EOp& e);
 // the updated vectors), parametrized ops numbered (aslot) in forward order,
 // and its plan (11 tile bits, 3 register bits).  nslots: parametrized ops;
 // pm: their largest parameter count.
+// `base` (optional) is the configuration snapshot whose other fields the
+// plan keeps; by default the library configuration.
 int adjoint_back_plan(int vq, const std::vector<EOp>& eops, std::vector<EOp>& back, Plan& plan, int& nslots,
-                      int& pm);
+                      int& pm, const TileConfig* base = nullptr);
 
 // Bytes of the per-tile partial sums adjoint_tile_run needs for `plan`.
 size_t adjoint_tile_partial_bytes(const Plan& plan, int vq, int nslots, int pm);

@@ -11,6 +11,27 @@ struct PauliMask {
     int ny = 0;
 };
 
+// One term of a Pauli sum H = sum_w c_w P_w on the device: P_w |i> =
+// phase_w(i) |i ^ f>, f = x | y bits, with (-1) for each y bit of i that is
+// 0 and each z bit that is 1, times i^ny (lambda_i = sum_w c_w phase_w(i)
+// psi_{i ^ f_w}).
+struct PWord {
+    uint64_t f, y, z;
+    int32_t ny, pad_;
+    double c;
+};
+
+__device__ __forceinline__ double2 pphase(uint64_t i, const PWord& w) {
+    int neg = (__popcll(~i & w.y) + __popcll(i & w.z)) & 1;
+    double s = neg ? -w.c : w.c;
+    switch (w.ny & 3) {
+        case 0: return make_double2(s, 0);
+        case 1: return make_double2(0, s);
+        case 2: return make_double2(-s, 0);
+        default: return make_double2(0, -s);
+    }
+}
+
 // Parse one Pauli word of nq letters; qubit q acts on bit nq-1-q.
 int parse_pauli(int nq, const char* word, PauliMask& pm);
 // probabilities into a device buffer of 2^nq doubles

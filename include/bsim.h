@@ -280,11 +280,26 @@ typedef struct bsim_task {
     uint64_t seed;
     double* expect_z;      /* out: qubit_count doubles */
     int64_t* samples;      /* out: shots indices (may be NULL if shots == 0) */
+    /* Gradient task (n_words > 0; parameter sweeps, no reference
+     * counterpart): the circuit's parametrized ops carry
+     * param_index / dmatrix as for bsim_adjoint_gradient, H = sum_w
+     * coeffs[w] * P_w (paulis as in bsim_expect_pauli), shots must be 0 and
+     * expect_z / samples are not used.  Outputs: grad (n_params doubles)
+     * and *expval = <H> (may be NULL). */
+    int32_t n_words;
+    int32_t n_params;
+    const char* paulis;
+    const double* coeffs;
+    double* grad;
+    double* expval;
 } bsim_task;
 
-/* Run many state-vector circuits in one persistent launch with a device
- * work queue.  latency_us (optional, n_tasks doubles) receives each task's
- * completion time from the launch start. */
+/* Run many state-vector circuits -- and adjoint gradients of parametrized
+ * circuits -- in one persistent launch with a device work queue.
+ * latency_us (optional, n_tasks doubles) receives each task's completion
+ * time from the launch start.  A gradient task gives the gradient of
+ * bsim_adjoint_gradient on the same circuit (same forward plan, backward
+ * plan and per-wave product sums), <H> up to summation order. */
 int bsim_batch_run(const bsim_task* tasks, int n_tasks, double* latency_us);
 
 /* ---- captured evolutions and tuning ------------------------------ */

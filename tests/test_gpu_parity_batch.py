@@ -121,3 +121,93 @@ def test_simulate_batch_persistent():
         assert r.measurements.shape == (100, 5)
         rows = {tuple(x) for x in r.measurements}
         assert rows <= {(0, 0, 0, 0, 1), (1, 1, 0, 0, 1)} and len(rows) == 2
+
+
+def test_batch_gradient_sweep_matches_serial():
+    """Parameter sets of the hardware-efficient ansatz (5..16 qubits, random
+    angles) as gradient tasks in the same launch as ordinary circuits: every
+    batched gradient equals bsim_adjoint_gradient on the same circuit within
+    1e-10 (the backward plans and per-wave sums are the same), <H> too, and
+    the circuits' results are unchanged by the company."""
+    from bsim import adjoint as A
+    from bsim.batch import GradientTask
+
+    items, serial = [], []
+    for k, n in enumerate([5, 6, 8, 11, 12, 13, 16, 16]):
+        ins, pmap, th = W.hea(n, 3 if n < 16 else 8, seed=40 + k)
+        words, coeffs = W.tfim(n, hx=0.3 + 0.1 * k)
+        items.append(GradientTask(n, ins, pmap, words, coeffs))
+        sim = StateVectorSimulator(n, 0)
+        serial.append(A.adjoint_gradient(sim, ins, pmap, words, coeffs))
+    spec = W.batch_tasks(40, seed=3, qmin=8, qmax=16)
+    circuits = [(n, W.brickwall(n, d, seed=s)) for n, d, s in spec]
+    mixed = []
+    for i in range(max(len(items), len(circuits))):
+        if i < len(circuits):
+            mixed.append(circuits[i])
+        if i < len(items):
+            mixed.append(items[i])
+    seeds = list(range(len(mixed)))
+    _say(f"batch: {len(items)} gradient tasks among {len(circuits)} circuits")
+    res = run_batch(mixed, shots=[0 if isinstance(m, GradientTask) else 500 for m in mixed], seeds=seeds)
+    gi = ci = 0
+    for m, r, sd in zip(mixed, res, seeds):
+        if isinstance(m, GradientTask):
+            ev, g = serial[gi]
+            assert abs(r.expval - ev) < TOL, gi
+            np.testing.assert_allclose(r.grad, g, atol=TOL, err_msg=f"gradient task {gi}")
+            assert r.latency_us > 0
+            gi += 1
+        else:
+            n, ins = m
+            psi = _gpu_state(n, ins)
+            assert np.max(np.abs(r.expect_z - _z_of(psi, n))) < 1e-12
+            np.testing.assert_array_equal(r.samples, O.cdf_shots(psi, n, 500, sd))
+            ci += 1
+    assert gi == len(items) and ci == len(circuits)
+
+
+def test_batch_gradient_vs_oracle_parameter_shift():
+    """A 7-qubit parameter set with controlled rotations and diagonal
+    gates: the batched gradient against the oracle's parameter shift."""
+    from bsim import adjoint as A
+    from bsim.batch import GradientTask
+
+    n = 7
+    rng = np.random.default_rng(8)
+    ins, pmap, th = [], {}, rng.uniform(-3, 3, 12)
+    for p in range(12):
+        q = p % n
+        g = [G.Rx, G.Ry, G.Rz][p % 3](float(th[p]))
+        pmap[(len(ins), 0)] = p
+        ins.append(Instruction(g, [q]))
+        ins.append(Instruction(G.CNot(), [q, (q + 3) % n]))
+        if p % 4 == 0:
+            ins.append(Instruction(G.CZ(), [(q + 1) % n, (q + 5) % n]))
+    words, coeffs = W.tfim(n)
+    _say("batch gradient vs oracle parameter shift")
+    r = run_batch([GradientTask(n, ins, pmap, words, coeffs)], shots=0)[0]
+
+    def energy(t):
+        psi = O.evolve_sv(n, A.with_params(ins, pmap, t))
+        return sum(c * O.sv_expect_pauli(psi, n, w) for w, c in zip(words, coeffs))
+
+    assert abs(r.expval - energy(th)) < TOL
+    for p in range(12):
+        tp, tm = th.copy(), th.copy()
+        tp[p] += np.pi / 2
+        tm[p] -= np.pi / 2
+        assert abs(r.grad[p] - 0.5 * (energy(tp) - energy(tm))) < TOL, p
+
+
+def test_batch_gradient_rejects():
+    from bsim.batch import GradientTask
+
+    ins, pmap, _ = W.hea(4, 1, seed=0)
+    words, coeffs = W.tfim(4)
+    with pytest.raises(bsim.BsimError, match="5..26"):
+        run_batch([GradientTask(4, ins, pmap, words, coeffs)], shots=0)
+    ins, pmap, _ = W.hea(6, 1, seed=0)
+    words, coeffs = W.tfim(6)
+    with pytest.raises(bsim.BsimError, match="gradient"):
+        run_batch([GradientTask(6, ins, pmap, words, coeffs)], shots=5)

@@ -13,9 +13,11 @@ seconds, against the GPU's plain per-op path plus an oracle sample):
       against parameter shift through the plain path on a parameter sample,
       within 1e-10; the 12-qubit noisy density-matrix variant against the
       oracle's parameter shift on a sample;
-  C5  4096 uneven brick-wall circuits in one persistent launch against one
-      launch per circuit (<Z_q> within 1e-10), and a sample of circuits
-      against the oracle (inverse-CDF shots bit-exact).
+  C5  4096 uneven brick-wall circuits and 256 parameter sets of the
+      16-qubit ansatz with their adjoint gradients in one persistent launch,
+      against one launch per circuit (<Z_q> within 1e-10) and the serial
+      adjoint gradient (within 1e-10), and a sample of circuits against the
+      oracle (inverse-CDF shots bit-exact).
 
 Each step prints what it runs, so a long step is never silent for long.
 """
@@ -199,12 +201,27 @@ def test_c4_noisy_dm_adjoint_12q():
 
 @pytest.mark.timeout(600)
 def test_c5_batch_4096():
-    count, shots = 4096, 1000
+    """The 4096 circuits and 256 parameter sets of the 16-qubit ansatz (each
+    with its adjoint gradient) in one persistent launch."""
+    from bsim.batch import GradientTask
+
+    count, shots, nsets = 4096, 1000, 256
     spec = W.batch_tasks(count, seed=0)
     circuits = [(n, W.brickwall_ops(n, d, s)) for n, d, s in spec]
     seeds = [s for _, _, s in spec]
-    _say(f"C5: {count} circuits in one persistent launch")
-    res = run_batch(circuits, shots, seeds)
+    words, coeffs = W.tfim(16)
+    sets = [W.hea(16, 8, seed=k) for k in range(nsets)]
+    grads = [GradientTask(16, ins, pmap, words, coeffs) for ins, pmap, _ in sets]
+    _say(f"C5: {count} circuits and {nsets} gradient tasks in one persistent launch")
+    allres = run_batch(circuits + grads, [shots] * count + [0] * nsets, seeds + [0] * nsets)
+    res, gres = allres[:count], allres[count:]
+    _say("C5: serial adjoint gradients")
+    gsim = StateVectorSimulator(16, 0)
+    for k, (ins, pmap, _) in enumerate(sets):
+        ev, g = A.adjoint_gradient(gsim, ins, pmap, words, coeffs)
+        assert abs(gres[k].expval - ev) < TOL, k
+        np.testing.assert_allclose(gres[k].grad, g, atol=TOL, err_msg=f"parameter set {k}")
+    del gsim
     _say("C5: one launch per circuit")
     sims = {}
     zmax = 0.0
