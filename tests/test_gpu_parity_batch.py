@@ -209,5 +209,8 @@ def test_batch_gradient_rejects():
         run_batch([GradientTask(4, ins, pmap, words, coeffs)], shots=0)
     ins, pmap, _ = W.hea(6, 1, seed=0)
     words, coeffs = W.tfim(6)
-    with pytest.raises(bsim.BsimError, match="gradient"):
-        run_batch([GradientTask(6, ins, pmap, words, coeffs)], shots=5)
+    u3 = np.linalg.qr(np.random.default_rng(0).normal(size=(8, 8)) + 0j)[0]
+    with pytest.raises(bsim.BsimError, match="fusable"):
+        run_batch([GradientTask(6, ins + [Instruction(G.Unitary(u3), [0, 1, 2])], pmap, words, coeffs)], shots=0)
+    with pytest.raises(bsim.BsimError, match="param_index"):
+        run_batch([GradientTask(6, ins, pmap, words, coeffs, n_params=3)], shots=0)
