@@ -136,6 +136,16 @@ typedef struct bsim_op {
  * launch per op (the reference's loop shape).  Results are the same. */
 int bsim_evolve(bsim_state* h, const bsim_op* ops, int n_ops, int mode);
 
+/* evolve! (fused mode) without device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc).  The plan's tables live in a
+ * fixed 4 MiB device array per GPU instead of the handle's workspace; plans
+ * whose tables exceed it, or that hold ops outside the tile passes (dense
+ * gates on 3+ targets, Kraus channels run on their own), return
+ * BSIM_ENOTSUP.  The array is shared by every call on a GPU: calls that
+ * overlap in time on one GPU race.  Returns after the evolution finished. */
+int bsim_evolve_noalloc(bsim_state* h, const bsim_op* ops, int n_ops);
+
 /* Planner statistics for the fused path, computed on the host only:
  * stats[0] = HBM passes, stats[1] = per-op launches outside tiles,
  * stats[2] = LDS sweeps, stats[3] = ops fused. */
