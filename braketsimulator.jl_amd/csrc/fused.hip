@@ -803,6 +803,18 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
         plan.steps.push_back({ true, (int)plan.passes.size(), -1 });
         plan.passes.push_back(P);
     }
+    // Sign ops carry a copy of their parameters in the fields every op's
+    // head load reads (nd: the pair distances, a byte each; cmask: jmask and,
+    // from bit 32, the pair count; m[0..3]: the pair masks, then the
+    // register-bit masks), so the kernel reads a sign op in one scalar round
+    // trip.  Kernels that read the named fields keep working.
+    for (TOp& op : plan.tops)
+        if (op.kind == 3) {
+            op.nd = (op.pd[0] & 0xFF) | (op.pd[1] & 0xFF) << 8 | (op.pd[2] & 0xFF) << 16 | (op.pd[3] & 0xFF) << 24;
+            op.cmask = (uint64_t)op.jmask | ((uint64_t)op.npairs << 32);
+            const uint64_t w[8] = { op.pm[0], op.pm[1], op.pm[2], op.pm[3], op.sgm[0], op.sgm[1], op.sgm[2], op.sgm[3] };
+            memcpy(op.m, w, sizeof(w));
+        }
     return BSIM_OK;
 }
 

@@ -105,6 +105,34 @@ __device__ __forceinline__ OpHead load_head(const TOp* __restrict__ op) {
     return h;
 }
 
+// One 2x2 complex matrix on the pair (a, b) in place: x = m00 a + m01 b,
+// y = m10 a + m11 b with P = m01 b and Q = m10 a formed first, so that each
+// output's last instruction is the last read of the input it replaces.
+__device__ __forceinline__ void pair_inplace(double& ar, double& ai, double& br, double& bi, c2 m00, c2 m10, c2 m01,
+                                             c2 m11) {
+    double pr, pi, qr, qi, t, t2;
+    asm("v_mul_f64 %[pr], %[br], %[m01r]\n\t"
+        "v_mul_f64 %[qr], %[ar], %[m10r]\n\t"
+        "v_mul_f64 %[pi], %[bi], %[m01r]\n\t"
+        "v_mul_f64 %[qi], %[ai], %[m10r]\n\t"
+        "v_fma_f64 %[pr], -%[bi], %[m01i], %[pr]\n\t"
+        "v_fma_f64 %[qr], -%[ai], %[m10i], %[qr]\n\t"
+        "v_fma_f64 %[pi], %[br], %[m01i], %[pi]\n\t"
+        "v_fma_f64 %[qi], %[ar], %[m10i], %[qi]\n\t"
+        "v_fma_f64 %[t], -%[ai], %[m00i], %[pr]\n\t"
+        "v_fma_f64 %[t2], -%[bi], %[m11i], %[qr]\n\t"
+        "v_fma_f64 %[ai], %[ai], %[m00r], %[pi]\n\t"
+        "v_fma_f64 %[bi], %[bi], %[m11r], %[qi]\n\t"
+        "v_fma_f64 %[ai], %[ar], %[m00i], %[ai]\n\t"
+        "v_fma_f64 %[bi], %[br], %[m11i], %[bi]\n\t"
+        "v_fma_f64 %[ar], %[ar], %[m00r], %[t]\n\t"
+        "v_fma_f64 %[br], %[br], %[m11r], %[t2]"
+        : [ar] "+v"(ar), [ai] "+v"(ai), [br] "+v"(br), [bi] "+v"(bi), [pr] "=&v"(pr), [pi] "=&v"(pi),
+          [qr] "=&v"(qr), [qi] "=&v"(qi), [t] "=&v"(t), [t2] "=&v"(t2)
+        : [m00r] "s"(m00.x), [m00i] "s"(m00.y), [m10r] "s"(m10.x), [m10i] "s"(m10.y), [m01r] "s"(m01.x),
+          [m01i] "s"(m01.y), [m11r] "s"(m11.x), [m11i] "s"(m11.y));
+}
+
 // Dense ops come in two instantiations: without controls (the common case)
 // and with the split control test; the choice is made once per op.
 template <int R, int A, bool CTL>
@@ -115,6 +143,21 @@ __device__ __forceinline__ void sw_dense1_t(SubCube<R>& s, const TOp* __restrict
     if constexpr (CTL) {
         tok = (s.gbase & op->tmask) == op->tval;  // this thread's part of the control test
         jm = op->jmask;                           // the registers' part
+    }
+    if constexpr (!CTL) {
+        sfor<0, (1 << R)>([&](auto J) {
+            constexpr int j = decltype(J)::value;
+            if constexpr ((j & (1 << A)) == 0) {
+                constexpr int k = j | (1 << A);
+                double ar = s.v[2 * j], ai = s.v[2 * j + 1], br = s.v[2 * k], bi = s.v[2 * k + 1];
+                pair_inplace(ar, ai, br, bi, m00, m10, m01, m11);
+                s.v[2 * j] = ar;
+                s.v[2 * j + 1] = ai;
+                s.v[2 * k] = br;
+                s.v[2 * k + 1] = bi;
+            }
+        });
+        return;
     }
     sfor<0, (1 << R)>([&](auto J) {
         constexpr int j = decltype(J)::value;
@@ -140,8 +183,71 @@ __device__ __forceinline__ void sw_dense1(SubCube<R>& s, const TOp* __restrict__
     else sw_dense1_t<R, A, false>(s, op, hd);
 }
 
+// acc = sum over c != i of M[i][c] a_c (12 operands at most: asm operand
+// lists stay short); M column-major m[r + 4c].
+__device__ __forceinline__ void offdiag_row(double& tr, double& ti, double a0r, double a0i, double a1r, double a1i,
+                                            double a2r, double a2i, c2 m0, c2 m1, c2 m2) {
+    asm("v_mul_f64 %[tr], %[a0r], %[m0r]\n\t"
+        "v_mul_f64 %[ti], %[a0i], %[m0r]\n\t"
+        "v_fma_f64 %[tr], -%[a0i], %[m0i], %[tr]\n\t"
+        "v_fma_f64 %[ti], %[a0r], %[m0i], %[ti]\n\t"
+        "v_fma_f64 %[tr], %[a1r], %[m1r], %[tr]\n\t"
+        "v_fma_f64 %[ti], %[a1i], %[m1r], %[ti]\n\t"
+        "v_fma_f64 %[tr], -%[a1i], %[m1i], %[tr]\n\t"
+        "v_fma_f64 %[ti], %[a1r], %[m1i], %[ti]\n\t"
+        "v_fma_f64 %[tr], %[a2r], %[m2r], %[tr]\n\t"
+        "v_fma_f64 %[ti], %[a2i], %[m2r], %[ti]\n\t"
+        "v_fma_f64 %[tr], -%[a2i], %[m2i], %[tr]\n\t"
+        "v_fma_f64 %[ti], %[a2r], %[m2i], %[ti]"
+        : [tr] "=&v"(tr), [ti] "=&v"(ti)
+        : [a0r] "v"(a0r), [a0i] "v"(a0i), [a1r] "v"(a1r), [a1i] "v"(a1i), [a2r] "v"(a2r), [a2i] "v"(a2i),
+          [m0r] "s"(m0.x), [m0i] "s"(m0.y), [m1r] "s"(m1.x), [m1i] "s"(m1.y), [m2r] "s"(m2.x), [m2i] "s"(m2.y));
+}
+
+// a <- m a + t in place (one scratch register pair).
+__device__ __forceinline__ void diag_inplace(double& ar, double& ai, double tr, double ti, c2 m) {
+    double u;
+    asm("v_fma_f64 %[u], -%[ai], %[mi], %[tr]\n\t"
+        "v_fma_f64 %[ai], %[ai], %[mr], %[ti]\n\t"
+        "v_fma_f64 %[ai], %[ar], %[mi], %[ai]\n\t"
+        "v_fma_f64 %[ar], %[ar], %[mr], %[u]"
+        : [ar] "+v"(ar), [ai] "+v"(ai), [u] "=&v"(u)
+        : [tr] "v"(tr), [ti] "v"(ti), [mr] "s"(m.x), [mi] "s"(m.y));
+}
+
 template <int R, int A, int B, bool CTL>
 __device__ __forceinline__ void sw_dense2_t(SubCube<R>& s, const TOp* __restrict__ op, const OpHead& hd) {
+    if constexpr (!CTL) {
+        // off-diagonal parts of every output first (all inputs intact), then
+        // each output in place of its input: no copies of the group
+        const c2 M[16] = { hd.m0, hd.m1, hd.m2, hd.m3, to_c2(op->m[4]), to_c2(op->m[5]), to_c2(op->m[6]),
+                           to_c2(op->m[7]), to_c2(op->m[8]), to_c2(op->m[9]), to_c2(op->m[10]), to_c2(op->m[11]),
+                           to_c2(op->m[12]), to_c2(op->m[13]), to_c2(op->m[14]), to_c2(op->m[15]) };
+        sfor<0, (1 << R)>([&](auto J) {
+            constexpr int j = decltype(J)::value;
+            if constexpr ((j & ((1 << A) | (1 << B))) == 0) {
+                constexpr int ix[4] = { j, j | (1 << A), j | (1 << B), j | (1 << A) | (1 << B) };
+                double ar[4], ai[4], tr[4], ti[4];
+                for (int q = 0; q < 4; ++q) {
+                    ar[q] = s.v[2 * ix[q]];
+                    ai[q] = s.v[2 * ix[q] + 1];
+                }
+                offdiag_row(tr[0], ti[0], ar[1], ai[1], ar[2], ai[2], ar[3], ai[3], M[4], M[8], M[12]);
+                offdiag_row(tr[1], ti[1], ar[0], ai[0], ar[2], ai[2], ar[3], ai[3], M[1], M[9], M[13]);
+                offdiag_row(tr[2], ti[2], ar[0], ai[0], ar[1], ai[1], ar[3], ai[3], M[2], M[6], M[14]);
+                offdiag_row(tr[3], ti[3], ar[0], ai[0], ar[1], ai[1], ar[2], ai[2], M[3], M[7], M[11]);
+                diag_inplace(ar[0], ai[0], tr[0], ti[0], M[0]);
+                diag_inplace(ar[1], ai[1], tr[1], ti[1], M[5]);
+                diag_inplace(ar[2], ai[2], tr[2], ti[2], M[10]);
+                diag_inplace(ar[3], ai[3], tr[3], ti[3], M[15]);
+                for (int q = 0; q < 4; ++q) {
+                    s.v[2 * ix[q]] = ar[q];
+                    s.v[2 * ix[q] + 1] = ai[q];
+                }
+            }
+        });
+        return;
+    }
     bool tok = true;
     uint32_t jm = 0xFFFFu;
     if constexpr (CTL) {
@@ -272,18 +378,33 @@ __device__ __forceinline__ uint32_t xor_and(uint32_t a, uint32_t b, uint32_t c) 
 // parity(gbase & sgm[bit]); the planner stores P(r_j) for every register j
 // in jmask.
 template <int R>
-__device__ __forceinline__ void sw_sign(SubCube<R>& s, const TOp* __restrict__ op) {
+__device__ __forceinline__ void sw_sign(SubCube<R>& s, const OpHead& hd) {
+    // the parameters come with the op head (the planner's copies): pair
+    // distances in nd, jmask and the pair count in cmask, pair masks and
+    // register-bit masks in m0..m3; unused pair masks are zero
+    uint32_t m = (uint32_t)hd.cmask;
+    const int np = (int)(hd.cmask >> 32);
+    const int pd = hd.nd;
+    const uint64_t pm0 = __builtin_bit_cast(uint64_t, hd.m0.x), pm1 = __builtin_bit_cast(uint64_t, hd.m0.y);
+    const uint64_t pm2 = __builtin_bit_cast(uint64_t, hd.m1.x), pm3 = __builtin_bit_cast(uint64_t, hd.m1.y);
+    const uint64_t sg0 = __builtin_bit_cast(uint64_t, hd.m2.x), sg1 = __builtin_bit_cast(uint64_t, hd.m2.y);
+    const uint64_t sg2 = __builtin_bit_cast(uint64_t, hd.m3.x), sg3 = __builtin_bit_cast(uint64_t, hd.m3.y);
     const uint64_t a = s.gbase;
-    uint32_t m = op->jmask;
-    uint32_t par = 0;
-    for (int k = 0; k < op->npairs; ++k) par += __popcll(a & (a >> op->pd[k]) & op->pm[k]);
+    uint32_t par = __popcll(a & (a >> (pd & 0xFF)) & pm0);
+    if (np > 1) {
+        par += __popcll(a & (a >> ((pd >> 8) & 0xFF)) & pm1);
+        if (np > 2) {
+            par += __popcll(a & (a >> ((pd >> 16) & 0xFF)) & pm2);
+            if (np > 3) par += __popcll(a & (a >> ((pd >> 24) & 0xFF)) & pm3);
+        }
+    }
     // all-ones where the parity bit is set (sign-extended bit 0), masked into m
     auto flip = [&](uint32_t p, uint32_t pattern) { m = xor_and(m, (uint32_t)__builtin_amdgcn_sbfe(p, 0, 1), pattern); };
     flip(par, (1u << (1 << R)) - 1);
-    flip(__popcll(a & op->sgm[0]), reg_pattern<R>(0));
-    flip(__popcll(a & op->sgm[1]), reg_pattern<R>(1));
-    flip(__popcll(a & op->sgm[2]), reg_pattern<R>(2));
-    if constexpr (R == 4) flip(__popcll(a & op->sgm[3]), reg_pattern<R>(3));
+    flip(__popcll(a & sg0), reg_pattern<R>(0));
+    flip(__popcll(a & sg1), reg_pattern<R>(1));
+    flip(__popcll(a & sg2), reg_pattern<R>(2));
+    if constexpr (R == 4) flip(__popcll(a & sg3), reg_pattern<R>(3));
     sfor<0, (1 << R)>([&](auto J) {
         constexpr int j = decltype(J)::value;
         const uint32_t t = m << (31 - j);  // bit j of m at the sign position (garbage below, masked off)
@@ -384,7 +505,7 @@ __device__ __forceinline__ int run_op(SubCube<R>& s, const TOp* __restrict__ op,
     }
     if constexpr (R == 4) {
         switch (hd.kind * 16 + hd.a * 4 + hd.b) {
-            case 48: sw_sign<R>(s, op); break;
+            case 48: sw_sign<R>(s, hd); break;
             case 0: sw_dense1<R, 0>(s, op, hd); break;
             case 4: sw_dense1<R, 1>(s, op, hd); break;
             case 8: sw_dense1<R, 2>(s, op, hd); break;
@@ -402,7 +523,7 @@ __device__ __forceinline__ int run_op(SubCube<R>& s, const TOp* __restrict__ op,
         }
     } else {
         switch (hd.kind * 16 + hd.a * 4 + hd.b) {
-            case 48: sw_sign<R>(s, op); break;
+            case 48: sw_sign<R>(s, hd); break;
             case 0: sw_dense1<R, 0>(s, op, hd); break;
             case 4: sw_dense1<R, 1>(s, op, hd); break;
             case 8: sw_dense1<R, 2>(s, op, hd); break;

@@ -1,3 +1,4 @@
+#include <string.h>
 // Test helper (CPU only): plans an instruction list with libbsim's own host
 // planner (lower_ops + make_plan) and interprets the plan -- every pass,
 // sweep and tile op with its documented meaning on global indices -- on a
@@ -95,6 +96,13 @@ static int check_sweep(const Plan& p, const Sweep& S, int R, const int* vb, uint
             int j = 0;
             for (int r = 0; r < R; ++r) j |= (int)((i >> vb[r]) & 1) << r;
             if (op.kind == 3) {
+                // the head copies the kernel reads (nd, cmask, m[0..3])
+                uint64_t w[8];
+                memcpy(w, op.m, sizeof(w));
+                for (int k = 0; k < 4; ++k)
+                    if (((op.nd >> (8 * k)) & 0xFF) != (op.pd[k] & 0xFF) || w[k] != op.pm[k] || w[4 + k] != op.sgm[k])
+                        return -16;
+                if ((uint32_t)op.cmask != op.jmask || (int)(op.cmask >> 32) != op.npairs) return -16;
                 int direct = 0, split = (int)((op.jmask >> j) & 1);
                 for (int k = 0; k < op.npairs; ++k) {
                     direct ^= __builtin_popcountll(i & (i >> op.pd[k]) & op.pm[k]) & 1;
