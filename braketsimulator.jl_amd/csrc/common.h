@@ -45,6 +45,9 @@ struct bsim_state {
     // reduction partials, gate scratch copy, alias table, planner buffers
     bsim::Workspace red, res, scratch, ap, alias, alias_work, ops, lam;
     bool alias_ready = false;
+    // ticket counter of the streaming tile pass: zero between passes (each
+    // pass's last ticket resets it), so it is set once, at creation
+    unsigned long long* tile_ctr = nullptr;
     uint64_t gen = 0;                   // unique per created state (process-wide counter)
     std::vector<bsim_graph*> graphs;    // captured evolutions on this state
 };

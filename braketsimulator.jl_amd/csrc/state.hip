@@ -136,6 +136,12 @@ int bsim_state_create(int qubit_count, int kind, bsim_state** out) {
         delete h;
         return hip_fail(e, "hipStreamCreate");
     }
+    e = hipMalloc((void**)&h->tile_ctr, 256);
+    if (e == hipSuccess) e = hipMemsetAsync(h->tile_ctr, 0, 256, h->stream);
+    if (e != hipSuccess) {
+        bsim_state_destroy(h);
+        return hip_fail(e, "hipMalloc(tile counter)");
+    }
     int rc = bsim_state_reset(h);
     if (rc != BSIM_OK) {
         bsim_state_destroy(h);
@@ -150,6 +156,7 @@ int bsim_state_destroy(bsim_state* h) {
     if (h->stream) hipStreamSynchronize(h->stream);
     detach_graphs(h);
     if (h->d) hipFree(h->d);
+    if (h->tile_ctr) hipFree(h->tile_ctr);
     free_ws(h->red);
     free_ws(h->res);
     free_ws(h->scratch);

@@ -662,4 +662,161 @@ __device__ __forceinline__ void tile_body(double2* __restrict__ sv, int vq, cons
     }
 }
 
+// ---------------------------------------------------------------------------
+// The streaming pass (T = 12, R = 4, no reference Kraus ops): a persistent
+// workgroup per slot (two per CU) takes tiles from a ticket counter and
+// loads the next tile's amplitudes into registers while the current tile
+// runs its last sweep, so one tile's HBM latency hides behind the previous
+// tile's arithmetic.  Tiles are handed out in ticket order: a slot that
+// runs behind (HBM channel contention, a busier XCD) takes fewer tiles
+// instead of holding up the end of the pass.
+//
+// Tickets: the first tile of workgroup b is tile b; after that each tile
+// costs one ticket v = atomicAdd(ctr, 1) and names tile gridDim.x + v.  A
+// pass of n tiles draws exactly n tickets (one per tile run, the last of
+// each workgroup naming a tile >= n, which ends it), so the workgroup that
+// draws v = n - 1 holds the last ticket and resets the counter to zero for
+// the next pass on the stream.  No host-side clearing is needed, which
+// keeps the pass valid inside a captured graph and across its replays.
+//
+// LEAN: the pass holds only uncontrolled dense 1-/2-register ops and sign
+// ops (the brick-wall shape): only those op paths are compiled in, which
+// frees registers and instruction cache for the rest.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int run_op_lean(SubCube<4>& s, const TOp* __restrict__ op) {
+    const OpHead hd = load_head(op);
+    switch (hd.kind * 16 + hd.a * 4 + hd.b) {
+        case 48: sw_sign<4>(s, hd); break;
+        case 0: sw_dense1_t<4, 0, false>(s, op, hd); break;
+        case 4: sw_dense1_t<4, 1, false>(s, op, hd); break;
+        case 8: sw_dense1_t<4, 2, false>(s, op, hd); break;
+        case 12: sw_dense1_t<4, 3, false>(s, op, hd); break;
+        case 16 + 1: sw_dense2_t<4, 0, 1, false>(s, op, hd); break;
+        case 16 + 2: sw_dense2_t<4, 0, 2, false>(s, op, hd); break;
+        case 16 + 3: sw_dense2_t<4, 0, 3, false>(s, op, hd); break;
+        case 16 + 6: sw_dense2_t<4, 1, 2, false>(s, op, hd); break;
+        case 16 + 7: sw_dense2_t<4, 1, 3, false>(s, op, hd); break;
+        default: sw_dense2_t<4, 2, 3, false>(s, op, hd); break;
+    }
+    return 1;
+}
+
+// LDS: tile (2^12 + 2^8 amplitudes), scat_lo[256], scat_hi[16], s_next.
+// The launch has min(ntiles, slots) workgroups of 256 threads.
+template <bool LEAN>
+__device__ __forceinline__ void stream_body(double2* __restrict__ sv, int vq, const Pass* __restrict__ pass,
+                                            const Sweep* __restrict__ sweeps, const TOp* __restrict__ tops,
+                                            uint64_t ntiles, unsigned long long* __restrict__ ctr,
+                                            double2* __restrict__ tile, uint64_t* __restrict__ scat_lo,
+                                            uint64_t* __restrict__ scat_hi, uint64_t* s_next) {
+    constexpr int R = 4, T = 12, TB = tile_threads<R>(), TBB = tile_thread_bits<R>(), NR = 1 << R;
+    static_assert(T - TBB == R, "one row of the tile per register");
+    const uint64_t lmask = pass->lmask;
+    {
+        const int t = threadIdx.x;
+        uint64_t sc = 0;
+        for (int b = 0; b < TBB; ++b)
+            if ((t >> b) & 1) sc |= 1ull << pass->lbits[b];
+        scat_lo[t] = sc;
+        if (t < NR) {
+            uint64_t h = 0;
+            for (int b = TBB; b < T; ++b)
+                if ((t >> (b - TBB)) & 1) h |= 1ull << pass->lbits[b];
+            scat_hi[t] = h;
+        }
+    }
+    __syncthreads();
+    // global base of tile w: w deposited into the non-tile bits
+    auto tile_base = [&](uint64_t w) {
+        uint64_t b = 0;
+        for (int bit = 0, k = 0; bit < vq; ++bit)
+            if (!((lmask >> bit) & 1)) {
+                b |= ((w >> k) & 1) << bit;
+                ++k;
+            }
+        return b;
+    };
+    const int nsw = pass->nsweeps;
+    dvec<2 * NR> pre;  // this thread's amplitudes of the next tile (row jj = local j = tid + TB jj)
+    auto load = [&](uint64_t b) {
+        const uint64_t s_t = b | scat_lo[threadIdx.x];
+        sfor<0, NR>([&](auto J) {
+            constexpr int jj = decltype(J)::value;
+            setv<jj>(pre, to_c2(sv[s_t | scat_hi[jj]]));
+        });
+    };
+    auto to_lds = [&]() {
+        sfor<0, NR>([&](auto J) {
+            constexpr int jj = decltype(J)::value;
+            tile[phys((int)threadIdx.x + (jj << TBB))] = to_d2(getv<jj>(pre));
+        });
+        __syncthreads();
+    };
+    // one sweep: a 12-bit tile holds exactly one sub-cube per thread
+    auto sweep = [&](int sidx, uint64_t base) {
+        const Sweep* sw = sweeps + pass->sweep_begin + sidx;
+        const int sb0 = sw->sb[0], sb1 = sw->sb[1], sb2 = sw->sb[2], sb3 = sw->sb[3];
+        const int ob = sw->op_begin, oe = sw->op_end;
+        auto gl = [&](int l) { return scat_lo[(1 << l) & (TB - 1)] | scat_hi[(1 << l) >> TBB]; };
+        const int o0 = 1 << sb0, o1 = 1 << sb1, o2 = 1 << sb2, o3 = 1 << sb3;
+        auto ins = [](int x, int b) { const int lo = x & ((1 << b) - 1); return ((x - lo) << 1) | lo; };
+        const int lb = ins(ins(ins(ins((int)threadIdx.x, sb0), sb1), sb2), sb3);
+        const int pl = phys(lb);
+        SubCube<R> sc;
+        sc.gbase = base | scat_lo[lb & (TB - 1)] | scat_hi[lb >> TBB];
+        sc.g0 = gl(sb0);
+        sc.g1 = gl(sb1);
+        sc.g2 = gl(sb2);
+        sc.g3 = gl(sb3);
+        sfor<0, NR>([&](auto J) {
+            constexpr int j = decltype(J)::value;
+            const int o = ((j & 1) ? o0 : 0) + ((j & 2) ? o1 : 0) + ((j & 4) ? o2 : 0) + ((j & 8) ? o3 : 0);
+            setv<j>(sc.v, to_c2(tile[pl + phys(o)]));
+        });
+        if constexpr (LEAN) {
+            for (int o = ob; o < oe;) o += run_op_lean(sc, tops + o);
+        } else {
+            for (int o = ob; o < oe;) o += run_op<R, false>(sc, tops + o, tile, pl, o0, o1, o2, o3);
+        }
+        sfor<0, NR>([&](auto J) {
+            constexpr int j = decltype(J)::value;
+            const int o = ((j & 1) ? o0 : 0) + ((j & 2) ? o1 : 0) + ((j & 4) ? o2 : 0) + ((j & 8) ? o3 : 0);
+            tile[pl + phys(o)] = to_d2(getv<j>(sc.v));
+        });
+        __syncthreads();
+    };
+    uint64_t base = tile_base(blockIdx.x);  // blockIdx.x < ntiles (grid <= ntiles)
+    load(base);
+    to_lds();
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const unsigned long long v = atomicAdd(ctr, 1ull);
+            if (v == ntiles - 1) atomicExch(ctr, 0ull);  // the pass's last ticket
+            *s_next = gridDim.x + v;
+        }
+        __syncthreads();
+        const uint64_t nt = *s_next;  // rewritten only after the store barrier below
+        // after the last tile, the prefetch re-reads this tile's old
+        // amplitudes (in bounds, discarded), so every path issues the same
+        // memory operations in the same order
+        const uint64_t nbase = nt < ntiles ? tile_base(nt) : base;
+        for (int sidx = 0; sidx + 1 < nsw; ++sidx) sweep(sidx, base);
+        load(nbase);  // in flight during the last sweep and the stores
+        if (nsw > 0) sweep(nsw - 1, base);
+        const uint64_t s_t = base | scat_lo[threadIdx.x];
+        sfor<0, NR>([&](auto J) {
+            constexpr int jj = decltype(J)::value;
+            const c2 a = to_c2(tile[phys((int)threadIdx.x + (jj << TBB))]);
+            double* p = (double*)(sv + (s_t | scat_hi[jj]));
+            __builtin_nontemporal_store(a.x, p);
+            __builtin_nontemporal_store(a.y, p + 1);
+        });
+        __syncthreads();  // the next tile rewrites LDS
+        if (nt >= ntiles) break;
+        base = nbase;
+        to_lds();
+    }
+}
+
 }  // namespace bsim
