@@ -1,9 +1,21 @@
-"""Benchmark: 30-qubit ComplexF64 brick-wall random circuit (depth 20, U on
-every qubit + nearest-neighbour CZ layers), <Z_i> for all i and 1,000,000
-alias-method shots, on one MI355X.
+"""Benchmark of the evolution hot path on one MI355X, ComplexF64 throughout,
+every state resident in HBM.  Prints one JSON line.
 
-One step = reinit to |0...0> + fused evolve + one-pass <Z_i> + alias table
-build + 1e6 shots, with the state resident in HBM.  Prints one JSON line.
+Headline (``value``, gates/s): the 30-qubit brick-wall random circuit of
+depth 20 (600 U + 290 CZ, 16 GiB state).  One step = reset to |0...0> +
+fused evolve + one-pass <Z_i> for all i + alias table + 1,000,000 shots;
+``value`` is the evolve's gates/s, the other phases are reported beside it.
+The tile configuration is chosen by the autotuner on the circuit's first
+five layers before the timed steps.
+
+Other legs (each a workload named by its sizes): the 20-qubit RY + QFT
+circuit with probabilities, <Z_i> and 10,000 shots; the 14-qubit noisy
+density matrix in both Kraus semantics; the 24- and 28-qubit
+hardware-efficient ansatz adjoint gradient against parameter shift; the
+4096-circuit uneven batch run together with 256 16-qubit gradient sets in
+one persistent launch; the per-gate and dense k-target kernels; HBM stream
+probes (the measured bandwidth ceiling); and the oracle (the reference's
+loops restated in C, OpenMP) on bounded samples of every workload.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--qubits 30] [--depth 20]
 """
@@ -31,44 +43,6 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n: int, ins, budget_s: float = 15.0):
-    """The oracle (C restatement of the reference's per-gate loops, OpenMP
-    over every host core) on the same 30-qubit circuit, on a bounded prefix
-    of its gates."""
-    import numpy as np
-
-    import oracle as O
-    from bsim.gates import kernel_call
-
-    cores = os.cpu_count() or 1
-    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
-    psi = np.zeros(2 ** n, dtype=np.complex128)
-    psi[0] = 1
-    t0 = time.perf_counter()
-    done = 0
-    bytes_ = 0.0
-    for i in ins:
-        call = kernel_call(i.operator, i.target)
-        O.apply_call(psi, n, call)
-        done += 1
-        # reference traffic: dense gates stream the whole state, controlled
-        # gates (CZ) the control-set half (src/gate_kernels.jl:579-606)
-        bytes_ += (16.0 if call.controls else 32.0) * 2 ** n
-        if time.perf_counter() - t0 > budget_s:
-            break
-    dt = time.perf_counter() - t0
-    del psi
-    return {
-        "value": bytes_ / dt / 1e9,
-        "unit": "GB/s",
-        "gates_per_s": done / dt,
-        "cores": int(os.environ.get("OMP_NUM_THREADS", cores)),
-        "kind": "port",
-        "sample": f"first {done} of {len(ins)} gates of the {n}-qubit brick-wall, reference per-gate loops "
-                  f"restated in C (oracle/refcpu.c), {dt:.1f} s; reference Julia unavailable",
-    }
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +52,8 @@ def main():
     ap.add_argument("--depth", type=int, default=20)
     ap.add_argument("--shots", type=int, default=1000000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--autotune", type=int, default=1, help="1: autotune the tile configuration first")
+    ap.add_argument("--qft-qubits", type=int, default=20, help="RY + QFT leg (0 skips it)")
     ap.add_argument("--adjoint-qubits", type=int, default=24, help="0 skips the adjoint leg")
     ap.add_argument("--adjoint-large", type=int, default=28,
                     help="qubits of the second adjoint workload (the same ansatz; 0 skips it)")
@@ -87,9 +63,10 @@ def main():
     ap.add_argument("--dm-qubits", type=int, default=14, help="noisy density-matrix leg (0 skips it)")
     ap.add_argument("--graph-reps", type=int, default=500, help="captured-evolution replays (0 skips the leg)")
     ap.add_argument("--per-gate", type=int, default=1, help="1: time the per-gate launch path on one layer")
+    ap.add_argument("--probe", type=int, default=1, help="1: HBM stream probes (the bandwidth ceiling)")
     ap.add_argument("--sweep-sets", type=int, default=256, help="16-qubit ansatz parameter sets with gradients")
     ap.add_argument("--pmc-probe", action="store_true",
-                    help="profiling aid: one fused evolve of the workload and one 8 GiB device copy, no JSON")
+                    help="profiling aid: one fused evolve of the workload and one 4 GiB stream copy, no JSON")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -98,7 +75,6 @@ def main():
 
     import numpy as np
 
-    import bsim
     from bsim import _lib as L
     from bsim import workloads as W
     from bsim.simulators import StateVectorSimulator, lower_instructions
@@ -110,26 +86,30 @@ def main():
     ins = W.brickwall(n, depth, seed=0)
     n_gates = len(ins)
     ol = lower_instructions(ins, L.BSIM_SV)
+    sim = StateVectorSimulator(n, a.shots)
+    lib = L.load()
+
+    tuned = None
+    if a.autotune:
+        tuned = autotune_leg(sim, ins)
     stats = (ctypes.c_int64 * 4)()
     L.call("bsim_plan_stats", n, 0, ol.ptr, ol.n, stats)
     passes = int(stats[0])
-    tile_bits, low_bits = ctypes.c_int(), ctypes.c_int()
-    L.call("bsim_get_tile_config", ctypes.byref(tile_bits), ctypes.byref(low_bits))
-    log(f"workload: {n}q brick-wall depth {depth}: {n_gates} gates -> {passes} fused passes "
-        f"(tile {tile_bits.value} bits, {low_bits.value} contiguous)")
+    cfg = current_config()
+    log(f"workload: {n}q brick-wall depth {depth}: {n_gates} gates -> {passes} fused passes ({cfg})")
 
-    sim = StateVectorSimulator(n, a.shots)
-    lib = L.load()
     phases = {"evolve": 0.0, "expect_z": 0.0, "alias_build": 0.0, "shots": 0.0}
 
     def step(timed):
         t = time.perf_counter()
         L.call("bsim_state_reset", sim.handle)
         L.call("bsim_evolve", sim.handle, ol.ptr, ol.n, 0)
+        L.call("bsim_synchronize", sim.handle)
         t1 = time.perf_counter()
         z = sim.expect_z_all()
         t2 = time.perf_counter()
         L.call("bsim_alias_build", sim.handle, L.ALIAS_PARALLEL, 0.0)
+        L.call("bsim_synchronize", sim.handle)
         t3 = time.perf_counter()
         out = np.empty(a.shots, dtype=np.int64)
         L.call("bsim_sample", sim.handle, a.shots, ctypes.c_uint64(7), L.i64ptr(out))
@@ -160,14 +140,14 @@ def main():
     pass_bytes = 32.0 * 2 ** n
     flop_per_launch = algorithmic_flops(ins, n) / max(passes, 1)
     evolve_s = phases["evolve"] / a.steps
-    value = passes * pass_bytes / evolve_s / 1e9
+    value = n_gates / evolve_s
     avg_kernel_ms = kms.value / max(launches.value, 1)
     achieved = pass_bytes / (avg_kernel_ms * 1e-3) / 1e9
     ms_per_step = dt / a.steps * 1e3
     rec = {
         "metric": METRIC,
-        "value": round(value, 2),
-        "unit": "GB/s",
+        "value": round(value, 1),
+        "unit": "gates/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
@@ -179,22 +159,27 @@ def main():
         "data": "synthetic (SplitMix64 angles, seed 0)",
         "config": {
             "workload": f"brick-wall random circuit, {n} qubits, depth {depth}, ComplexF64 state "
-                        f"({2 ** n * 16 / 2 ** 30:.0f} GiB), <Z_i> all i + {a.shots} alias-method shots",
+                        f"({2 ** n * 16 / 2 ** 30:.0f} GiB), <Z_i> all i + {a.shots} alias-method shots; "
+                        f"value = gates/s of the fused evolve",
             "qubits": n, "depth": depth, "gates": n_gates, "shots": a.shots,
-            "fused_passes": passes, "tile_bits": tile_bits.value, "low_bits": low_bits.value,
+            "fused_passes": passes, **cfg,
         },
-        "gates_per_s": round(n_gates / evolve_s, 1),
+        "evolve_ms": round(evolve_s * 1e3, 3),
+        "hbm_GBps_algorithmic": round(passes * pass_bytes / evolve_s / 1e9, 1),
         "phase_ms": {k: round(v / a.steps * 1e3, 3) for k, v in phases.items()},
+        "expect_z_GBps": round(16.0 * 2 ** n / (phases["expect_z"] / a.steps) / 1e9, 1),
         "shots_per_s": round(a.shots / (phases["shots"] / a.steps), 1),
+        "autotune": tuned,
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_tile (fused LDS-tiled gate pass)",
+            "kernel": "k_tile_stream_lean (fused LDS-tiled gate pass, persistent, next tile prefetched)",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(n, depth),
             "traffic_source": "profiles/k_tile_brickwall_30q.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+            "algorithmic_bytes_per_launch": int(pass_bytes),
             "avg_kernel_ms": round(avg_kernel_ms, 4),
             "launches_timed": int(launches.value),
             # the pass does many gates per HBM round trip and is bound by FP64
@@ -210,8 +195,15 @@ def main():
         "z_checksum": float(np.sum(z)),
     }
     del sim
+    if a.probe:
+        rec["stream_ceiling"] = probe_leg()
+        ceil = rec["stream_ceiling"]["copy_GBps"]
+        rec["roofline"]["frac_of_stream_copy"] = round(achieved / ceil, 4)
+        rec["expect_z_frac_of_stream_read"] = round(rec["expect_z_GBps"] / rec["stream_ceiling"]["read_GBps"], 4)
     if a.per_gate:
-        rec["per_gate"] = per_gate_leg(n, ins)
+        rec["per_gate"] = per_gate_leg(n, ins, rec.get("stream_ceiling"))
+    if a.qft_qubits > 0:
+        rec["qft"] = qft_leg(a.qft_qubits)
     if a.adjoint_qubits > 0:
         rec["adjoint"] = adjoint_leg(a.adjoint_qubits, a.ps_sample)
     if a.adjoint_large > 0:
@@ -221,13 +213,354 @@ def main():
     if a.batch > 0:
         rec["batch"] = batch_leg(a.batch, a.sweep_sets)
     if a.dm_qubits > 0:
-        rec["density_matrix"] = dm_leg(a.dm_qubits)
+        rec["density_matrix"] = dm_leg(a.dm_qubits, a.autotune)
     if a.graph_reps > 0:
         rec["captured"] = graph_leg(a.graph_reps)
     if not a.no_cpu:
         log("cpu baseline (oracle) ...")
-        rec["cpu_baseline"] = cpu_baseline(n, ins)
+        rec["cpu_baseline"] = cpu_baselines(n, depth, ins, rec)
     print(json.dumps(rec), flush=True)
+
+
+def cpu_baselines(n: int, depth: int, ins, rec: dict) -> dict:
+    """The oracle -- the reference's per-gate, per-channel and result loops
+    restated in C (oracle/refcpu.c), OpenMP over the host cores -- on a
+    bounded sample of every workload, each phase timed and extrapolated to
+    the full workload where the sample is a part of it.  The top-level value
+    is the 30-qubit evolve in gates/s, the headline's unit."""
+    cores = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    os.environ["OMP_NUM_THREADS"] = str(cores)
+    configs = {}
+    for name, fn in (("C1_qft_20q", lambda: _cpu_qft(20)),
+                     ("C2_brickwall_30q", lambda: _cpu_brickwall(n, ins)),
+                     ("C3_noisy_dm_14q", lambda: _cpu_dm(14)),
+                     ("C4_hea_adjoint_24q", lambda: _cpu_hea(24)),
+                     ("C5_batch", lambda: _cpu_batch())):
+        t = time.perf_counter()
+        configs[name] = fn()
+        configs[name]["cpu_wall_s"] = round(time.perf_counter() - t, 2)
+        log(f"cpu baseline {name}: {configs[name]}")
+    c2 = configs["C2_brickwall_30q"]
+    return {
+        "value": c2["evolve_gates_per_s"],
+        "unit": "gates/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"30-qubit brick-wall: the first {c2['evolve_sample_gates']} of {len(ins)} gates through the "
+                  f"reference per-gate loops restated in C (oracle/refcpu.c, {cores} OpenMP threads); reference "
+                  f"Julia unavailable on the host.  Per-config records below (their samples named in each)",
+        "configs": configs,
+    }
+
+
+def _timed(f, *args):
+    t = time.perf_counter()
+    r = f(*args)
+    return r, time.perf_counter() - t
+
+
+def _cpu_qft(nq: int) -> dict:
+    """C1 in full, median of 3: evolve, probabilities, <Z_i> for every
+    qubit, the reference-order alias table and 10,000 shots."""
+    import numpy as np
+
+    import oracle as O
+    from bsim import workloads as W
+
+    ins, _ = W.ry_qft(nq, seed=0)
+    runs = []
+    for rep in range(3):
+        psi, t_ev = _timed(O.evolve_sv, nq, ins)
+        p, t_p = _timed(O.sv_probabilities, psi, nq)
+        t = time.perf_counter()
+        for q in range(nq):
+            O.sv_expect_pauli(psi, nq, "I" * q + "Z" + "I" * (nq - 1 - q))
+        t_z = time.perf_counter() - t
+        t = time.perf_counter()
+        ap, al = O.alias_reference(p, float(p.sum()))
+        idx, u = O.shot_draws(rep, nq, 10000)
+        O.lookup(ap, al, idx, u)
+        t_s = time.perf_counter() - t
+        runs.append((t_ev, t_p, t_z, t_s))
+    m = np.median(np.array(runs), axis=0)
+    return {"sample": f"full workload ({len(ins)} gates), median of 3", "evolve_ms": round(m[0] * 1e3, 2),
+            "probabilities_ms": round(m[1] * 1e3, 2), "expect_z_ms": round(m[2] * 1e3, 2),
+            "alias_and_shots_ms": round(m[3] * 1e3, 2), "total_ms": round(float(m.sum()) * 1e3, 2),
+            "evolve_gates_per_s": round(len(ins) / m[0], 1)}
+
+
+def _cpu_brickwall(n: int, ins, budget_s: float = 8.0) -> dict:
+    """C2: the evolve on the gates that fit in ``budget_s`` (extrapolated to
+    the circuit), then on that state: one <Z_q> (x n), the probabilities,
+    the reference-order alias table on 2^26 of them (x 2^(n-26)) and the
+    reference's n >= 30 direct sampling (StatsBase.sample, a linear scan)
+    on 3 shots, extrapolated to 1,000,000."""
+    import numpy as np
+
+    import oracle as O
+    from bsim.gates import kernel_call
+
+    psi = np.zeros(2 ** n, dtype=np.complex128)
+    psi.fill(0)  # touch every page outside the timed region
+    psi[0] = 1
+    t0 = time.perf_counter()
+    done = 0
+    for i in ins:
+        O.apply_call(psi, n, kernel_call(i.operator, i.target))
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    t_ev = time.perf_counter() - t0
+    _, t_z = _timed(O.sv_expect_pauli, psi, n, "Z" + "I" * (n - 1))
+    p, t_p = _timed(O.sv_probabilities, psi, n)
+    del psi
+    m = min(26, n)
+    sub = np.ascontiguousarray(p[: 2 ** m])
+    _, t_a = _timed(O.alias_reference, sub, float(sub.sum()))
+    del sub
+    u = np.array([0.25, 0.5, 0.75])
+    _, t_d = _timed(O.direct_samples, p, float(p.sum()), u)
+    del p
+    rate = done / t_ev
+    per_shot = t_d / len(u)
+    return {"sample": f"evolve: first {done} of {len(ins)} gates; <Z>: 1 of {n} qubits; alias: 2^{m} of 2^{n} "
+                      f"weights; direct sampling: 3 shots (u = 0.25, 0.5, 0.75) of 10^6",
+            "evolve_sample_gates": done, "evolve_gates_per_s": round(rate, 3),
+            "evolve_s_extrapolated": round(len(ins) / rate, 2),
+            "expect_z_all_s_extrapolated": round(t_z * n, 3),
+            "probabilities_s": round(t_p, 3),
+            "alias_table_s_extrapolated": round(t_a * 2 ** (n - m), 3),
+            "direct_sampling_s_per_shot": round(per_shot, 4),
+            "direct_sampling_1e6_shots_s_extrapolated": round(per_shot * 1e6, 1)}
+
+
+def _cpu_dm(nq: int) -> dict:
+    """C3, reference Kraus semantics: the first op of each kind of the noisy
+    brick-wall (U, Depolarizing, CZ, TwoQubitDepolarizing,
+    AmplitudeDamping) applied to the 4 GiB density matrix (best of two), the evolve
+    estimated as the sum over the circuit of its ops' kind times; one
+    <Z_i Z_i+1> (x (n-1)) and the diagonal probabilities."""
+    import numpy as np
+
+    import oracle as O
+    from bsim import workloads as W
+
+    ins = W.brickwall(nq, 10, seed=0, noise=True)
+    kinds = {}
+    for i in ins:
+        kinds.setdefault(type(i.operator).__name__, []).append(i)
+    rho = np.zeros(4 ** nq, dtype=np.complex128)
+    rho.fill(0)  # touch every page outside the timed ops
+    rho[0] = 1
+    per_kind = {}
+    for k, lst in kinds.items():  # best of two applications
+        per_kind[k] = min(_timed(O.evolve_dm, nq, lst[:1], "reference", rho)[1] for _ in range(2))
+    est = sum(per_kind[k] * len(lst) for k, lst in kinds.items())
+    _, t_zz = _timed(O.dm_expect_pauli, rho, nq, "ZZ" + "I" * (nq - 2))
+    _, t_p = _timed(O.dm_probabilities, rho, nq)
+    del rho
+    return {"sample": "one op of each kind (" + ", ".join(f"{k} x{len(v)}" for k, v in kinds.items())
+                      + f" in the circuit); one of {nq - 1} <Z_i Z_i+1>",
+            "op_ms": {k: round(v * 1e3, 1) for k, v in per_kind.items()},
+            "evolve_s_estimated": round(est, 2), "ops_per_s": round(len(ins) / est, 2),
+            "zz_all_s_extrapolated": round(t_zz * (nq - 1), 3), "probabilities_s": round(t_p, 3)}
+
+
+def _cpu_hea(nq: int) -> dict:
+    """C4: one energy evaluation of the 24-qubit ansatz -- the first layer's
+    gates (x 8 layers) and one ZZ and one X term of <H> (x their counts) --
+    then the parameter-shift gradient (2 energies per parameter) built from
+    it; the reference has no adjoint."""
+    import numpy as np
+
+    import oracle as O
+    from bsim import workloads as W
+    from bsim.gates import kernel_call
+
+    ins, pmap, th = W.hea(nq, 8, seed=0)
+    words, _ = W.tfim(nq)
+    layer = len(ins) // 8
+    psi = np.zeros(2 ** nq, dtype=np.complex128)
+    psi.fill(0)
+    psi[0] = 1
+    t = time.perf_counter()
+    for i in ins[:layer]:
+        O.apply_call(psi, nq, kernel_call(i.operator, i.target))
+    t_layer = time.perf_counter() - t
+    zz = [w for w in words if "Z" in w]
+    xx = [w for w in words if "X" in w]
+    _, t_zz = _timed(O.sv_expect_pauli, psi, nq, zz[0])
+    _, t_x = _timed(O.sv_expect_pauli, psi, nq, xx[0])
+    del psi
+    t_energy = 8 * t_layer + len(zz) * t_zz + len(xx) * t_x
+    t_ps = 2 * len(th) * t_energy
+    return {"sample": f"first layer of 8 ({layer} of {len(ins)} gates); 1 of {len(zz)} ZZ and 1 of {len(xx)} X terms",
+            "energy_s_estimated": round(t_energy, 3),
+            "param_shift_gradient_s_estimated": round(t_ps, 1),
+            "param_shift_grads_per_s": round(1.0 / t_ps, 6), "params": len(th)}
+
+
+def _cpu_batch() -> dict:
+    """C5: 16 circuits of the uneven batch (evolve + <Z_q> for every qubit +
+    reference-order alias + 1000 shots each), extrapolated to the batch by
+    gate count; one energy of the 16-qubit ansatz, extrapolated to the
+    parameter-shift gradients of 256 sets (2 energies per parameter)."""
+    import numpy as np
+
+    import oracle as O
+    from bsim import workloads as W
+
+    count = 4096
+    spec = W.batch_tasks(count, seed=0)
+    pick = spec[:: count // 16][:16]
+    t = time.perf_counter()
+    g_sample = 0
+    for n, d, s in pick:
+        ins = W.brickwall(n, d, s)
+        g_sample += len(ins)
+        psi = O.evolve_sv(n, ins)
+        for q in range(n):
+            O.sv_expect_pauli(psi, n, "I" * q + "Z" + "I" * (n - 1 - q))
+        p = O.sv_probabilities(psi, n)
+        ap, al = O.alias_reference(p, float(p.sum()))
+        idx, u = O.shot_draws(s, n, 1000)
+        O.lookup(ap, al, idx, u)
+    t_sample = time.perf_counter() - t
+    # brick-wall gates: n U per layer, then n // 2 CZ (even layers) or (n - 1) // 2 (odd)
+    g_total = sum(n * d + (n // 2) * ((d + 1) // 2) + ((n - 1) // 2) * (d // 2) for n, d, _ in spec)
+    t_circ = t_sample * g_total / g_sample
+    words, _ = W.tfim(16)
+    ins, pmap, th = W.hea(16, 8, seed=0)
+    t = time.perf_counter()
+    psi = O.evolve_sv(16, ins)
+    for w in words:
+        O.sv_expect_pauli(psi, 16, w)
+    t_energy = time.perf_counter() - t
+    t_grad = 256 * 2 * len(th) * t_energy
+    total = t_circ + t_grad
+    return {"sample": f"16 of {count} circuits ({g_sample} gates, extrapolated by gate count); one 16-qubit energy "
+                      f"(x 2 x {len(th)} parameters x 256 sets)",
+            "circuits_s_extrapolated": round(t_circ, 2), "gradients_s_extrapolated": round(t_grad, 2),
+            "tasks_per_s": round((count + 256) / total, 2)}
+
+
+def current_config() -> dict:
+    """The library's tile configuration: tile bits, contiguous low bits,
+    register bits per sweep, fusion limit."""
+    from bsim import _lib as L
+
+    t, lo, rb, fl = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    L.call("bsim_get_tile_config", ctypes.byref(t), ctypes.byref(lo))
+    L.call("bsim_set_sweep_bits", 0, ctypes.byref(rb))  # 0: read only
+    L.call("bsim_set_fusion_limit", 0, ctypes.byref(fl))
+    L.call("bsim_set_fusion_limit", fl.value, None)
+    return {"tile_bits": t.value, "low_bits": lo.value, "sweep_bits": rb.value, "fusion_limit": fl.value}
+
+
+def set_config(c: dict):
+    from bsim import _lib as L
+
+    L.call("bsim_set_tile_config", c["tile_bits"], c["low_bits"])
+    L.call("bsim_set_sweep_bits", c["sweep_bits"], None)
+    L.call("bsim_set_fusion_limit", c["fusion_limit"], None)
+
+
+def autotune_leg(sim, ins) -> dict:
+    """bsim_autotune on the workload (every tile width, contiguous low bits
+    and sweep width, then the fusion width at the best shape); the winner
+    becomes the library configuration.  Reports it with the default
+    configuration's evolve time."""
+    from bsim import _lib as L
+    from bsim.simulators import lower_instructions
+
+    default = current_config()
+    ol = lower_instructions(ins, L.BSIM_SV)
+
+    def timed():
+        L.call("bsim_state_reset", sim.handle)
+        L.call("bsim_synchronize", sim.handle)
+        t = time.perf_counter()
+        L.call("bsim_evolve", sim.handle, ol.ptr, ol.n, 0)
+        L.call("bsim_synchronize", sim.handle)
+        return time.perf_counter() - t
+
+    timed()
+    t_default = min(timed() for _ in range(2))
+    t0 = time.perf_counter()
+    best = sim.autotune(ins)
+    t_search = time.perf_counter() - t0
+    log(f"autotune: {best} in {t_search:.1f} s; default {default} {t_default * 1e3:.1f} ms")
+    return {"chosen": {k: best[k] for k in ("tile_bits", "low_bits", "sweep_bits", "fusion_limit")},
+            "chosen_ms": round(best["ms"], 3), "default": default, "default_ms": round(t_default * 1e3, 3),
+            "search_s": round(t_search, 2)}
+
+
+def probe_leg() -> dict:
+    """The HBM ceiling: bsim_stream_probe's plain copy, read and write
+    kernels over 4 GiB (copy: 4 GiB read + 4 GiB written), best of 5
+    launches at 4, 8 and 16 workgroups per CU."""
+    from bsim import _lib as L
+
+    cus = ctypes.c_int()
+    L.call("bsim_device_info", None, 0, ctypes.byref(cus), None)
+    nbytes = 2 ** 32
+    out = {"bytes": nbytes}
+    for mode, name, moved in ((0, "copy", 2 * nbytes), (1, "read", nbytes), (2, "write", nbytes)):
+        best = float("inf")
+        for per_cu in (4, 8, 16):
+            ms = ctypes.c_double()
+            L.call("bsim_stream_probe", mode, nbytes, per_cu * cus.value, 5, ctypes.byref(ms))
+            best = min(best, ms.value)
+        out[f"{name}_GBps"] = round(moved / (best * 1e-3) / 1e9, 1)
+        out[f"{name}_frac_of_peak"] = round(moved / (best * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    log(f"stream probes: copy {out['copy_GBps']} GB/s, read {out['read_GBps']}, write {out['write_GBps']}")
+    return out
+
+
+def qft_leg(nq: int) -> dict:
+    """The 20-qubit RY + textbook QFT circuit (240 gates): fused evolve,
+    probabilities, <Z_i> for every qubit and 10,000 alias-method shots
+    (exact-integer parallel table), median of 5 after a warm-up; then once
+    with the reference-order table.  The 16 MiB state stays in cache:
+    launch-bound."""
+    import numpy as np
+
+    from bsim import _lib as L
+    from bsim import workloads as W
+    from bsim.simulators import StateVectorSimulator, lower_instructions
+
+    ins, _ = W.ry_qft(nq, seed=0)
+    ol = lower_instructions(ins, L.BSIM_SV)
+    sim = StateVectorSimulator(nq, 10000)
+    runs = []
+    for rep in range(6):
+        t0 = time.perf_counter()
+        L.call("bsim_state_reset", sim.handle)
+        L.call("bsim_evolve", sim.handle, ol.ptr, ol.n, 0)
+        L.call("bsim_synchronize", sim.handle)
+        t1 = time.perf_counter()
+        p = sim.probabilities()
+        t2 = time.perf_counter()
+        z = sim.expect_z_all()
+        t3 = time.perf_counter()
+        shots = sim.samples(seed=rep)
+        t4 = time.perf_counter()
+        if rep:
+            runs.append((t1 - t0, t2 - t1, t3 - t2, t4 - t3, t4 - t0))
+    med = np.median(np.array(runs), axis=0)
+    # the reference-order table (bit-compatible with make_alias_table!): a
+    # sequential LIFO pairing, one lane
+    t = time.perf_counter()
+    sim.samples(seed=0, alias_mode=L.ALIAS_REFERENCE)
+    t_ref = time.perf_counter() - t
+    del sim
+    log(f"qft {nq}q: evolve {med[0] * 1e3:.2f} ms, total {med[4] * 1e3:.2f} ms")
+    return {"workload": f"RY + QFT, {nq} qubits ({len(ins)} gates), probabilities + <Z_i> + 10000 shots",
+            "evolve_ms": round(med[0] * 1e3, 3), "probabilities_ms": round(med[1] * 1e3, 3),
+            "expect_z_ms": round(med[2] * 1e3, 3), "alias_and_shots_ms": round(med[3] * 1e3, 3),
+            "total_ms": round(med[4] * 1e3, 3), "gates_per_s": round(len(ins) / med[0], 1),
+            "alias_reference_order_and_shots_ms": round(t_ref * 1e3, 3),
+            "z_checksum": float(np.sum(z)), "prob_sum": float(np.sum(p)), "shots_drawn": int(shots.size)}
 
 
 def adjoint_leg(nq: int, ps_sample: int):
@@ -387,14 +720,16 @@ def measured_traffic(n: int, depth: int):
     return None
 
 
-def dm_leg(nq: int):
+def dm_leg(nq: int, autotune: int):
     """Noisy density matrix: the brick-wall of the state-vector leg at nq
     qubits, depth 10, with Depolarizing(0.01) after every U,
     TwoQubitDepolarizing(0.02) after every CZ and AmplitudeDamping(0.005) on
     every qubit per layer; <Z_i Z_i+1> for all pairs and the diagonal
-    probabilities, in both noise semantics.  Then the 12-qubit noisy
-    hardware-efficient ansatz (depolarizing 0.005) with its full adjoint
-    gradient through the checkpointed density-matrix adjoint."""
+    probabilities, in both noise semantics, each with the configuration the
+    autotuner picks for it (restored afterwards).  Then
+    the 12-qubit noisy hardware-efficient ansatz (depolarizing 0.005) with
+    its full adjoint gradient through the checkpointed density-matrix
+    adjoint."""
     import numpy as np
 
     from bsim import _lib as L
@@ -406,27 +741,40 @@ def dm_leg(nq: int):
     words = ["I" * i + "ZZ" + "I" * (nq - 2 - i) for i in range(nq - 1)]
     out = {"workload": f"noisy brick-wall {nq} qubits depth 10 ({len(ins)} ops), "
                        f"{2 ** (2 * nq) * 16 / 2 ** 30:.0f} GiB density matrix, <Z_i Z_i+1> + probabilities"}
+    keep = current_config()
     for mode in ("reference", "physical"):
         sim = DensityMatrixSimulator(nq, 0, noise_mode=mode)
+        tuned = None
+        if autotune:
+            t0 = time.perf_counter()
+            best = sim.autotune(ins)
+            tuned = {k: best[k] for k in ("tile_bits", "low_bits", "sweep_bits", "fusion_limit")}
+            tuned["search_s"] = round(time.perf_counter() - t0, 2)
         ol = lower_instructions(ins, L.BSIM_DM, mode)
         stats = (ctypes.c_int64 * 4)()
         L.call("bsim_plan_stats", 2 * nq, 1, ol.ptr, ol.n, stats)
+        L.call("bsim_state_reset", sim.handle)
         L.call("bsim_evolve", sim.handle, ol.ptr, ol.n, 0)  # warm-up
         L.call("bsim_synchronize", sim.handle)
-        t = time.perf_counter()
-        L.call("bsim_state_reset", sim.handle)
-        L.call("bsim_evolve", sim.handle, ol.ptr, ol.n, 0)
-        L.call("bsim_synchronize", sim.handle)
-        t_ev = time.perf_counter() - t
+        best_ev = float("inf")
+        for _ in range(2):
+            L.call("bsim_state_reset", sim.handle)
+            L.call("bsim_synchronize", sim.handle)
+            t = time.perf_counter()
+            L.call("bsim_evolve", sim.handle, ol.ptr, ol.n, 0)
+            L.call("bsim_synchronize", sim.handle)
+            best_ev = min(best_ev, time.perf_counter() - t)
         t = time.perf_counter()
         zz = sim.expect_pauli_words(words)
         p = sim.probabilities()
         t_res = time.perf_counter() - t
-        out[mode] = {"evolve_ms": round(t_ev * 1e3, 3), "results_ms": round(t_res * 1e3, 3),
-                     "passes": int(stats[0]), "single_launches": int(stats[1]),
+        out[mode] = {"evolve_ms": round(best_ev * 1e3, 3), "results_ms": round(t_res * 1e3, 3),
+                     "ops_per_s": round(len(ins) / best_ev, 1),
+                     "passes": int(stats[0]), "single_launches": int(stats[1]), "autotune": tuned,
                      "zz_checksum": float(np.sum(zz)), "trace": float(np.sum(p))}
-        log(f"density matrix {nq}q {mode}: evolve {t_ev * 1e3:.1f} ms")
+        log(f"density matrix {nq}q {mode}: evolve {best_ev * 1e3:.1f} ms ({tuned})")
         del sim
+        set_config(keep)
     n12 = 12
     hins, pmap, th = W.hea(n12, 8, seed=0, noise_p=0.005)
     hw, hc = W.tfim(n12)
@@ -442,7 +790,7 @@ def dm_leg(nq: int):
     return out
 
 
-def per_gate_leg(n: int, ins):
+def per_gate_leg(n: int, ins, ceiling: dict | None):
     """The per-gate launch path (bsim_evolve mode 1, the reference's loop
     shape) on the workload's first layer: its 30 U gates (each one HBM pass,
     32 B x 2^n) and its CZ gates (a diagonal over the control-set half,
@@ -467,6 +815,8 @@ def per_gate_leg(n: int, ins):
         gbs = unit_bytes / per / 1e9
         out[name] = {"gates": len(part), "ms_per_gate": round(per * 1e3, 3), "pass_GBps": round(gbs, 1),
                      "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4)}
+        if ceiling:
+            out[name]["frac_of_stream_copy"] = round(gbs / ceiling["copy_GBps"], 4)
         log(f"per-gate {name}: {per * 1e3:.2f} ms per gate, {gbs:.0f} GB/s")
     out["note"] = "pass_GBps: algorithmic bytes per gate (U: every amplitude read and written once; CZ: the control-set half)"
     del sim
@@ -570,9 +920,10 @@ def graph_leg(reps: int):
 
 
 def pmc_probe(n: int, depth: int):
-    """Kernels for hardware-counter runs: the fused passes of one evolve
-    (k_tile) and a plain device copy of 2^29 doubles (4 GiB read + 4 GiB
-    written), whose FETCH_SIZE / WRITE_SIZE calibrate the counters."""
+    """Kernels for hardware-counter runs: the fused passes of one evolve and
+    the stream probe's kernels over 4 GiB (k_probe_write fills the source:
+    4 GiB written; k_probe_copy twice: 4 GiB read + 4 GiB written each),
+    whose FETCH_SIZE / WRITE_SIZE calibrate the counters."""
     from bsim import _lib as L
     from bsim import workloads as W
     from bsim.simulators import StateVectorSimulator, lower_instructions
@@ -583,41 +934,48 @@ def pmc_probe(n: int, depth: int):
     L.call("bsim_synchronize", sim.handle)
     log(f"pmc probe: evolve {n}q done")
     del sim
-    cp = _DeviceCopy(2 ** 32)
-    cp.copy_seconds()
-    cp.close()
-    log("pmc probe: copy of 4 GiB done")
+    ms = ctypes.c_double()
+    L.call("bsim_stream_probe", 0, 2 ** 32, 0, 1, ctypes.byref(ms))
+    log(f"pmc probe: stream copy of 4 GiB done ({ms.value:.3f} ms)")
 
 
 def batch_leg(count: int, sweep_sets: int, shots: int = 1000):
-    """Uneven batch: ``count`` brick-wall circuits, qubits uniform in 8..22,
-    depth uniform in 10..40, <Z_q> + 1000 shots each, in one persistent
-    launch, against the per-circuit launch path on the same circuits; then
-    ``sweep_sets`` parameter sets of the 16-qubit 8-layer ansatz, each with
-    its adjoint gradient."""
+    """The uneven batch of one persistent launch: ``count`` brick-wall
+    circuits (qubits uniform in 8..22, depth uniform in 10..40, <Z_q> + 1000
+    shots each) together with ``sweep_sets`` parameter sets of the 16-qubit
+    8-layer ansatz, each with its adjoint gradient of the TFIM <H>.  Against
+    the serial path on the same tasks: one reset + fused evolve + <Z> +
+    alias table + shots launch sequence per circuit, then one
+    adjoint_gradient call per parameter set.  Latency = completion time of
+    each task from the start (persistent: the device clock from the launch
+    start); throughput = tasks / wall time, host lowering included."""
     import numpy as np
 
     from bsim import _lib as L
     from bsim import adjoint as A
     from bsim import workloads as W
-    from bsim.batch import run_batch
+    from bsim.batch import GradientTask, run_batch
     from bsim.simulators import StateVectorSimulator
 
     spec = W.batch_tasks(count, seed=0)
     circuits = [(n, W.brickwall_ops(n, d, s)) for n, d, s in spec]
     seeds = [s for _, _, s in spec]
     gates = sum(c[1].n for c in circuits)
-    log(f"batch: {count} circuits, {gates} gates")
-    run_batch(circuits[:64], shots, seeds[:64])  # warm-up
+    words, coeffs = W.tfim(16)
+    sets = [W.hea(16, 8, seed=k) for k in range(sweep_sets)]
+    grads = [GradientTask(16, ins, pmap, words, coeffs) for ins, pmap, _ in sets]
+    log(f"batch: {count} circuits ({gates} gates) + {sweep_sets} gradient sets in one launch")
+    nw = min(4, sweep_sets)
+    run_batch(circuits[:64] + grads[:nw], [shots] * 64 + [0] * nw, seeds[:64] + [0] * nw)  # warm-up
     t = time.perf_counter()
-    res = run_batch(circuits, shots, seeds)
+    res = run_batch(circuits + grads, [shots] * count + [0] * sweep_sets, seeds + [0] * sweep_sets)
     t_batch = time.perf_counter() - t
-    lat = np.array([r.latency_us for r in res]) * 1e-3
-    # per-circuit launches: reset, fused evolve, <Z>, parallel alias, shots
-    log("batch: per-circuit launch path")
+    lat_c = np.array([r.latency_us for r in res[:count]]) * 1e-3
+    lat_g = np.array([r.latency_us for r in res[count:]]) * 1e-3
+    log("batch: serial path")
     sims = {n: StateVectorSimulator(n, shots) for n in sorted({c[0] for c in circuits})}
     out = np.empty(shots, dtype=np.int64)
-    done = np.empty(count)
+    done_c = np.empty(count)
     z_diff = 0.0
     t = time.perf_counter()
     for i, ((n, ops), seed) in enumerate(zip(circuits, seeds)):
@@ -627,42 +985,41 @@ def batch_leg(count: int, sweep_sets: int, shots: int = 1000):
         z = sim.expect_z_all()
         L.call("bsim_alias_build", sim.handle, L.ALIAS_PARALLEL, 0.0)
         L.call("bsim_sample", sim.handle, shots, ctypes.c_uint64(seed), L.i64ptr(out))
-        done[i] = time.perf_counter() - t
+        done_c[i] = time.perf_counter() - t
         z_diff = max(z_diff, float(np.max(np.abs(z - res[i].expect_z))))
-    t_single = done[-1]
-    del sims
-    rec = {
-        "workload": f"{count} brick-wall SV circuits, qubits U{{8..22}}, depth U{{10..40}}, <Z_i> + {shots} shots each",
+    done_g = np.empty(sweep_sets)
+    g_diff = 0.0
+    gsim = sims.get(16) or StateVectorSimulator(16, 0)
+    for k, (ins, pmap, _) in enumerate(sets):
+        ev, g = A.adjoint_gradient(gsim, ins, pmap, words, coeffs)
+        done_g[k] = time.perf_counter() - t
+        g_diff = max(g_diff, float(np.max(np.abs(g - res[count + k].grad))), abs(ev - res[count + k].expval))
+    t_serial = time.perf_counter() - t
+    del sims, gsim
+
+    def pct(x):
+        return {"p50": round(float(np.percentile(x, 50)), 3), "p99": round(float(np.percentile(x, 99)), 3)} \
+            if len(x) else None
+
+    ntask = count + sweep_sets
+    log(f"batch: persistent {t_batch:.3f} s, serial {t_serial:.3f} s")
+    return {
+        "workload": f"{count} brick-wall SV circuits (qubits U{{8..22}}, depth U{{10..40}}, <Z_i> + {shots} shots "
+                    f"each) + {sweep_sets} parameter sets of the 16-qubit 8-layer ansatz with adjoint gradients "
+                    f"(TFIM H), one persistent launch",
         "gates": int(gates),
         "persistent_s": round(t_batch, 4),
-        "persistent_circuits_per_s": round(count / t_batch, 1),
-        "persistent_kernel_ms": round(float(lat.max()), 3),
-        "persistent_latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
-        "persistent_latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
-        "per_circuit_s": round(t_single, 4),
-        "per_circuit_circuits_per_s": round(count / t_single, 1),
-        "per_circuit_latency_ms_p50": round(float(np.percentile(done, 50)) * 1e3, 3),
-        "per_circuit_latency_ms_p99": round(float(np.percentile(done, 99)) * 1e3, 3),
-        "speedup": round(t_single / t_batch, 2),
-        "max_abs_z_diff_vs_per_circuit": z_diff,
-        "latency_note": "completion time from the start of the batch; persistent = device clock from the launch",
+        "persistent_tasks_per_s": round(ntask / t_batch, 1),
+        "persistent_kernel_ms": round(float(max(lat_c.max(initial=0), lat_g.max(initial=0))), 3),
+        "persistent_latency_ms": {"circuit": pct(lat_c), "gradient": pct(lat_g)},
+        "serial_s": round(t_serial, 4),
+        "serial_tasks_per_s": round(ntask / t_serial, 1),
+        "serial_latency_ms": {"circuit": pct(done_c * 1e3), "gradient": pct(done_g * 1e3)},
+        "speedup": round(t_serial / t_batch, 2),
+        "max_abs_z_diff_vs_serial": z_diff,
+        "max_abs_grad_diff_vs_serial": g_diff,
+        "latency_note": "completion time from the start; persistent = device clock from the launch start",
     }
-    if sweep_sets > 0:
-        words, coeffs = W.tfim(16)
-        sim = StateVectorSimulator(16, 0)
-        sets = [W.hea(16, 8, seed=k) for k in range(sweep_sets)]
-        A.adjoint_gradient(sim, sets[0][0], sets[0][1], words, coeffs)
-        log(f"batch: {sweep_sets} parameter sets x 16-qubit ansatz gradients")
-        t = time.perf_counter()
-        for ins, pmap, _ in sets:
-            A.adjoint_gradient(sim, ins, pmap, words, coeffs)
-        t_sw = time.perf_counter() - t
-        rec["sweep"] = {
-            "workload": f"{sweep_sets} parameter sets, HEA 16 qubits x 8 layers ({len(sets[0][2])} params), TFIM H",
-            "s": round(t_sw, 4),
-            "grads_per_s": round(sweep_sets / t_sw, 1),
-        }
-    return rec
 
 
 if __name__ == "__main__":

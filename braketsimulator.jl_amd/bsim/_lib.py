@@ -105,6 +105,7 @@ _SIGS = {
     "bsim_set_fusion_limit": (C.c_int, [C.c_int, _ip]),
     "bsim_set_profiling": (C.c_int, [C.c_int]),
     "bsim_kernel_stats": (C.c_int, [C.c_int, _i64p, _dp]),
+    "bsim_stream_probe": (C.c_int, [C.c_int, C.c_int64, C.c_int, C.c_int, _dp]),
     "bsim_dm_kraus": (C.c_int, [C.c_void_p, C.c_int, _ip, C.c_int, _dp, C.c_int]),
     "bsim_dm_phaseflip": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     "bsim_probabilities": (C.c_int, [C.c_void_p, _dp]),

@@ -852,7 +852,7 @@ __global__ __launch_bounds__(256) void k_tile_kraus(double2* __restrict__ sv, in
 
 // The streaming pass (12-bit tiles, 4 register bits): persistent workgroups,
 // two per CU, taking tiles by ticket with the next tile prefetched.
-template <bool LEAN>
+template <bool LEAN, bool KRAUS>
 __device__ __forceinline__ void stream_pass(double2* __restrict__ sv, int vq, const Pass* __restrict__ pass,
                                             const Sweep* __restrict__ sweeps, const TOp* __restrict__ tops,
                                             uint64_t ntiles, unsigned long long* __restrict__ ctr) {
@@ -860,21 +860,28 @@ __device__ __forceinline__ void stream_pass(double2* __restrict__ sv, int vq, co
     __shared__ uint64_t scat_lo[tile_threads<4>()];
     __shared__ uint64_t scat_hi[16];
     __shared__ uint64_t s_next;
-    stream_body<LEAN>(sv, vq, pass, sweeps, tops, ntiles, ctr, tile, scat_lo, scat_hi, &s_next);
+    stream_body<LEAN, KRAUS>(sv, vq, pass, sweeps, tops, ntiles, ctr, tile, scat_lo, scat_hi, &s_next);
 }
 __global__ __launch_bounds__(256, 2) void k_tile_stream(double2* __restrict__ sv, int vq,
                                                         const Pass* __restrict__ pass,
                                                         const Sweep* __restrict__ sweeps,
                                                         const TOp* __restrict__ tops, uint64_t ntiles,
                                                         unsigned long long* __restrict__ ctr) {
-    stream_pass<false>(sv, vq, pass, sweeps, tops, ntiles, ctr);
+    stream_pass<false, false>(sv, vq, pass, sweeps, tops, ntiles, ctr);
 }
 __global__ __launch_bounds__(256, 2) void k_tile_stream_lean(double2* __restrict__ sv, int vq,
                                                              const Pass* __restrict__ pass,
                                                              const Sweep* __restrict__ sweeps,
                                                              const TOp* __restrict__ tops, uint64_t ntiles,
                                                              unsigned long long* __restrict__ ctr) {
-    stream_pass<true>(sv, vq, pass, sweeps, tops, ntiles, ctr);
+    stream_pass<true, false>(sv, vq, pass, sweeps, tops, ntiles, ctr);
+}
+__global__ __launch_bounds__(256, 2) void k_tile_stream_kraus(double2* __restrict__ sv, int vq,
+                                                              const Pass* __restrict__ pass,
+                                                              const Sweep* __restrict__ sweeps,
+                                                              const TOp* __restrict__ tops, uint64_t ntiles,
+                                                              unsigned long long* __restrict__ ctr) {
+    stream_pass<true, true>(sv, vq, pass, sweeps, tops, ntiles, ctr);
 }
 
 }  // namespace bsim
@@ -948,37 +955,50 @@ int upload_plan(const Plan& plan, char* base, hipStream_t s) {
     return BSIM_OK;
 }
 
-// The streaming kernel takes 12-bit, 4-register-bit passes without reference
-// Kraus ops; its lean instantiation those whose ops are all uncontrolled
-// dense 1-/2-register ops and sign ops.
-enum class StreamKind { none, full, lean };
+// The streaming kernels take 12-bit, 4-register-bit passes: the lean one
+// those whose ops are all uncontrolled dense 1-/2-register ops and sign ops,
+// the lean Kraus one the same plus reference Kraus ops (density-matrix
+// plans), the full one any other pass without Kraus ops.
+enum class StreamKind { none, full, lean, lean_kraus };
 
 static StreamKind stream_kind(const Plan& plan, const Pass& P, int vq) {
-    if (P.R != 4 || P.T != 12 || vq < 12 || plan.has_kraus) return StreamKind::none;
+    if (P.R != 4 || P.T != 12 || vq < 12) return StreamKind::none;
+    bool lean = true;
     for (int s = P.sweep_begin; s < P.sweep_begin + P.nsweeps; ++s)
         for (int o = plan.sweeps[s].op_begin; o < plan.sweeps[s].op_end; ++o) {
             const TOp& t = plan.tops[o];
-            if (t.kind != 3 && ((t.kind != 0 && t.kind != 1) || t.cmask != 0)) return StreamKind::full;
+            if (t.kind == 4) {
+                o += t.nd;  // a 4-bit Kraus op's matrices follow its head
+                continue;
+            }
+            if (t.kind != 3 && ((t.kind != 0 && t.kind != 1) || t.cmask != 0)) lean = false;
         }
-    return StreamKind::lean;
+    if (plan.has_kraus) return lean ? StreamKind::lean_kraus : StreamKind::none;
+    return lean ? StreamKind::lean : StreamKind::full;
 }
 
 // Workgroup slots of a streaming kernel on the current device: resident
 // workgroups per CU (two: 69.6 KiB of LDS and up to 256 VGPRs each) times
 // the CU count.
-static int stream_slots(bool lean) {
-    static std::atomic<int> cache[64][2];
+static const void* stream_kernel(StreamKind k) {
+    return k == StreamKind::lean ? (const void*)k_tile_stream_lean
+           : k == StreamKind::lean_kraus ? (const void*)k_tile_stream_kraus
+                                          : (const void*)k_tile_stream;
+}
+
+static int stream_slots(StreamKind kind) {
+    static std::atomic<int> cache[64][4];
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
-    int s = cache[dev][lean].load();
+    int s = cache[dev][(int)kind].load();
     if (s) return s;
-    const void* k = lean ? (const void*)k_tile_stream_lean : (const void*)k_tile_stream;
+    const void* k = stream_kernel(kind);
     int per_cu = 0;
     const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
         &per_cu, k, 256, sizeof(double2) * (size_t)((1 << 12) + (1 << 8)));
     if (e != hipSuccess || per_cu < 1) per_cu = 1;
     s = per_cu * device_cu_count();
-    cache[dev][lean].store(s);
+    cache[dev][(int)kind].store(s);
     return s;
 }
 
@@ -997,11 +1017,13 @@ int launch_plan(bsim_state* h, double2* d, const std::vector<EOp>& eops, const P
             const bool r3 = P.R == 3;
             const StreamKind sk = stream_kind(plan, P, h->vq);
             if (sk != StreamKind::none) {
-                const bool lean = sk == StreamKind::lean;
                 const uint64_t ntiles = grid;
-                const unsigned g = (unsigned)std::min<uint64_t>(ntiles, (uint64_t)stream_slots(lean));
-                hipLaunchKernelGGL(lean ? k_tile_stream_lean : k_tile_stream, dim3(g), dim3(256), shmem, h->stream, d,
-                                   h->vq, (const Pass*)base + st.pass, (const Sweep*)(base + off_s),
+                const unsigned g = (unsigned)std::min<uint64_t>(ntiles, (uint64_t)stream_slots(sk));
+                auto kern = sk == StreamKind::lean ? k_tile_stream_lean
+                            : sk == StreamKind::lean_kraus ? k_tile_stream_kraus
+                                                           : k_tile_stream;
+                hipLaunchKernelGGL(kern, dim3(g), dim3(256), shmem, h->stream, d, h->vq,
+                                   (const Pass*)base + st.pass, (const Sweep*)(base + off_s),
                                    (const TOp*)(base + off_o), ntiles, h->tile_ctr);
             } else {
                 hipLaunchKernelGGL(r3 ? k_tile_r3 : (plan.has_kraus ? k_tile_kraus : k_tile), dim3(grid),
@@ -1033,7 +1055,8 @@ int prepare_tile_kernel() {
     // allow > 64 KiB of dynamic LDS: a padded 13-bit tile (the kernel's
     // static scatter tables take another 2.5 KiB of the 160 KiB)
     for (const void* k : { (const void*)k_tile, (const void*)k_tile_r3, (const void*)k_tile_kraus,
-                           (const void*)k_tile_stream, (const void*)k_tile_stream_lean })
+                           (const void*)k_tile_stream, (const void*)k_tile_stream_lean,
+                           (const void*)k_tile_stream_kraus })
         BSIM_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)(sizeof(double2) * ((1 << 13) + (1 << 9)))));
     g_shmem_set = true;

@@ -119,6 +119,9 @@ static int time_config(bsim_state* h, const std::vector<EOp>& eops, const TileCo
     best = DBL_MAX;
     Plan plan;
     if (int rc = make_plan(h->vq, eops, cfg, plan)) return rc;
+    // one warm-up run, then the best of two timed runs -- or of one when a
+    // run takes over 20 ms (large states: run-to-run spread is well under
+    // the differences between candidates there)
     for (int rep = 0; rep < 3; ++rep) {
         if (int rc = bsim_state_reset(h)) return rc;
         BSIM_HIP(hipEventRecord(e0, h->stream));
@@ -128,6 +131,7 @@ static int time_config(bsim_state* h, const std::vector<EOp>& eops, const TileCo
         float ms = 0;
         BSIM_HIP(hipEventElapsedTime(&ms, e0, e1));
         if (rep > 0 && ms < best) best = ms;
+        if (rep == 1 && ms > 20.f) break;
     }
     return BSIM_OK;
 }

@@ -681,7 +681,8 @@ __device__ __forceinline__ void tile_body(double2* __restrict__ sv, int vq, cons
 //
 // LEAN: the pass holds only uncontrolled dense 1-/2-register ops and sign
 // ops (the brick-wall shape): only those op paths are compiled in, which
-// frees registers and instruction cache for the rest.
+// frees registers and instruction cache for the rest.  LEAN + KRAUS: the
+// same op set plus reference Kraus ops (density-matrix plans).
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ int run_op_lean(SubCube<4>& s, const TOp* __restrict__ op) {
@@ -702,9 +703,45 @@ __device__ __forceinline__ int run_op_lean(SubCube<4>& s, const TOp* __restrict_
     return 1;
 }
 
+// The lean op set plus the reference Kraus ops of density-matrix plans.
+__device__ __forceinline__ int run_op_lean_kraus(SubCube<4>& s, const TOp* __restrict__ op,
+                                                 double2* __restrict__ tile, int pl, int o0, int o1, int o2,
+                                                 int o3) {
+    const OpHead hd = load_head(op);
+    if (hd.kind == 4) {
+        if (hd.nd) {
+            sw_kraus4(s, op, tile, pl, o0, o1, o2, o3);
+            return 1 + hd.nd;
+        }
+        switch (hd.a * 4 + hd.b) {
+            case 1: sw_kraus2<0, 1>(s, op, hd); break;
+            case 2: sw_kraus2<0, 2>(s, op, hd); break;
+            case 3: sw_kraus2<0, 3>(s, op, hd); break;
+            case 6: sw_kraus2<1, 2>(s, op, hd); break;
+            case 7: sw_kraus2<1, 3>(s, op, hd); break;
+            default: sw_kraus2<2, 3>(s, op, hd); break;
+        }
+        return 1;
+    }
+    switch (hd.kind * 16 + hd.a * 4 + hd.b) {
+        case 48: sw_sign<4>(s, hd); break;
+        case 0: sw_dense1_t<4, 0, false>(s, op, hd); break;
+        case 4: sw_dense1_t<4, 1, false>(s, op, hd); break;
+        case 8: sw_dense1_t<4, 2, false>(s, op, hd); break;
+        case 12: sw_dense1_t<4, 3, false>(s, op, hd); break;
+        case 16 + 1: sw_dense2_t<4, 0, 1, false>(s, op, hd); break;
+        case 16 + 2: sw_dense2_t<4, 0, 2, false>(s, op, hd); break;
+        case 16 + 3: sw_dense2_t<4, 0, 3, false>(s, op, hd); break;
+        case 16 + 6: sw_dense2_t<4, 1, 2, false>(s, op, hd); break;
+        case 16 + 7: sw_dense2_t<4, 1, 3, false>(s, op, hd); break;
+        default: sw_dense2_t<4, 2, 3, false>(s, op, hd); break;
+    }
+    return 1;
+}
+
 // LDS: tile (2^12 + 2^8 amplitudes), scat_lo[256], scat_hi[16], s_next.
 // The launch has min(ntiles, slots) workgroups of 256 threads.
-template <bool LEAN>
+template <bool LEAN, bool KRAUS>
 __device__ __forceinline__ void stream_body(double2* __restrict__ sv, int vq, const Pass* __restrict__ pass,
                                             const Sweep* __restrict__ sweeps, const TOp* __restrict__ tops,
                                             uint64_t ntiles, unsigned long long* __restrict__ ctr,
@@ -774,7 +811,9 @@ __device__ __forceinline__ void stream_body(double2* __restrict__ sv, int vq, co
             const int o = ((j & 1) ? o0 : 0) + ((j & 2) ? o1 : 0) + ((j & 4) ? o2 : 0) + ((j & 8) ? o3 : 0);
             setv<j>(sc.v, to_c2(tile[pl + phys(o)]));
         });
-        if constexpr (LEAN) {
+        if constexpr (LEAN && KRAUS) {
+            for (int o = ob; o < oe;) o += run_op_lean_kraus(sc, tops + o, tile, pl, o0, o1, o2, o3);
+        } else if constexpr (LEAN) {
             for (int o = ob; o < oe;) o += run_op_lean(sc, tops + o);
         } else {
             for (int o = ob; o < oe;) o += run_op<R, false>(sc, tops + o, tile, pl, o0, o1, o2, o3);

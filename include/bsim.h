@@ -173,6 +173,15 @@ int bsim_set_fusion_limit(int max_cost, int* old);
 int bsim_set_profiling(int enable);
 int bsim_kernel_stats(int reset, int64_t* launches, double* total_ms);
 
+/* HBM stream probe (no reference counterpart; measurement only): allocates
+ * a buffer of `bytes` (two for a copy), runs a plain streaming kernel over
+ * it -- mode 0 copy (read + write every 16 B element), 1 read (reduce), 2
+ * write (fill) -- `reps` times after one warm-up on `blocks` workgroups of
+ * 256 threads (0 = 8 per CU), and returns the fastest launch in *best_ms.
+ * The ceiling the fused passes and per-gate kernels are judged against.
+ * bytes: a positive multiple of 16 KiB. */
+int bsim_stream_probe(int mode, int64_t bytes, int blocks, int reps, double* best_ms);
+
 /* ---- noise (src/noise_kernels.jl) -------------------------------- */
 
 /* apply_noise!(::Kraus, dm, targets...): src/noise_kernels.jl:105-196.

@@ -50,6 +50,7 @@ _SIGS = {
     "ref_dot": [_dp, _dp, C.c_int64, _dp],
     "ref_marginal_probability": [_dp, C.c_int, C.c_int, _ip, _dp],
     "ref_make_alias_table": [_dp, C.c_int64, C.c_double, _dp, _i64p, _i64p, _i64p],
+    "ref_direct_sample": [_dp, C.c_int64, C.c_double, C.c_int64, _dp, _i64p],
     "ref_alias_lookup": [_dp, _i64p, C.c_int64, _i64p, _dp, _i64p],
     "ref_philox4x32_10": [_u32p, C.c_uint32, C.c_uint32, _u32p],
     "ref_shot_draws": [C.c_uint64, C.c_int, C.c_int64, C.c_int64, _i64p, _dp],
@@ -485,6 +486,16 @@ def alias_reference(p: np.ndarray, wsum: float):
     lib().ref_make_alias_table(_d(np.ascontiguousarray(p)), n, float(wsum), _d(ap), alias.ctypes.data_as(_i64p),
                                lw.ctypes.data_as(_i64p), sw.ctypes.data_as(_i64p))
     return ap, alias - 1
+
+
+def direct_samples(p: np.ndarray, wsum: float, u: np.ndarray) -> np.ndarray:
+    """StatsBase.sample per shot, the reference's n >= 30 sampler
+    (src/result_types.jl:73-78): a linear scan of the running weight sum to
+    u * wsum; 0-based shots."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.empty(u.size, dtype=np.int64)
+    lib().ref_direct_sample(_d(np.ascontiguousarray(p)), p.size, float(wsum), u.size, _d(u), out.ctypes.data_as(_i64p))
+    return out
 
 
 def alias_exact(p: np.ndarray, wsum: float):

@@ -598,6 +598,25 @@ void ref_alias_lookup(const double* ap, const int64_t* alias, int64_t nshots, co
     }
 }
 
+/* result_types.jl:73-78, the n >= 30 branch: one StatsBase.sample(rng,
+ * weights) per shot.  StatsBase.jl 0.34 (Project.toml:45, not vendored)
+ * publishes sample(rng, wv::AbstractWeights) as a linear scan:
+ * t = rand(rng) * sum(wv); i = 1; cw = wv[1]; while cw < t && i < n:
+ * i += 1, cw += wv[i]; return i.  Here the uniforms u are supplied and the
+ * shots are returned 0-based.  Sequential by construction (a running sum). */
+void ref_direct_sample(const double* w, int64_t n, double wsum, int64_t nshots, const double* u, int64_t* out) {
+    for (int64_t s = 0; s < nshots; ++s) {
+        const double t = u[s] * wsum;
+        int64_t i = 1;
+        double cw = w[0];
+        while (cw < t && i < n) {
+            ++i;
+            cw += w[i - 1];
+        }
+        out[s] = i - 1;
+    }
+}
+
 /* BraketSimulator.jl:88-95: shot index -> bits, qubit 0 first. */
 void ref_index_to_endian_bits(int64_t ix, int nq, int32_t* bits) {
     for (int q = 0; q < nq; ++q) bits[nq - 1 - q] = (int32_t)((ix >> q) & 1);

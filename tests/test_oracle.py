@@ -309,6 +309,26 @@ def test_alias_exact_special_cases():
         np.testing.assert_allclose(O.alias_distribution(ap, al), p, atol=1e-9)
 
 
+@pytest.mark.parametrize("n", [1, 4, 10])
+def test_direct_sample_is_cdf_search(n):
+    """StatsBase.sample's linear scan (the reference's n >= 30 sampler) is
+    the first index whose running sum reaches u * sum(w): the same index as a
+    binary search of numpy's sequential cumsum, clamped to the last entry."""
+    p = _probs(n + 7, n)
+    rng = np.random.default_rng(n)
+    u = np.concatenate([rng.random(2000), [0.0, 1.0 - 2 ** -53, 0.5]])
+    got = O.direct_samples(p, float(p.sum()), u)
+    exp = np.minimum(np.searchsorted(np.cumsum(p), u * p.sum(), side="left"), p.size - 1)
+    np.testing.assert_array_equal(got, exp)
+
+
+def test_direct_sample_hand_cases():
+    w = np.array([0.0, 0.25, 0.0, 0.5, 0.25])
+    u = np.array([0.0, 0.1, 0.25, 0.26, 0.74, 0.75, 0.76, 0.999])
+    # t = u: index 0 only at t = 0 (cw = 0 is not < 0); zero weights are skipped otherwise
+    np.testing.assert_array_equal(O.direct_samples(w, 1.0, u), [0, 1, 1, 3, 3, 3, 4, 4])
+
+
 def test_shot_rule():
     ap = np.array([1.0, 0.25, 0.5, 1.0])
     al = np.array([0, 3, 0, 3])
