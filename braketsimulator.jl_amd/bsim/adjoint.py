@@ -55,7 +55,9 @@ def with_params(instructions, param_map, theta):
 
 def adjoint_gradient(sim, instructions, param_map, words, coeffs, n_params=None, checkpoint_every=0, fused=True):
     """(<H>, dH/dtheta) for the circuit from |0...0>.  ``param_map`` maps
-    (instruction index, angle index) -> parameter index.  On a
+    (instruction index, angle index) -> parameter index.  ``instructions``
+    may also be an op list lowered with that map (``lower_instructions(...,
+    param_map=...)``), which skips the lowering.  On a
     ``DensityMatrixSimulator`` the circuit may hold noise; the forward pass is
     checkpointed every ``checkpoint_every`` ops (0: ceil(sqrt(ops))) and
     recomputed segment by segment.  State vectors: ``fused`` runs the
@@ -63,8 +65,11 @@ def adjoint_gradient(sim, instructions, param_map, words, coeffs, n_params=None,
     kernel per op.  The simulator keeps the final state."""
     n_params = (max(param_map.values()) + 1 if param_map else 0) if n_params is None else n_params
     kind = getattr(sim, "kind", L.BSIM_SV)
-    ol = lower_instructions(list(instructions), kind, noise_mode=getattr(sim, "noise_mode", "reference"),
-                            param_map=param_map)
+    if hasattr(instructions, "ptr"):  # already lowered (with param_map), e.g. reused across calls
+        ol = instructions
+    else:
+        ol = lower_instructions(list(instructions), kind, noise_mode=getattr(sim, "noise_mode", "reference"),
+                                param_map=param_map)
     old = C.c_int()
     L.call("bsim_set_checkpoint_interval", int(checkpoint_every), C.byref(old))
     old_f = C.c_int()

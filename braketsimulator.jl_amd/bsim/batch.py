@@ -42,7 +42,8 @@ class GradientTask:
     """One parameter set of a sweep: the adjoint gradient of <H> =
     sum_w coeffs[w] <P_w> for ``instructions`` from |0...0>, with
     ``param_map`` as for adjoint.adjoint_gradient ((instruction index,
-    angle index) -> parameter index)."""
+    angle index) -> parameter index).  ``instructions`` may be an op list
+    already lowered with that map."""
 
     qubit_count: int
     instructions: list
@@ -73,7 +74,10 @@ def run_batch(circuits, shots=1000, seeds=None):
             nq = item.qubit_count
             pm = item.param_map
             npar = (max(pm.values()) + 1 if pm else 0) if item.n_params is None else int(item.n_params)
-            ol = lower_instructions(list(item.instructions), L.BSIM_SV, param_map=pm)
+            if hasattr(item.instructions, "ptr"):  # lowered with param_map already
+                ol = item.instructions
+            else:
+                ol = lower_instructions(list(item.instructions), L.BSIM_SV, param_map=pm)
             for w in item.words:
                 if len(w) != nq:
                     raise ValueError(f"Pauli word {w!r} must have {nq} letters")

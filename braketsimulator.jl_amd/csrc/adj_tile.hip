@@ -38,6 +38,23 @@ __global__ __launch_bounds__(AT_TB) void k_adj_tile(double2* __restrict__ psi, d
     adj_tile_body(psi, lam, vq, pass, sweeps, tops, partials, pm, ntiles, blockIdx.x, lds, scat_lo, scat_hi);
 }
 
+// 11-bit passes: persistent workgroups (two per CU) taking tiles by ticket,
+// the next tile prefetched during the last sweep.
+__global__ __launch_bounds__(AT_TB, 2) void k_adj_tile_stream(double2* __restrict__ psi, double2* __restrict__ lam,
+                                                              int vq, const Pass* __restrict__ pass,
+                                                              const Sweep* __restrict__ sweeps,
+                                                              const TOp* __restrict__ tops,
+                                                              double* __restrict__ partials, int pm, int64_t ntiles,
+                                                              uint64_t pass_tiles,
+                                                              unsigned long long* __restrict__ ctr) {
+    extern __shared__ double2 lds[];
+    __shared__ uint64_t scat_lo[AT_TB];
+    __shared__ uint64_t scat_hi[64];
+    __shared__ uint64_t s_next;
+    adj_stream_body(psi, lam, vq, pass, sweeps, tops, partials, pm, ntiles, pass_tiles, ctr, lds, scat_lo, scat_hi,
+                    &s_next);
+}
+
 // out[slot * 3 + p] = sum over the n partial sums of row slot * pm + p, in a
 // fixed order.
 __global__ __launch_bounds__(256) void k_adj_tile_finish(const double* __restrict__ partials, int64_t n, int pm,
@@ -107,10 +124,21 @@ int adjoint_back_plan(int vq, const std::vector<EOp>& eops, std::vector<EOp>& ba
 
 static std::atomic<bool> g_adj_shmem{ false };
 
+static std::atomic<int> g_adj_slots{ 0 };
+
 static int adjoint_tile_prepare() {
     if (g_adj_shmem) return BSIM_OK;
-    BSIM_HIP(hipFuncSetAttribute((const void*)k_adj_tile, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)(2 * sizeof(double2) * ((1 << 11) + (1 << 7)))));
+    const int shmem = (int)(2 * sizeof(double2) * ((1 << 11) + (1 << 7)));
+    for (const void* k : { (const void*)k_adj_tile, (const void*)k_adj_tile_stream })
+        BSIM_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, shmem));
+    // workgroup slots of the streaming kernel: resident workgroups per CU
+    // (two: 72 KiB of LDS each) times the CU count
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_adj_tile_stream, AT_TB, shmem) !=
+            hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    g_adj_slots = per_cu * device_cu_count();
     g_adj_shmem = true;
     return BSIM_OK;
 }
@@ -150,9 +178,16 @@ int adjoint_tile_run(bsim_state* h, double2* psi, double2* lam, const Plan& plan
             const Pass& P = plan.passes[st.pass];
             const unsigned grid = (unsigned)(1ull << (h->vq - P.T));
             const size_t shmem = 2 * sizeof(double2) * (size_t)((1 << P.T) + ((1 << P.T) >> 4));
-            hipLaunchKernelGGL(k_adj_tile, dim3(grid), dim3(AT_TB), shmem, h->stream, psi, lam, h->vq,
-                               (const Pass*)tables + st.pass, (const Sweep*)(tables + off_s),
-                               (const TOp*)(tables + off_o), partials, pm, nt);
+            if (P.T == 11 && h->vq >= 11) {
+                const unsigned g = std::min<unsigned>(grid, (unsigned)g_adj_slots.load());
+                hipLaunchKernelGGL(k_adj_tile_stream, dim3(g), dim3(AT_TB), shmem, h->stream, psi, lam, h->vq,
+                                   (const Pass*)tables + st.pass, (const Sweep*)(tables + off_s),
+                                   (const TOp*)(tables + off_o), partials, pm, nt, (uint64_t)grid, h->tile_ctr);
+            } else {
+                hipLaunchKernelGGL(k_adj_tile, dim3(grid), dim3(AT_TB), shmem, h->stream, psi, lam, h->vq,
+                                   (const Pass*)tables + st.pass, (const Sweep*)(tables + off_s),
+                                   (const TOp*)(tables + off_o), partials, pm, nt);
+            }
             BSIM_HIP(hipGetLastError());
         } else {
             if (int rc = single(st.op)) return rc;

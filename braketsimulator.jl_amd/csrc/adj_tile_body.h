@@ -217,6 +217,91 @@ __device__ __forceinline__ void run_op_both(SubCube<3>& sp, SubCube<3>& sl, cons
     }
 }
 
+// One sweep of a backward pass over the tile in LDS (tp: psi, tl: lambda,
+// 2^T amplitudes each, T <= 11): every thread carries one sub-cube of each
+// through the sweep's ops and writes its wave's product sums to `partials`
+// under `tile_index`.  Ends with a barrier.
+__device__ __forceinline__ void adj_sweep(const Pass* __restrict__ pass, const Sweep* __restrict__ sweeps,
+                                          const TOp* __restrict__ tops, int sidx, uint64_t base, int nloc,
+                                          double2* __restrict__ tp, double2* __restrict__ tl,
+                                          const uint64_t* __restrict__ scat_lo, const uint64_t* __restrict__ scat_hi,
+                                          double* __restrict__ partials, int pm, int64_t ntiles,
+                                          uint64_t tile_index) {
+    const int nsub = nloc >> 3;
+    const int c = threadIdx.x;
+    const bool valid = c < nsub;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const Sweep* sw = sweeps + pass->sweep_begin + sidx;
+    const int sb0 = sw->sb[0], sb1 = sw->sb[1], sb2 = sw->sb[2];
+    const int ob = sw->op_begin, oe = sw->op_end;
+    auto gl = [&](int l) { return scat_lo[(1 << l) & (AT_TB - 1)] | scat_hi[(1 << l) >> AT_BITS]; };
+    const uint64_t g0 = gl(sb0), g1 = gl(sb1), g2 = gl(sb2);
+    const int o0 = 1 << sb0, o1 = 1 << sb1, o2 = 1 << sb2;
+    auto ins = [](int x, int b) { const int lo = x & ((1 << b) - 1); return ((x - lo) << 1) | lo; };
+    const int lb = valid ? ins(ins(ins(c, sb0), sb1), sb2) : 0;
+    const int pl = phys(lb);
+    SubCube<3> sp, sl;
+    sp.gbase = sl.gbase = base | scat_lo[lb & (AT_TB - 1)] | scat_hi[lb >> AT_BITS];
+    sp.g0 = sl.g0 = g0;
+    sp.g1 = sl.g1 = g1;
+    sp.g2 = sl.g2 = g2;
+    sp.g3 = sl.g3 = 0;
+    sfor<0, 8>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        const int o = ((j & 1) ? o0 : 0) + ((j & 2) ? o1 : 0) + ((j & 4) ? o2 : 0);
+        setv<j>(sp.v, valid ? to_c2(tp[pl + phys(o)]) : c2{ 0.0, 0.0 });
+        setv<j>(sl.v, valid ? to_c2(tl[pl + phys(o)]) : c2{ 0.0, 0.0 });
+    });
+    for (int o = ob; o < oe;) {
+        const TOp* op = tops + o;
+        const int np = op->nparam;
+        run_op_both(sp, sl, op);
+        for (int p = 0; p < np; ++p) {
+            const TOp* d = op + 1 + p;
+            const double v = wave_sum63(adj_dot(sp, sl, op, d));
+            if (lane == 63) partials[((size_t)(d->a * pm + d->b) * ntiles + tile_index) * AT_W + wave] = v;
+        }
+        o += 1 + np;
+    }
+    if (valid) {
+        sfor<0, 8>([&](auto J) {
+            constexpr int j = decltype(J)::value;
+            const int o = ((j & 1) ? o0 : 0) + ((j & 2) ? o1 : 0) + ((j & 4) ? o2 : 0);
+            tp[pl + phys(o)] = to_d2(getv<j>(sp.v));
+            tl[pl + phys(o)] = to_d2(getv<j>(sl.v));
+        });
+    }
+    __syncthreads();
+}
+
+// The scatter tables of a pass: scat_lo[t] the vector bits of local bits
+// 0..AT_BITS-1 of t, scat_hi[h] those of local bits AT_BITS.. of h.
+__device__ __forceinline__ void adj_scatter(const Pass* __restrict__ pass, int T, uint64_t* __restrict__ scat_lo,
+                                            uint64_t* __restrict__ scat_hi) {
+    const int t = threadIdx.x;
+    uint64_t s = 0;
+    for (int b = 0; b < AT_BITS && b < T; ++b)
+        if ((t >> b) & 1) s |= 1ull << pass->lbits[b];
+    scat_lo[t] = s;
+    if (t < 64) {
+        uint64_t h = 0;
+        for (int b = AT_BITS; b < T; ++b)
+            if ((t >> (b - AT_BITS)) & 1) h |= 1ull << pass->lbits[b];
+        scat_hi[t] = h;
+    }
+}
+
+// Global base of tile w: w deposited into the non-tile bits.
+__device__ __forceinline__ uint64_t adj_tile_base(uint64_t lmask, int vq, uint64_t w) {
+    uint64_t base = 0;
+    for (int bit = 0, k = 0; bit < vq; ++bit)
+        if (!((lmask >> bit) & 1)) {
+            base |= ((w >> k) & 1) << bit;
+            ++k;
+        }
+    return base;
+}
+
 // One tile of one backward pass (T <= 11 bits, 3 register bits per sweep),
 // AT_TB threads.  LDS: `lds` holds the two tiles (2 (2^T + 2^T/16)
 // amplitudes), scat_lo[AT_TB] and scat_hi[64] the scatter tables.
@@ -230,26 +315,8 @@ __device__ __forceinline__ void adj_tile_body(double2* __restrict__ psi, double2
     const int stride_t = nloc + (nloc >> 4);
     double2* tp = lds;             // psi tile
     double2* tl = lds + stride_t;  // lambda tile
-    const uint64_t lmask = pass->lmask;
-    {
-        const int t = threadIdx.x;
-        uint64_t s = 0;
-        for (int b = 0; b < AT_BITS && b < T; ++b)
-            if ((t >> b) & 1) s |= 1ull << pass->lbits[b];
-        scat_lo[t] = s;
-        if (t < 64) {
-            uint64_t h = 0;
-            for (int b = AT_BITS; b < T; ++b)
-                if ((t >> (b - AT_BITS)) & 1) h |= 1ull << pass->lbits[b];
-            scat_hi[t] = h;
-        }
-    }
-    uint64_t w = tile_index, base = 0;
-    for (int bit = 0, k = 0; bit < vq; ++bit)
-        if (!((lmask >> bit) & 1)) {
-            base |= ((w >> k) & 1) << bit;
-            ++k;
-        }
+    adj_scatter(pass, T, scat_lo, scat_hi);
+    const uint64_t base = adj_tile_base(pass->lmask, vq, tile_index);
     __syncthreads();
     const uint64_t s_t = base | scat_lo[threadIdx.x];
     const int iters = nloc >> AT_BITS;
@@ -266,54 +333,8 @@ __device__ __forceinline__ void adj_tile_body(double2* __restrict__ psi, double2
         tl[l] = lam[g];
     }
     __syncthreads();
-    const int nsub = nloc >> 3;
-    const int c = threadIdx.x;
-    const bool valid = c < nsub;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int sidx = 0; sidx < pass->nsweeps; ++sidx) {
-        const Sweep* sw = sweeps + pass->sweep_begin + sidx;
-        const int sb0 = sw->sb[0], sb1 = sw->sb[1], sb2 = sw->sb[2];
-        const int ob = sw->op_begin, oe = sw->op_end;
-        auto gl = [&](int l) { return scat_lo[(1 << l) & (AT_TB - 1)] | scat_hi[(1 << l) >> AT_BITS]; };
-        const uint64_t g0 = gl(sb0), g1 = gl(sb1), g2 = gl(sb2);
-        const int o0 = 1 << sb0, o1 = 1 << sb1, o2 = 1 << sb2;
-        auto ins = [](int x, int b) { const int lo = x & ((1 << b) - 1); return ((x - lo) << 1) | lo; };
-        const int lb = valid ? ins(ins(ins(c, sb0), sb1), sb2) : 0;
-        const int pl = phys(lb);
-        SubCube<3> sp, sl;
-        sp.gbase = sl.gbase = base | scat_lo[lb & (AT_TB - 1)] | scat_hi[lb >> AT_BITS];
-        sp.g0 = sl.g0 = g0;
-        sp.g1 = sl.g1 = g1;
-        sp.g2 = sl.g2 = g2;
-        sp.g3 = sl.g3 = 0;
-        sfor<0, 8>([&](auto J) {
-            constexpr int j = decltype(J)::value;
-            const int o = ((j & 1) ? o0 : 0) + ((j & 2) ? o1 : 0) + ((j & 4) ? o2 : 0);
-            setv<j>(sp.v, valid ? to_c2(tp[pl + phys(o)]) : c2{ 0.0, 0.0 });
-            setv<j>(sl.v, valid ? to_c2(tl[pl + phys(o)]) : c2{ 0.0, 0.0 });
-        });
-        for (int o = ob; o < oe;) {
-            const TOp* op = tops + o;
-            const int np = op->nparam;
-            run_op_both(sp, sl, op);
-            for (int p = 0; p < np; ++p) {
-                const TOp* d = op + 1 + p;
-                const double v = wave_sum63(adj_dot(sp, sl, op, d));
-                if (lane == 63)
-                    partials[((size_t)(d->a * pm + d->b) * ntiles + tile_index) * AT_W + wave] = v;
-            }
-            o += 1 + np;
-        }
-        if (valid) {
-            sfor<0, 8>([&](auto J) {
-                constexpr int j = decltype(J)::value;
-                const int o = ((j & 1) ? o0 : 0) + ((j & 2) ? o1 : 0) + ((j & 4) ? o2 : 0);
-                tp[pl + phys(o)] = to_d2(getv<j>(sp.v));
-                tl[pl + phys(o)] = to_d2(getv<j>(sl.v));
-            });
-        }
-        __syncthreads();
-    }
+    for (int sidx = 0; sidx < pass->nsweeps; ++sidx)
+        adj_sweep(pass, sweeps, tops, sidx, base, nloc, tp, tl, scat_lo, scat_hi, partials, pm, ntiles, tile_index);
     if (iters == 0) {
         if ((int)threadIdx.x < nloc) {
             psi[s_t] = tp[phys(threadIdx.x)];
@@ -325,6 +346,81 @@ __device__ __forceinline__ void adj_tile_body(double2* __restrict__ psi, double2
         const int l = phys((int)threadIdx.x + (i << AT_BITS));
         psi[g] = tp[l];
         lam[g] = tl[l];
+    }
+}
+
+// The streaming backward pass (T = 11): a persistent workgroup per slot
+// takes tiles by ticket -- its first tile is its index, then each ticket v
+// = atomicAdd(ctr, 1) names tile gridDim.x + v, and the workgroup drawing
+// the pass's last ticket (v = ntiles - 1) resets the counter -- and loads
+// the next tile's psi and lambda rows into registers while the current
+// tile runs its last sweep.  The same sweeps, products and partial slots
+// as adj_tile_body, tile by tile.  LDS as adj_tile_body plus s_next.
+__device__ __forceinline__ void adj_stream_body(double2* __restrict__ psi, double2* __restrict__ lam, int vq,
+                                                const Pass* __restrict__ pass, const Sweep* __restrict__ sweeps,
+                                                const TOp* __restrict__ tops, double* __restrict__ partials, int pm,
+                                                int64_t ntiles, uint64_t pass_tiles,
+                                                unsigned long long* __restrict__ ctr, double2* __restrict__ lds,
+                                                uint64_t* __restrict__ scat_lo, uint64_t* __restrict__ scat_hi,
+                                                uint64_t* s_next) {
+    constexpr int T = 11, NR = 1 << (T - AT_BITS);  // 8 rows of AT_TB amplitudes
+    constexpr int nloc = 1 << T, stride_t = nloc + (nloc >> 4);
+    double2* tp = lds;
+    double2* tl = lds + stride_t;
+    adj_scatter(pass, T, scat_lo, scat_hi);
+    const uint64_t lmask = pass->lmask;
+    __syncthreads();
+    const int nsw = pass->nsweeps;
+    dvec<2 * NR> pp, pq;  // the next tile's psi and lambda rows (row i = local tid + AT_TB i)
+    auto load = [&](uint64_t b) {
+        const uint64_t s_t = b | scat_lo[threadIdx.x];
+        sfor<0, NR>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            const uint64_t g = s_t | scat_hi[i];
+            setv<i>(pp, to_c2(psi[g]));
+            setv<i>(pq, to_c2(lam[g]));
+        });
+    };
+    auto to_lds = [&]() {
+        sfor<0, NR>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            const int l = phys((int)threadIdx.x + (i << AT_BITS));
+            tp[l] = to_d2(getv<i>(pp));
+            tl[l] = to_d2(getv<i>(pq));
+        });
+        __syncthreads();
+    };
+    uint64_t t = blockIdx.x;  // < pass_tiles (grid <= pass_tiles)
+    uint64_t base = adj_tile_base(lmask, vq, t);
+    load(base);
+    to_lds();
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const unsigned long long v = atomicAdd(ctr, 1ull);
+            if (v == pass_tiles - 1) atomicExch(ctr, 0ull);  // the pass's last ticket
+            *s_next = gridDim.x + v;
+        }
+        __syncthreads();
+        const uint64_t nt = *s_next;  // rewritten only after the store barrier below
+        const uint64_t nbase = nt < pass_tiles ? adj_tile_base(lmask, vq, nt) : base;
+        for (int sidx = 0; sidx + 1 < nsw; ++sidx)
+            adj_sweep(pass, sweeps, tops, sidx, base, nloc, tp, tl, scat_lo, scat_hi, partials, pm, ntiles, t);
+        load(nbase);  // in flight during the last sweep and the stores
+        if (nsw > 0)
+            adj_sweep(pass, sweeps, tops, nsw - 1, base, nloc, tp, tl, scat_lo, scat_hi, partials, pm, ntiles, t);
+        const uint64_t s_t = base | scat_lo[threadIdx.x];
+        sfor<0, NR>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            const int l = phys((int)threadIdx.x + (i << AT_BITS));
+            const uint64_t g = s_t | scat_hi[i];
+            psi[g] = tp[l];
+            lam[g] = tl[l];
+        });
+        __syncthreads();  // the next tile rewrites LDS
+        if (nt >= pass_tiles) break;
+        t = nt;
+        base = nbase;
+        to_lds();
     }
 }
 

@@ -112,14 +112,39 @@ __global__ __launch_bounds__(RBLOCK) void k_zsums_sv(const double2* __restrict__
     // Layout of v: [0..16) sums of bits lstride+b (the bits of the iteration
     // count k), [16..36) sums of bits 0..19 (thread constants), [36] total.
     const int hi_bits = vq > lstride ? vq - lstride : 0;
-    int64_t k = 0;
-    for (int64_t i = tid; i < len; i += stride, ++k) {
-        double2 a = sv[i];
-        double p = a.x * a.x + a.y * a.y;
-        v[36] += p;
+    if (hi_bits >= 3) {
+        // eight iterations per trip, their loads issued together: bits 0-2
+        // of the iteration count vary inside a trip (fixed pattern), the
+        // higher bits are the trip's, so those sums take the trip total
+        const int64_t iters = (int64_t)1 << hi_bits;
+        const double2* __restrict__ base = sv + tid;
+        for (int64_t k = 0; k < iters; k += 8) {
+            double2 a[8];
 #pragma unroll
-        for (int b = 0; b < 16; ++b)
-            if (b < hi_bits && ((k >> b) & 1)) v[b] += p;
+            for (int u = 0; u < 8; ++u) a[u] = base[(k + u) << lstride];
+            double p[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) p[u] = a[u].x * a[u].x + a[u].y * a[u].y;
+            const double s01 = p[0] + p[1], s23 = p[2] + p[3], s45 = p[4] + p[5], s67 = p[6] + p[7];
+            const double tot = (s01 + s23) + (s45 + s67);
+            v[36] += tot;
+            v[0] += (p[1] + p[3]) + (p[5] + p[7]);
+            v[1] += (p[2] + p[3]) + (p[6] + p[7]);
+            v[2] += s45 + s67;
+#pragma unroll
+            for (int b = 3; b < 16; ++b)
+                if (b < hi_bits && ((k >> b) & 1)) v[b] += tot;
+        }
+    } else {
+        int64_t k = 0;
+        for (int64_t i = tid; i < len; i += stride, ++k) {
+            double2 a = sv[i];
+            double p = a.x * a.x + a.y * a.y;
+            v[36] += p;
+#pragma unroll
+            for (int b = 0; b < 16; ++b)
+                if (b < hi_bits && ((k >> b) & 1)) v[b] += p;
+        }
     }
 #pragma unroll
     for (int b = 0; b < 20; ++b)

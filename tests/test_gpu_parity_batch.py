@@ -128,17 +128,20 @@ def test_batch_gradient_sweep_matches_serial():
     angles) as gradient tasks in the same launch as ordinary circuits: every
     batched gradient equals bsim_adjoint_gradient on the same circuit within
     1e-10 (the backward plans and per-wave sums are the same), <H> too, and
-    the circuits' results are unchanged by the company."""
+    the circuits' results are unchanged by the company.  Odd tasks pass op
+    lists lowered once (with the parameter map) to both paths."""
     from bsim import adjoint as A
     from bsim.batch import GradientTask
+    from bsim.simulators import lower_instructions
 
     items, serial = [], []
     for k, n in enumerate([5, 6, 8, 11, 12, 13, 16, 16]):
         ins, pmap, th = W.hea(n, 3 if n < 16 else 8, seed=40 + k)
         words, coeffs = W.tfim(n, hx=0.3 + 0.1 * k)
-        items.append(GradientTask(n, ins, pmap, words, coeffs))
+        src = lower_instructions(list(ins), L.BSIM_SV, param_map=pmap) if k % 2 else ins
+        items.append(GradientTask(n, src, pmap, words, coeffs))
         sim = StateVectorSimulator(n, 0)
-        serial.append(A.adjoint_gradient(sim, ins, pmap, words, coeffs))
+        serial.append(A.adjoint_gradient(sim, src, pmap, words, coeffs))
     spec = W.batch_tasks(40, seed=3, qmin=8, qmax=16)
     circuits = [(n, W.brickwall(n, d, seed=s)) for n, d, s in spec]
     mixed = []
