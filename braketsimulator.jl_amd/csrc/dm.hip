@@ -4,7 +4,8 @@
 // 2^(n-k) padded indices p and replaces the block rho[S(p), S(p)] -- the
 // rows and columns that agree on every non-target bit -- by
 // sum_m K_m B K_m^dagger (src/noise_kernels.jl:105-196).  One workgroup
-// owns one block; the d x d block and each product stay in LDS.
+// owns one block; the d x d block and each product stay in LDS (k <= 5) or,
+// for k = 6..8, in a slice of device memory of the workgroup's own.
 //
 // Physical semantics (mode 1) and PhaseFlip are lowered by the planner to a
 // superoperator / a 2-bit diagonal on the 4^n vector and run through the
@@ -24,11 +25,13 @@ struct KrausArgs {
     int od[8];   // sorted endian targets
 };
 
+template <bool GLOBAL>
 __global__ __launch_bounds__(1024) void k_kraus_ref(double2* __restrict__ dm, int nq, KrausArgs a, int nK,
-                                                    const double2* __restrict__ K) {
-    extern __shared__ double2 sh[];
+                                                    const double2* __restrict__ K, double2* __restrict__ work) {
+    extern __shared__ double2 lds[];
     const int k = a.k;
     const int d = 1 << k, dd = d * d;
+    double2* sh = GLOBAL ? work + (size_t)blockIdx.x * 3 * dd : lds;
     double2* B = sh;
     double2* Tm = sh + dd;
     double2* O = sh + 2 * dd;
@@ -78,7 +81,7 @@ __global__ __launch_bounds__(1024) void k_kraus_ref(double2* __restrict__ dm, in
 
 int launch_kraus_ref(bsim_state* h, double2* d, const EOp& e) {
     const int k = e.kq, nq = h->nq;
-    if (k > 5) return fail(BSIM_ENOTSUP, "Kraus channels act on at most 5 qubits");
+    if (k > 8) return fail(BSIM_ENOTSUP, "Kraus channels act on at most 8 qubits");
     const int D = 1 << k, dd = D * D;
     KrausArgs a;
     memset(&a, 0, sizeof(a));
@@ -90,9 +93,19 @@ int launch_kraus_ref(bsim_state* h, double2* d, const EOp& e) {
     BSIM_HIP(hipMemcpyAsync(h->scratch.ptr, e.kraus.data(), bytes, hipMemcpyHostToDevice, h->stream));
     int64_t nblk = ((int64_t)1 << nq) >> k;
     int block = dd < 64 ? 64 : (dd > 1024 ? 1024 : dd);
-    int64_t grid = nblk < 65536 ? nblk : 65536;
-    hipLaunchKernelGGL(k_kraus_ref, dim3((unsigned)grid), dim3(block), sizeof(double2) * 3 * dd, h->stream, d, nq, a,
-                       e.nkraus, (const double2*)h->scratch.ptr);
+    if (k <= 5) {
+        int64_t grid = nblk < 65536 ? nblk : 65536;
+        hipLaunchKernelGGL(k_kraus_ref<false>, dim3((unsigned)grid), dim3(block), sizeof(double2) * 3 * dd, h->stream,
+                           d, nq, a, e.nkraus, (const double2*)h->scratch.ptr, (double2*)nullptr);
+    } else {
+        // block, product and sum of each workgroup in device memory (3 d^2
+        // amplitudes per workgroup: 12 MiB for 64 workgroups at k = 6)
+        const int64_t grid = nblk < 64 ? nblk : 64;
+        if (int rc = ensure_ws(h->alias_work, sizeof(double2) * 3 * (size_t)dd * (size_t)grid)) return rc;
+        h->alias_ready = false;
+        hipLaunchKernelGGL(k_kraus_ref<true>, dim3((unsigned)grid), dim3(block), 0, h->stream, d, nq, a, e.nkraus,
+                           (const double2*)h->scratch.ptr, (double2*)h->alias_work.ptr);
+    }
     BSIM_HIP(hipGetLastError());
     return BSIM_OK;
 }
@@ -106,7 +119,7 @@ extern "C" {
 int bsim_dm_kraus(bsim_state* h, int k, const int* targets, int n_kraus, const double* kraus, int mode) {
     if (int rc = check_state(h)) return rc;
     if (h->kind != BSIM_DM) return fail(BSIM_EINVAL, "Kraus channels need a density matrix");
-    if (k < 1 || k > 5 || !targets || !kraus || n_kraus < 1) return fail(BSIM_EINVAL, "bad Kraus arguments");
+    if (k < 1 || k > 8 || !targets || !kraus || n_kraus < 1) return fail(BSIM_EINVAL, "bad Kraus arguments");
     bsim_op op = {};
     op.type = mode ? BSIM_OP_KRAUS_PHYS : BSIM_OP_KRAUS_REF;
     op.n_targets = k;
@@ -125,7 +138,7 @@ int bsim_dm_kraus(bsim_state* h, int k, const int* targets, int n_kraus, const d
 int bsim_dm_apply_superop(bsim_state* h, int k, const int* targets, const double* S) {
     if (int rc = check_state(h)) return rc;
     if (h->kind != BSIM_DM) return fail(BSIM_EINVAL, "superoperators need a density matrix");
-    if (k < 1 || k > 3 || !targets || !S) return fail(BSIM_EINVAL, "bad superoperator arguments");
+    if (k < 1 || k > 5 || !targets || !S) return fail(BSIM_EINVAL, "bad superoperator arguments");
     bsim_op op = {};
     op.type = BSIM_OP_SUPEROP;
     op.n_targets = k;

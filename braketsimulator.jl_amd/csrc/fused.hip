@@ -139,7 +139,7 @@ int lower_ops(int nq, int kind, const bsim_op* ops, int n_ops, std::vector<EOp>&
                 if (int rc = make_spec(vq, 2, t, 0, nullptr, nullptr, d, true, false, false, e.g)) return rc;
                 out.push_back(std::move(e));
             } else if (op.type == BSIM_OP_KRAUS_REF) {
-                if (k > 5) return fail(BSIM_ENOTSUP, "Kraus channels act on at most 5 qubits");
+                if (k > 8) return fail(BSIM_ENOTSUP, "Kraus channels act on at most 8 qubits");
                 if (op.n_kraus < 1) return fail(BSIM_EINVAL, "Kraus channel without matrices");
                 EOp e;
                 e.kind = E_KRAUS_REF;
@@ -186,7 +186,7 @@ int lower_ops(int nq, int kind, const bsim_op* ops, int n_ops, std::vector<EOp>&
                 out.push_back(std::move(e));
             } else {
                 // physical channel: superoperator sum_m K_m (column bits) (x) conj(K_m) (row bits)
-                if (k > 3) return fail(BSIM_ENOTSUP, "physical Kraus channels act on at most 3 qubits");
+                if (k > 5) return fail(BSIM_ENOTSUP, "physical Kraus channels act on at most 5 qubits");
                 if (op.n_kraus < 1) return fail(BSIM_EINVAL, "Kraus channel without matrices");
                 int D = 1 << k, DD = D * D;
                 // channel matrix bit b -> qubit: k=1 (q); k=2 t1 = MSB (noise_kernels.jl:145-155);
@@ -212,7 +212,7 @@ int lower_ops(int nq, int kind, const bsim_op* ops, int n_ops, std::vector<EOp>&
                             sup[idx + 1] += ar * bi + ai * br;
                         }
                 }
-                int t[8];
+                int t[10];
                 for (int b = 0; b < k; ++b) {
                     t[b] = qb[b] + nq;  // row bit of the channel's matrix bit b
                     t[k + b] = qb[b];   // column bit
@@ -230,8 +230,8 @@ int lower_ops(int nq, int kind, const bsim_op* ops, int n_ops, std::vector<EOp>&
             // column index c on vector qubits nq + t (storage is rho^T)
             if (kind != BSIM_DM) return fail(BSIM_EINVAL, "superoperators need a density matrix");
             if (nc) return fail(BSIM_EINVAL, "superoperators take no controls");
-            if (k > 3) return fail(BSIM_ENOTSUP, "superoperators act on at most 3 qubits");
-            int t[6];
+            if (k > 5) return fail(BSIM_ENOTSUP, "superoperators act on at most 5 qubits");
+            int t[10];
             for (int b = 0; b < k; ++b) {
                 t[b] = op.targets[b];
                 t[k + b] = op.targets[b] + nq;

@@ -11,7 +11,7 @@ namespace bsim {
 
 struct GateSpec {
     int k = 0;                 // number of matrix bits
-    int tb[8] = {0};           // vector bit position of matrix bit b
+    int tb[16] = {0};          // vector bit position of matrix bit b
     uint64_t cmask = 0;        // control bits (vector positions)
     uint64_t cval = 0;         // required values of the control bits
     bool diag = false;         // m holds 2^k diagonal entries

@@ -34,8 +34,9 @@ namespace bsim {
 
 int make_spec(int vq, int k, const int* targets, int nc, const int* controls, const int* cvals, const double* m,
               bool diag, bool conj, bool sorted, GateSpec& g) {
-    if (k < 1 || k > 8) return fail(BSIM_ENOTSUP, "gate target count must be 1..8");
+    if (k < 1 || k > (diag ? 8 : 12)) return fail(BSIM_ENOTSUP, "gate target count must be 1..12 (diagonal 1..8)");
     if (nc < 0 || nc > BSIM_MAX_CONTROLS) return fail(BSIM_ENOTSUP, "at most 8 controls");
+    if (k + nc > 16) return fail(BSIM_ENOTSUP, "at most 16 targets and controls together");
     if (!m) return fail(BSIM_EINVAL, "matrix pointer is null");
     uint64_t used = 0;
     g = GateSpec();
@@ -182,6 +183,76 @@ __global__ __launch_bounds__(256) void k_dense_lds(double2* __restrict__ sv, Ins
             double2 acc = cmul(M[r], col[0]);
             for (int c = 1; c < D; ++c) acc = cfma(M[r + D * c], col[c], acc);
             sv[off] = acc;
+        }
+    }
+}
+
+// K = 7..12 targets (the k-qubit fallback beyond the matrix-core kernels).
+// A workgroup holds GB groups (GB D = 2048 amplitudes, or one group of D =
+// 4096) in LDS and computes NOUT = GB D / 256 outputs per thread: for D <=
+// 256 thread t owns row t % D of NOUT groups, so each matrix element it
+// reads (column-major, consecutive threads on consecutive rows) serves NOUT
+// outputs; for D > 256 it owns rows t, t + 256, ... of every group.  The
+// matrix (4^K x 16 B) streams through L2 once per workgroup trip.
+template <int K>
+__global__ __launch_bounds__(256) void k_dense_big(double2* __restrict__ sv, InsArgs a,
+                                                   const double2* __restrict__ mg, int4 tlo, int4 tmid, int4 thi) {
+    constexpr int D = 1 << K;
+    constexpr int GB = D >= 2048 ? 1 : 2048 / D;
+    constexpr int NA = GB * D;  // amplitudes staged per trip
+    constexpr int NOUT = NA / 256;
+    extern __shared__ double2 A[];  // NA
+    const int tb[12] = { tlo.x, tlo.y, tlo.z, tlo.w, tmid.x, tmid.y, tmid.z, tmid.w, thi.x, thi.y, thi.z, thi.w };
+    auto offset = [&](int j) {  // target-bit part of local amplitude j of a group
+        uint64_t o = 0;
+#pragma unroll
+        for (int b = 0; b < K; ++b)
+            if ((j >> b) & 1) o |= 1ull << tb[b];
+        return o;
+    };
+    const int64_t trips = (a.ngroups + GB - 1) / GB;
+    for (int64_t tr = blockIdx.x; tr < trips; tr += gridDim.x) {
+        const int64_t g0 = tr * GB;
+        __syncthreads();  // the previous trip's reads of A are done
+        for (int q = threadIdx.x; q < NA; q += 256) {
+            const int64_t g = g0 + q / D;
+            if (g < a.ngroups) A[q] = sv[group_base((uint64_t)g, a) | offset(q % D)];
+        }
+        __syncthreads();
+        double2 acc[NOUT];
+#pragma unroll
+        for (int i = 0; i < NOUT; ++i) acc[i] = make_double2(0.0, 0.0);
+        if constexpr (D <= 256) {
+            const int r = threadIdx.x % D, gi0 = threadIdx.x / D;
+            for (int c = 0; c < D; ++c) {
+                const double2 m = mg[r + (size_t)D * c];
+#pragma unroll
+                for (int i = 0; i < NOUT; ++i) acc[i] = cfma(m, A[(gi0 + i * (256 / D)) * D + c], acc[i]);
+            }
+#pragma unroll
+            for (int i = 0; i < NOUT; ++i) {
+                const int64_t g = g0 + gi0 + i * (256 / D);
+                if (g < a.ngroups) sv[group_base((uint64_t)g, a) | offset(r)] = acc[i];
+            }
+        } else {
+            constexpr int RPT = D / 256;  // rows per thread, per group
+            for (int c = 0; c < D; ++c) {
+#pragma unroll
+                for (int j = 0; j < RPT; ++j) {
+                    const double2 m = mg[threadIdx.x + 256 * j + (size_t)D * c];
+#pragma unroll
+                    for (int gi = 0; gi < GB; ++gi) acc[gi * RPT + j] = cfma(m, A[gi * D + c], acc[gi * RPT + j]);
+                }
+            }
+#pragma unroll
+            for (int gi = 0; gi < GB; ++gi) {
+                const int64_t g = g0 + gi;
+                if (g < a.ngroups) {
+                    const uint64_t base = group_base((uint64_t)g, a);
+#pragma unroll
+                    for (int j = 0; j < RPT; ++j) sv[base | offset(threadIdx.x + 256 * j)] = acc[gi * RPT + j];
+                }
+            }
         }
     }
 }
@@ -592,8 +663,39 @@ int launch_gate(double2* d, int vq, const GateSpec& g, Workspace& scratch, hipSt
         size_t shmem = sizeof(double2) * (D * D + 256);  // 5 KiB
         hipLaunchKernelGGL(k_dense_lds, dim3((unsigned)nblk), dim3(256), shmem, s, d, a, (const double2*)scratch.ptr,
                            g.k, make_int4(tb[0], tb[1], tb[2], tb[3]), make_int2(tb[4], tb[5]));
+    } else if (g.k <= 12) {
+        const size_t D = (size_t)1 << g.k;
+        if (int rc = ensure_ws(scratch, sizeof(double2) * D * D)) return rc;
+        BSIM_HIP(hipMemcpyAsync(scratch.ptr, g.m.data(), sizeof(double2) * D * D, hipMemcpyHostToDevice, s));
+        int tb[12] = {0};
+        for (int b = 0; b < g.k; ++b) tb[b] = g.tb[b];
+        const int4 tlo = make_int4(tb[0], tb[1], tb[2], tb[3]), tmid = make_int4(tb[4], tb[5], tb[6], tb[7]),
+                   thi = make_int4(tb[8], tb[9], tb[10], tb[11]);
+        const int64_t gb = D >= 2048 ? 1 : 2048 / (int64_t)D;
+        const int64_t trips = (a.ngroups + gb - 1) / gb;
+        const unsigned nblk = (unsigned)std::min<int64_t>(trips, cap_blocks());
+        const size_t shmem = sizeof(double2) * (size_t)(gb * D);
+        const double2* mg = (const double2*)scratch.ptr;
+        switch (g.k) {
+#define BIG_CASE(K)                                                                                            \
+    case K:                                                                                                    \
+        BSIM_HIP(hipFuncSetAttribute((const void*)k_dense_big<K>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                     (int)shmem));                                                             \
+        hipLaunchKernelGGL(k_dense_big<K>, dim3(nblk), dim3(256), shmem, s, d, a, mg, tlo, tmid, thi);         \
+        break;
+            BIG_CASE(7)
+            BIG_CASE(8)
+            BIG_CASE(9)
+            BIG_CASE(10)
+            BIG_CASE(11)
+            BIG_CASE(12)
+#undef BIG_CASE
+        }
+        // the matrix was copied from a host vector that ends with this call
+        BSIM_HIP(hipGetLastError());
+        BSIM_HIP(hipStreamSynchronize(s));
     } else {
-        return fail(BSIM_ENOTSUP, "dense gates act on at most 6 targets");
+        return fail(BSIM_ENOTSUP, "dense gates act on at most 12 targets");
     }
     BSIM_HIP(hipGetLastError());
     return BSIM_OK;
@@ -653,7 +755,7 @@ int bsim_apply_controlled(bsim_state* h, int nc, const int* controls, const int*
 
 int bsim_apply_kq(bsim_state* h, int k, const int* targets, const double* m, int conj) {
     if (int rc = check_state(h)) return rc;
-    if (k < 1 || k > 6) return fail(BSIM_ENOTSUP, "k-qubit dense gates take 1..6 targets");
+    if (k < 1 || k > 12) return fail(BSIM_ENOTSUP, "k-qubit dense gates take 1..12 targets");
     GateSpec g;
     if (int rc = make_spec(h->vq, k, targets, 0, nullptr, nullptr, m, false, conj != 0, true, g)) return rc;
     h->alias_ready = false;

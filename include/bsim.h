@@ -91,12 +91,13 @@ int bsim_apply_controlled(bsim_state* h, int nc, const int* controls, const int*
                           const int* targets, const double* m, int conj);
 /* k-qubit dense fallback apply_gate!(::SMatrix{N,N}, sv, targets...):
  * src/gate_kernels.jl:753-775.  Bit j of the matrix index is the j-th
- * smallest endian target: the given order is ignored.  k <= 6. */
+ * smallest endian target: the given order is ignored.  k <= 12 (4..6 on the
+ * FP64 matrix cores, 7..12 through LDS-staged groups). */
 int bsim_apply_kq(bsim_state* h, int k, const int* targets, const double* m, int conj);
 /* Diagonal gates (Diagonal{SVector} through the kernels above and
  * custom_gates.jl:364-388 for MultiRZ / GPhase): amplitude i is multiplied
  * by d[j], where bit b of j is the amplitude bit of qubit targets[b].
- * k <= 10. */
+ * k <= 8. */
 int bsim_apply_diagonal(bsim_state* h, int k, const int* targets, const double* d, int conj);
 /* DoubleExcitation kernel: src/custom_gates.jl:390-416 (angle * pow). */
 int bsim_apply_double_excitation(bsim_state* h, const int* targets4, double angle);
@@ -109,7 +110,7 @@ int bsim_apply_double_excitation(bsim_state* h, const int* targets4, double angl
 #define BSIM_OP_KRAUS_REF 2 /* DM only: Kraus channel, reference block-diagonal semantics */
 #define BSIM_OP_KRAUS_PHYS 3 /* DM only: Kraus channel on every block */
 #define BSIM_OP_PHASEFLIP 4 /* DM only: PhaseFlip(p) elementwise, matrix[0] = p */
-#define BSIM_OP_SUPEROP 5   /* DM only: 4^k x 4^k superoperator, k <= 3 (bsim_dm_apply_superop) */
+#define BSIM_OP_SUPEROP 5   /* DM only: 4^k x 4^k superoperator, k <= 5 (bsim_dm_apply_superop) */
 
 #define BSIM_MAX_TARGETS 8
 #define BSIM_MAX_CONTROLS 8
@@ -188,8 +189,8 @@ int bsim_stream_probe(int mode, int64_t bytes, int blocks, int reps, double* bes
  * mode 0 = reference semantics (only blocks whose row and column agree on
  * every non-target bit; 2-qubit block order (00, t2, t1, 11), t1 = MSB;
  * k > 2: sorted endian targets), mode 1 = physical channel on every block
- * of the stored matrix (S <- sum conj(K) S K^T, S = rho^T).  k <= 5 in
- * mode 0, k <= 3 in mode 1. */
+ * of the stored matrix (S <- sum conj(K) S K^T, S = rho^T).  k <= 8 in
+ * mode 0, k <= 5 in mode 1. */
 int bsim_dm_kraus(bsim_state* h, int k, const int* targets, int n_kraus, const double* kraus, int mode);
 /* apply_noise!(::PhaseFlip, dm, q): src/noise_kernels.jl:7-23. */
 int bsim_dm_phaseflip(bsim_state* h, int q, double p);
@@ -199,7 +200,7 @@ int bsim_dm_phaseflip(bsim_state* h, int q, double p);
  * 4^k x 4^k column-major Liouville matrix of the map on the physical rho
  * under column stacking: rho'[r, c] = sum S[r + 2^k c, r' + 2^k c'] rho[r', c'],
  * matrix bit b of r and c <-> qubit targets[b] (so U rho U^dagger has
- * S = conj(U) (x) U, Kraus K_m have S = sum conj(K_m) (x) K_m).  k <= 3. */
+ * S = conj(U) (x) U, Kraus K_m have S = sum conj(K_m) (x) K_m).  k <= 5. */
 int bsim_dm_apply_superop(bsim_state* h, int k, const int* targets, const double* S);
 
 /* ---- results (src/result_types.jl, src/sv_simulator.jl:240-271,

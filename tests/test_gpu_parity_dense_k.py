@@ -159,3 +159,109 @@ def test_dense_k_large_state_roundtrip(tg):
             assert np.max(np.abs(mid - psi0)) > 1e-3
             sim.evolve([bwd], mode=1)
             np.testing.assert_allclose(sim.state_vector(), psi0, atol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# beyond the matrix-core kernels: 7..12 targets (k_dense_big), Kraus
+# channels on 6..8 qubits (reference semantics, blocks in device memory)
+# and 4..5 qubits (physical semantics, 8- and 10-bit superoperators)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("k", [7, 8])
+@pytest.mark.parametrize("n,controls", [(9, 0), (10, 1), (12, 0)])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_dense_big_vs_oracle(k, n, controls, mode):
+    """7- and 8-target Unitary gates (random, unsorted targets; with a
+    control: Control of the Unitary) as non-tile steps of a fused evolve and
+    per gate, against the oracle's k-qubit fallback at 1e-12."""
+    if k + controls > n:
+        pytest.skip("not enough qubits")
+    rng = np.random.default_rng(500 * k + 10 * n + controls)
+    ins, want = _circuit(n, k, rng, controls)
+    _say(f"dense big k={k} n={n} controls={controls} mode={mode}")
+    sim = StateVectorSimulator(n, 0)
+    sim.evolve(ins, mode=mode)
+    np.testing.assert_allclose(sim.state_vector(), want, atol=TOL)
+
+
+@pytest.mark.parametrize("k,n", [(7, 7), (9, 11), (10, 11), (11, 12)])
+def test_apply_kq_up_to_11_targets(k, n):
+    """bsim_apply_kq (apply_gate of a Unitary) on 7..11 targets, including
+    every qubit of the register (one group), against the oracle."""
+    rng = np.random.default_rng(900 + k)
+    u = random_unitary(k, rng)
+    t = [int(x) for x in rng.choice(n, k, replace=False)]
+    pre = [Instruction(G.Ry(float(a)), [q]) for q, a in enumerate(rng.uniform(0.1, 3.0, n))]
+    _say(f"apply_kq k={k} n={n} targets={t}")
+    sim = StateVectorSimulator(n, 0)
+    sim.evolve(pre)
+    sim.apply_gate(G.Unitary(u), *t)
+    np.testing.assert_allclose(sim.state_vector(), O.evolve_sv(n, pre + [Instruction(G.Unitary(u), t)]), atol=TOL)
+
+
+def test_dense_big_density_matrix():
+    """A 7-target Unitary on an 8-qubit density matrix (U on the column bits,
+    conj(U) on the row bits: two 7-bit passes over the 16-bit vector)."""
+    rng = np.random.default_rng(4242)
+    m, k = 8, 7
+    t = [int(x) for x in rng.choice(m, k, replace=False)]
+    u = random_unitary(k, rng)
+    pre = [Instruction(G.H(), [q]) for q in range(m)] + [Instruction(G.Rz(0.4 * q + 0.1), [q]) for q in range(m)]
+    ins = pre + [Instruction(G.Unitary(u), t)]
+    ref = O.evolve_dm(m, ins, "reference")
+    for mode in (0, 1):
+        _say(f"dense big density matrix mode={mode}")
+        dm = DensityMatrixSimulator(m, 0)
+        dm.evolve(ins, mode=mode)
+        np.testing.assert_allclose(dm.density_matrix().ravel(order="F"), ref, atol=TOL)
+
+
+def _random_channel(k, nk, rng):
+    """nk Kraus matrices of a random k-qubit channel: the d-row blocks of a
+    random isometry (nk d x d), so sum K^dagger K = I."""
+    d = 2 ** k
+    a = rng.normal(size=(nk * d, d)) + 1j * rng.normal(size=(nk * d, d))
+    v, _ = np.linalg.qr(a)  # orthonormal columns
+    return [v[i * d:(i + 1) * d, :] for i in range(nk)]
+
+
+@pytest.mark.parametrize("k,n", [(6, 7), (7, 8), (8, 8)])
+def test_kraus_reference_semantics_6_to_8_qubits(k, n):
+    """Reference-semantics Kraus channels on 6..8 qubits (block-diagonal
+    update, blocks in device memory), in a noisy circuit, against the
+    oracle's restatement of src/noise_kernels.jl:170-196."""
+    from bsim import noises as Nz
+
+    rng = np.random.default_rng(60 + k)
+    t = [int(x) for x in rng.choice(n, k, replace=False)]
+    ins = [Instruction(G.H(), [q]) for q in range(n)] + [Instruction(G.Ry(0.3 * q + 0.2), [q]) for q in range(n)]
+    ins += [Instruction(Nz.Kraus(_random_channel(k, 2, rng)), t), Instruction(G.CNot(), [0, n - 1]),
+            Instruction(Nz.Kraus(_random_channel(k, 2, rng)), list(reversed(t)))]
+    ref = O.evolve_dm(n, ins, "reference")
+    for mode in (0, 1):
+        _say(f"kraus reference k={k} n={n} mode={mode}")
+        dm = DensityMatrixSimulator(n, 0, noise_mode="reference")
+        dm.evolve(ins, mode=mode)
+        np.testing.assert_allclose(dm.density_matrix().ravel(order="F"), ref, atol=TOL)
+
+
+@pytest.mark.parametrize("k", [4, 5])
+def test_kraus_physical_4_and_5_qubits(k):
+    """Physical Kraus channels on 4 and 5 qubits (8- and 10-bit
+    superoperators through the dense kernels) against the oracle's block
+    form of sum_m K rho K^dagger; the trace is kept."""
+    from bsim import noises as Nz
+
+    n = 6
+    rng = np.random.default_rng(80 + k)
+    t = [int(x) for x in rng.choice(n, k, replace=False)]
+    ins = [Instruction(G.H(), [q]) for q in range(n)] + [Instruction(G.Rx(0.5 * q + 0.1), [q]) for q in range(n)]
+    ins += [Instruction(Nz.Kraus(_random_channel(k, 3, rng)), t), Instruction(G.CZ(), [1, 4])]
+    ref = O.evolve_dm(n, ins, "physical")
+    for mode in (0, 1):
+        _say(f"kraus physical k={k} mode={mode}")
+        dm = DensityMatrixSimulator(n, 0, noise_mode="physical")
+        dm.evolve(ins, mode=mode)
+        got = dm.density_matrix()
+        np.testing.assert_allclose(got.ravel(order="F"), ref, atol=TOL)
+        assert abs(np.trace(got).real - 1.0) < 1e-12
