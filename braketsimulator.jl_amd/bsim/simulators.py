@@ -12,11 +12,13 @@ a result is read back.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 
 import numpy as np
 
 from . import _lib as L
+from . import gates as G
 from .gates import Gate, GPhase, KernelCall, Ry, Unitary, H as HGate, kernel_call, derivative_matrices
 from .noises import Noise, PhaseFlip
 from .observables import H as HObs
@@ -186,13 +188,43 @@ class OpList:
         return C.cast(self._arr, C.POINTER(L.Op))
 
 
-def lower_instructions(instructions, kind: int, noise_mode: str = "reference", param_map=None) -> OpList:
+# Gates whose kernel call depends only on (class, exponent, targets): their
+# calls are reused across lowerings (the matrices are only read).
+_FIXED_GATES = tuple(getattr(G, n) for n in ("I", "H", "X", "Y", "Z", "V", "Vi", "S", "Si", "T", "Ti", "CNot", "CY",
+                                             "CZ", "CV", "CSwap", "CCNot", "Swap", "ISwap", "ECR"))
+_CALL_CACHE: dict = {}
+# Pauli rotations at exponent 1: matrix raw(theta) and derivative
+# raw(theta + pi) / 2, the numbers kernel_call / derivative_matrices give
+_ROTATIONS = {G.Rx: G._rx, G.Ry: G._ry, G.Rz: G._rz}
+
+
+def lower_instructions(instructions, kind: int, noise_mode: str = "reference", param_map=None,
+                       fast: bool = True) -> OpList:
     """Instructions -> bsim_op array.  ``param_map`` (adjoint) maps
-    (instruction index, angle index) -> global parameter index."""
+    (instruction index, angle index) -> global parameter index.  ``fast``
+    takes the shortcuts above (the same op array; False: every gate through
+    kernel_call / derivative_matrices)."""
     ol = OpList(len(instructions))
     for i, ins in enumerate(instructions):
         op = ins.operator
         if isinstance(op, NO_OPS) or op is None:
+            continue
+        tp = type(op)
+        if fast and tp in _ROTATIONS and op.pow_exponent == 1.0 and len(ins.target) == 1:
+            raw, th = _ROTATIONS[tp], op.angles[0]
+            params = dmats = None
+            if param_map is not None and (i, 0) in param_map:
+                params, dmats = [param_map[(i, 0)]], [0.5 * raw(th + math.pi)]
+            ol.add(L.OP_GATE, [int(ins.target[0])], raw(th), params=params, dmats=dmats)
+            continue
+        if fast and tp in _FIXED_GATES:
+            key = (tp, op.pow_exponent, tuple(int(t) for t in ins.target))
+            call = _CALL_CACHE.get(key)
+            if call is None:
+                if len(_CALL_CACHE) > 65536:
+                    _CALL_CACHE.clear()
+                call = _CALL_CACHE[key] = kernel_call(op, ins.target)
+            ol.add_call(call)
             continue
         if isinstance(op, Gate):
             call = kernel_call(op, ins.target)

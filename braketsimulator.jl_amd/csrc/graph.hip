@@ -159,36 +159,45 @@ int bsim_autotune(bsim_state* h, const bsim_op* ops, int n_ops, int* tile_bits, 
     TileConfig best_cfg = keep;
     double best = DBL_MAX;
     const int Tmax = h->vq < 13 ? h->vq : 13;
-    // stage 1: tile width, contiguous low bits and sweep width at the
-    // current fusion limit
+    // A coordinate search: stage 1, the tile width and sweep width at the
+    // current contiguous low bits; stage 2, the low bits at the best shape;
+    // stage 3, the fusion width.
+    auto trial = [&](const TileConfig& c) -> int {
+        double t = DBL_MAX;
+        if (int rc = time_config(h, eops, c, e0, e1, t)) return rc;
+        if (t < best) {
+            best = t;
+            best_cfg = c;
+        }
+        return BSIM_OK;
+    };
     for (int RB = 4; RB >= 3; --RB)
-        for (int T = 10; T <= Tmax; ++T)
-            for (int L = 2; L <= 6 && L <= T - 2; ++L) {
-                TileConfig c = keep;
-                c.tile_bits = T;
-                c.low_bits = L;
-                c.reg_bits = RB;
-                double t = DBL_MAX;
-                if (int rc = time_config(h, eops, c, e0, e1, t)) return rc;
-                if (t < best) {
-                    best = t;
-                    best_cfg = c;
-                }
-            }
-    // stage 2: the fusion width (FP64 operations per amplitude one pass may
-    // spend; 0 = unlimited) at the best tile shape
+        for (int T = 10; T <= Tmax; ++T) {
+            TileConfig c = keep;
+            c.tile_bits = T;
+            c.low_bits = std::min(keep.low_bits, T - 2);
+            c.reg_bits = RB;
+            if (int rc = trial(c)) return rc;
+        }
+    if (best < DBL_MAX) {
+        const TileConfig shape = best_cfg;
+        for (int L = 2; L <= 6 && L <= shape.tile_bits - 2; ++L) {
+            if (L == shape.low_bits) continue;
+            TileConfig c = shape;
+            c.low_bits = L;
+            if (int rc = trial(c)) return rc;
+        }
+    }
+    // stage 3: the fusion width (FP64 operations per amplitude one pass may
+    // spend; 0 = unlimited) at the best tile configuration
     if (best < DBL_MAX) {
         static const int limits[] = { 0, 48, 96, 192, 384 };
+        const TileConfig tiled = best_cfg;
         for (int lim : limits) {
-            if (lim == best_cfg.max_cost) continue;
-            TileConfig c = best_cfg;
+            if (lim == tiled.max_cost) continue;
+            TileConfig c = tiled;
             c.max_cost = lim;
-            double t = DBL_MAX;
-            if (int rc = time_config(h, eops, c, e0, e1, t)) return rc;
-            if (t < best) {
-                best = t;
-                best_cfg = c;
-            }
+            if (int rc = trial(c)) return rc;
         }
     }
     if (best == DBL_MAX) {  // no candidate fits this state: keep the current configuration

@@ -342,10 +342,12 @@ int bsim_graph_destroy(bsim_graph* g);
 /* Tile autotuner (the reference has no tunables besides CHUNK_SIZE,
  * src/BraketSimulator.jl:85-86): times the fused evolution of ops on h's
  * state (reset to |0...0> before every run, left in an unspecified state
- * afterwards) for tile_bits 10..min(13, vector qubits), contiguous low bits
- * 2..min(6, tile_bits-2) and 3 or 4 sweep register bits, then -- at the
- * best of those -- the fusion width (bsim_set_fusion_limit: 0 = unlimited,
- * 48, 96, 192, 384 FP64 operations per amplitude and pass), keeps the
+ * afterwards) in a coordinate search: tile_bits 10..min(13, vector qubits)
+ * with 3 or 4 sweep register bits at the current contiguous low bits, then
+ * low bits 2..min(6, tile_bits-2) at the best shape, then the fusion width
+ * (bsim_set_fusion_limit: 0 = unlimited, 48, 96, 192, 384 FP64 operations
+ * per amplitude and pass); each candidate is one warm-up run and the best
+ * of two timed runs (one when a run takes over 20 ms).  It keeps the
  * fastest as the library configuration and returns it (outputs may be
  * NULL; *best_ms = 0 when no candidate fits and the configuration is
  * unchanged).  Candidates are planned with private configurations: other

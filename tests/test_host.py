@@ -208,3 +208,34 @@ def test_multi_qubit_pauli_channel():
     np.testing.assert_allclose(sum(k.conj().T @ k for k in ka), np.eye(4), atol=1e-15)
     with pytest.raises(NotImplementedError):
         Nz.MultiQubitPauliChannel({"XYZ": 0.1}).kraus_rep()
+
+
+def test_fast_lowering_matches_general_path():
+    """lower_instructions' shortcuts (Pauli rotations at exponent 1 and
+    fixed gates with reused kernel calls) produce the op records and matrix
+    bytes of the general kernel_call / derivative_matrices path, with and
+    without a parameter map."""
+    import numpy as np
+
+    from bsim import _lib as L
+    from bsim import gates as G
+    from bsim import workloads as W
+    from bsim.simulators import Instruction, lower_instructions
+
+    from circuits import random_circuit
+
+    rng = np.random.default_rng(3)
+    ins, pmap, _ = W.hea(9, 3, seed=4)
+    cases = [(ins, pmap), (ins, None), (random_circuit(7, 200, rng), None),
+             ([Instruction(G.Rx(0.3, pow_exponent=2.0), [1]), Instruction(G.CNot(pow_exponent=0.5), [0, 2]),
+               Instruction(G.Rz(-1.1), [3]), Instruction(G.H(), [4]), Instruction(G.CCNot(), [0, 1, 2])], None)]
+    for instrs, pm in cases:
+        a = lower_instructions(list(instrs), L.BSIM_SV, param_map=pm, fast=True)
+        b = lower_instructions(list(instrs), L.BSIM_SV, param_map=pm, fast=False)
+        a._finish()
+        b._finish()
+        assert a.n == b.n
+        for f in ("type", "n_targets", "n_controls", "targets", "controls", "control_values", "n_params",
+                  "param_index"):
+            np.testing.assert_array_equal(a._rec[f], b._rec[f], err_msg=f)
+        assert a._buf.tobytes() == b._buf.tobytes()
