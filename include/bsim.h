@@ -178,9 +178,10 @@ int bsim_kernel_stats(int reset, int64_t* launches, double* total_ms);
  * a buffer of `bytes` (two for a copy), runs a plain streaming kernel over
  * it -- mode 0 copy (read + write every 16 B element), 1 read (reduce), 2
  * write (fill) -- `reps` times after one warm-up on `blocks` workgroups of
- * 256 threads (0 = 8 per CU), and returns the fastest launch in *best_ms.
- * The ceiling the fused passes and per-gate kernels are judged against.
- * bytes: a positive multiple of 16 KiB. */
+ * 256 threads (0 = 8 per CU), each thread moving 4, 8 or 16 x 16 B per
+ * trip, and returns the fastest launch in *best_ms.  The ceiling the fused
+ * passes and per-gate kernels are judged against.  bytes: a positive
+ * multiple of 64 KiB. */
 int bsim_stream_probe(int mode, int64_t bytes, int blocks, int reps, double* best_ms);
 
 /* ---- noise (src/noise_kernels.jl) -------------------------------- */
