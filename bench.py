@@ -69,8 +69,9 @@ def main():
                     help="profiling aid: one fused evolve of the workload and one 4 GiB stream copy, no JSON")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: libbsim has no multi-GPU path (a state "
+              "would shard by its top index bits with qubit swaps over xGMI); run with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
