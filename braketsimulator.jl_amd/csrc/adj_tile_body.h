@@ -58,6 +58,26 @@ __device__ __forceinline__ double dot_dense1(const SubCube<3>& sp, const SubCube
     return acc;
 }
 
+// dot_dense1 for an uncontrolled op whose payload is real (nd = 1).
+template <int A>
+__device__ __forceinline__ double dot_dense1_real(const SubCube<3>& sp, const SubCube<3>& sl,
+                                                  const TOp* __restrict__ d) {
+    const double m00 = d->m[0].x, m10 = d->m[1].x, m01 = d->m[2].x, m11 = d->m[3].x;
+    double acc = 0.0;
+    sfor<0, 8>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        if constexpr ((j & (1 << A)) == 0) {
+            constexpr int k = j | (1 << A);
+            const c2 a = getv<j>(sp.v), b = getv<k>(sp.v);
+            const c2 la = getv<j>(sl.v), lb = getv<k>(sl.v);
+            const double xr = m00 * a.x + m01 * b.x, xi = m00 * a.y + m01 * b.y;
+            const double yr = m10 * a.x + m11 * b.x, yi = m10 * a.y + m11 * b.y;
+            acc += la.x * xr + la.y * xi + lb.x * yr + lb.y * yi;
+        }
+    });
+    return acc;
+}
+
 template <int A, int B>
 __device__ __forceinline__ double dot_dense2(const SubCube<3>& sp, const SubCube<3>& sl, const TOp* __restrict__ op,
                                              const TOp* __restrict__ d) {
@@ -124,16 +144,29 @@ __device__ __forceinline__ double dot_rdiag(const SubCube<3>& sp, const SubCube<
     return acc;
 }
 
+// dot_rdiag for an uncontrolled register diagonal whose payload is
+// imaginary (nd = 2): Re conj(l) (i mu v) = mu (l.y v.x - l.x v.y).
+__device__ __forceinline__ double dot_rdiag_imag(const SubCube<3>& sp, const SubCube<3>& sl,
+                                                 const TOp* __restrict__ d) {
+    double acc = 0.0;
+    sfor<0, 8>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        const c2 v = getv<j>(sp.v), l = getv<j>(sl.v);
+        acc += d->m[j].y * (l.y * v.x - l.x * v.y);
+    });
+    return acc;
+}
+
 __device__ __forceinline__ double adj_dot(const SubCube<3>& sp, const SubCube<3>& sl, const TOp* __restrict__ op,
                                           const TOp* __restrict__ d) {
     switch (op->kind * 16 + op->a * 4 + op->b) {
-        case 0: return dot_dense1<0>(sp, sl, op, d);
-        case 4: return dot_dense1<1>(sp, sl, op, d);
-        case 8: return dot_dense1<2>(sp, sl, op, d);
+        case 0: return d->nd == 1 ? dot_dense1_real<0>(sp, sl, d) : dot_dense1<0>(sp, sl, op, d);
+        case 4: return d->nd == 1 ? dot_dense1_real<1>(sp, sl, d) : dot_dense1<1>(sp, sl, op, d);
+        case 8: return d->nd == 1 ? dot_dense1_real<2>(sp, sl, d) : dot_dense1<2>(sp, sl, op, d);
         case 16 + 1: return dot_dense2<0, 1>(sp, sl, op, d);
         case 16 + 2: return dot_dense2<0, 2>(sp, sl, op, d);
         case 16 + 6: return dot_dense2<1, 2>(sp, sl, op, d);
-        case 96: return dot_rdiag(sp, sl, op, d);
+        case 96: return d->nd == 2 ? dot_rdiag_imag(sp, sl, d) : dot_rdiag(sp, sl, op, d);
         default: return dot_diag(sp, sl, op, d);
     }
 }
@@ -194,9 +227,33 @@ __device__ __forceinline__ void run_op_both(SubCube<3>& sp, SubCube<3>& sl, cons
     const OpHead hd = load_head(op);
     switch (hd.kind * 16 + hd.a * 4 + hd.b) {
         case 48: sign_both(sp, sl, op); break;
-        case 0: sw_dense1<3, 0>(sp, op, hd); sw_dense1<3, 0>(sl, op, hd); break;
-        case 4: sw_dense1<3, 1>(sp, op, hd); sw_dense1<3, 1>(sl, op, hd); break;
-        case 8: sw_dense1<3, 2>(sp, op, hd); sw_dense1<3, 2>(sl, op, hd); break;
+        case 0:
+            if (hd.nd == 1) {
+                sw_dense1_real<3, 0>(sp, hd);
+                sw_dense1_real<3, 0>(sl, hd);
+            } else {
+                sw_dense1<3, 0>(sp, op, hd);
+                sw_dense1<3, 0>(sl, op, hd);
+            }
+            break;
+        case 4:
+            if (hd.nd == 1) {
+                sw_dense1_real<3, 1>(sp, hd);
+                sw_dense1_real<3, 1>(sl, hd);
+            } else {
+                sw_dense1<3, 1>(sp, op, hd);
+                sw_dense1<3, 1>(sl, op, hd);
+            }
+            break;
+        case 8:
+            if (hd.nd == 1) {
+                sw_dense1_real<3, 2>(sp, hd);
+                sw_dense1_real<3, 2>(sl, hd);
+            } else {
+                sw_dense1<3, 2>(sp, op, hd);
+                sw_dense1<3, 2>(sl, op, hd);
+            }
+            break;
         case 16 + 1: sw_dense2<3, 0, 1>(sp, op, hd); sw_dense2<3, 0, 1>(sl, op, hd); break;
         case 16 + 2: sw_dense2<3, 0, 2>(sp, op, hd); sw_dense2<3, 0, 2>(sl, op, hd); break;
         case 16 + 6: sw_dense2<3, 1, 2>(sp, op, hd); sw_dense2<3, 1, 2>(sl, op, hd); break;

@@ -815,6 +815,33 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
             const uint64_t w[8] = { op.pm[0], op.pm[1], op.pm[2], op.pm[3], op.sgm[0], op.sgm[1], op.sgm[2], op.sgm[3] };
             memcpy(op.m, w, sizeof(w));
         }
+    // Forms the adjoint tile body has cheaper paths for, in fields its dense
+    // and register-diagonal ops do not otherwise use (nd): an uncontrolled
+    // dense 1-register op whose matrix is real (nd = 1: RY, H, X ...); the
+    // product payloads U^dagger dU of such ops that are real to rounding
+    // (nd = 1: RY's -i Y / 2), and of uncontrolled register diagonals that are
+    // imaginary to rounding (nd = 2: RZ's -i Z / 2).  A payload entry below
+    // 1e-15 of the largest is rounding of an exact zero; the cheap path
+    // leaves it out.
+    auto small = [](double x, double scale) { return std::fabs(x) <= 1e-15 * scale; };
+    for (size_t o = 0; o < plan.tops.size(); ++o) {
+        TOp& op = plan.tops[o];
+        if ((op.kind != 0 && op.kind != 6) || op.cmask != 0) continue;
+        const int nm = op.kind == 0 ? 4 : 16;  // register diagonals: unused entries are zero
+        if (op.kind == 0) {
+            bool real = true;
+            for (int j = 0; j < 4; ++j) real = real && op.m[j].y == 0.0;
+            if (real) op.nd = 1;
+        }
+        for (int p = 0; p < op.nparam; ++p) {
+            TOp& d = plan.tops[o + 1 + p];
+            double scale = 0.0;
+            for (int j = 0; j < nm; ++j) scale = std::max(scale, std::max(std::fabs(d.m[j].x), std::fabs(d.m[j].y)));
+            bool form = scale > 0.0;
+            for (int j = 0; j < nm && form; ++j) form = small(op.kind == 0 ? d.m[j].y : d.m[j].x, scale);
+            if (form) d.nd = op.kind == 0 ? 1 : 2;
+        }
+    }
     return BSIM_OK;
 }
 

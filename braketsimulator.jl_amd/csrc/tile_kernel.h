@@ -133,6 +133,42 @@ __device__ __forceinline__ void pair_inplace(double& ar, double& ai, double& br,
           [m01i] "s"(m01.y), [m11r] "s"(m11.x), [m11i] "s"(m11.y));
 }
 
+// The same with a real matrix (m00, m10, m01, m11 real): 8 instructions.
+__device__ __forceinline__ void pair_inplace_real(double& ar, double& ai, double& br, double& bi, double m00,
+                                                  double m10, double m01, double m11) {
+    double pr, pi, qr, qi;
+    asm("v_mul_f64 %[pr], %[br], %[m01]\n\t"
+        "v_mul_f64 %[pi], %[bi], %[m01]\n\t"
+        "v_mul_f64 %[qr], %[ar], %[m10]\n\t"
+        "v_mul_f64 %[qi], %[ai], %[m10]\n\t"
+        "v_fma_f64 %[ar], %[ar], %[m00], %[pr]\n\t"
+        "v_fma_f64 %[ai], %[ai], %[m00], %[pi]\n\t"
+        "v_fma_f64 %[br], %[br], %[m11], %[qr]\n\t"
+        "v_fma_f64 %[bi], %[bi], %[m11], %[qi]"
+        : [ar] "+v"(ar), [ai] "+v"(ai), [br] "+v"(br), [bi] "+v"(bi), [pr] "=&v"(pr), [pi] "=&v"(pi),
+          [qr] "=&v"(qr), [qi] "=&v"(qi)
+        : [m00] "s"(m00), [m10] "s"(m10), [m01] "s"(m01), [m11] "s"(m11));
+}
+
+// An uncontrolled dense 1-register op with a real matrix (the planner marks
+// it with nd = 1).
+template <int R, int A>
+__device__ __forceinline__ void sw_dense1_real(SubCube<R>& s, const OpHead& hd) {
+    const double m00 = hd.m0.x, m10 = hd.m1.x, m01 = hd.m2.x, m11 = hd.m3.x;
+    sfor<0, (1 << R)>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        if constexpr ((j & (1 << A)) == 0) {
+            constexpr int k = j | (1 << A);
+            double ar = s.v[2 * j], ai = s.v[2 * j + 1], br = s.v[2 * k], bi = s.v[2 * k + 1];
+            pair_inplace_real(ar, ai, br, bi, m00, m10, m01, m11);
+            s.v[2 * j] = ar;
+            s.v[2 * j + 1] = ai;
+            s.v[2 * k] = br;
+            s.v[2 * k + 1] = bi;
+        }
+    });
+}
+
 // Dense ops come in two instantiations: without controls (the common case)
 // and with the split control test; the choice is made once per op.
 template <int R, int A, bool CTL>
@@ -840,9 +876,16 @@ __device__ __forceinline__ void stream_body(double2* __restrict__ sv, int vq, co
         // amplitudes (in bounds, discarded), so every path issues the same
         // memory operations in the same order
         const uint64_t nbase = nt < ntiles ? tile_base(nt) : base;
-        for (int sidx = 0; sidx + 1 < nsw; ++sidx) sweep(sidx, base);
-        load(nbase);  // in flight during the last sweep and the stores
-        if (nsw > 0) sweep(nsw - 1, base);
+        if constexpr (KRAUS) {
+            // density-matrix passes (long 16 x 16 channel sweeps): the next
+            // tile's loads in flight during every sweep
+            load(nbase);
+            for (int sidx = 0; sidx < nsw; ++sidx) sweep(sidx, base);
+        } else {
+            for (int sidx = 0; sidx + 1 < nsw; ++sidx) sweep(sidx, base);
+            load(nbase);  // in flight during the last sweep and the stores
+            if (nsw > 0) sweep(nsw - 1, base);
+        }
         const uint64_t s_t = base | scat_lo[threadIdx.x];
         sfor<0, NR>([&](auto J) {
             constexpr int jj = decltype(J)::value;
