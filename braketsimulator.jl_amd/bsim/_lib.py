@@ -99,6 +99,9 @@ _SIGS = {
     "bsim_apply_diagonal": (C.c_int, [C.c_void_p, C.c_int, _ip, _dp, C.c_int]),
     "bsim_apply_double_excitation": (C.c_int, [C.c_void_p, _ip, C.c_double]),
     "bsim_evolve": (C.c_int, [C.c_void_p, C.POINTER(Op), C.c_int, C.c_int]),
+    "bsim_ops_pack": (C.c_int, [C.POINTER(Op), C.c_int, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "bsim_ops_unpack": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(Op), C.c_int, _ip]),
+    "bsim_evolve_packed": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
     "bsim_plan_stats": (C.c_int, [C.c_int, C.c_int, C.POINTER(Op), C.c_int, _i64p]),
     "bsim_set_tile_config": (C.c_int, [C.c_int, C.c_int]),
     "bsim_get_tile_config": (C.c_int, [_ip, _ip]),

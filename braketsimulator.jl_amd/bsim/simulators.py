@@ -188,6 +188,37 @@ class OpList:
         return C.cast(self._arr, C.POINTER(L.Op))
 
 
+class OpImage:
+    """A compact op-list image (bsim_ops_pack): self-contained bytes that can
+    be stored or sent and evolved, captured or batched elsewhere.  ``n`` and
+    ``ptr`` (ops pointing into the image, bsim_ops_unpack) as for OpList;
+    ``tobytes()`` gives the image."""
+
+    def __init__(self, data):
+        self.buf = np.frombuffer(bytes(data), dtype=np.uint8).copy()  # an aligned private copy
+        n = C.c_int()
+        L.call("bsim_ops_unpack", self.buf.ctypes.data, self.buf.size, None, 0, C.byref(n))
+        self._ops = (L.Op * max(n.value, 1))()
+        L.call("bsim_ops_unpack", self.buf.ctypes.data, self.buf.size, self._ops, n.value, C.byref(n))
+        self.n = n.value
+
+    @classmethod
+    def pack(cls, ops) -> "OpImage":
+        """The image of an op list (anything with ``n`` and ``ptr``)."""
+        size = C.c_size_t()
+        L.call("bsim_ops_pack", ops.ptr, ops.n, None, 0, C.byref(size))
+        buf = np.zeros(size.value, dtype=np.uint8)
+        L.call("bsim_ops_pack", ops.ptr, ops.n, buf.ctypes.data, buf.size, C.byref(size))
+        return cls(buf.tobytes())
+
+    @property
+    def ptr(self):
+        return C.cast(self._ops, C.POINTER(L.Op))
+
+    def tobytes(self) -> bytes:
+        return self.buf.tobytes()
+
+
 # Gates whose kernel call depends only on (class, exponent, targets): their
 # calls are reused across lowerings (the matrices are only read).
 _FIXED_GATES = tuple(getattr(G, n) for n in ("I", "H", "X", "Y", "Z", "V", "Vi", "S", "Si", "T", "Ti", "CNot", "CY",

@@ -137,6 +137,51 @@ typedef struct bsim_op {
  * launch per op (the reference's loop shape).  Results are the same. */
 int bsim_evolve(bsim_state* h, const bsim_op* ops, int n_ops, int mode);
 
+/* ---- compact op-list image ------------------------------------------
+ * The reference builds its instruction stream from OpenQASM 3 in Julia
+ * (to_circuit, src/circuit.jl:277-310) and hands it to evolve! as objects.
+ * A bsim_op array holds host pointers; its packed image is self-contained
+ * bytes (no pointers) that can be stored, sent to another process and
+ * evolved, captured or batched from there.  Layout, little-endian, 8-byte
+ * aligned:
+ *   bsim_packed_header;
+ *   n_ops x bsim_packed_op  (matrix / dmatrix: index of the first complex
+ *                            value in the value array, -1 when absent);
+ *   n_values x complex128   (re, im) -- per op its matrix (gate 4^k, diag
+ *                            2^k, Kraus n_kraus 4^k, PhaseFlip 1 (p, 0),
+ *                            superoperator 16^k values), then its
+ *                            n_params derivative matrices (4^k each). */
+#define BSIM_PACK_VERSION 1
+typedef struct bsim_packed_header {
+    char magic[4];          /* "BSOP" */
+    uint32_t version;       /* BSIM_PACK_VERSION */
+    uint32_t n_ops;
+    uint32_t record_bytes;  /* sizeof(bsim_packed_op) */
+    uint64_t n_values;
+} bsim_packed_header;
+typedef struct bsim_packed_op {
+    int32_t type, n_targets, n_controls, n_kraus;
+    int32_t targets[BSIM_MAX_TARGETS];
+    int32_t controls[BSIM_MAX_CONTROLS];
+    int32_t control_values[BSIM_MAX_CONTROLS];
+    int64_t matrix;
+    int32_t n_params;
+    int32_t param_index[3];
+    int64_t dmatrix[3];
+} bsim_packed_op;
+/* Image of ops[0..n_ops) into buf (cap bytes; 8-byte aligned); *size gets
+ * the bytes needed (buf = NULL: size only).  BSIM_EINVAL when cap is
+ * smaller or an op is malformed (null matrix, counts out of range). */
+int bsim_ops_pack(const bsim_op* ops, int n_ops, void* buf, size_t cap, size_t* size);
+/* Validates an image of `size` bytes (magic, version, record size, every op's
+ * counts and value ranges inside the buffer) and fills ops[0..*n_ops) with
+ * pointers into buf -- no copy: the ops are valid while buf lives unchanged.
+ * ops = NULL: *n_ops only; BSIM_EINVAL when cap < *n_ops or the image is
+ * malformed.  Qubit ranges are checked by the call that takes the ops. */
+int bsim_ops_unpack(const void* buf, size_t size, bsim_op* ops, int cap, int* n_ops);
+/* bsim_evolve of an image. */
+int bsim_evolve_packed(bsim_state* h, const void* buf, size_t size, int mode);
+
 /* evolve! (fused mode) without device allocation.  No counterpart in the
  * reference: for callers that cannot allocate device memory (tight serving
  * loops, harnesses that forbid hipMalloc).  The plan's tables live in a
