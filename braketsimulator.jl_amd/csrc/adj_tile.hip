@@ -119,6 +119,7 @@ int adjoint_back_plan(int vq, const std::vector<EOp>& eops, std::vector<EOp>& ba
     TileConfig cfg = base ? *base : tile_config();
     cfg.tile_bits = 11;
     cfg.reg_bits = 3;
+    cfg.merge_dense = false;  // single steps run by index in the reversed list
     return make_plan(vq, back, cfg, plan);
 }
 

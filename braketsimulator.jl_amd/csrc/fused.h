@@ -101,12 +101,17 @@ struct Plan {
         bool tile;
         int pass;              // index into passes
         int op;                // index into eops (single launch)
+        int merged = -1;       // >= 0: single launch of merged[merged] instead
     };
     std::vector<Step> steps;
     std::vector<Pass> passes;
     std::vector<Sweep> sweeps;
     std::vector<TOp> tops;
     std::vector<int> top_src;  // eop index each TOp came from
+    // runs of adjacent dense ops around a 3+-bit one, multiplied into one
+    // op on at most 6 bits (one HBM pass on the matrix cores instead of
+    // several)
+    std::vector<EOp> merged;
     int64_t fused_ops = 0;
     bool has_kraus = false;    // holds kind-4 ops: launch k_tile_kraus
 };
@@ -123,6 +128,9 @@ struct TileConfig {
     // = 8, dense 2q = 32, diagonal = 4, sign = 1); 0 = unlimited.  Bounds
     // the fusion width so that a pass stays HBM-bound.
     int max_cost = 0;
+    // merge runs of adjacent dense ops that do not fit a tile pass (3+
+    // target bits) with their neighbours into one op of at most 6 bits
+    bool merge_dense = true;
 };
 // A snapshot of the library-wide configuration (taken under a lock), and
 // its setter; every entry point plans with one snapshot.

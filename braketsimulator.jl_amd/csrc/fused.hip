@@ -477,6 +477,72 @@ static void fuse_sweep(const std::vector<EOp>& eops, const std::vector<int>& tak
     for (auto& b : pend) flush(b);
 }
 
+// Merging of adjacent dense ops into one op of at most kMergeBits bits: on
+// vectors of at least kMergeMinBits bits, where each launch is an HBM pass
+// (28 qubits, 3- and 4-target runs: 130 -> 76 ms), and 6-bit products from
+// kMerge6MinBits (on small vectors the 6-bit matrix-core launch is
+// latency-bound: an 8-qubit density matrix's 3-target gates took 16x longer
+// merged).
+constexpr int kMergeBits = 6;
+constexpr int kMergeMinBits = 20;
+constexpr int kMerge6MinBits = 22;
+
+static bool mergeable(const EOp& e) {
+    return (e.kind == E_DENSE || e.kind == E_DIAG) && e.g.cmask == 0 && e.nparams == 0 && e.aslot < 0;
+}
+
+// The group's product as one dense op on the union of its bits (ascending):
+// op j of the group acts on union bits pos(tb), M <- Op_j M.
+static EOp merge_group(const std::vector<EOp>& eops, const std::vector<size_t>& group, uint64_t U) {
+    int ub[kMergeBits], ku = 0;
+    for (int bit = 0; bit < 64; ++bit)
+        if ((U >> bit) & 1) ub[ku++] = bit;
+    const int D = 1 << ku;
+    std::vector<std::complex<double>> M((size_t)D * D, 0.0), T((size_t)D * D);
+    for (int i = 0; i < D; ++i) M[i + (size_t)D * i] = 1.0;
+    for (size_t gi : group) {
+        const GateSpec& g = eops[gi].g;
+        int pos[16];
+        uint64_t om = 0;  // union bits the op acts on
+        for (int b = 0; b < g.k; ++b) {
+            for (int u = 0; u < ku; ++u)
+                if (ub[u] == g.tb[b]) pos[b] = u;
+            om |= 1ull << pos[b];
+        }
+        const int d = 1 << g.k;
+        auto sub = [&](int x) {  // the op's matrix index of union index x
+            int s = 0;
+            for (int b = 0; b < g.k; ++b) s |= ((x >> pos[b]) & 1) << b;
+            return s;
+        };
+        // T = Op M: Op[r][c] is nonzero only where r and c agree off the op's bits
+        for (int r = 0; r < D; ++r)
+            for (int c = 0; c < D; ++c) {
+                std::complex<double> acc = 0.0;
+                const int rs = sub(r);
+                if (g.diag) {
+                    acc = std::complex<double>(g.m[rs].x, g.m[rs].y) * M[r + (size_t)D * c];
+                } else {
+                    for (int cs = 0; cs < d; ++cs) {
+                        int k2 = r & ~(int)om;
+                        for (int b = 0; b < g.k; ++b) k2 |= ((cs >> b) & 1) << pos[b];
+                        const double2 o = g.m[rs + (size_t)d * cs];
+                        acc += std::complex<double>(o.x, o.y) * M[k2 + (size_t)D * c];
+                    }
+                }
+                T[r + (size_t)D * c] = acc;
+            }
+        M.swap(T);
+    }
+    EOp e;
+    e.kind = E_DENSE;
+    e.g.k = ku;
+    for (int u = 0; u < ku; ++u) e.g.tb[u] = ub[u];
+    e.g.m.resize((size_t)D * D);
+    for (size_t i = 0; i < M.size(); ++i) e.g.m[i] = make_double2(M[i].real(), M[i].imag());
+    return e;
+}
+
 int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan& plan) {
     plan = Plan();
     const size_t n = eops.size();
@@ -495,6 +561,32 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
         while (head < n && done[head]) ++head;
         if (head >= n) break;
         if (!eops[head].fusable()) {
+            // a dense op on 3+ bits: take the adjacent dense ops that either
+            // fit inside its bits or are themselves 3+-bit ops, while the
+            // union stays within kMergeBits, into one product
+            const int max_bits = vq >= kMerge6MinBits ? kMergeBits : kMergeBits - 1;
+            if (cfg.merge_dense && vq >= kMergeMinBits && eops[head].kind == E_DENSE && mergeable(eops[head]) &&
+                eops[head].g.k <= max_bits) {
+                std::vector<size_t> group{ head };
+                uint64_t U = target_mask(eops[head]);
+                for (size_t j = head + 1; j < n; ++j) {
+                    if (done[j]) continue;  // taken ahead of head by an earlier pass
+                    const EOp& e = eops[j];
+                    if (!mergeable(e)) break;
+                    const uint64_t t = target_mask(e);
+                    if (popc(U | t) > max_bits || (e.fusable() && (t & ~U))) break;
+                    group.push_back(j);
+                    U |= t;
+                }
+                if (group.size() > 1) {
+                    plan.merged.push_back(merge_group(eops, group, U));
+                    Plan::Step st{ false, -1, (int)head };
+                    st.merged = (int)plan.merged.size() - 1;
+                    plan.steps.push_back(st);
+                    for (size_t j : group) done[j] = 1;
+                    continue;
+                }
+            }
             plan.steps.push_back({ false, -1, (int)head });
             done[head] = 1;
             continue;
@@ -1061,7 +1153,7 @@ int launch_plan(bsim_state* h, double2* d, const std::vector<EOp>& eops, const P
             if (timed && prof().on)
                 if (int rc = prof_mark(h->stream, &e1)) return rc;
         } else {
-            if (int rc = launch_eop(h, d, eops[st.op])) return rc;
+            if (int rc = launch_eop(h, d, st.merged >= 0 ? plan.merged[st.merged] : eops[st.op])) return rc;
         }
     }
     return BSIM_OK;

@@ -129,6 +129,8 @@ static int check_sweep(const Plan& p, const Sweep& S, int R, const int* vb, uint
     return 0;
 }
 
+static void apply_eop_cpu(const EOp& e, uint64_t N, cd* v);
+
 static int run(int nq, int kind, const bsim_op* ops, int n, double* state) {
     std::vector<EOp> e;
     if (int rc = lower_ops(nq, kind, ops, n, e, false)) return rc;
@@ -138,7 +140,13 @@ static int run(int nq, int kind, const bsim_op* ops, int n, double* state) {
     cd* v = (cd*)state;
     const uint64_t N = 1ull << vq;
     for (auto& st : p.steps) {
-        if (!st.tile) { printf("non-tile step\n"); return -99; }
+        if (!st.tile) {
+            // a single launch: the op itself, or the product of a merged run
+            const EOp& se = st.merged >= 0 ? p.merged[st.merged] : e[st.op];
+            if (se.kind == E_KRAUS_REF) { printf("Kraus step\n"); return -99; }
+            apply_eop_cpu(se, N, v);
+            continue;
+        }
         const Pass& P = p.passes[st.pass];
         for (int s = 0; s < P.nsweeps; ++s) {
             const Sweep& S = p.sweeps[P.sweep_begin + s];
@@ -166,6 +174,16 @@ extern "C" int plan_singles(int nq, int kind, const bsim_op* ops, int n) {
     if (int rc = make_plan(kind == BSIM_SV ? nq : 2 * nq, e, tile_config(), p)) return rc;
     int k = 0;
     for (auto& st : p.steps) k += st.tile ? 0 : 1;
+    return k;
+}
+// number of single steps that launch a merged run of ops
+extern "C" int plan_merged(int nq, int kind, const bsim_op* ops, int n) {
+    std::vector<EOp> e;
+    if (int rc = lower_ops(nq, kind, ops, n, e, false)) return rc;
+    Plan p;
+    if (int rc = make_plan(kind == BSIM_SV ? nq : 2 * nq, e, tile_config(), p)) return rc;
+    int k = 0;
+    for (auto& st : p.steps) k += (!st.tile && st.merged >= 0) ? 1 : 0;
     return k;
 }
 

@@ -265,3 +265,46 @@ def test_kraus_physical_4_and_5_qubits(k):
         got = dm.density_matrix()
         np.testing.assert_allclose(got.ravel(order="F"), ref, atol=TOL)
         assert abs(np.trace(got).real - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("n", [22, 24])
+def test_merged_dense_runs_vs_oracle(n):
+    """On vectors of 20+ bits the planner multiplies runs of adjacent dense
+    ops around 3+-target gates into one op of at most 6 bits (one
+    matrix-core launch instead of several passes): the fused evolve against
+    the oracle and against the per-op path."""
+    rng = np.random.default_rng(90 + n)
+    ins = []
+    for _ in range(6):
+        k = int(rng.integers(3, 5))
+        q = [int(x) for x in rng.choice(n, k, replace=False)]
+        ins.append(Instruction(G.Unitary(random_unitary(k, rng)), q))
+        ins.append(Instruction(G.Rx(float(rng.uniform(-3, 3))), [q[0]]))
+        q2 = [q[1]] + [int(x) for x in rng.choice([x for x in range(n) if x not in q], 2, replace=False)]
+        ins.append(Instruction(G.Unitary(random_unitary(3, rng)), q2))
+        ins.append(Instruction(G.H(), [int(rng.integers(n))]))
+    _say(f"merged dense runs n={n}")
+    sim = StateVectorSimulator(n, 0)
+    sim.evolve(ins, mode=0)
+    got = sim.state_vector()
+    np.testing.assert_allclose(got, O.evolve_sv(n, ins), atol=1e-10)
+    sim.reinit(n, 0)
+    sim.evolve(ins, mode=1)
+    np.testing.assert_allclose(got, sim.state_vector(), atol=1e-12)
+
+
+def test_merged_dm_gate_halves_vs_oracle():
+    """3-target gates on an 11-qubit density matrix (22-bit vector): each
+    gate's column and row halves merge into one 6-bit op."""
+    rng = np.random.default_rng(123)
+    m = 11
+    ins = [Instruction(G.H(), [q]) for q in range(m)]
+    for _ in range(4):
+        ins.append(Instruction(G.Unitary(random_unitary(3, rng)), [int(x) for x in rng.choice(m, 3, replace=False)]))
+        ins.append(Instruction(G.Ry(float(rng.uniform(-2, 2))), [int(rng.integers(m))]))
+    ref = O.evolve_dm(m, ins, "reference")
+    for mode in (0, 1):
+        _say(f"merged dm halves mode={mode}")
+        dm = DensityMatrixSimulator(m, 0)
+        dm.evolve(ins, mode=mode)
+        np.testing.assert_allclose(dm.density_matrix().ravel(order="F"), ref, atol=1e-10)

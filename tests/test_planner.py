@@ -50,6 +50,8 @@ def interp():
         f.argtypes = [ctypes.c_int, ctypes.POINTER(L.Op), ctypes.c_int, ctypes.c_void_p]
     h.plan_singles.restype = ctypes.c_int
     h.plan_singles.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(L.Op), ctypes.c_int]
+    h.plan_merged.restype = ctypes.c_int
+    h.plan_merged.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(L.Op), ctypes.c_int]
     h.plan_kind_count.restype = ctypes.c_int
     h.plan_kind_count.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(L.Op), ctypes.c_int, ctypes.c_int,
                                   ctypes.c_int]
@@ -338,3 +340,53 @@ def test_adjoint_back_plan_general_gates(interp, n):
         fd[k] = (_energy(n, A.with_params(ins, pmap, tp), words, coeffs)
                  - _energy(n, A.with_params(ins, pmap, tm), words, coeffs)) / (2 * hstep)
     np.testing.assert_allclose(grad, fd, atol=1e-8)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_planner_merges_dense_runs(interp, seed):
+    """Runs of adjacent dense ops around 3+-target gates are multiplied into
+    one op of at most 6 bits (a matrix-core launch instead of several HBM
+    passes; vectors of 20+ bits): the interpreted plan, merged steps
+    included, matches the oracle on random circuits with 3- and 4-target
+    unitaries, and merging happens; below 20 bits nothing is merged."""
+    from circuits import random_unitary
+
+    rng = np.random.default_rng(700 + seed)
+    n = 20
+    ins = []
+    for _ in range(5):
+        k = int(rng.integers(3, 5))
+        q = [int(x) for x in rng.choice(n, k, replace=False)]
+        ins.append(Instruction(G.Unitary(random_unitary(k, rng)), q))
+        # neighbours: inside the block, another 3-target gate sharing a qubit, a far gate
+        ins.append(Instruction(G.Rx(float(rng.uniform(-3, 3))), [q[0]]))
+        if rng.random() < 0.6:
+            q2 = [q[1]] + [int(x) for x in rng.choice([x for x in range(n) if x not in q], 2, replace=False)]
+            ins.append(Instruction(G.Unitary(random_unitary(3, rng)), q2))
+        ins.append(Instruction(G.CZ(), [int(x) for x in rng.choice(n, 2, replace=False)]))
+    ol = lower_instructions(ins, L.BSIM_SV)
+    assert interp.plan_merged(n, L.BSIM_SV, ol.ptr, ol.n) > 0
+    assert np.max(np.abs(_run(interp, n, ins) - O.evolve_sv(n, ins))) < 1e-12
+    small = lower_instructions([Instruction(i.operator, [q % 9 for q in i.target]) for i in ins
+                                if len(set(q % 9 for q in i.target)) == len(i.target)], L.BSIM_SV)
+    assert interp.plan_merged(9, L.BSIM_SV, small.ptr, small.n) == 0
+
+
+def test_planner_merges_dm_gate_halves(interp):
+    """A 3-target gate on an 11-qubit density matrix is U on 3 column bits
+    and conj(U) on 3 row bits: one merged 6-bit op (22-bit vector); the
+    interpreted plan matches the oracle."""
+    from circuits import random_unitary
+
+    rng = np.random.default_rng(5)
+    n = 11
+    ins = [Instruction(G.H(), [q]) for q in range(n)]
+    ins += [Instruction(G.Unitary(random_unitary(3, rng)), [0, 3, 4]), Instruction(G.Ry(0.7), [2]),
+            Instruction(G.Unitary(random_unitary(3, rng)), [4, 1, 2])]
+    ol = lower_instructions(ins, L.BSIM_DM)
+    assert interp.plan_merged(n, L.BSIM_DM, ol.ptr, ol.n) >= 2
+    rho = np.zeros(4 ** n, dtype=np.complex128)
+    rho[0] = 1
+    rc = interp.planexec_dm(n, ol.ptr, ol.n, rho.ctypes.data)
+    assert rc == 0
+    np.testing.assert_allclose(rho, O.evolve_dm(n, ins), atol=1e-12)
