@@ -455,7 +455,10 @@ def current_config() -> dict:
     L.call("bsim_set_sweep_bits", 0, ctypes.byref(rb))  # 0: read only
     L.call("bsim_set_fusion_limit", 0, ctypes.byref(fl))
     L.call("bsim_set_fusion_limit", fl.value, None)
-    return {"tile_bits": t.value, "low_bits": lo.value, "sweep_bits": rb.value, "fusion_limit": fl.value}
+    mw = ctypes.c_int()
+    L.call("bsim_set_merge_width", -1, ctypes.byref(mw))
+    return {"tile_bits": t.value, "low_bits": lo.value, "sweep_bits": rb.value, "fusion_limit": fl.value,
+            "merge_width": mw.value}
 
 
 def set_config(c: dict):
@@ -464,6 +467,7 @@ def set_config(c: dict):
     L.call("bsim_set_tile_config", c["tile_bits"], c["low_bits"])
     L.call("bsim_set_sweep_bits", c["sweep_bits"], None)
     L.call("bsim_set_fusion_limit", c["fusion_limit"], None)
+    L.call("bsim_set_merge_width", c["merge_width"], None)
 
 
 def autotune_leg(sim, ins) -> dict:
@@ -491,7 +495,7 @@ def autotune_leg(sim, ins) -> dict:
     best = sim.autotune(ins)
     t_search = time.perf_counter() - t0
     log(f"autotune: {best} in {t_search:.1f} s; default {default} {t_default * 1e3:.1f} ms")
-    return {"chosen": {k: best[k] for k in ("tile_bits", "low_bits", "sweep_bits", "fusion_limit")},
+    return {"chosen": {k: best[k] for k in ("tile_bits", "low_bits", "sweep_bits", "fusion_limit", "merge_width")},
             "chosen_ms": round(best["ms"], 3), "default": default, "default_ms": round(t_default * 1e3, 3),
             "search_s": round(t_search, 2)}
 
@@ -749,7 +753,7 @@ def dm_leg(nq: int, autotune: int):
         if autotune:
             t0 = time.perf_counter()
             best = sim.autotune(ins)
-            tuned = {k: best[k] for k in ("tile_bits", "low_bits", "sweep_bits", "fusion_limit")}
+            tuned = {k: best[k] for k in ("tile_bits", "low_bits", "sweep_bits", "fusion_limit", "merge_width")}
             tuned["search_s"] = round(time.perf_counter() - t0, 2)
         ol = lower_instructions(ins, L.BSIM_DM, mode)
         stats = (ctypes.c_int64 * 4)()

@@ -135,6 +135,7 @@ _SIGS = {
     "bsim_graph_destroy": (C.c_int, [C.c_void_p]),
     "bsim_autotune": (C.c_int, [C.c_void_p, C.POINTER(Op), C.c_int, _ip, _ip, _ip, _ip, _dp]),
     "bsim_set_max_blocks": (C.c_int, [C.c_int, _ip]),
+    "bsim_set_merge_width": (C.c_int, [C.c_int, _ip]),
     "bsim_evolve_noalloc": (C.c_int, [C.c_void_p, C.POINTER(Op), C.c_int]),
     "bsim_partial_trace": (C.c_int, [C.c_void_p, C.c_int, _ip, _dp]),
     "bsim_set_sweep_bits": (C.c_int, [C.c_int, _ip]),

@@ -443,13 +443,15 @@ class _Simulator:
         """Time the fused evolution of ``operations`` for every tile
         configuration, then every fusion width at the best one, and keep the
         fastest (bsim_autotune).  Returns {tile_bits, low_bits, sweep_bits,
-        fusion_limit, ms}; the state is left unspecified."""
+        fusion_limit, merge_width, ms}; the state is left unspecified."""
         ol = lower_instructions(list(operations), self.kind, getattr(self, "noise_mode", "reference"))
         t, lo, rb, fl, ms = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_double()
         L.call("bsim_autotune", self._h, ol.ptr, ol.n, C.byref(t), C.byref(lo), C.byref(rb), C.byref(fl),
                C.byref(ms))
+        mw = C.c_int()
+        L.call("bsim_set_merge_width", -1, C.byref(mw))
         return {"tile_bits": t.value, "low_bits": lo.value, "sweep_bits": rb.value, "fusion_limit": fl.value,
-                "ms": ms.value}
+                "merge_width": mw.value, "ms": ms.value}
 
     def _apply_call(self, call: KernelCall, vq_targets, conj: int):
         lib = L.load()

@@ -129,8 +129,10 @@ struct TileConfig {
     // the fusion width so that a pass stays HBM-bound.
     int max_cost = 0;
     // merge runs of adjacent dense ops that do not fit a tile pass (3+
-    // target bits) with their neighbours into one op of at most 6 bits
+    // target bits) with their neighbours into one op of at most merge_bits
+    // bits (0: no merging; 3..6)
     bool merge_dense = true;
+    int merge_bits = 6;
 };
 // A snapshot of the library-wide configuration (taken under a lock), and
 // its setter; every entry point plans with one snapshot.

@@ -564,8 +564,9 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
             // a dense op on 3+ bits: take the adjacent dense ops that either
             // fit inside its bits or are themselves 3+-bit ops, while the
             // union stays within kMergeBits, into one product
-            const int max_bits = vq >= kMerge6MinBits ? kMergeBits : kMergeBits - 1;
-            if (cfg.merge_dense && vq >= kMergeMinBits && eops[head].kind == E_DENSE && mergeable(eops[head]) &&
+            const int max_bits = std::min(cfg.merge_bits, vq >= kMerge6MinBits ? kMergeBits : kMergeBits - 1);
+            if (cfg.merge_dense && max_bits >= 3 && vq >= kMergeMinBits && eops[head].kind == E_DENSE &&
+                mergeable(eops[head]) &&
                 eops[head].g.k <= max_bits) {
                 std::vector<size_t> group{ head };
                 uint64_t U = target_mask(eops[head]);
@@ -1216,6 +1217,15 @@ int bsim_set_sweep_bits(int bits, int* old) {
     std::lock_guard<std::mutex> lock(g_cfg_mu);
     if (old) *old = g_cfg.reg_bits;
     if (bits) g_cfg.reg_bits = bits;
+    return BSIM_OK;
+}
+
+int bsim_set_merge_width(int bits, int* old) {
+    if (bits < -1 || bits == 1 || bits == 2 || bits > 6)
+        return fail(BSIM_EINVAL, "merge width must be 0 (off) or 3..6 (-1: read only)");
+    std::lock_guard<std::mutex> lock(g_cfg_mu);
+    if (old) *old = g_cfg.merge_bits;
+    if (bits >= 0) g_cfg.merge_bits = bits;
     return BSIM_OK;
 }
 

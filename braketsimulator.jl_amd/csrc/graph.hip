@@ -188,8 +188,10 @@ int bsim_autotune(bsim_state* h, const bsim_op* ops, int n_ops, int* tile_bits, 
             if (int rc = trial(c)) return rc;
         }
     }
-    // stage 3: the fusion width (FP64 operations per amplitude one pass may
-    // spend; 0 = unlimited) at the best tile configuration
+    // stage 3: the fusion width of the tile passes (FP64 operations per
+    // amplitude one pass may spend; 0 = unlimited) at the best tile
+    // configuration; stage 4: the width of the merged dense blocks (0 = no
+    // merging, 4, 5, 6 bits), only when the plan has merged steps
     if (best < DBL_MAX) {
         static const int limits[] = { 0, 48, 96, 192, 384 };
         const TileConfig tiled = best_cfg;
@@ -198,6 +200,17 @@ int bsim_autotune(bsim_state* h, const bsim_op* ops, int n_ops, int* tile_bits, 
             TileConfig c = tiled;
             c.max_cost = lim;
             if (int rc = trial(c)) return rc;
+        }
+        Plan probe;
+        if (int rc = make_plan(h->vq, eops, best_cfg, probe)) return rc;
+        if (!probe.merged.empty()) {
+            const TileConfig fused = best_cfg;
+            for (int mb : { 0, 4, 5, 6 }) {
+                if (mb == fused.merge_bits) continue;
+                TileConfig c = fused;
+                c.merge_bits = mb;
+                if (int rc = trial(c)) return rc;
+            }
         }
     }
     if (best == DBL_MAX) {  // no candidate fits this state: keep the current configuration

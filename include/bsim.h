@@ -211,6 +211,13 @@ int bsim_set_sweep_bits(int bits, int* old);
  * may spend (1-qubit gate 8, 2-qubit 32, diagonal 4, CZ-type sign 1);
  * 0 = unlimited.  Returns the previous value in *old (may be NULL). */
 int bsim_set_fusion_limit(int max_cost, int* old);
+/* Width of the merged dense blocks of the fused path: on vectors of 20+
+ * bits, runs of adjacent dense ops around a 3+-target op are multiplied
+ * into one op of at most `bits` bits (6 by default, 5 below 22 bits) that
+ * runs on the FP64 matrix cores; 0 = no merging, -1 = read only.  *old
+ * (may be NULL) receives the previous value.  No reference counterpart
+ * (a tuning knob; the autotuner searches it). */
+int bsim_set_merge_width(int bits, int* old);
 
 /* Kernel timing of the fused path (no reference counterpart; measurement
  * only): when enabled, every tile-pass launch is bracketed by HIP events on
@@ -392,8 +399,10 @@ int bsim_graph_destroy(bsim_graph* g);
  * with 3 or 4 sweep register bits at the current contiguous low bits, then
  * low bits 2..min(6, tile_bits-2) at the best shape, then the fusion width
  * (bsim_set_fusion_limit: 0 = unlimited, 48, 96, 192, 384 FP64 operations
- * per amplitude and pass); each candidate is one warm-up run and the best
- * of two timed runs (one when a run takes over 20 ms).  It keeps the
+ * per amplitude and pass), then -- when the plan has merged dense blocks --
+ * their width (bsim_set_merge_width: off, 4, 5, 6 bits; read it back with
+ * bsim_set_merge_width(-1, &w)); each candidate is one warm-up run and the
+ * best of two timed runs (one when a run takes over 20 ms).  It keeps the
  * fastest as the library configuration and returns it (outputs may be
  * NULL; *best_ms = 0 when no candidate fits and the configuration is
  * unchanged).  Candidates are planned with private configurations: other

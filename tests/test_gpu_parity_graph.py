@@ -39,10 +39,13 @@ def keep_tile_config():
     L.call("bsim_set_sweep_bits", 0, ctypes.byref(rb))
     L.call("bsim_set_fusion_limit", 0, ctypes.byref(fl))
     L.call("bsim_set_fusion_limit", fl.value, None)
+    mw = ctypes.c_int()
+    L.call("bsim_set_merge_width", -1, ctypes.byref(mw))
     yield
     L.call("bsim_set_tile_config", t.value, lo.value)
     L.call("bsim_set_sweep_bits", rb.value, None)
     L.call("bsim_set_fusion_limit", fl.value, None)
+    L.call("bsim_set_merge_width", mw.value, None)
 
 
 def _plan_singles(n, kind, ins, mode="physical"):
@@ -213,3 +216,26 @@ def test_graph_after_reinit_is_refused():
     assert lib.bsim_graph_launch(graph) == L.BSIM_EINVAL
     assert "destroyed" in L.last_error()
     assert lib.bsim_graph_destroy(graph) == 0
+
+
+def test_autotune_merge_width(keep_tile_config):
+    """A 21-qubit circuit of 3- and 4-target unitaries: the autotuner also
+    searches the merged block width (off, 4, 5 bits here), and every width
+    gives the oracle's state."""
+    n = 21
+    rng = np.random.default_rng(17)
+    ins = []
+    for _ in range(5):
+        k = int(rng.integers(3, 5))
+        q = [int(x) for x in rng.choice(n, k, replace=False)]
+        ins.append(Instruction(G.Unitary(random_unitary(k, rng)), q))
+        ins.append(Instruction(G.Rx(0.4), [q[0]]))
+    exp = O.evolve_sv(n, ins)
+    _say("autotune merge width 21q")
+    r = StateVectorSimulator(n, 0).autotune(ins)
+    assert r["merge_width"] in (0, 4, 5, 6) and r["ms"] > 0
+    for mw in (0, 4, 5, 6):
+        L.call("bsim_set_merge_width", mw, None)
+        sim = StateVectorSimulator(n, 0)
+        sim.evolve(ins)
+        np.testing.assert_allclose(sim.state_vector(), exp, atol=1e-10, err_msg=f"merge width {mw}")

@@ -239,3 +239,27 @@ def test_fast_lowering_matches_general_path():
                   "param_index"):
             np.testing.assert_array_equal(a._rec[f], b._rec[f], err_msg=f)
         assert a._buf.tobytes() == b._buf.tobytes()
+
+
+def test_merge_width_setting():
+    """bsim_set_merge_width: -1 reads, 0 and 3..6 set, anything else is refused."""
+    import ctypes
+
+    import pytest
+
+    from bsim import _lib as L
+
+    old = ctypes.c_int()
+    L.call("bsim_set_merge_width", -1, ctypes.byref(old))
+    assert old.value in (0, 3, 4, 5, 6)
+    try:
+        for v in (0, 3, 5, 6):
+            L.call("bsim_set_merge_width", v, None)
+            got = ctypes.c_int()
+            L.call("bsim_set_merge_width", -1, ctypes.byref(got))
+            assert got.value == v
+        for bad in (-2, 1, 2, 7):
+            with pytest.raises(L.BsimError):
+                L.call("bsim_set_merge_width", bad, None)
+    finally:
+        L.call("bsim_set_merge_width", old.value, None)
