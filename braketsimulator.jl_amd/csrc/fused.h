@@ -27,6 +27,12 @@ struct EOp {
     // ((i ^ (i >> kshift)) & kmask) == 0.
     uint64_t kmask = 0;
     int kshift = 0;
+    // kpres: the superoperator links no block entry whose stored row and
+    // column agree on the channel qubits with one where they differ.  Such a
+    // predicated op commutes with every diagonal op and every other kpres op
+    // on other bits (both keep each other's predicate); without it the op
+    // commutes with nothing, since its predicate reads every other bit.
+    bool kpres = false;
     // dense on 4 bits that may run as a tile op (the 16x16 superoperator of a
     // 2-qubit physical channel); other 3+-bit dense ops launch on their own
     bool tile4 = false;
@@ -36,6 +42,9 @@ struct EOp {
     std::vector<double2> dm[3];
     int aslot = -1;              // fused adjoint: partial-sum slot of the op's parameters
     uint64_t touched() const;    // every vector bit the op reads, writes or depends on
+                                 // (a kpres Kraus op: its row/column bits; its predicate
+                                 // is accounted for by pair_diag)
+    bool pair_diag() const;      // keeps every row/column pair predicate: diagonal ops, kpres Kraus
     bool fusable() const;        // fits a tile sweep (dense k <= 2, diag k <= 4, reference Kraus kq <= 2)
 };
 

@@ -37,11 +37,13 @@ void set_tile_config(const TileConfig& c) {
 }
 
 uint64_t EOp::touched() const {
-    if (kind == E_KRAUS_REF) return ~0ull;
+    if (kind == E_KRAUS_REF && !kpres) return ~0ull;
     uint64_t t = g.cmask;
     for (int b = 0; b < g.k; ++b) t |= 1ull << g.tb[b];
     return t;
 }
+
+bool EOp::pair_diag() const { return kind == E_DIAG || (kind == E_KRAUS_REF && kpres); }
 
 bool EOp::fusable() const {
     if (kind == E_DENSE) return g.k <= 2 || (g.k == 4 && tile4);
@@ -182,6 +184,15 @@ int lower_ops(int nq, int kind, const bsim_op* ops, int n_ops, std::vector<EOp>&
                         return rc;
                     e.kmask = (((1ull << nq) - 1) & ~rowbits);
                     e.kshift = nq;
+                    // exact zeros only: a rounding-level link keeps the op a full barrier
+                    e.kpres = true;
+                    for (size_t Jo = 0; Jo < DD && e.kpres; ++Jo)
+                        for (size_t Ji = 0; Ji < DD; ++Ji)
+                            if ((Jo % D == Jo / D) != (Ji % D == Ji / D) &&
+                                (sup[2 * (Jo + DD * Ji)] != 0.0 || sup[2 * (Jo + DD * Ji) + 1] != 0.0)) {
+                                e.kpres = false;
+                                break;
+                            }
                 }
                 out.push_back(std::move(e));
             } else {
@@ -543,6 +554,27 @@ static EOp merge_group(const std::vector<EOp>& eops, const std::vector<size_t>& 
     return e;
 }
 
+// Ops held back while a pass (or a sweep) takes later ops ahead of them.  A
+// later op may move ahead only if it commutes with every held-back op: its
+// bits are disjoint from theirs, and -- since a reference Kraus op's
+// predicate reads every other row/column pair -- a Kraus op passes only
+// pair-keeping ops (pair_diag), and only pair-keeping ops pass a Kraus op.
+struct HeldBack {
+    uint64_t bits = 0;
+    bool kraus = false;   // a kpres Kraus op is held back
+    bool mixing = false;  // an op that is not pair_diag is held back
+    bool blocks(const EOp& e) const {
+        if (e.touched() & bits) return true;
+        if (e.kind == E_KRAUS_REF && mixing) return true;
+        return kraus && !e.pair_diag();
+    }
+    void add(const EOp& e) {
+        bits |= e.touched();
+        kraus |= e.kind == E_KRAUS_REF;
+        mixing |= !e.pair_diag();
+    }
+};
+
 int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan& plan) {
     plan = Plan();
     const size_t n = eops.size();
@@ -598,19 +630,18 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
         uint64_t S = (L > 0) ? ((1ull << L) - 1) : 0;
         if ((eops[head].kind == E_DENSE || eops[head].kind == E_KRAUS_REF) && popc(S | target_mask(eops[head])) > T)
             S = 0;
-        uint64_t blocked = 0;
+        HeldBack held;
         int cost = 0;
         inc.clear();
         for (size_t i = head; i < n; ++i) {
             if (done[i]) continue;
             const EOp& e = eops[i];
-            uint64_t t = e.touched();
             int c = op_cost(e);
             bool over = cfg.max_cost > 0 && !inc.empty() && cost + c > cfg.max_cost;
-            if (t & blocked) {
-                blocked |= t;
+            if (held.blocks(e)) {
+                held.add(e);
             } else if (!e.fusable() || over) {
-                blocked |= t;
+                held.add(e);
             } else if (e.kind == E_DENSE || e.kind == E_KRAUS_REF) {
                 uint64_t need = target_mask(e);
                 if ((need & ~S) == 0 || popc(S | need) <= T) {
@@ -618,13 +649,13 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
                     inc.push_back((int)i);
                     cost += c;
                 } else {
-                    blocked |= t;
+                    held.add(e);
                 }
             } else {
                 inc.push_back((int)i);
                 cost += c;
             }
-            if ((blocked & all) == all) break;
+            if ((held.bits & all) == all) break;
         }
         if (inc.empty()) {  // cannot happen (the head op always fits); never loop without progress
             plan.steps.push_back({ false, -1, (int)head });
@@ -658,14 +689,13 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
         std::vector<int> next;
         while (!rem.empty()) {
             uint32_t W = 0;  // local bits
-            uint64_t blk = 0;
+            HeldBack blk;
             std::vector<int> take;
             next.clear();
             for (int i : rem) {
                 const EOp& e = eops[i];
-                uint64_t t = e.touched();
-                if (t & blk) {
-                    blk |= t;
+                if (blk.blocks(e)) {
+                    blk.add(e);
                     next.push_back(i);
                     continue;
                 }
@@ -676,7 +706,7 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
                         W |= need;
                         take.push_back(i);
                     } else {
-                        blk |= t;
+                        blk.add(e);
                         next.push_back(i);
                     }
                 } else {

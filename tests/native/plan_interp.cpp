@@ -176,6 +176,14 @@ extern "C" int plan_singles(int nq, int kind, const bsim_op* ops, int n) {
     for (auto& st : p.steps) k += st.tile ? 0 : 1;
     return k;
 }
+// number of tile passes
+extern "C" int plan_passes(int nq, int kind, const bsim_op* ops, int n) {
+    std::vector<EOp> e;
+    if (int rc = lower_ops(nq, kind, ops, n, e, false)) return rc;
+    Plan p;
+    if (int rc = make_plan(kind == BSIM_SV ? nq : 2 * nq, e, tile_config(), p)) return rc;
+    return (int)p.passes.size();
+}
 // number of single steps that launch a merged run of ops
 extern "C" int plan_merged(int nq, int kind, const bsim_op* ops, int n) {
     std::vector<EOp> e;

@@ -52,6 +52,8 @@ def interp():
     h.plan_singles.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(L.Op), ctypes.c_int]
     h.plan_merged.restype = ctypes.c_int
     h.plan_merged.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(L.Op), ctypes.c_int]
+    h.plan_passes.restype = ctypes.c_int
+    h.plan_passes.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(L.Op), ctypes.c_int]
     h.plan_kind_count.restype = ctypes.c_int
     h.plan_kind_count.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(L.Op), ctypes.c_int, ctypes.c_int,
                                   ctypes.c_int]
@@ -236,6 +238,63 @@ def test_planner_dm_noisy_brickwall(interp, tile_config, tile, low, mode):
     L.call("bsim_set_tile_config", tile, low)
     ins = W.brickwall(6, 3, seed=2, noise=True)
     np.testing.assert_allclose(_run_dm(interp, 6, ins, mode), O.evolve_dm(6, ins, mode), atol=1e-12)
+
+
+def _generic_channel(k, rng):
+    """Two Kraus matrices from a random isometry: a channel whose block
+    superoperator links row == column states of its qubits with the rest,
+    so the planner must keep it a full barrier."""
+    d = 2 ** k
+    q = np.linalg.qr(rng.normal(size=(2 * d, d)) + 1j * rng.normal(size=(2 * d, d)))[0]
+    return Nz.Kraus([q[:d], q[d:]])
+
+
+@pytest.mark.parametrize("mode", ["reference", "physical"])
+@pytest.mark.parametrize("tile,low", [(6, 0), (8, 3)])
+@pytest.mark.parametrize("seed", range(4))
+def test_planner_dm_kraus_reordering(interp, tile_config, tile, low, mode, seed):
+    """Reference-semantics Kraus ops move past each other and past diagonal
+    ops on other qubits when both keep the row/column pair predicates
+    (depolarizing, damping, dephasing, Pauli channels), and stay barriers
+    when they do not (generic channels).  Small tiles force many passes, so
+    ops are taken ahead of held-back ones; the interpreted plan must match
+    the oracle's loops in program order."""
+    L.call("bsim_set_tile_config", tile, low)
+    rng = np.random.default_rng(900 + seed)
+    n = 6
+    ins = []
+    for _ in range(3):
+        ins += random_circuit(n, 8, rng, kinds=("1q", "2q", "diag"))
+        for q in rng.permutation(n):
+            r = rng.random()
+            if r < 0.25:
+                ins.append(Instruction(_generic_channel(1, rng), [int(q)]))
+            elif r < 0.6:
+                ins.append(Instruction(Nz.AmplitudeDamping(float(rng.uniform(0, 0.4))), [int(q)]))
+            else:
+                ins.append(Instruction(Nz.Depolarizing(float(rng.uniform(0, 0.3))), [int(q)]))
+        for q in range(seed % 2, n - 1, 2):
+            ins.append(Instruction(G.CZ(), [q, q + 1]))
+            ins.append(Instruction(_generic_channel(2, rng) if rng.random() < 0.3 else
+                                   Nz.TwoQubitDepolarizing(float(rng.uniform(0, 0.2))), [q, q + 1]))
+        ins.append(Instruction(Nz.PhaseFlip(0.1), [int(rng.integers(n))]))
+    np.testing.assert_allclose(_run_dm(interp, n, ins, mode), O.evolve_dm(n, ins, mode), atol=1e-12)
+
+
+def test_planner_dm_kraus_pass_count(interp):
+    """The 14-qubit noisy brick-wall of depth 10 in reference semantics: with
+    the channels commuting past each other the plan needs 51 passes (90
+    when every Kraus op is a barrier); the same circuit with generic
+    channels in place of AmplitudeDamping keeps them barriers."""
+    ins = W.brickwall(14, 10, seed=0, noise=True)
+    ol = lower_instructions(ins, L.BSIM_DM, "reference")
+    good = interp.plan_passes(14, L.BSIM_DM, ol.ptr, ol.n)
+    assert 0 < good <= 52
+    rng = np.random.default_rng(5)
+    gen = [Instruction(_generic_channel(1, rng), g.target) if isinstance(g.operator, Nz.AmplitudeDamping) else g
+           for g in ins]
+    ol = lower_instructions(gen, L.BSIM_DM, "reference")
+    assert interp.plan_passes(14, L.BSIM_DM, ol.ptr, ol.n) > good + 10
 
 
 # ---- fused adjoint: the backward plan ---------------------------------------
