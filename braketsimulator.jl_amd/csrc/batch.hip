@@ -392,6 +392,7 @@ static void plan_task(const bsim_task& t, const TileConfig& base, TaskPlan& out)
     }
     TileConfig cfg = base;
     cfg.reg_bits = 4;  // the batch kernel runs 256-thread, 16-amplitude sweeps
+    cfg.chains = false;  // and the full tile op set, which has no register chains
     if ((out.rc = make_plan(t.qubit_count, eops, cfg, out.plan)) != BSIM_OK) {
         out.msg = last_error_text();
         return;

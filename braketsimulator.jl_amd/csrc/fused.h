@@ -59,7 +59,16 @@ struct TOp {
     // 0 dense1, 1 dense2, 2 diag, 3 sign parity, 4 predicated superoperator,
     // 5 its matrix, 6 diagonal on register bits only (m[j] = the factor of
     // register j, so every thread reads the same entries), 7 X on register a
-    // (swaps, 3-register sweeps only; m holds the X matrix)
+    // (swaps, 3-register sweeps only; m holds the X matrix).
+    // Register chains (12-bit, 4-register passes of the lean streaming
+    // kernels): each thread keeps a 2x2 matrix A and, with
+    // t(mask) = parity(gbase & mask) read from its sub-cube base,
+    //   8  A <- U Z^t(cmask)             (U = m[0..3], column-major)
+    //   9  A <- U Z^t(cmask) A
+    //   10 A <- Z^t(cmask) A, then A is applied to register a
+    //   11 A <- (-1)^P(gbase) A, P the sign op's pair form (its packed
+    //      copy: nd = pair distances, cmask >> 32 = pair count, m[0..1] =
+    //      pair masks)
     int32_t kind;
     int32_t a, b;      // dense: register bits (a < b for dense2)
     int32_t nd;        // diag: bit count; kind 4: TOps that follow with the matrix (0 or 15)
@@ -68,7 +77,11 @@ struct TOp {
     // sign: amplitude i is negated when sum_k popcount(i & (i >> pd[k]) & pm[k]) is odd
     uint64_t pm[4];
     int32_t pd[4];
-    int32_t npairs, pad_;
+    int32_t npairs;
+    // sign op left by the register-chain rewrite (the per-thread sign and
+    // the Z factors of registers without a dense op): defined by its split
+    // form below only, its pairs hold just the per-thread part
+    int32_t split_only;
     // Split of the index tests over a sweep (computed by the planner): a
     // register j of a sub-cube has index gbase | r_j, gbase the sub-cube's
     // base and r_j the global bits of j's register bits.
@@ -142,6 +155,10 @@ struct TileConfig {
     // bits (0: no merging; 3..6)
     bool merge_dense = true;
     int merge_bits = 6;
+    // rewrite the sweeps of 12-bit, 4-register passes into register chains
+    // (tile op kinds 8..11, run by the lean streaming kernels only); callers
+    // whose kernels take the full op set (the batch kernel) turn it off
+    bool chains = true;
 };
 // A snapshot of the library-wide configuration (taken under a lock), and
 // its setter; every entry point plans with one snapshot.

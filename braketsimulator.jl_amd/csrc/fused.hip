@@ -575,6 +575,262 @@ struct HeldBack {
     }
 };
 
+// ---- register chains -------------------------------------------------------
+//
+// A sign op of a sweep factors, for register j of a sub-cube with base g,
+// into (-1)^P(g) (one sign per thread), the uniform pattern jmask, and a
+// per-thread Z on each register bit r, Z_r^t with t = parity(g & sgm[r]).
+// When jmask is affine in the register bits (no CZ between two register
+// bits; its linear terms are uniform Z_r), every factor acts on one register
+// bit or on the whole sub-cube, so the sweep splits into one chain per
+// register bit -- its 1-register dense ops and Z factors in order -- and the
+// chains commute with each other; a 2-register dense op closes the chains of
+// its two bits.  A chain
+//   A = Z^t_end U_n Z^t_(n-1) ... U_1 Z^t_0
+// is multiplied per thread (kinds 8, 9: a sign flip of a row or column and
+// 32 FP64 instructions per further U) and applied once (kind 10: 128), where
+// the unchained sweep applies every U to the sub-cube (128 each) and every
+// sign op (about 100 VALU instructions each).  The per-thread sign P of all
+// sign ops rides on one chain (kind 11).
+using cd = std::complex<double>;
+
+namespace {
+struct ChainItem {
+    bool u;          // a uniform 2x2 (column-major) or a per-thread Z
+    cd m[4];
+    uint64_t mask;   // Z: t = parity(gbase & mask)
+};
+}  // namespace
+
+static void mat2_mul(const cd* a, const cd* b, cd* c) {  // c = a b, column-major
+    cd t[4];
+    t[0] = a[0] * b[0] + a[2] * b[1];
+    t[1] = a[1] * b[0] + a[3] * b[1];
+    t[2] = a[0] * b[2] + a[2] * b[3];
+    t[3] = a[1] * b[2] + a[3] * b[3];
+    for (int i = 0; i < 4; ++i) c[i] = t[i];
+}
+
+static TOp chain_top(int kind, int a, const cd* m, uint64_t mask) {
+    TOp op;
+    memset(&op, 0, sizeof(op));
+    op.kind = kind;
+    op.a = a;
+    op.cmask = mask;
+    if (m)
+        for (int i = 0; i < 4; ++i) op.m[i] = make_double2(m[i].real(), m[i].imag());
+    return op;
+}
+
+// VALU instructions per thread of a sweep's tile ops (the chain rule's cost
+// comparison): dense 1- and 2-register ops, sign ops, chain ops.
+static int sweep_valu(const TOp* t, size_t n) {
+    int c = 0;
+    for (size_t i = 0; i < n; ++i) {
+        switch (t[i].kind) {
+            case 0: c += 150; break;
+            case 1: c += 290; break;
+            case 3: c += 100; break;
+            case 8: c += 20; break;
+            case 9: c += 45; break;
+            case 10: c += 145; break;
+            case 11: c += 20; break;
+            default: c += 150; break;
+        }
+    }
+    return c;
+}
+
+// Rewrite one sweep's tile ops [in, in + n) into chains; false (out
+// untouched) when the sweep does not qualify or the chains cost more.
+static bool chain_sweep(const TOp* in, size_t n, std::vector<TOp>& out) {
+    std::vector<ChainItem> ch[4];
+    std::vector<TOp> res;
+    uint64_t ppm[4] = { 0, 0, 0, 0 }, rsgm[4] = { 0, 0, 0, 0 };
+    int ppd[4] = { 0, 0, 0, 0 }, pn = 0;
+    bool any_sign = false;
+    const cd Zm[4] = { 1.0, 0.0, 0.0, -1.0 };
+    auto addP = [&](uint64_t pm, int pd) {
+        for (int k = 0; k < pn; ++k)
+            if (ppd[k] == pd) {
+                ppm[k] ^= pm;
+                return true;
+            }
+        if (pn == 4) return false;
+        ppm[pn] = pm;
+        ppd[pn++] = pd;
+        return true;
+    };
+    // emit the chain of register r; at_end: a chain without a dense op (Z
+    // factors only) joins the sweep's closing sign op, otherwise (a 2-register
+    // op follows on r) it is a sign op of its own here
+    auto flush = [&](int r, bool at_end) {
+        std::vector<ChainItem> v;  // normalized: adjacent U multiplied, adjacent Z combined
+        for (const ChainItem& it : ch[r]) {
+            if (it.u) {
+                if (!v.empty() && v.back().u) mat2_mul(it.m, v.back().m, v.back().m);
+                else v.push_back(it);
+            } else {
+                if (!v.empty() && !v.back().u) {
+                    v.back().mask ^= it.mask;
+                    if (!v.back().mask) v.pop_back();
+                } else if (it.mask) {
+                    v.push_back(it);
+                }
+                // a zero mask between two dense ops lets them multiply
+                if (v.size() >= 2 && v.back().u && v[v.size() - 2].u) {
+                    mat2_mul(v.back().m, v[v.size() - 2].m, v[v.size() - 2].m);
+                    v.pop_back();
+                }
+            }
+        }
+        ch[r].clear();
+        bool has_u = false;
+        for (const ChainItem& it : v) has_u = has_u || it.u;
+        if (!has_u) {
+            if (v.empty()) return;
+            if (at_end) {
+                rsgm[r] ^= v[0].mask;
+            } else {
+                TOp s;
+                memset(&s, 0, sizeof(s));
+                s.kind = 3;
+                s.split_only = 1;
+                s.sgm[r] = v[0].mask;
+                res.push_back(s);
+            }
+            return;
+        }
+        size_t i = 0;
+        uint64_t z0 = 0;
+        if (!v[0].u) z0 = v[i++].mask;
+        if (v.size() - i == 1 && !z0) {  // one dense op, no Z: a plain dense op
+            TOp op = chain_top(0, r, v[i].m, 0);
+            res.push_back(op);
+            return;
+        }
+        res.push_back(chain_top(8, r, v[i].m, z0));
+        ++i;
+        uint64_t zend = 0;
+        while (i < v.size()) {
+            const uint64_t z = v[i++].mask;  // v alternates: a Z, then a U or the end
+            if (i < v.size()) {
+                res.push_back(chain_top(9, r, v[i].m, z));
+                ++i;
+            } else {
+                zend = z;
+            }
+        }
+        res.push_back(chain_top(10, r, nullptr, zend));
+    };
+    for (size_t o = 0; o < n; ++o) {
+        const TOp& t = in[o];
+        if (t.nparam) return false;
+        if (t.kind == 0 && t.cmask == 0) {
+            ChainItem it{ true, {}, 0 };
+            for (int i = 0; i < 4; ++i) it.m[i] = cd(t.m[i].x, t.m[i].y);
+            ch[t.a].push_back(it);
+        } else if (t.kind == 1 && t.cmask == 0) {
+            flush(t.a, false);
+            flush(t.b, false);
+            res.push_back(t);
+        } else if (t.kind == 3) {
+            any_sign = true;
+            // jmask affine in the register bits: bit j = XOR of c_r over the set bits r of j
+            const uint32_t jm = t.jmask;
+            if (jm & 1u) return false;
+            int c[4];
+            for (int r = 0; r < 4; ++r) c[r] = (jm >> (1u << r)) & 1u;
+            for (int j = 0; j < 16; ++j) {
+                int b = 0;
+                for (int r = 0; r < 4; ++r)
+                    if ((j >> r) & 1) b ^= c[r];
+                if (((jm >> j) & 1u) != (uint32_t)b) return false;
+            }
+            for (int r = 0; r < 4; ++r) {
+                if (c[r]) ch[r].push_back(ChainItem{ true, { Zm[0], Zm[1], Zm[2], Zm[3] }, 0 });
+                if (t.sgm[r]) ch[r].push_back(ChainItem{ false, {}, t.sgm[r] });
+            }
+            for (int k = 0; k < t.npairs; ++k)
+                if (t.pm[k] && !addP(t.pm[k], t.pd[k])) return false;
+        } else {
+            return false;
+        }
+    }
+    if (!any_sign) return false;  // nothing to gain: the planner's own fusion stands
+    for (int r = 0; r < 4; ++r) flush(r, true);
+    bool p_done = pn == 0;
+    if (!p_done)
+        for (size_t i = 0; i < res.size(); ++i)
+            if (res[i].kind == 8) {
+                TOp p;
+                memset(&p, 0, sizeof(p));
+                p.kind = 11;
+                for (int k = 0; k < pn; ++k) {
+                    p.nd |= (ppd[k] & 0xFF) << (8 * k);
+                    reinterpret_cast<uint64_t*>(p.m)[k] = ppm[k];
+                }
+                p.cmask = (uint64_t)pn << 32;
+                res.insert(res.begin() + (long)i + 1, p);
+                p_done = true;
+                break;
+            }
+    if (!p_done || rsgm[0] || rsgm[1] || rsgm[2] || rsgm[3]) {
+        TOp s;
+        memset(&s, 0, sizeof(s));
+        s.kind = 3;
+        s.split_only = 1;
+        if (!p_done) {
+            s.npairs = pn;
+            for (int k = 0; k < pn; ++k) {
+                s.pm[k] = ppm[k];
+                s.pd[k] = ppd[k];
+            }
+        }
+        for (int r = 0; r < 4; ++r) s.sgm[r] = rsgm[r];
+        res.push_back(s);
+    }
+    if (sweep_valu(res.data(), res.size()) >= sweep_valu(in, n)) return false;
+    out.insert(out.end(), res.begin(), res.end());
+    return true;
+}
+
+// Chains for every sweep of pass P when the pass runs on a lean streaming
+// kernel (12 tile bits, 4 register bits, only uncontrolled dense 1-/2-register
+// ops and sign ops, plus reference Kraus ops in sweeps left as they are).
+static void chain_pass(Plan& plan, const Pass& P) {
+    const int ob = plan.sweeps[P.sweep_begin].op_begin;
+    const int oe = plan.sweeps[P.sweep_begin + P.nsweeps - 1].op_end;
+    for (int o = ob; o < oe; ++o) {
+        const TOp& t = plan.tops[o];
+        if (t.nparam) return;
+        if (t.kind == 4) {
+            o += t.nd;
+            continue;
+        }
+        if (t.kind != 3 && ((t.kind != 0 && t.kind != 1) || t.cmask != 0)) return;
+    }
+    std::vector<TOp> tops(plan.tops.begin(), plan.tops.begin() + ob);
+    std::vector<int> src(plan.top_src.begin(), plan.top_src.begin() + ob);
+    bool changed = false;
+    for (int s = P.sweep_begin; s < P.sweep_begin + P.nsweeps; ++s) {
+        Sweep& sw = plan.sweeps[s];
+        const int b = (int)tops.size();
+        if (chain_sweep(plan.tops.data() + sw.op_begin, (size_t)(sw.op_end - sw.op_begin), tops)) {
+            changed = true;
+            src.insert(src.end(), tops.size() - b, plan.top_src[sw.op_begin]);
+        } else {
+            tops.insert(tops.end(), plan.tops.begin() + sw.op_begin, plan.tops.begin() + sw.op_end);
+            src.insert(src.end(), plan.top_src.begin() + sw.op_begin, plan.top_src.begin() + sw.op_end);
+        }
+        sw.op_begin = b;
+        sw.op_end = (int)tops.size();
+    }
+    if (!changed) return;
+    plan.tops.swap(tops);
+    plan.top_src.swap(src);
+}
+
 int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan& plan) {
     plan = Plan();
     const size_t n = eops.size();
@@ -922,6 +1178,8 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
             rem.swap(next);
         }
         P.nsweeps = (int)plan.sweeps.size() - P.sweep_begin;
+        // the shape the streaming kernels take (stream_kind below)
+        if (cfg.chains && P.T == 12 && P.R == 4 && vq >= 12 && P.nsweeps > 0) chain_pass(plan, P);
         for (int i : inc) done[i] = 1;
         plan.steps.push_back({ true, (int)plan.passes.size(), -1 });
         plan.passes.push_back(P);
@@ -1121,6 +1379,7 @@ static StreamKind stream_kind(const Plan& plan, const Pass& P, int vq) {
                 o += t.nd;  // a 4-bit Kraus op's matrices follow its head
                 continue;
             }
+            if (t.kind >= 8) continue;  // register chains: lean op sets only
             if (t.kind != 3 && ((t.kind != 0 && t.kind != 1) || t.cmask != 0)) lean = false;
         }
     if (plan.has_kraus) return lean ? StreamKind::lean_kraus : StreamKind::none;

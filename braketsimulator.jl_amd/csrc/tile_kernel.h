@@ -721,8 +721,135 @@ __device__ __forceinline__ void tile_body(double2* __restrict__ sv, int vq, cons
 // same op set plus reference Kraus ops (density-matrix plans).
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ int run_op_lean(SubCube<4>& s, const TOp* __restrict__ op) {
+// Register chains (kinds 8..11, planned for the lean passes): the thread's
+// 2x2 accumulator A (column-major), multiplied with uniform matrices and
+// per-thread Z factors t = parity(gbase & mask), then applied once.
+struct Acc {
+    double a00r, a00i, a10r, a10i, a01r, a01i, a11r, a11i;
+};
+
+// parity(g & m) in bit 31, the sign bit of a double's high word
+__device__ __forceinline__ uint32_t par31(uint64_t g, uint64_t m) { return (uint32_t)__popcll(g & m) << 31; }
+
+__device__ __forceinline__ double negif(double x, uint32_t s31) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, x);
+    return __builtin_bit_cast(double, b ^ ((uint64_t)s31 << 32));
+}
+
+// u a + v b (complex, uniform u and v)
+__device__ __forceinline__ void cmadd2(double& re, double& im, c2 u, double ar, double ai, c2 v, double br,
+                                       double bi) {
+    re = fma(v.x, br, fma(-v.y, bi, fma(u.x, ar, -u.y * ai)));
+    im = fma(v.x, bi, fma(v.y, br, fma(u.x, ai, u.y * ar)));
+}
+
+// The 2x2 per-thread matrix on register pair (a, b) in place, as
+// pair_inplace with the matrix in VGPRs.
+__device__ __forceinline__ void pair_inplace_v(double& ar, double& ai, double& br, double& bi, const Acc& A) {
+    double pr, pi, qr, qi, t, t2;
+    asm("v_mul_f64 %[pr], %[br], %[m01r]\n\t"
+        "v_mul_f64 %[qr], %[ar], %[m10r]\n\t"
+        "v_mul_f64 %[pi], %[bi], %[m01r]\n\t"
+        "v_mul_f64 %[qi], %[ai], %[m10r]\n\t"
+        "v_fma_f64 %[pr], -%[bi], %[m01i], %[pr]\n\t"
+        "v_fma_f64 %[qr], -%[ai], %[m10i], %[qr]\n\t"
+        "v_fma_f64 %[pi], %[br], %[m01i], %[pi]\n\t"
+        "v_fma_f64 %[qi], %[ar], %[m10i], %[qi]\n\t"
+        "v_fma_f64 %[t], -%[ai], %[m00i], %[pr]\n\t"
+        "v_fma_f64 %[t2], -%[bi], %[m11i], %[qr]\n\t"
+        "v_fma_f64 %[ai], %[ai], %[m00r], %[pi]\n\t"
+        "v_fma_f64 %[bi], %[bi], %[m11r], %[qi]\n\t"
+        "v_fma_f64 %[ai], %[ar], %[m00i], %[ai]\n\t"
+        "v_fma_f64 %[bi], %[br], %[m11i], %[bi]\n\t"
+        "v_fma_f64 %[ar], %[ar], %[m00r], %[t]\n\t"
+        "v_fma_f64 %[br], %[br], %[m11r], %[t2]"
+        : [ar] "+v"(ar), [ai] "+v"(ai), [br] "+v"(br), [bi] "+v"(bi), [pr] "=&v"(pr), [pi] "=&v"(pi),
+          [qr] "=&v"(qr), [qi] "=&v"(qi), [t] "=&v"(t), [t2] "=&v"(t2)
+        : [m00r] "v"(A.a00r), [m00i] "v"(A.a00i), [m10r] "v"(A.a10r), [m10i] "v"(A.a10i), [m01r] "v"(A.a01r),
+          [m01i] "v"(A.a01i), [m11r] "v"(A.a11r), [m11i] "v"(A.a11i));
+}
+
+template <int R, int AB>
+__device__ __forceinline__ void chain_apply(SubCube<R>& s, const Acc& A) {
+    sfor<0, (1 << R)>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        if constexpr ((j & (1 << AB)) == 0) {
+            constexpr int k = j | (1 << AB);
+            double ar = s.v[2 * j], ai = s.v[2 * j + 1], br = s.v[2 * k], bi = s.v[2 * k + 1];
+            pair_inplace_v(ar, ai, br, bi, A);
+            s.v[2 * j] = ar;
+            s.v[2 * j + 1] = ai;
+            s.v[2 * k] = br;
+            s.v[2 * k + 1] = bi;
+        }
+    });
+}
+
+__device__ __forceinline__ void run_chain(SubCube<4>& s, Acc& A, const OpHead& hd) {
+    if (hd.kind == 8) {  // A = U Z^t
+        const uint32_t z = par31(s.gbase, hd.cmask);
+        A.a00r = hd.m0.x;
+        A.a00i = hd.m0.y;
+        A.a10r = hd.m1.x;
+        A.a10i = hd.m1.y;
+        A.a01r = negif(hd.m2.x, z);
+        A.a01i = negif(hd.m2.y, z);
+        A.a11r = negif(hd.m3.x, z);
+        A.a11i = negif(hd.m3.y, z);
+    } else if (hd.kind == 9) {  // A = U Z^t A
+        const uint32_t z = par31(s.gbase, hd.cmask);
+        const double b10r = negif(A.a10r, z), b10i = negif(A.a10i, z), b11r = negif(A.a11r, z),
+                     b11i = negif(A.a11i, z);
+        double n00r, n00i, n10r, n10i, n01r, n01i, n11r, n11i;
+        cmadd2(n00r, n00i, hd.m0, A.a00r, A.a00i, hd.m2, b10r, b10i);
+        cmadd2(n10r, n10i, hd.m1, A.a00r, A.a00i, hd.m3, b10r, b10i);
+        cmadd2(n01r, n01i, hd.m0, A.a01r, A.a01i, hd.m2, b11r, b11i);
+        cmadd2(n11r, n11i, hd.m1, A.a01r, A.a01i, hd.m3, b11r, b11i);
+        A = Acc{ n00r, n00i, n10r, n10i, n01r, n01i, n11r, n11i };
+    } else if (hd.kind == 10) {  // A = Z^t A, applied to register a
+        if (hd.cmask) {
+            const uint32_t z = par31(s.gbase, hd.cmask);
+            A.a10r = negif(A.a10r, z);
+            A.a10i = negif(A.a10i, z);
+            A.a11r = negif(A.a11r, z);
+            A.a11i = negif(A.a11i, z);
+        }
+        switch (hd.a) {
+            case 0: chain_apply<4, 0>(s, A); break;
+            case 1: chain_apply<4, 1>(s, A); break;
+            case 2: chain_apply<4, 2>(s, A); break;
+            default: chain_apply<4, 3>(s, A); break;
+        }
+    } else {  // 11: A = (-1)^P A
+        const int np = (int)(hd.cmask >> 32);
+        const int pd = hd.nd;
+        const uint64_t a = s.gbase;
+        uint32_t par = __popcll(a & (a >> (pd & 0xFF)) & __builtin_bit_cast(uint64_t, hd.m0.x));
+        if (np > 1) {
+            par += __popcll(a & (a >> ((pd >> 8) & 0xFF)) & __builtin_bit_cast(uint64_t, hd.m0.y));
+            if (np > 2) {
+                par += __popcll(a & (a >> ((pd >> 16) & 0xFF)) & __builtin_bit_cast(uint64_t, hd.m1.x));
+                if (np > 3) par += __popcll(a & (a >> ((pd >> 24) & 0xFF)) & __builtin_bit_cast(uint64_t, hd.m1.y));
+            }
+        }
+        const uint32_t z = par << 31;
+        A.a00r = negif(A.a00r, z);
+        A.a00i = negif(A.a00i, z);
+        A.a10r = negif(A.a10r, z);
+        A.a10i = negif(A.a10i, z);
+        A.a01r = negif(A.a01r, z);
+        A.a01i = negif(A.a01i, z);
+        A.a11r = negif(A.a11r, z);
+        A.a11i = negif(A.a11i, z);
+    }
+}
+
+__device__ __forceinline__ int run_op_lean(SubCube<4>& s, Acc& A, const TOp* __restrict__ op) {
     const OpHead hd = load_head(op);
+    if (hd.kind >= 8) {
+        run_chain(s, A, hd);
+        return 1;
+    }
     switch (hd.kind * 16 + hd.a * 4 + hd.b) {
         case 48: sw_sign<4>(s, hd); break;
         case 0: sw_dense1_t<4, 0, false>(s, op, hd); break;
@@ -740,10 +867,14 @@ __device__ __forceinline__ int run_op_lean(SubCube<4>& s, const TOp* __restrict_
 }
 
 // The lean op set plus the reference Kraus ops of density-matrix plans.
-__device__ __forceinline__ int run_op_lean_kraus(SubCube<4>& s, const TOp* __restrict__ op,
+__device__ __forceinline__ int run_op_lean_kraus(SubCube<4>& s, Acc& A, const TOp* __restrict__ op,
                                                  double2* __restrict__ tile, int pl, int o0, int o1, int o2,
                                                  int o3) {
     const OpHead hd = load_head(op);
+    if (hd.kind >= 8) {
+        run_chain(s, A, hd);
+        return 1;
+    }
     if (hd.kind == 4) {
         if (hd.nd) {
             sw_kraus4(s, op, tile, pl, o0, o1, o2, o3);
@@ -811,6 +942,7 @@ __device__ __forceinline__ void stream_body(double2* __restrict__ sv, int vq, co
         return b;
     };
     const int nsw = pass->nsweeps;
+    Acc acc{};  // register chains of the lean op sets
     dvec<2 * NR> pre;  // this thread's amplitudes of the next tile (row jj = local j = tid + TB jj)
     auto load = [&](uint64_t b) {
         const uint64_t s_t = b | scat_lo[threadIdx.x];
@@ -848,9 +980,9 @@ __device__ __forceinline__ void stream_body(double2* __restrict__ sv, int vq, co
             setv<j>(sc.v, to_c2(tile[pl + phys(o)]));
         });
         if constexpr (LEAN && KRAUS) {
-            for (int o = ob; o < oe;) o += run_op_lean_kraus(sc, tops + o, tile, pl, o0, o1, o2, o3);
+            for (int o = ob; o < oe;) o += run_op_lean_kraus(sc, acc, tops + o, tile, pl, o0, o1, o2, o3);
         } else if constexpr (LEAN) {
-            for (int o = ob; o < oe;) o += run_op_lean(sc, tops + o);
+            for (int o = ob; o < oe;) o += run_op_lean(sc, acc, tops + o);
         } else {
             for (int o = ob; o < oe;) o += run_op<R, false>(sc, tops + o, tile, pl, o0, o1, o2, o3);
         }

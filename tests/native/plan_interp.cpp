@@ -68,6 +68,18 @@ static void apply_top(const Plan& p, int o, int R, const int* vb, uint64_t G, ui
             for (int r = 0; r < R; ++r) j |= (int)((i >> vb[r]) & 1) << r;
             v[i] *= M(j);
         }
+    } else if (op.kind == 3 && op.split_only) {
+        // defined by the split form: jmask, per-thread pairs, register terms
+        for (uint64_t i = 0; i < N; ++i) {
+            const uint64_t g = i & ~G;
+            int j = 0;
+            for (int r = 0; r < R; ++r) j |= (int)((i >> vb[r]) & 1) << r;
+            int par = (int)((op.jmask >> j) & 1);
+            for (int k = 0; k < op.npairs; ++k) par ^= __builtin_popcountll(g & (g >> op.pd[k]) & op.pm[k]) & 1;
+            for (int r = 0; r < R; ++r)
+                if ((j >> r) & 1) par ^= __builtin_popcountll(g & op.sgm[r]) & 1;
+            if (par) v[i] = -v[i];
+        }
     } else if (op.kind == 3) {
         for (uint64_t i = 0; i < N; ++i) {
             int par = 0;
@@ -103,6 +115,7 @@ static int check_sweep(const Plan& p, const Sweep& S, int R, const int* vb, uint
                     if (((op.nd >> (8 * k)) & 0xFF) != (op.pd[k] & 0xFF) || w[k] != op.pm[k] || w[4 + k] != op.sgm[k])
                         return -16;
                 if ((uint32_t)op.cmask != op.jmask || (int)(op.cmask >> 32) != op.npairs) return -16;
+                if (op.split_only) continue;  // no pair form to compare with
                 int direct = 0, split = (int)((op.jmask >> j) & 1);
                 for (int k = 0; k < op.npairs; ++k) {
                     direct ^= __builtin_popcountll(i & (i >> op.pd[k]) & op.pm[k]) & 1;
@@ -131,6 +144,63 @@ static int check_sweep(const Plan& p, const Sweep& S, int R, const int* vb, uint
 
 static void apply_eop_cpu(const EOp& e, uint64_t N, cd* v);
 
+// A register chain (kinds 8, 9 and 11 up to its kind 10 at index `end`) with
+// its documented per-sub-cube meaning: A = U Z^t ..., t = parity(gbase &
+// mask), applied to register a of every sub-cube.  Returns the index after
+// the chain, or -1 when it is malformed.
+static int apply_chain(const Plan& p, int o, int oe, int R, const int* vb, uint64_t G, uint64_t N, cd* v) {
+    int end = o;
+    while (end < oe && p.tops[end].kind != 10) ++end;
+    if (end == oe) return -1;
+    const int a = p.tops[end].a;
+    if (a < 0 || a >= R) return -1;
+    for (uint64_t g = 0; g < N; ++g) {
+        if (g & G) continue;
+        cd A[4] = {1.0, 0.0, 0.0, 1.0};
+        for (int q = o; q <= end; ++q) {
+            const TOp& op = p.tops[q];
+            const int t = __builtin_popcountll(g & op.cmask) & 1;
+            auto M = [&](int j) { return cd(op.m[j].x, op.m[j].y); };
+            if (op.kind == 8) {
+                if (q != o) return -1;
+                A[0] = M(0); A[1] = M(1); A[2] = t ? -M(2) : M(2); A[3] = t ? -M(3) : M(3);
+            } else if (op.kind == 9 || op.kind == 10) {
+                if (op.kind == 9 && !op.cmask) return -1;
+                cd B[4] = {A[0], t ? -A[1] : A[1], A[2], t ? -A[3] : A[3]};
+                if (op.kind == 10) {
+                    for (int k = 0; k < 4; ++k) A[k] = B[k];
+                } else {
+                    A[0] = M(0) * B[0] + M(2) * B[1];
+                    A[1] = M(1) * B[0] + M(3) * B[1];
+                    A[2] = M(0) * B[2] + M(2) * B[3];
+                    A[3] = M(1) * B[2] + M(3) * B[3];
+                }
+            } else if (op.kind == 11) {
+                const int np = (int)(op.cmask >> 32);
+                uint64_t pm[4];
+                memcpy(pm, op.m, sizeof(pm));
+                int par = 0;
+                for (int k = 0; k < np; ++k) par ^= __builtin_popcountll(g & (g >> ((op.nd >> (8 * k)) & 0xFF)) & pm[k]) & 1;
+                if (par)
+                    for (int k = 0; k < 4; ++k) A[k] = -A[k];
+            } else {
+                return -1;
+            }
+        }
+        for (int j = 0; j < (1 << R); ++j) {
+            if ((j >> a) & 1) continue;
+            uint64_t x = g;
+            for (int r = 0; r < R; ++r)
+                if ((j >> r) & 1) x |= 1ull << vb[r];
+            const uint64_t y = x | (1ull << vb[a]);
+            const cd u = v[x], w = v[y];
+            v[x] = A[0] * u + A[2] * w;
+            v[y] = A[1] * u + A[3] * w;
+        }
+    }
+    return end + 1;
+}
+
 static int run(int nq, int kind, const bsim_op* ops, int n, double* state) {
     std::vector<EOp> e;
     if (int rc = lower_ops(nq, kind, ops, n, e, false)) return rc;
@@ -157,7 +227,18 @@ static int run(int nq, int kind, const bsim_op* ops, int n, double* state) {
             uint64_t G = 0;
             for (int r = 0; r < R; ++r) G |= 1ull << vb[r];
             if (int rc = check_sweep(p, S, R, vb, G, N)) return rc;
-            for (int o = S.op_begin; o < S.op_end; ++o) apply_top(p, o, R, vb, G, N, v);
+            for (int o = S.op_begin; o < S.op_end;) {
+                const int k = p.tops[o].kind;
+                if (k >= 8 && k <= 11) {
+                    if (k != 8) return -17;  // a chain starts with kind 8
+                    const int nx = apply_chain(p, o, S.op_end, R, vb, G, N, v);
+                    if (nx < 0) return -17;
+                    o = nx;
+                } else {
+                    apply_top(p, o, R, vb, G, N, v);
+                    ++o;
+                }
+            }
         }
     }
     return 0;
