@@ -69,6 +69,10 @@ struct TOp {
     //   11 A <- (-1)^P(gbase) A, P the sign op's pair form (its packed
     //      copy: nd = pair distances, cmask >> 32 = pair count, m[0..1] =
     //      pair masks)
+    //   12 a whole chain of nd = 1..4 matrices in one op, applied to
+    //      register a: A = Z^t(sgm[3]) U_nd ... Z^t(sgm[0]) U_1 Z^t(cmask)
+    //      (U_k = m[4k-4..4k-1]), times (-1)^P(gbase) with P = pairs
+    //      (pm, pd, npairs) when npairs > 0
     int32_t kind;
     int32_t a, b;      // dense: register bits (a < b for dense2)
     int32_t nd;        // diag: bit count; kind 4: TOps that follow with the matrix (0 or 15)

@@ -635,6 +635,7 @@ static int sweep_valu(const TOp* t, size_t n) {
             case 9: c += 45; break;
             case 10: c += 145; break;
             case 11: c += 20; break;
+            case 12: c += 150 + 45 * (t[i].nd - 1); break;
             default: c += 150; break;
         }
     }
@@ -709,6 +710,23 @@ static bool chain_sweep(const TOp* in, size_t n, std::vector<TOp>& out) {
             res.push_back(op);
             return;
         }
+        int nu = 0;
+        for (const ChainItem& it : v) nu += it.u ? 1 : 0;
+        if (nu <= 4) {  // the whole chain in one op (kind 12)
+            TOp op = chain_top(12, r, v[i].m, z0);
+            op.nd = nu;
+            int k = 0;
+            for (++i; i < v.size(); ++i) {
+                if (!v[i].u) {
+                    op.sgm[i + 1 < v.size() ? k : 3] = v[i].mask;  // between U_(k+1) and U_(k+2), or after the last
+                    continue;
+                }
+                ++k;
+                for (int e = 0; e < 4; ++e) op.m[4 * k + e] = make_double2(v[i].m[e].real(), v[i].m[e].imag());
+            }
+            res.push_back(op);
+            return;
+        }
         res.push_back(chain_top(8, r, v[i].m, z0));
         ++i;
         uint64_t zend = 0;
@@ -762,7 +780,15 @@ static bool chain_sweep(const TOp* in, size_t n, std::vector<TOp>& out) {
     bool p_done = pn == 0;
     if (!p_done)
         for (size_t i = 0; i < res.size(); ++i)
-            if (res[i].kind == 8) {
+            if (res[i].kind == 12) {
+                res[i].npairs = pn;
+                for (int k = 0; k < pn; ++k) {
+                    res[i].pm[k] = ppm[k];
+                    res[i].pd[k] = ppd[k];
+                }
+                p_done = true;
+                break;
+            } else if (res[i].kind == 8) {
                 TOp p;
                 memset(&p, 0, sizeof(p));
                 p.kind = 11;

@@ -785,8 +785,66 @@ __device__ __forceinline__ void chain_apply(SubCube<R>& s, const Acc& A) {
     });
 }
 
-__device__ __forceinline__ void run_chain(SubCube<4>& s, Acc& A, const OpHead& hd) {
-    if (hd.kind == 8) {  // A = U Z^t
+// A = U Z^t A (one further matrix of a chain)
+__device__ __forceinline__ void chain_mul(Acc& A, uint32_t z, c2 u0, c2 u1, c2 u2, c2 u3) {
+    const double b10r = negif(A.a10r, z), b10i = negif(A.a10i, z), b11r = negif(A.a11r, z), b11i = negif(A.a11i, z);
+    double n00r, n00i, n10r, n10i, n01r, n01i, n11r, n11i;
+    cmadd2(n00r, n00i, u0, A.a00r, A.a00i, u2, b10r, b10i);
+    cmadd2(n10r, n10i, u1, A.a00r, A.a00i, u3, b10r, b10i);
+    cmadd2(n01r, n01i, u0, A.a01r, A.a01i, u2, b11r, b11i);
+    cmadd2(n11r, n11i, u1, A.a01r, A.a01i, u3, b11r, b11i);
+    A = Acc{ n00r, n00i, n10r, n10i, n01r, n01i, n11r, n11i };
+}
+
+__device__ __forceinline__ void run_chain(SubCube<4>& s, Acc& A, const TOp* __restrict__ op, const OpHead& hd) {
+    if (hd.kind == 12) {  // a whole chain
+        const int n = hd.nd, np = op->npairs;
+        const uint64_t a = s.gbase;
+        uint32_t z = par31(a, hd.cmask);
+        if (np) {  // the sweep's per-thread sign rides on this chain
+            uint32_t par = __popcll(a & (a >> op->pd[0]) & op->pm[0]);
+            if (np > 1) {
+                par += __popcll(a & (a >> op->pd[1]) & op->pm[1]);
+                if (np > 2) {
+                    par += __popcll(a & (a >> op->pd[2]) & op->pm[2]);
+                    if (np > 3) par += __popcll(a & (a >> op->pd[3]) & op->pm[3]);
+                }
+            }
+            const uint32_t p = par << 31;
+            A.a00r = negif(hd.m0.x, p);
+            A.a00i = negif(hd.m0.y, p);
+            A.a10r = negif(hd.m1.x, p);
+            A.a10i = negif(hd.m1.y, p);
+            z ^= p;
+        } else {
+            A.a00r = hd.m0.x;
+            A.a00i = hd.m0.y;
+            A.a10r = hd.m1.x;
+            A.a10i = hd.m1.y;
+        }
+        A.a01r = negif(hd.m2.x, z);
+        A.a01i = negif(hd.m2.y, z);
+        A.a11r = negif(hd.m3.x, z);
+        A.a11i = negif(hd.m3.y, z);
+        if (n > 1) {
+            chain_mul(A, par31(a, op->sgm[0]), to_c2(op->m[4]), to_c2(op->m[5]), to_c2(op->m[6]), to_c2(op->m[7]));
+            if (n > 2) {
+                chain_mul(A, par31(a, op->sgm[1]), to_c2(op->m[8]), to_c2(op->m[9]), to_c2(op->m[10]),
+                          to_c2(op->m[11]));
+                if (n > 3)
+                    chain_mul(A, par31(a, op->sgm[2]), to_c2(op->m[12]), to_c2(op->m[13]), to_c2(op->m[14]),
+                              to_c2(op->m[15]));
+            }
+        }
+        const uint64_t ze = op->sgm[3];
+        if (ze) {
+            const uint32_t zz = par31(a, ze);
+            A.a10r = negif(A.a10r, zz);
+            A.a10i = negif(A.a10i, zz);
+            A.a11r = negif(A.a11r, zz);
+            A.a11i = negif(A.a11i, zz);
+        }
+    } else if (hd.kind == 8) {  // A = U Z^t
         const uint32_t z = par31(s.gbase, hd.cmask);
         A.a00r = hd.m0.x;
         A.a00i = hd.m0.y;
@@ -796,16 +854,10 @@ __device__ __forceinline__ void run_chain(SubCube<4>& s, Acc& A, const OpHead& h
         A.a01i = negif(hd.m2.y, z);
         A.a11r = negif(hd.m3.x, z);
         A.a11i = negif(hd.m3.y, z);
+        return;
     } else if (hd.kind == 9) {  // A = U Z^t A
-        const uint32_t z = par31(s.gbase, hd.cmask);
-        const double b10r = negif(A.a10r, z), b10i = negif(A.a10i, z), b11r = negif(A.a11r, z),
-                     b11i = negif(A.a11i, z);
-        double n00r, n00i, n10r, n10i, n01r, n01i, n11r, n11i;
-        cmadd2(n00r, n00i, hd.m0, A.a00r, A.a00i, hd.m2, b10r, b10i);
-        cmadd2(n10r, n10i, hd.m1, A.a00r, A.a00i, hd.m3, b10r, b10i);
-        cmadd2(n01r, n01i, hd.m0, A.a01r, A.a01i, hd.m2, b11r, b11i);
-        cmadd2(n11r, n11i, hd.m1, A.a01r, A.a01i, hd.m3, b11r, b11i);
-        A = Acc{ n00r, n00i, n10r, n10i, n01r, n01i, n11r, n11i };
+        chain_mul(A, par31(s.gbase, hd.cmask), hd.m0, hd.m1, hd.m2, hd.m3);
+        return;
     } else if (hd.kind == 10) {  // A = Z^t A, applied to register a
         if (hd.cmask) {
             const uint32_t z = par31(s.gbase, hd.cmask);
@@ -813,12 +865,6 @@ __device__ __forceinline__ void run_chain(SubCube<4>& s, Acc& A, const OpHead& h
             A.a10i = negif(A.a10i, z);
             A.a11r = negif(A.a11r, z);
             A.a11i = negif(A.a11i, z);
-        }
-        switch (hd.a) {
-            case 0: chain_apply<4, 0>(s, A); break;
-            case 1: chain_apply<4, 1>(s, A); break;
-            case 2: chain_apply<4, 2>(s, A); break;
-            default: chain_apply<4, 3>(s, A); break;
         }
     } else {  // 11: A = (-1)^P A
         const int np = (int)(hd.cmask >> 32);
@@ -841,13 +887,22 @@ __device__ __forceinline__ void run_chain(SubCube<4>& s, Acc& A, const OpHead& h
         A.a01i = negif(A.a01i, z);
         A.a11r = negif(A.a11r, z);
         A.a11i = negif(A.a11i, z);
+        return;
+    }
+    // kinds 10 and 12: apply A (the switch stays in this function: moved
+    // into a helper, the sub-cube went to scratch memory)
+    switch (hd.a) {
+        case 0: chain_apply<4, 0>(s, A); break;
+        case 1: chain_apply<4, 1>(s, A); break;
+        case 2: chain_apply<4, 2>(s, A); break;
+        default: chain_apply<4, 3>(s, A); break;
     }
 }
 
 __device__ __forceinline__ int run_op_lean(SubCube<4>& s, Acc& A, const TOp* __restrict__ op) {
     const OpHead hd = load_head(op);
     if (hd.kind >= 8) {
-        run_chain(s, A, hd);
+        run_chain(s, A, op, hd);
         return 1;
     }
     switch (hd.kind * 16 + hd.a * 4 + hd.b) {
@@ -872,7 +927,7 @@ __device__ __forceinline__ int run_op_lean_kraus(SubCube<4>& s, Acc& A, const TO
                                                  int o3) {
     const OpHead hd = load_head(op);
     if (hd.kind >= 8) {
-        run_chain(s, A, hd);
+        run_chain(s, A, op, hd);
         return 1;
     }
     if (hd.kind == 4) {

@@ -148,6 +148,43 @@ static void apply_eop_cpu(const EOp& e, uint64_t N, cd* v);
 // its documented per-sub-cube meaning: A = U Z^t ..., t = parity(gbase &
 // mask), applied to register a of every sub-cube.  Returns the index after
 // the chain, or -1 when it is malformed.
+// A whole chain in one op (kind 12).
+static int apply_chain12(const Plan& p, int o, int R, const int* vb, uint64_t G, uint64_t N, cd* v) {
+    const TOp& op = p.tops[o];
+    if (op.a < 0 || op.a >= R || op.nd < 1 || op.nd > 4 || op.npairs < 0 || op.npairs > 4) return -17;
+    auto M = [&](int j) { return cd(op.m[j].x, op.m[j].y); };
+    for (uint64_t g = 0; g < N; ++g) {
+        if (g & G) continue;
+        auto t = [&](uint64_t mask) { return __builtin_popcountll(g & mask) & 1; };
+        const int t0 = t(op.cmask);
+        cd A[4] = {M(0), M(1), t0 ? -M(2) : M(2), t0 ? -M(3) : M(3)};
+        for (int k = 1; k < op.nd; ++k) {
+            const int tk = t(op.sgm[k - 1]);
+            const cd B[4] = {A[0], tk ? -A[1] : A[1], A[2], tk ? -A[3] : A[3]};
+            A[0] = M(4 * k) * B[0] + M(4 * k + 2) * B[1];
+            A[1] = M(4 * k + 1) * B[0] + M(4 * k + 3) * B[1];
+            A[2] = M(4 * k) * B[2] + M(4 * k + 2) * B[3];
+            A[3] = M(4 * k + 1) * B[2] + M(4 * k + 3) * B[3];
+        }
+        if (t(op.sgm[3])) { A[1] = -A[1]; A[3] = -A[3]; }
+        int par = 0;
+        for (int k = 0; k < op.npairs; ++k) par ^= __builtin_popcountll(g & (g >> op.pd[k]) & op.pm[k]) & 1;
+        if (par)
+            for (int k = 0; k < 4; ++k) A[k] = -A[k];
+        for (int j = 0; j < (1 << R); ++j) {
+            if ((j >> op.a) & 1) continue;
+            uint64_t x = g;
+            for (int r = 0; r < R; ++r)
+                if ((j >> r) & 1) x |= 1ull << vb[r];
+            const uint64_t y = x | (1ull << vb[op.a]);
+            const cd u = v[x], w = v[y];
+            v[x] = A[0] * u + A[2] * w;
+            v[y] = A[1] * u + A[3] * w;
+        }
+    }
+    return 0;
+}
+
 static int apply_chain(const Plan& p, int o, int oe, int R, const int* vb, uint64_t G, uint64_t N, cd* v) {
     int end = o;
     while (end < oe && p.tops[end].kind != 10) ++end;
@@ -229,7 +266,10 @@ static int run(int nq, int kind, const bsim_op* ops, int n, double* state) {
             if (int rc = check_sweep(p, S, R, vb, G, N)) return rc;
             for (int o = S.op_begin; o < S.op_end;) {
                 const int k = p.tops[o].kind;
-                if (k >= 8 && k <= 11) {
+                if (k == 12) {
+                    if (int rc = apply_chain12(p, o, R, vb, G, N, v)) return rc;
+                    ++o;
+                } else if (k >= 8 && k <= 11) {
                     if (k != 8) return -17;  // a chain starts with kind 8
                     const int nx = apply_chain(p, o, S.op_end, R, vb, G, N, v);
                     if (nx < 0) return -17;
