@@ -76,8 +76,11 @@ struct TOp {
     int32_t kind;
     int32_t a, b;      // dense: register bits (a < b for dense2)
     int32_t nd;        // diag: bit count; kind 4: TOps that follow with the matrix (0 or 15)
-    int32_t dbit[4];   // diag: vector bit of diagonal index bit i
     uint64_t cmask, cval;
+    // the matrix right after the head fields: an op's head load (kind .. nd,
+    // cmask, m[0..3]) touches the first two 64-byte lines of its record
+    double2 m[16];
+    int32_t dbit[4];   // diag: vector bit of diagonal index bit i
     // sign: amplitude i is negated when sum_k popcount(i & (i >> pd[k]) & pm[k]) is odd
     uint64_t pm[4];
     int32_t pd[4];
@@ -104,7 +107,6 @@ struct TOp {
     // TOps (m = the EOp's dm in this op's layout, a = partial-sum slot,
     // b = parameter)
     int32_t nparam;
-    double2 m[16];
 };
 
 struct Sweep {

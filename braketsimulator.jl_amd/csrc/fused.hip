@@ -1293,8 +1293,8 @@ __device__ __forceinline__ void stream_pass(double2* __restrict__ sv, int vq, co
     extern __shared__ double2 tile[];
     __shared__ uint64_t scat_lo[tile_threads<4>()];
     __shared__ uint64_t scat_hi[16];
-    __shared__ uint64_t s_next;
-    stream_body<LEAN, KRAUS>(sv, vq, pass, sweeps, tops, ntiles, ctr, tile, scat_lo, scat_hi, &s_next);
+    __shared__ uint64_t s_next[2];
+    stream_body<LEAN, KRAUS>(sv, vq, pass, sweeps, tops, ntiles, ctr, tile, scat_lo, scat_hi, s_next);
 }
 __global__ __launch_bounds__(256, 2) void k_tile_stream(double2* __restrict__ sv, int vq,
                                                         const Pass* __restrict__ pass,

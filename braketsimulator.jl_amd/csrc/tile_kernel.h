@@ -901,11 +901,15 @@ __device__ __forceinline__ void run_chain(SubCube<4>& s, Acc& A, const TOp* __re
 
 __device__ __forceinline__ int run_op_lean(SubCube<4>& s, Acc& A, const TOp* __restrict__ op) {
     const OpHead hd = load_head(op);
-    if (hd.kind >= 8) {
+    // one key for both tests, so that every head field is read in the op's
+    // first scalar round trip (testing the kind alone let the compiler load
+    // b in a second one)
+    const int key = hd.kind * 16 + hd.a * 4 + hd.b;
+    if (key >= 8 * 16) {  // register chains: kind >= 8 (a < 4, b = 0)
         run_chain(s, A, op, hd);
         return 1;
     }
-    switch (hd.kind * 16 + hd.a * 4 + hd.b) {
+    switch (key) {
         case 48: sw_sign<4>(s, hd); break;
         case 0: sw_dense1_t<4, 0, false>(s, op, hd); break;
         case 4: sw_dense1_t<4, 1, false>(s, op, hd); break;
@@ -926,7 +930,8 @@ __device__ __forceinline__ int run_op_lean_kraus(SubCube<4>& s, Acc& A, const TO
                                                  double2* __restrict__ tile, int pl, int o0, int o1, int o2,
                                                  int o3) {
     const OpHead hd = load_head(op);
-    if (hd.kind >= 8) {
+    const int key = hd.kind * 16 + hd.a * 4 + hd.b;  // as in run_op_lean
+    if (key >= 8 * 16) {
         run_chain(s, A, op, hd);
         return 1;
     }
@@ -945,7 +950,7 @@ __device__ __forceinline__ int run_op_lean_kraus(SubCube<4>& s, Acc& A, const TO
         }
         return 1;
     }
-    switch (hd.kind * 16 + hd.a * 4 + hd.b) {
+    switch (key) {
         case 48: sw_sign<4>(s, hd); break;
         case 0: sw_dense1_t<4, 0, false>(s, op, hd); break;
         case 4: sw_dense1_t<4, 1, false>(s, op, hd); break;
@@ -961,7 +966,14 @@ __device__ __forceinline__ int run_op_lean_kraus(SubCube<4>& s, Acc& A, const TO
     return 1;
 }
 
-// LDS: tile (2^12 + 2^8 amplitudes), scat_lo[256], scat_hi[16], s_next.
+// Workgroup barrier over LDS traffic only: this wave's LDS reads and writes
+// (and scalar loads) have completed when it passes; its global loads and
+// stores may still be in flight.  __syncthreads() also waits for those
+// (vmcnt(0)), which would make every sweep wait for the next tile's
+// prefetch.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// LDS: tile (2^12 + 2^8 amplitudes), scat_lo[256], scat_hi[16], s_next[2].
 // The launch has min(ntiles, slots) workgroups of 256 threads.
 template <bool LEAN, bool KRAUS>
 __device__ __forceinline__ void stream_body(double2* __restrict__ sv, int vq, const Pass* __restrict__ pass,
@@ -985,7 +997,7 @@ __device__ __forceinline__ void stream_body(double2* __restrict__ sv, int vq, co
             scat_hi[t] = h;
         }
     }
-    __syncthreads();
+    lds_barrier();
     // global base of tile w: w deposited into the non-tile bits
     auto tile_base = [&](uint64_t w) {
         uint64_t b = 0;
@@ -1011,7 +1023,7 @@ __device__ __forceinline__ void stream_body(double2* __restrict__ sv, int vq, co
             constexpr int jj = decltype(J)::value;
             tile[phys((int)threadIdx.x + (jj << TBB))] = to_d2(getv<jj>(pre));
         });
-        __syncthreads();
+        lds_barrier();
     };
     // one sweep: a 12-bit tile holds exactly one sub-cube per thread
     auto sweep = [&](int sidx, uint64_t base) {
@@ -1046,33 +1058,36 @@ __device__ __forceinline__ void stream_body(double2* __restrict__ sv, int vq, co
             const int o = ((j & 1) ? o0 : 0) + ((j & 2) ? o1 : 0) + ((j & 4) ? o2 : 0) + ((j & 8) ? o3 : 0);
             tile[pl + phys(o)] = to_d2(getv<j>(sc.v));
         });
-        __syncthreads();
+        lds_barrier();
     };
+    // Tiles: the workgroup's first tile is blockIdx.x; each ticket v names
+    // tile gridDim.x + v.  The ticket of the tile after next is drawn once
+    // the current tile's sweeps are done and before its stores, so the
+    // atomic's wait (an in-order vmcnt) finds the memory pipe nearly idle.
     uint64_t base = tile_base(blockIdx.x);  // blockIdx.x < ntiles (grid <= ntiles)
-    load(base);
-    to_lds();
-    for (;;) {
+    auto ticket = [&](int slot) {
         if (threadIdx.x == 0) {
             const unsigned long long v = atomicAdd(ctr, 1ull);
             if (v == ntiles - 1) atomicExch(ctr, 0ull);  // the pass's last ticket
-            *s_next = gridDim.x + v;
+            s_next[slot] = gridDim.x + v;
         }
-        __syncthreads();
-        const uint64_t nt = *s_next;  // rewritten only after the store barrier below
-        // after the last tile, the prefetch re-reads this tile's old
-        // amplitudes (in bounds, discarded), so every path issues the same
-        // memory operations in the same order
-        const uint64_t nbase = nt < ntiles ? tile_base(nt) : base;
-        if constexpr (KRAUS) {
-            // density-matrix passes (long 16 x 16 channel sweeps): the next
-            // tile's loads in flight during every sweep
-            load(nbase);
-            for (int sidx = 0; sidx < nsw; ++sidx) sweep(sidx, base);
-        } else {
-            for (int sidx = 0; sidx + 1 < nsw; ++sidx) sweep(sidx, base);
-            load(nbase);  // in flight during the last sweep and the stores
-            if (nsw > 0) sweep(nsw - 1, base);
-        }
+    };
+    load(base);
+    ticket(0);
+    to_lds();  // its barrier publishes s_next[0]
+    uint64_t next = s_next[0];
+    for (int it = 1;; ++it) {
+        // after the last tile, the prefetch re-reads this tile's amplitudes
+        // (in bounds, discarded), so every path issues the same memory
+        // operations in the same order
+        const uint64_t nbase = next < ntiles ? tile_base(next) : base;
+        load(nbase);  // in flight during every sweep (their barriers wait for LDS traffic only)
+        for (int sidx = 0; sidx < nsw; ++sidx) sweep(sidx, base);
+        // a workgroup stops drawing after its first ticket past the end, so
+        // a pass draws exactly ntiles tickets; slots alternate, so a write
+        // never meets a read of the previous value that has not passed the
+        // store barrier
+        if (next < ntiles) ticket(it & 1);
         const uint64_t s_t = base | scat_lo[threadIdx.x];
         sfor<0, NR>([&](auto J) {
             constexpr int jj = decltype(J)::value;
@@ -1081,9 +1096,10 @@ __device__ __forceinline__ void stream_body(double2* __restrict__ sv, int vq, co
             __builtin_nontemporal_store(a.x, p);
             __builtin_nontemporal_store(a.y, p + 1);
         });
-        __syncthreads();  // the next tile rewrites LDS
-        if (nt >= ntiles) break;
+        lds_barrier();  // the next tile rewrites LDS; s_next[it & 1] visible
+        if (next >= ntiles) break;
         base = nbase;
+        next = s_next[it & 1];
         to_lds();
     }
 }
