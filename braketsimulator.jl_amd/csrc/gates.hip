@@ -153,7 +153,7 @@ __global__ __launch_bounds__(256) void k_1q_lane(double2* __restrict__ sv, int64
     }
 }
 
-// K = 3..6: matrix (D x D) and a block's amplitude groups in LDS.
+// K = 3: matrix (D x D) and a block's amplitude groups in LDS.
 __global__ __launch_bounds__(256) void k_dense_lds(double2* __restrict__ sv, InsArgs a, const double2* __restrict__ mg,
                                                    int K, int4 tlo, int2 thi) {
     extern __shared__ double2 smem[];
@@ -545,6 +545,11 @@ static int64_t cap_blocks() {
     return m > 0 ? (int64_t)m : (int64_t)device_cu_count() * 64;
 }
 
+// Matrices and operands go to the handle's scratch workspace with
+// hipMemcpyAsync from host vectors that end with this call: a copy from
+// pageable host memory has consumed its source when the call returns (it is
+// staged), and the stream orders it after the kernels that still read the
+// previous contents, so no gate waits for the device.
 int launch_gate(double2* d, int vq, const GateSpec& g, Workspace& scratch, hipStream_t s) {
     if (g.diag) {
         if (g.k > 8) return fail(BSIM_ENOTSUP, "diagonal gate on more than 8 targets");
@@ -649,9 +654,6 @@ int launch_gate(double2* d, int vq, const GateSpec& g, Workspace& scratch, hipSt
             hipLaunchKernelGGL(k_dense_mfma6, dim3((unsigned)nblk), dim3(512), 0, s, d, a,
                                (const double*)scratch.ptr, tlo, thi6, lay, hi);
         }
-        // the operands were copied from a host vector that ends with this call:
-        // wait until the copy (and the kernel) has completed
-        BSIM_HIP(hipStreamSynchronize(s));
     } else if (g.k == 3) {
         size_t D = (size_t)1 << g.k;
         if (int rc = ensure_ws(scratch, sizeof(double2) * D * D)) return rc;
@@ -691,9 +693,6 @@ int launch_gate(double2* d, int vq, const GateSpec& g, Workspace& scratch, hipSt
             BIG_CASE(12)
 #undef BIG_CASE
         }
-        // the matrix was copied from a host vector that ends with this call
-        BSIM_HIP(hipGetLastError());
-        BSIM_HIP(hipStreamSynchronize(s));
     } else {
         return fail(BSIM_ENOTSUP, "dense gates act on at most 12 targets");
     }
