@@ -974,6 +974,8 @@ def batch_leg(count: int, sweep_sets: int, shots: int = 1000):
     # both paths take the same lowered op lists (as the circuits' packed ops)
     t = time.perf_counter()
     lowered = [lower_instructions(list(ins), L.BSIM_SV, param_map=pmap) for ins, pmap, _ in sets]
+    for ol in lowered:
+        ol.ptr  # an OpList writes its bsim_op records on first use: part of the lowering
     t_lower = time.perf_counter() - t
     grads = [GradientTask(16, ol, pmap, words, coeffs) for ol, (_, pmap, _) in zip(lowered, sets)]
     log(f"batch: {count} circuits ({gates} gates) + {sweep_sets} gradient sets in one launch")

@@ -53,70 +53,123 @@ class GradientTask:
     n_params: int | None = None
 
 
+_TASK_DTYPE = None
+
+
+def _task_dtype():
+    """numpy layout of bsim_task (the ctypes ``Task``): the batch's task
+    array is filled by vector operations instead of one ctypes assignment
+    per field and task."""
+    global _TASK_DTYPE
+    if _TASK_DTYPE is None:
+        import ctypes as C
+
+        names = [n for n, _ in L.Task._fields_]
+        fmt = {"qubit_count": "<i4", "n_ops": "<i4", "shots": "<i8", "n_words": "<i4", "n_params": "<i4"}
+        _TASK_DTYPE = np.dtype({"names": names, "formats": [fmt.get(n, "<u8") for n in names],
+                                "offsets": [getattr(L.Task, n).offset for n in names],
+                                "itemsize": C.sizeof(L.Task)})
+    return _TASK_DTYPE
+
+
+def _is_op_array(x) -> bool:
+    """An already lowered op array (OpList, PackedOps): tested on the type,
+    since reading ``ptr`` on an OpList builds its record array."""
+    return hasattr(type(x), "ptr")
+
+
+def _address(ol) -> int:
+    """Address of an op array (OpList, PackedOps or anything with ``ptr``)."""
+    import ctypes as C
+
+    a = getattr(ol, "addr", None)
+    if isinstance(a, int):
+        return a
+    return C.cast(ol.ptr, C.c_void_p).value or 0
+
+
 def run_batch(circuits, shots=1000, seeds=None):
     """circuits: list of (qubit_count, instructions) -- or (qubit_count, ops)
     with ops already a bsim_op array (``n`` and ``ptr``, as OpList and
     workloads.PackedOps have) -- and GradientTask items, all run in the same
     persistent launch.  ``shots`` and ``seeds`` are scalars or per-item
     sequences (gradient tasks take no shots).  Returns a list of BatchResult."""
+    import ctypes as C
+
     n = len(circuits)
     if n == 0:
         return []
-    shots_l = list(shots) if np.ndim(shots) else [int(shots)] * n
-    seeds_l = list(seeds) if seeds is not None and np.ndim(seeds) else [
-        (0 if seeds is None else int(seeds)) + i for i in range(n)]
-    tasks = (L.Task * n)()
-    keep, outs = [], []
-    for i, item in enumerate(circuits):
+    shots_a = np.broadcast_to(np.asarray(shots, dtype=np.int64), (n,)).copy()
+    if seeds is None:
+        seeds_a = np.arange(n, dtype=np.uint64)
+    elif np.ndim(seeds):
+        seeds_a = np.array([int(s) & 0xFFFFFFFFFFFFFFFF for s in seeds], dtype=np.uint64)
+    else:
+        seeds_a = (np.uint64(int(seeds) & 0xFFFFFFFFFFFFFFFF) + np.arange(n, dtype=np.uint64))
+    tasks = np.zeros(n, dtype=_task_dtype())
+    tasks["seed"] = seeds_a
+    is_grad = np.array([isinstance(c, GradientTask) for c in circuits], dtype=bool)
+    keep = []
+    nq = np.zeros(n, dtype=np.int64)
+    nops = np.zeros(n, dtype=np.int64)
+    addr = np.zeros(n, dtype=np.uint64)
+    grads = {}
+    for i in np.flatnonzero(is_grad):
+        item = circuits[i]
+        q = item.qubit_count
+        pm = item.param_map
+        npar = (max(pm.values()) + 1 if pm else 0) if item.n_params is None else int(item.n_params)
+        ol = item.instructions if _is_op_array(item.instructions) else \
+            lower_instructions(list(item.instructions), L.BSIM_SV, param_map=pm)
+        for w in item.words:
+            if len(w) != q:
+                raise ValueError(f"Pauli word {w!r} must have {q} letters")
+        buf = C.create_string_buffer("".join(item.words).encode())
+        c = np.ascontiguousarray(item.coeffs, dtype=np.float64)
+        g = np.zeros(max(npar, 1), dtype=np.float64)
+        ev = np.zeros(1, dtype=np.float64)
+        keep.append((ol, buf, c))
+        grads[i] = (g[:npar], ev)
+        nq[i], nops[i], addr[i] = q, ol.n, _address(ol)
         t = tasks[i]
-        t.seed = int(seeds_l[i]) & 0xFFFFFFFFFFFFFFFF
-        if isinstance(item, GradientTask):
-            nq = item.qubit_count
-            pm = item.param_map
-            npar = (max(pm.values()) + 1 if pm else 0) if item.n_params is None else int(item.n_params)
-            if hasattr(item.instructions, "ptr"):  # lowered with param_map already
-                ol = item.instructions
-            else:
-                ol = lower_instructions(list(item.instructions), L.BSIM_SV, param_map=pm)
-            for w in item.words:
-                if len(w) != nq:
-                    raise ValueError(f"Pauli word {w!r} must have {nq} letters")
-            buf = "".join(item.words).encode()
-            c = np.ascontiguousarray(item.coeffs, dtype=np.float64)
-            g = np.zeros(max(npar, 1), dtype=np.float64)
-            ev = np.zeros(1, dtype=np.float64)
-            keep.append((ol, buf, c))
-            outs.append(("grad", g[:npar], ev))
-            t.qubit_count = int(nq)
-            t.n_ops = ol.n
-            t.ops = ol.ptr
-            t.shots = 0
-            t.n_words = len(item.words)
-            t.n_params = npar
-            t.paulis = buf
-            t.coeffs = L.dptr(c)
-            t.grad = L.dptr(g)
-            t.expval = L.dptr(ev)
-            continue
-        nq, ins = item
-        ol = ins if hasattr(ins, "ptr") else lower_instructions(ins, L.BSIM_SV)
+        t["shots"] = 0
+        t["n_words"] = len(item.words)
+        t["n_params"] = npar
+        t["paulis"] = C.addressof(buf)
+        t["coeffs"] = c.ctypes.data
+        t["grad"] = g.ctypes.data
+        t["expval"] = ev.ctypes.data
+    circ = np.flatnonzero(~is_grad)
+    for i in circ:
+        q, ins = circuits[i]
+        ol = ins if _is_op_array(ins) else lower_instructions(ins, L.BSIM_SV)
         keep.append(ol)
-        z = np.zeros(nq, dtype=np.float64)
-        s = np.zeros(max(int(shots_l[i]), 1), dtype=np.int64)
-        outs.append(("circuit", z, s[: int(shots_l[i])]))
-        t.qubit_count = int(nq)
-        t.n_ops = ol.n
-        t.ops = ol.ptr
-        t.shots = int(shots_l[i])
-        t.expect_z = L.dptr(z)
-        t.samples = L.i64ptr(s)
+        nq[i], nops[i], addr[i] = q, ol.n, _address(ol)
+    # the circuits' outputs in two blocks: <Z_q> of every circuit, then shots
+    shots_a[is_grad] = 0
+    zoff = np.zeros(n + 1, dtype=np.int64)
+    zoff[1:] = np.cumsum(np.where(is_grad, 0, nq))
+    soff = np.zeros(n + 1, dtype=np.int64)
+    soff[1:] = np.cumsum(shots_a)
+    z = np.zeros(max(int(zoff[-1]), 1), dtype=np.float64)
+    s = np.zeros(max(int(soff[-1]), 1), dtype=np.int64)
+    tasks["qubit_count"] = nq
+    tasks["n_ops"] = nops
+    tasks["ops"] = addr
+    tasks["shots"] = shots_a
+    tasks["expect_z"][circ] = np.uint64(z.ctypes.data) + (zoff[:-1][circ] * 8).astype(np.uint64)
+    tasks["samples"][circ] = np.where(shots_a[circ] > 0,
+                                      np.uint64(s.ctypes.data) + (soff[:-1][circ] * 8).astype(np.uint64),
+                                      np.uint64(0))
     lat = np.zeros(n, dtype=np.float64)
-    L.call("bsim_batch_run", tasks, n, L.dptr(lat))
+    L.call("bsim_batch_run", tasks.ctypes.data_as(C.POINTER(L.Task)), n, L.dptr(lat))
+    del keep
     res = []
-    for i, o in enumerate(outs):
-        if o[0] == "grad":
-            res.append(BatchResult(np.zeros(0), np.zeros(0, dtype=np.int64), float(lat[i]), o[1].copy(),
-                                   float(o[2][0])))
+    empty_z, empty_s = np.zeros(0), np.zeros(0, dtype=np.int64)
+    for i in range(n):
+        if is_grad[i]:
+            g, ev = grads[i]
+            res.append(BatchResult(empty_z, empty_s, float(lat[i]), g.copy(), float(ev[0])))
         else:
-            res.append(BatchResult(o[1], o[2], float(lat[i])))
+            res.append(BatchResult(z[zoff[i]:zoff[i + 1]], s[soff[i]:soff[i] + shots_a[i]], float(lat[i])))
     return res

@@ -123,6 +123,7 @@ class PackedOps:
     def __init__(self, rec: np.ndarray, mats: np.ndarray):
         self.rec, self.mats = rec, mats
         self.n = len(rec)
+        self.addr = rec.__array_interface__["data"][0]  # address of the first record
 
     @property
     def ptr(self):
