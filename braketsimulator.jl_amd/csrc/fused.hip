@@ -647,20 +647,19 @@ static int sweep_valu(const TOp* t, size_t n) {
 static bool chain_sweep(const TOp* in, size_t n, std::vector<TOp>& out) {
     std::vector<ChainItem> ch[4];
     std::vector<TOp> res;
-    uint64_t ppm[4] = { 0, 0, 0, 0 }, rsgm[4] = { 0, 0, 0, 0 };
-    int ppd[4] = { 0, 0, 0, 0 }, pn = 0;
+    uint64_t rsgm[4] = { 0, 0, 0, 0 };
+    // the per-thread sign of all sign ops: pair masks by pair distance (a
+    // sign op takes four distances; more go to further sign ops)
+    std::vector<std::pair<int, uint64_t>> pp;
     bool any_sign = false;
     const cd Zm[4] = { 1.0, 0.0, 0.0, -1.0 };
     auto addP = [&](uint64_t pm, int pd) {
-        for (int k = 0; k < pn; ++k)
-            if (ppd[k] == pd) {
-                ppm[k] ^= pm;
-                return true;
+        for (auto& q : pp)
+            if (q.first == pd) {
+                q.second ^= pm;
+                return;
             }
-        if (pn == 4) return false;
-        ppm[pn] = pm;
-        ppd[pn++] = pd;
-        return true;
+        pp.push_back({ pd, pm });
     };
     // emit the chain of register r; at_end: a chain without a dense op (Z
     // factors only) joins the sweep's closing sign op, otherwise (a 2-register
@@ -770,50 +769,60 @@ static bool chain_sweep(const TOp* in, size_t n, std::vector<TOp>& out) {
                 if (t.sgm[r]) ch[r].push_back(ChainItem{ false, {}, t.sgm[r] });
             }
             for (int k = 0; k < t.npairs; ++k)
-                if (t.pm[k] && !addP(t.pm[k], t.pd[k])) return false;
+                if (t.pm[k]) addP(t.pm[k], t.pd[k]);
         } else {
             return false;
         }
     }
     if (!any_sign) return false;  // nothing to gain: the planner's own fusion stands
     for (int r = 0; r < 4; ++r) flush(r, true);
-    bool p_done = pn == 0;
-    if (!p_done)
+    {
+        std::vector<std::pair<int, uint64_t>> live;
+        for (const auto& q : pp)
+            if (q.second) live.push_back(q);
+        pp.swap(live);
+    }
+    // the first four pair distances ride on the first chain; the sign is a
+    // factor of the whole sub-cube, so it commutes with every op of the sweep
+    size_t pk = 0;
+    if (!pp.empty())
         for (size_t i = 0; i < res.size(); ++i)
             if (res[i].kind == 12) {
-                res[i].npairs = pn;
-                for (int k = 0; k < pn; ++k) {
-                    res[i].pm[k] = ppm[k];
-                    res[i].pd[k] = ppd[k];
+                for (; pk < pp.size() && pk < 4; ++pk) {
+                    res[i].pd[pk] = pp[pk].first;
+                    res[i].pm[pk] = pp[pk].second;
                 }
-                p_done = true;
+                res[i].npairs = (int)pk;
                 break;
             } else if (res[i].kind == 8) {
                 TOp p;
                 memset(&p, 0, sizeof(p));
                 p.kind = 11;
-                for (int k = 0; k < pn; ++k) {
-                    p.nd |= (ppd[k] & 0xFF) << (8 * k);
-                    reinterpret_cast<uint64_t*>(p.m)[k] = ppm[k];
+                for (; pk < pp.size() && pk < 4; ++pk) {
+                    p.nd |= (pp[pk].first & 0xFF) << (8 * pk);
+                    reinterpret_cast<uint64_t*>(p.m)[pk] = pp[pk].second;
                 }
-                p.cmask = (uint64_t)pn << 32;
+                p.cmask = (uint64_t)pk << 32;
                 res.insert(res.begin() + (long)i + 1, p);
-                p_done = true;
                 break;
             }
-    if (!p_done || rsgm[0] || rsgm[1] || rsgm[2] || rsgm[3]) {
+    // what no chain carries: sign ops at the end of the sweep (split form
+    // only), the Z factors of registers without a dense op in the first
+    bool first = true;
+    while (pk < pp.size() || (first && (rsgm[0] || rsgm[1] || rsgm[2] || rsgm[3]))) {
         TOp s;
         memset(&s, 0, sizeof(s));
         s.kind = 3;
         s.split_only = 1;
-        if (!p_done) {
-            s.npairs = pn;
-            for (int k = 0; k < pn; ++k) {
-                s.pm[k] = ppm[k];
-                s.pd[k] = ppd[k];
-            }
+        int k = 0;
+        for (; pk < pp.size() && k < 4; ++pk, ++k) {
+            s.pd[k] = pp[pk].first;
+            s.pm[k] = pp[pk].second;
         }
-        for (int r = 0; r < 4; ++r) s.sgm[r] = rsgm[r];
+        s.npairs = k;
+        if (first)
+            for (int r = 0; r < 4; ++r) s.sgm[r] = rsgm[r];
+        first = false;
         res.push_back(s);
     }
     if (sweep_valu(res.data(), res.size()) >= sweep_valu(in, n)) return false;

@@ -233,3 +233,37 @@ def random_noise(n, rng):
     k1 = random_unitary(1, rng) * math.sqrt(0.7)
     k2 = random_unitary(1, rng) * math.sqrt(0.3)
     return Instruction(Nz.Kraus([k1, k2]), [q])
+
+
+def chain_circuit(n, layers, seed, n_hot=1):
+    """Register-chain workload of the fused planner: ``n_hot`` qubits get a
+    U gate per layer, each followed by a CZ to a random other qubit (a Z on
+    the register that depends on a thread bit), plus a CZ between two other
+    qubits (a per-thread sign) and now and then a Z on a hot qubit (a
+    uniform Z).  chain_circuit(13, 9, 0, 2) plans into every chain form:
+    chains of more than four matrices, per-thread signs on a long chain and
+    sign ops left at the end of a sweep."""
+    rng = np.random.default_rng(seed)
+    ins = [Instruction(G.H(), [q]) for q in range(n)]
+    hot = list(range(n - n_hot, n))
+    cold = list(range(0, n - n_hot))
+    for _ in range(layers):
+        for q in hot:
+            ins.append(Instruction(G.U(*rng.uniform(-3, 3, 3)), [q]))
+            ins.append(Instruction(G.CZ(), [q, int(rng.choice(cold))]))
+        a, b = rng.choice(cold, 2, replace=False)
+        ins.append(Instruction(G.CZ(), [int(a), int(b)]))
+        if rng.random() < 0.3:
+            ins.append(Instruction(G.Z(), [int(rng.choice(hot))]))
+    return ins
+
+
+def dm_sparse_noise(n, depth, seed):
+    """A density-matrix brick-wall with two noise channels in the middle:
+    the noise-free passes around them plan into register chains."""
+    from bsim import workloads as W
+
+    ins = W.brickwall(n, depth, seed=seed)
+    ins.append(Instruction(Nz.Depolarizing(0.05), [0]))
+    ins.append(Instruction(Nz.AmplitudeDamping(0.1), [n - 1]))
+    return ins + W.brickwall(n, 2, seed=seed + 1)

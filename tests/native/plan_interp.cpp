@@ -460,3 +460,19 @@ extern "C" int plan_kind_count(int nq, int kind, const bsim_op* ops, int n, int 
     for (auto& t : p.tops) k += (t.kind == tkind && (!ctl || t.cmask)) ? 1 : 0;
     return k;
 }
+
+// register-chain statistics of the plan: out[0..4] = tile ops of kinds
+// 8..12, out[5] = split-only sign ops, out[6] = the longest kind-12 chain
+extern "C" int plan_chain_stats(int nq, int kind, const bsim_op* ops, int n, int* out) {
+    std::vector<EOp> e;
+    if (int rc = lower_ops(nq, kind, ops, n, e, false)) return rc;
+    Plan p;
+    if (int rc = make_plan(kind == BSIM_SV ? nq : 2 * nq, e, tile_config(), p)) return rc;
+    for (int i = 0; i < 7; ++i) out[i] = 0;
+    for (auto& t : p.tops) {
+        if (t.kind >= 8 && t.kind <= 12) ++out[t.kind - 8];
+        if (t.kind == 3 && t.split_only) ++out[5];
+        if (t.kind == 12 && t.nd > out[6]) out[6] = t.nd;
+    }
+    return 0;
+}

@@ -20,7 +20,7 @@ from bsim import noises as Nz
 from bsim import workloads as W
 from bsim.simulators import Instruction, lower_instructions
 
-from circuits import grcs16, random_circuit
+from circuits import chain_circuit, dm_sparse_noise, grcs16, random_circuit
 from conftest import GOLDEN, ROOT
 
 HERE = Path(__file__).resolve().parent
@@ -54,6 +54,9 @@ def interp():
     h.plan_merged.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(L.Op), ctypes.c_int]
     h.plan_passes.restype = ctypes.c_int
     h.plan_passes.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(L.Op), ctypes.c_int]
+    h.plan_chain_stats.restype = ctypes.c_int
+    h.plan_chain_stats.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(L.Op), ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_int)]
     h.plan_kind_count.restype = ctypes.c_int
     h.plan_kind_count.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(L.Op), ctypes.c_int, ctypes.c_int,
                                   ctypes.c_int]
@@ -449,3 +452,54 @@ def test_planner_merges_dm_gate_halves(interp):
     rc = interp.planexec_dm(n, ol.ptr, ol.n, rho.ctypes.data)
     assert rc == 0
     np.testing.assert_allclose(rho, O.evolve_dm(n, ins), atol=1e-12)
+
+
+# ---- register chains ---------------------------------------------------------
+
+def _chain_stats(h, n, ins, kind=L.BSIM_SV, mode="reference"):
+    ol = lower_instructions(ins, kind, noise_mode=mode)
+    out = (ctypes.c_int * 7)()
+    assert h.plan_chain_stats(n, kind, ol.ptr, ol.n, out) == 0
+    return {"start": out[0], "mul": out[1], "apply": out[2], "psign": out[3], "chain": out[4], "split_sign": out[5],
+            "longest": out[6]}
+
+
+def test_planner_chain_forms(interp):
+    """Every register-chain form, interpreted with its documented
+    per-sub-cube meaning, against the oracle: chains of up to four
+    matrices in one op, longer chains (start / multiply / per-thread sign /
+    apply), and the sign ops a rewrite leaves at the end of a sweep."""
+    ins = chain_circuit(13, 9, 0, 2)
+    s = _chain_stats(interp, 13, ins)
+    assert s["start"] and s["mul"] and s["apply"] and s["psign"], s
+    assert s["chain"] and s["split_sign"] and s["longest"] == 4, s
+    assert np.max(np.abs(_run(interp, 13, ins) - O.evolve_sv(13, ins))) < 1e-12
+    ins = W.brickwall(14, 12, seed=4)
+    s = _chain_stats(interp, 14, ins)
+    assert s["chain"] >= 10 and s["start"] == 0, s
+    assert np.max(np.abs(_run(interp, 14, ins) - O.evolve_sv(14, ins))) < 1e-12
+
+
+def test_planner_chain_random(interp):
+    """Random chain circuits (with or without chains, depending on the
+    sweeps the planner forms) against the oracle."""
+    chains = 0
+    for seed in range(8):
+        n = 12 + seed % 2
+        ins = chain_circuit(n, 6 + seed, 10 + seed, 1 + seed % 3)
+        s = _chain_stats(interp, n, ins)
+        chains += s["chain"] + s["start"]
+        assert np.max(np.abs(_run(interp, n, ins) - O.evolve_sv(n, ins))) < 1e-12, seed
+    assert chains > 0
+
+
+@pytest.mark.parametrize("mode", ["reference", "physical"])
+def test_planner_chains_density_matrix(interp, mode):
+    """Density matrices of 6 and 7 qubits (12 and 14 vector bits): chains in
+    the noise-free passes, the Kraus passes around them unchanged."""
+    ins = W.brickwall(6, 8, seed=6)
+    assert _chain_stats(interp, 6, ins, L.BSIM_DM, mode)["chain"] > 0
+    np.testing.assert_allclose(_run_dm(interp, 6, ins, mode), O.evolve_dm(6, ins, mode), atol=1e-12)
+    ins = dm_sparse_noise(7, 6, 7)
+    assert _chain_stats(interp, 7, ins, L.BSIM_DM, mode)["chain"] > 0
+    np.testing.assert_allclose(_run_dm(interp, 7, ins, mode), O.evolve_dm(7, ins, mode), atol=1e-12)
