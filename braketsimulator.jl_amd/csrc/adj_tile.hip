@@ -20,6 +20,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
+#include <cmath>
+#include <complex>
 #include <functional>
 #include <vector>
 
@@ -69,8 +72,221 @@ __global__ void k_adj_zero(double* __restrict__ p, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0.0;
 }
 
+namespace {
+
+using cplx = std::complex<double>;
+using M2 = std::array<cplx, 4>;  // 2 x 2, column-major: m[r + 2 c]
+
+bool one_qubit(const EOp& e) {
+    return (e.kind == E_DENSE || e.kind == E_DIAG) && e.g.k == 1 && e.g.cmask == 0;
+}
+
+M2 mat1(const EOp& e) {
+    auto c = [](double2 x) { return cplx(x.x, x.y); };
+    if (e.g.diag) return { c(e.g.m[0]), 0.0, 0.0, c(e.g.m[1]) };
+    return { c(e.g.m[0]), c(e.g.m[1]), c(e.g.m[2]), c(e.g.m[3]) };
+}
+
+M2 mat1(const std::vector<double2>& d) {
+    return { cplx(d[0].x, d[0].y), cplx(d[1].x, d[1].y), cplx(d[2].x, d[2].y), cplx(d[3].x, d[3].y) };
+}
+
+M2 mul(const M2& a, const M2& b) {
+    M2 r;
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) r[i + 2 * j] = a[i] * b[2 * j] + a[i + 2] * b[2 * j + 1];
+    return r;
+}
+
+double maxabs(const M2& m) {
+    double s = 0.0;
+    for (const cplx& x : m) s = std::max({ s, std::fabs(x.real()), std::fabs(x.imag()) });
+    return s;
+}
+
+// a m = m a to 1e-14 of m's size.  The payloads are formed in floating
+// point from derivative matrices, so the exact commutation of a Pauli
+// rotation's -i P / 2 with the rotation shows up as a residue of rounding.
+bool commutes(const M2& a, const M2& m) {
+    const M2 x = mul(a, m), y = mul(m, a);
+    M2 d;
+    for (int i = 0; i < 4; ++i) d[i] = x[i] - y[i];
+    return maxabs(d) <= 1e-14 * maxabs(m);
+}
+
+}  // namespace
+
+// Pauli form of a product payload (2 x 2, column-major): m = -i s P with
+// P = X (1), Y (2) or Z (3) and s real; 0 when m has no such form.
+// Entries below 1e-15 of the largest are rounding of an exact zero, and
+// entries that agree to that much are taken as equal.
+int pauli_form(const double2* m, double& s) {
+    double sc = 0.0;
+    for (int j = 0; j < 4; ++j) sc = std::max({ sc, std::fabs(m[j].x), std::fabs(m[j].y) });
+    if (sc == 0.0) return 0;
+    const double eps = 1e-15 * sc;
+    auto z = [&](double x) { return std::fabs(x) <= eps; };
+    auto eq = [&](double x, double y) { return std::fabs(x - y) <= eps; };
+    const double2 m00 = m[0], m10 = m[1], m01 = m[2], m11 = m[3];
+    const bool diag0 = z(m00.x) && z(m00.y) && z(m11.x) && z(m11.y);
+    const bool off0 = z(m01.x) && z(m01.y) && z(m10.x) && z(m10.y);
+    if (diag0 && z(m01.x) && z(m10.x) && eq(m01.y, m10.y)) {  // -i s X: m01 = m10 = -i s
+        s = -m01.y;
+        return 1;
+    }
+    if (diag0 && z(m01.y) && z(m10.y) && eq(m01.x, -m10.x)) {  // -i s Y = s [[0, -1], [1, 0]]
+        s = m10.x;
+        return 2;
+    }
+    if (off0 && z(m00.x) && z(m11.x) && eq(m00.y, -m11.y)) {  // -i s Z: m00 = -i s, m11 = i s
+        s = -m00.y;
+        return 3;
+    }
+    return 0;
+}
+
+// Runs of uncontrolled one-qubit ops on one qubit, each the next op on
+// that qubit in the backward list (the ops between act on other qubits and
+// commute with the run), become one dense op: the run's product W.  A
+// product is taken after W when its payload commutes with the run's ops
+// after its own, or before W when it commutes with the run's ops up to its
+// own (its value is unchanged: <lam'|M|psi'> over the moved vectors equals
+// the original when the unitaries between commute with M).  The kernels
+// take the products of a merged op in two shapes: all after W (any form,
+// at most 3), or one before W and at most one after, both of Pauli form
+// (the coded path).  The RZ^dagger RY^dagger pairs of a hardware-efficient
+// ansatz merge this way (the Z product before, the Y product after).
+static void merge_one_qubit_runs(std::vector<EOp>& back, std::vector<int>& src) {
+    struct Prod {
+        M2 m;        // payload (after-form of its own op)
+        M2 upto;     // the run's product through its own op
+        int slot, prm;
+        bool pre;
+        std::vector<double2> dm;
+    };
+    const size_t n = back.size();
+    std::vector<char> dead(n, 0);
+    auto admissible = [](const std::vector<Prod>& ps) {
+        int npre = 0, npost = 0, post_coded = 0;
+        for (const Prod& p : ps) {
+            double s;
+            if (p.pre) ++npre;
+            else {
+                ++npost;
+                post_coded += pauli_form(p.dm.data(), s) ? 1 : 0;
+            }
+        }
+        if ((int)ps.size() > 3) return false;
+        return npre == 0 || (npre == 1 && npost <= 1 && post_coded == npost);
+    };
+    for (size_t i = 0; i < n; ++i) {
+        if (dead[i] || !one_qubit(back[i])) continue;
+        const int q = back[i].g.tb[0];
+        const uint64_t qb = 1ull << q;
+        M2 W = mat1(back[i]);
+        std::vector<Prod> prods;
+        auto own = [&](const EOp& e, std::vector<Prod>& ps) {
+            for (int p = 0; p < e.nparams; ++p)
+                ps.push_back({ mat1(e.dm[p]), W, e.pslot[p] >= 0 ? e.pslot[p] : e.aslot,
+                               e.pslot[p] >= 0 ? e.pprm[p] : p, e.ppre[p], e.dm[p] });
+        };
+        own(back[i], prods);
+        int absorbed = 0;
+        for (size_t k = i + 1; k < n; ++k) {
+            if (dead[k] || !(back[k].touched() & qb)) continue;
+            const EOp& x = back[k];
+            if (!one_qubit(x) || x.g.tb[0] != q) break;
+            const M2 X = mat1(x);
+            std::vector<Prod> t = prods;
+            bool ok = true;
+            for (Prod& p : t) {
+                if (p.pre || commutes(X, p.m)) continue;
+                double s;
+                if (commutes(p.upto, p.m) && pauli_form(p.dm.data(), s)) p.pre = true;
+                else ok = false;
+            }
+            if (!ok) break;
+            const M2 W2 = mul(X, W);
+            const M2 W0 = W;
+            W = W2;
+            own(x, t);
+            if (!admissible(t)) {
+                W = W0;
+                break;
+            }
+            prods.swap(t);
+            dead[k] = 1;
+            ++absorbed;
+        }
+        if (!absorbed) continue;
+        EOp m;
+        m.kind = E_DENSE;
+        m.g.k = 1;
+        m.g.tb[0] = q;
+        m.g.diag = false;
+        m.g.m.resize(4);
+        for (int j = 0; j < 4; ++j) m.g.m[j] = make_double2(W[j].real(), W[j].imag());
+        m.nparams = (int)prods.size();
+        for (int p = 0; p < m.nparams; ++p) {
+            m.dm[p] = prods[p].dm;
+            m.pslot[p] = prods[p].slot;
+            m.pprm[p] = prods[p].prm;
+            m.ppre[p] = prods[p].pre;
+        }
+        m.aslot = m.nparams ? prods[0].slot : -1;
+        back[i] = std::move(m);
+        src[i] = -1;
+    }
+    size_t w = 0;
+    for (size_t i = 0; i < n; ++i)
+        if (!dead[i]) {
+            if (w != i) {
+                back[w] = std::move(back[i]);
+                src[w] = src[i];
+            }
+            ++w;
+        }
+    back.resize(w);
+    src.resize(w);
+}
+
+// Pauli-coded products (the TOp comment gives the layout): every
+// uncontrolled dense 1-register op whose products are one before its
+// update and at most one after, or one after, all of Pauli form.
+static int code_products(Plan& plan, int pm) {
+    for (size_t o = 0; o < plan.tops.size(); ++o) {
+        TOp& op = plan.tops[o];
+        if (op.kind != 0 || op.cmask != 0) continue;
+        op.cval = 0;
+        const int np = op.nparam;
+        if (np < 1) continue;
+        int npre = 0, npost = 0;
+        bool coded = np <= 2;
+        uint32_t w[2] = { 0, 0 };
+        double sc[2] = { 0.0, 0.0 };
+        for (int p = 0; p < np && coded; ++p) {
+            const TOp& d = plan.tops[o + 1 + p];
+            double s = 0.0;
+            const int f = pauli_form(d.m, s);
+            const int side = d.pre ? 1 : 0;
+            (d.pre ? npre : npost) += 1;
+            const int64_t row = (int64_t)d.a * pm + d.b;
+            coded = f != 0 && row < (1ll << 30) && w[side] == 0;
+            w[side] = (uint32_t)(row << 2) | (uint32_t)f;
+            sc[side] = s;
+        }
+        if (!coded) {
+            if (npre) return fail(BSIM_EINVAL, "adjoint plan: a product before its op must be of Pauli form");
+            continue;
+        }
+        op.cval = (uint64_t)w[0] | ((uint64_t)w[1] << 32);
+        op.m[4] = make_double2(sc[1], sc[0]);
+    }
+    return BSIM_OK;
+}
+
 int adjoint_back_plan(int vq, const std::vector<EOp>& eops, std::vector<EOp>& back, Plan& plan, int& nslots,
-                      int& pm, const TileConfig* base) {
+                      int& pm, const TileConfig* base, std::vector<int>* src) {
     const int L = (int)eops.size();
     nslots = 0;
     pm = 0;
@@ -116,11 +332,16 @@ int adjoint_back_plan(int vq, const std::vector<EOp>& eops, std::vector<EOp>& ba
         }
         back.push_back(std::move(b));
     }
+    std::vector<int> idx(back.size());
+    for (int i = 0; i < (int)back.size(); ++i) idx[i] = L - 1 - i;
+    merge_one_qubit_runs(back, idx);
+    if (src) src->swap(idx);
     TileConfig cfg = base ? *base : tile_config();
     cfg.tile_bits = 11;
     cfg.reg_bits = 3;
-    cfg.merge_dense = false;  // single steps run by index in the reversed list
-    return make_plan(vq, back, cfg, plan);
+    cfg.merge_dense = false;  // single steps run by index in `back`
+    if (int rc = make_plan(vq, back, cfg, plan)) return rc;
+    return code_products(plan, pm);
 }
 
 static std::atomic<bool> g_adj_shmem{ false };

@@ -157,6 +157,45 @@ __device__ __forceinline__ double dot_rdiag_imag(const SubCube<3>& sp, const Sub
     return acc;
 }
 
+// Im <l| P |v> over the sub-cube for a Pauli P on register A (P = 1 X,
+// 2 Y, 3 Z): with Im(x* y) = x.x y.y - x.y y.x and Re(x* y) = x.x y.x + x.y y.y
+// over each pair (a, b) of v and (la, lb) of l,
+//   X: Im(la* b) + Im(lb* a),  Y: Re(lb* a) - Re(la* b),  Z: Im(la* a) - Im(lb* b).
+// Re <l| -i s P |v> = s Im <l| P |v>: the product of a Pauli-coded payload.
+template <int A, int P>
+__device__ __forceinline__ double pauli_im(const SubCube<3>& sp, const SubCube<3>& sl) {
+    double acc = 0.0;
+    sfor<0, 8>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        if constexpr ((j & (1 << A)) == 0) {
+            constexpr int k = j | (1 << A);
+            const c2 a = getv<j>(sp.v), b = getv<k>(sp.v);
+            const c2 la = getv<j>(sl.v), lb = getv<k>(sl.v);
+            if constexpr (P == 1) acc += (la.x * b.y - la.y * b.x) + (lb.x * a.y - lb.y * a.x);
+            else if constexpr (P == 2) acc += (lb.x * a.x + lb.y * a.y) - (la.x * b.x + la.y * b.y);
+            else acc += (la.x * a.y - la.y * a.x) - (lb.x * b.y - lb.y * b.x);
+        }
+    });
+    return acc;
+}
+
+template <int A>
+__device__ __forceinline__ double pauli_im_a(const SubCube<3>& sp, const SubCube<3>& sl, uint32_t p) {
+    switch (p) {
+        case 1: return pauli_im<A, 1>(sp, sl);
+        case 2: return pauli_im<A, 2>(sp, sl);
+        default: return pauli_im<A, 3>(sp, sl);
+    }
+}
+
+__device__ __forceinline__ double pauli_im_any(const SubCube<3>& sp, const SubCube<3>& sl, int a, uint32_t p) {
+    switch (a) {
+        case 0: return pauli_im_a<0>(sp, sl, p);
+        case 1: return pauli_im_a<1>(sp, sl, p);
+        default: return pauli_im_a<2>(sp, sl, p);
+    }
+}
+
 __device__ __forceinline__ double adj_dot(const SubCube<3>& sp, const SubCube<3>& sl, const TOp* __restrict__ op,
                                           const TOp* __restrict__ d) {
     switch (op->kind * 16 + op->a * 4 + op->b) {
@@ -309,14 +348,28 @@ __device__ __forceinline__ void adj_sweep(const Pass* __restrict__ pass, const S
         setv<j>(sp.v, valid ? to_c2(tp[pl + phys(o)]) : c2{ 0.0, 0.0 });
         setv<j>(sl.v, valid ? to_c2(tl[pl + phys(o)]) : c2{ 0.0, 0.0 });
     });
+    auto put = [&](uint32_t row, double v) {
+        if (lane == 63) partials[((size_t)row * ntiles + tile_index) * AT_W + wave] = v;
+    };
     for (int o = ob; o < oe;) {
         const TOp* op = tops + o;
         const int np = op->nparam;
-        run_op_both(sp, sl, op);
-        for (int p = 0; p < np; ++p) {
-            const TOp* d = op + 1 + p;
-            const double v = wave_sum63(adj_dot(sp, sl, op, d));
-            if (lane == 63) partials[((size_t)(d->a * pm + d->b) * ntiles + tile_index) * AT_W + wave] = v;
+        const uint64_t pc = op->kind == 0 && op->cmask == 0 ? op->cval : 0;
+        if (pc) {
+            // Pauli-coded products, read from the op's head (the payload
+            // TOps that follow are skipped): one before the update, one after
+            const int a = op->a;
+            const uint32_t wpre = (uint32_t)(pc >> 32), wpost = (uint32_t)pc;
+            const double2 sc = op->m[4];
+            if (wpre) put(wpre >> 2, sc.x * wave_sum63(pauli_im_any(sp, sl, a, wpre & 3u)));
+            run_op_both(sp, sl, op);
+            if (wpost) put(wpost >> 2, sc.y * wave_sum63(pauli_im_any(sp, sl, a, wpost & 3u)));
+        } else {
+            run_op_both(sp, sl, op);
+            for (int p = 0; p < np; ++p) {
+                const TOp* d = op + 1 + p;
+                put((uint32_t)(d->a * pm + d->b), wave_sum63(adj_dot(sp, sl, op, d)));
+            }
         }
         o += 1 + np;
     }

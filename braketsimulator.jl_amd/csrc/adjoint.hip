@@ -662,19 +662,20 @@ int bsim_adjoint_gradient(bsim_state* h, const bsim_op* ops, int n_ops, int n_wo
     // parameter products reduced inside the pass), unless their per-tile
     // partial sums would outgrow max(4 GiB, the state's size): then one
     // kernel per op
-    const int L = (int)eops.size();
     std::vector<EOp> back;
+    std::vector<int> back_src;  // eops index of each op of `back` (-1: merged run)
     Plan bplan;
     int ns = 0, pm = 0;  // ns == nslots
     bool fused = g_adj_fusion && h->vq >= 5;
     if (fused) {
-        if (int rc = adjoint_back_plan(h->vq, eops, back, bplan, ns, pm)) return rc;
+        if (int rc = adjoint_back_plan(h->vq, eops, back, bplan, ns, pm, nullptr, &back_src)) return rc;
         const size_t limit = std::max<size_t>((size_t)4 << 30, sizeof(double2) * (size_t)len);
         fused = adjoint_tile_partial_bytes(bplan, h->vq, nslots, pm) <= limit;
     }
     if (fused) {
         auto single = [&](int i) -> int {
-            const EOp& e = eops[L - 1 - i];
+            if (back_src[i] < 0) return fail(BSIM_EINVAL, "adjoint: a merged run outside the tile passes");
+            const EOp& e = eops[back_src[i]];
             if (e.nparams > 0)
                 return fail(BSIM_ENOTSUP, "adjoint: parametrized gates act on at most 2 dense or 4 diagonal targets");
             return adjoint_step(h, work, lam, e, nullptr, grid);  // parameter-free: no partials

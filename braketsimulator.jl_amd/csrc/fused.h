@@ -41,6 +41,12 @@ struct EOp {
     int pidx[3] = { -1, -1, -1 };
     std::vector<double2> dm[3];
     int aslot = -1;              // fused adjoint: partial-sum slot of the op's parameters
+    // fused adjoint, a run of one-qubit ops merged into one op: product p
+    // belongs to parameter pprm[p] of slot pslot[p] (pslot[p] < 0: parameter
+    // p of aslot) and is taken before the op's update when ppre[p]
+    int pslot[3] = { -1, -1, -1 };
+    int pprm[3] = { 0, 0, 0 };
+    bool ppre[3] = { false, false, false };
     uint64_t touched() const;    // every vector bit the op reads, writes or depends on
                                  // (a kpres Kraus op: its row/column bits; its predicate
                                  // is accounted for by pair_diag)
@@ -55,7 +61,9 @@ int lower_ops(int nq, int kind, const bsim_op* ops, int n_ops, std::vector<EOp>&
 
 // Device-side tile program ------------------------------------------------
 
-struct TOp {
+// 64-byte aligned (tables start at 256-byte offsets): an op's head load
+// (kind .. nd, cmask, cval, m[0..4]) reads exactly its first two lines.
+struct alignas(64) TOp {
     // 0 dense1, 1 dense2, 2 diag, 3 sign parity, 4 predicated superoperator,
     // 5 its matrix, 6 diagonal on register bits only (m[j] = the factor of
     // register j, so every thread reads the same entries), 7 X on register a
@@ -105,8 +113,14 @@ struct TOp {
     uint32_t jmask;
     // fused adjoint: the op's product matrices follow as `nparam` kind-5
     // TOps (m = the EOp's dm in this op's layout, a = partial-sum slot,
-    // b = parameter)
+    // b = parameter, pre = 1: taken before the op's update).
+    // Pauli-coded products of an uncontrolled dense 1-register op (kind 0,
+    // cmask 0, cval != 0): payload -i s P with P = X, Y or Z, read from the
+    // head alone: cval = post | pre << 32, a word = row << 2 | P (P = 1 X,
+    // 2 Y, 3 Z; 0 no product; row = slot * PM + parameter), m[4] = (s of the
+    // pre product, s of the post product); the kind-5 TOps follow as well
     int32_t nparam;
+    int32_t pre;
 };
 
 struct Sweep {
@@ -198,9 +212,16 @@ int launch_kraus_ref(bsim_state* h, double2* d, const EOp& e);
 // and its plan (11 tile bits, 3 register bits).  nslots: parametrized ops;
 // pm: their largest parameter count.
 // `base` (optional) is the configuration snapshot whose other fields the
-// plan keeps; by default the library configuration.
+// plan keeps; by default the library configuration.  Runs of uncontrolled
+// one-qubit ops on one qubit are merged into one op of `back` (their
+// products taken before or after it); `src` (optional) receives the eops
+// index of each op of `back` (-1: a merged run).
 int adjoint_back_plan(int vq, const std::vector<EOp>& eops, std::vector<EOp>& back, Plan& plan, int& nslots,
-                      int& pm, const TileConfig* base = nullptr);
+                      int& pm, const TileConfig* base = nullptr, std::vector<int>* src = nullptr);
+
+// Pauli form of a product payload (2 x 2, column-major): m = -i s P with
+// P = X (1), Y (2), Z (3) and s real, to rounding; 0 when it has none.
+int pauli_form(const double2* m, double& s);
 
 // Bytes of the per-tile partial sums adjoint_tile_run needs for `plan`.
 size_t adjoint_tile_partial_bytes(const Plan& plan, int vq, int nslots, int pm);

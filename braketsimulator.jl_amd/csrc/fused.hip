@@ -785,8 +785,8 @@ static bool chain_sweep(const TOp* in, size_t n, std::vector<TOp>& out) {
     // the first four pair distances ride on the first chain; the sign is a
     // factor of the whole sub-cube, so it commutes with every op of the sweep
     size_t pk = 0;
-    if (!pp.empty())
-        for (size_t i = 0; i < res.size(); ++i)
+    if (!pp.empty()) {
+        for (size_t i = 0; i < res.size(); ++i) {
             if (res[i].kind == 12) {
                 for (; pk < pp.size() && pk < 4; ++pk) {
                     res[i].pd[pk] = pp[pk].first;
@@ -806,6 +806,8 @@ static bool chain_sweep(const TOp* in, size_t n, std::vector<TOp>& out) {
                 res.insert(res.begin() + (long)i + 1, p);
                 break;
             }
+        }
+    }
     // what no chain carries: sign ops at the end of the sweep (split form
     // only), the Z factors of registers without a dense op in the first
     bool first = true;
@@ -1143,8 +1145,9 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
                         TOp d;
                         memset(&d, 0, sizeof(d));
                         d.kind = 5;
-                        d.a = e.aslot;
-                        d.b = p;
+                        d.a = e.pslot[p] >= 0 ? e.pslot[p] : e.aslot;
+                        d.b = e.pslot[p] >= 0 ? e.pprm[p] : p;
+                        d.pre = e.ppre[p] ? 1 : 0;
                         const int D = 1 << e.g.k;
                         if (op.kind == 2) {
                             for (int j = 0; j < D; ++j) d.m[j] = e.dm[p][j + D * j];

@@ -267,3 +267,53 @@ def dm_sparse_noise(n, depth, seed):
     ins.append(Instruction(Nz.Depolarizing(0.05), [0]))
     ins.append(Instruction(Nz.AmplitudeDamping(0.1), [n - 1]))
     return ins + W.brickwall(n, 2, seed=seed + 1)
+
+
+def one_qubit_runs(n, layers, seed):
+    """Parametrized circuit whose backward list holds every shape of
+    one-qubit run the adjoint planner merges or leaves alone, per layer and
+    qubit q (shape q % 7): RZ RY (Z product before the merged op, Y after),
+    RX RX (same axis: both products after), RY H RZ (a parameter-free op
+    inside the run), RZ RY RX (three rotations: the third stays apart),
+    PhaseShift RY (a payload of no Pauli form: no merge), U (three products)
+    and RY Control(RZ) (a controlled op ends the run); then CZ on (q, q+1)
+    for even q of the layer.  Returns (instructions, param_map, thetas)."""
+    rng = np.random.default_rng(seed)
+    ins, pmap = [], {}
+    p = 0
+
+    def rot(g, qs, k=1):
+        nonlocal p
+        a = [float(x) for x in rng.uniform(-3, 3, k)]
+        ins.append(Instruction(g(*a), qs))
+        for j in range(k):
+            pmap[(len(ins) - 1, j)] = p
+            p += 1
+
+    for layer in range(layers):
+        for q in range(n):
+            shape = (q + layer) % 7
+            if shape == 0:
+                rot(G.Rz, [q]); rot(G.Ry, [q])
+            elif shape == 1:
+                rot(G.Rx, [q]); rot(G.Rx, [q])
+            elif shape == 2:
+                rot(G.Ry, [q]); ins.append(Instruction(G.H(), [q])); rot(G.Rz, [q])
+            elif shape == 3:
+                rot(G.Rz, [q]); rot(G.Ry, [q]); rot(G.Rx, [q])
+            elif shape == 4:
+                rot(G.PhaseShift, [q]); rot(G.Ry, [q])
+            elif shape == 5:
+                rot(G.U, [q], 3)
+            else:
+                rot(G.Ry, [q])
+                a = float(rng.uniform(-3, 3))
+                ins.append(Instruction(G.Control(G.Rz(a), (1,)), [(q + 1) % n, q]))
+                pmap[(len(ins) - 1, 0)] = p
+                p += 1
+        for q in range(layer % 2, n - 1, 2):
+            ins.append(Instruction(G.CZ(), [q, q + 1]))
+    th = np.zeros(p)
+    for (i, j), k in pmap.items():
+        th[k] = ins[i].operator.angles[j]
+    return ins, pmap, th

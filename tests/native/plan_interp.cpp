@@ -5,6 +5,8 @@
 // full state vector, without tiles.  Comparing the result with the oracle
 // checks the planner's lowering, ordering, fusion and index-test tables on
 // the CPU.
+#include <algorithm>
+#include <cmath>
 #include <complex>
 #include <cstdio>
 #include <vector>
@@ -361,6 +363,24 @@ static double dot_top(const TOp& op, const TOp& d, const int* vb, uint64_t N, co
     return acc;
 }
 
+// Im <lam| P |psi> for a Pauli P (1 X, 2 Y, 3 Z) on vector bit `bit`, from
+// its definition: (X v)_i = v_(i^b), (Y v)_i = -i v_(i^b) if bit b of i is 0
+// else i v_(i^b), (Z v)_i = -v_i if bit b of i is set.
+static double pauli_im(int bit, int P, uint64_t N, const cd* psi, const cd* lam) {
+    const uint64_t b = 1ull << bit;
+    const cd I(0.0, 1.0);
+    cd acc = 0.0;
+    for (uint64_t i = 0; i < N; ++i) {
+        const bool one = (i & b) != 0;
+        cd y;
+        if (P == 1) y = psi[i ^ b];
+        else if (P == 2) y = (one ? I : -I) * psi[i ^ b];
+        else y = one ? -psi[i] : psi[i];
+        acc += std::conj(lam[i]) * y;
+    }
+    return acc.imag();
+}
+
 // A non-tile step: the elementary op itself (dense or diagonal, any width).
 static void apply_eop_cpu(const EOp& e, uint64_t N, cd* v) {
     const GateSpec& g = e.g;
@@ -424,15 +444,44 @@ extern "C" int adjexec(int nq, const bsim_op* ops, int n, int n_params, double* 
                 const TOp& op = p.tops[o];
                 if (op.kind == 5 || op.kind == 4) return -22;
                 if (o + 1 + op.nparam > S.op_end) return -23;
-                // both vectors first, then the products with the payloads
-                apply_top(p, o, 3, vb, G, N, psi);
-                apply_top(p, o, 3, vb, G, N, lam);
+                // Pauli-coded products: the head's words must name the
+                // payloads that follow (one before the update, at most one
+                // after), and the coded form must give the payload's value
+                const bool coded = op.kind == 0 && op.cmask == 0 && op.cval != 0;
+                const uint32_t words[2] = { (uint32_t)op.cval, (uint32_t)(op.cval >> 32) };  // after, before
+                int named[2] = { 0, 0 };
+                int rc = 0;
+                auto product = [&](const TOp& d) {
+                    double v = dot_top(op, d, vb, N, psi, lam);
+                    if (coded) {
+                        const int side = d.pre ? 1 : 0;
+                        const uint32_t w = words[side];
+                        ++named[side];
+                        if (w == 0 || (w >> 2) != (uint32_t)(d.a * pm + d.b)) rc = -26;
+                        const double sc = d.pre ? op.m[4].x : op.m[4].y;
+                        const double c = sc * pauli_im(vb[op.a], (int)(w & 3u), N, psi, lam);
+                        if (std::fabs(c - v) > 1e-12 * std::max(1.0, std::fabs(v))) rc = -27;
+                        v = c;  // the kernel's number
+                    } else if (d.pre) {
+                        rc = -28;  // products before their op are coded only
+                    }
+                    out[3 * (size_t)d.a + d.b] += v;
+                    ++seen[3 * (size_t)d.a + d.b];
+                };
                 for (int q = 0; q < op.nparam; ++q) {
                     const TOp& d = p.tops[o + 1 + q];
-                    if (d.kind != 5 || d.a < 0 || d.a >= ns || d.b != q) return -24;
-                    out[3 * (size_t)d.a + q] += dot_top(op, d, vb, N, psi, lam);
-                    ++seen[3 * (size_t)d.a + q];
+                    if (d.kind != 5 || d.a < 0 || d.a >= ns || d.b < 0 || d.b >= pm) return -24;
                 }
+                for (int q = 0; q < op.nparam; ++q)
+                    if (p.tops[o + 1 + q].pre) product(p.tops[o + 1 + q]);
+                apply_top(p, o, 3, vb, G, N, psi);
+                apply_top(p, o, 3, vb, G, N, lam);
+                for (int q = 0; q < op.nparam; ++q)
+                    if (!p.tops[o + 1 + q].pre) product(p.tops[o + 1 + q]);
+                if (rc) return rc;
+                if (coded && ((words[0] != 0) != (named[0] == 1) || (words[1] != 0) != (named[1] == 1) ||
+                              named[0] > 1 || named[1] > 1))
+                    return -26;
                 o += 1 + op.nparam;
             }
         }
@@ -473,6 +522,31 @@ extern "C" int plan_chain_stats(int nq, int kind, const bsim_op* ops, int n, int
         if (t.kind >= 8 && t.kind <= 12) ++out[t.kind - 8];
         if (t.kind == 3 && t.split_only) ++out[5];
         if (t.kind == 12 && t.nd > out[6]) out[6] = t.nd;
+    }
+    return 0;
+}
+
+// backward-plan statistics of the fused adjoint: out[0] = ops of the
+// backward list, out[1] = merged one-qubit runs among them, out[2] = tile
+// ops with Pauli-coded products, out[3] = coded products taken before
+// their op, out[4] = coded products taken after
+extern "C" int plan_adj_stats(int nq, const bsim_op* ops, int n, int* out) {
+    std::vector<EOp> e;
+    if (int rc = lower_ops(nq, BSIM_SV, ops, n, e, true)) return rc;
+    std::vector<EOp> back;
+    std::vector<int> src;
+    Plan p;
+    int ns = 0, pm = 0;
+    if (int rc = adjoint_back_plan(nq, e, back, p, ns, pm, nullptr, &src)) return rc;
+    for (int k = 0; k < 5; ++k) out[k] = 0;
+    out[0] = (int)back.size();
+    for (int s : src) out[1] += s < 0 ? 1 : 0;
+    for (size_t o = 0; o < p.tops.size(); ++o) {
+        const TOp& t = p.tops[o];
+        if (t.kind != 0 || t.cmask != 0 || t.cval == 0) continue;
+        ++out[2];
+        out[3] += (t.cval >> 32) ? 1 : 0;
+        out[4] += (uint32_t)t.cval ? 1 : 0;
     }
     return 0;
 }

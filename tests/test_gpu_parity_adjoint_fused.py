@@ -140,3 +140,33 @@ def test_fused_adjoint_without_parameters():
     psi = O.evolve_sv(n, ins)
     assert grad.shape == (0,)
     assert abs(ev - sum(c * O.sv_expect_pauli(psi, n, w) for w, c in zip(words, coeffs))) < TOL
+
+
+@pytest.mark.parametrize("n,layers,seed", [(7, 2, 0), (12, 3, 1), (14, 2, 2)])
+def test_fused_adjoint_merged_runs(n, layers, seed):
+    """Merged one-qubit runs of every shape (Pauli-coded products before and
+    after the merged op, payload products after it, runs ended by a
+    parameter-free op, a non-Pauli payload or a controlled op): fused vs
+    per-op, and at n = 7 vs the oracle's central differences."""
+    from circuits import one_qubit_runs
+
+    _say(f"fused adjoint merged runs n={n} layers={layers} seed={seed}")
+    ins, pmap, th = one_qubit_runs(n, layers, seed)
+    words, coeffs = W.tfim(n)
+    words = list(words) + ["Y" + "I" * (n - 2) + "X"]
+    coeffs = list(coeffs) + [0.3]
+    ev_f, g_f, ev_p, g_p = _both(n, ins, pmap, words, coeffs)
+    assert ev_f == ev_p
+    np.testing.assert_allclose(g_f, g_p, atol=TOL)
+    if n == 7:
+        def energy(t):
+            psi = O.evolve_sv(n, A.with_params(ins, pmap, t))
+            return sum(c * O.sv_expect_pauli(psi, n, w) for w, c in zip(words, coeffs))
+
+        hstep = 1e-5
+        fd = np.zeros(len(th))
+        for k in range(len(th)):
+            tp = th.copy(); tp[k] += hstep
+            tm = th.copy(); tm[k] -= hstep
+            fd[k] = (energy(tp) - energy(tm)) / (2 * hstep)
+        np.testing.assert_allclose(g_f, fd, atol=1e-8)

@@ -63,6 +63,8 @@ def interp():
     h.adjexec.restype = ctypes.c_int
     h.adjexec.argtypes = [ctypes.c_int, ctypes.POINTER(L.Op), ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                           ctypes.c_void_p, ctypes.c_void_p]
+    h.plan_adj_stats.restype = ctypes.c_int
+    h.plan_adj_stats.argtypes = [ctypes.c_int, ctypes.POINTER(L.Op), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     return h
 
 
@@ -402,6 +404,52 @@ def test_adjoint_back_plan_general_gates(interp, n):
         fd[k] = (_energy(n, A.with_params(ins, pmap, tp), words, coeffs)
                  - _energy(n, A.with_params(ins, pmap, tm), words, coeffs)) / (2 * hstep)
     np.testing.assert_allclose(grad, fd, atol=1e-8)
+
+
+def _adj_stats(h, n, ins, pmap):
+    ol = lower_instructions(list(ins), L.BSIM_SV, param_map=pmap)
+    out = (ctypes.c_int * 5)()
+    assert h.plan_adj_stats(n, ol.ptr, ol.n, out) == 0
+    return list(out)
+
+
+@pytest.mark.parametrize("n,layers,seed", [(7, 2, 0), (12, 3, 1), (13, 2, 2)])
+def test_adjoint_merged_one_qubit_runs(interp, n, layers, seed):
+    """Runs of one-qubit ops on one qubit merge into one backward op, their
+    products taken before or after it (Pauli-coded) or after it (payloads):
+    every run shape of circuits.one_qubit_runs, interpreted on the CPU
+    (coded products cross-checked against their payloads inside adjexec),
+    vs central differences of the oracle; and the merges happen."""
+    from bsim import adjoint as A
+    from circuits import one_qubit_runs
+
+    ins, pmap, th = one_qubit_runs(n, layers, seed)
+    words, coeffs = W.tfim(n)
+    words = list(words) + ["Y" + "I" * (n - 2) + "X"]
+    coeffs = list(coeffs) + [0.3]
+    st = _adj_stats(interp, n, ins, pmap)
+    assert st[1] >= n * layers // 2, st   # merged runs
+    assert st[3] > 0 and st[4] > 0, st    # coded products on both sides
+    grad = _adj_interp(interp, n, ins, pmap, words, coeffs, len(th))
+    hstep = 1e-5
+    fd = np.zeros(len(th))
+    for k in range(len(th)):
+        tp = th.copy(); tp[k] += hstep
+        tm = th.copy(); tm[k] -= hstep
+        fd[k] = (_energy(n, A.with_params(ins, pmap, tp), words, coeffs)
+                 - _energy(n, A.with_params(ins, pmap, tm), words, coeffs)) / (2 * hstep)
+    np.testing.assert_allclose(grad, fd, atol=1e-8)
+
+
+def test_adjoint_merge_hea_counts(interp):
+    """The hardware-efficient ansatz's RZ^dagger RY^dagger pairs: every pair
+    one merged op with its Z product before and its Y product after."""
+    n, layers = 10, 3
+    ins, pmap, th = W.hea(n, layers, seed=3)
+    st = _adj_stats(interp, n, ins, pmap)
+    assert st[1] == n * layers and st[2] == n * layers, st
+    assert st[3] == n * layers and st[4] == n * layers, st
+    assert st[0] == len(ins) - n * layers, st
 
 
 @pytest.mark.parametrize("seed", range(3))
