@@ -116,6 +116,12 @@ class OpList:
         self._where = []  # (op index, slot): slot 0 = matrix, 1 + p = dmatrix[p]
         self._rec = None
         self._arr = None
+        # parameters bind() rewrites: Pauli rotations (op index, 0 Rx / 1 Ry /
+        # 2 Rz, parameter) and other gates (op index, instruction, {angle: parameter})
+        self._rot = []
+        self._gen = []
+        self._rot_arrays = None
+        self._offset = None
 
     def _mat(self, m, op: int, slot: int):
         a = np.asarray(m, dtype=np.complex128)
@@ -154,7 +160,8 @@ class OpList:
         if self._mats:
             buf = np.concatenate(self._mats)
             sizes = np.fromiter((m.size for m in self._mats), dtype=np.int64, count=len(self._mats))
-            addr = buf.ctypes.data + 16 * (np.cumsum(sizes) - sizes)
+            self._offset = np.cumsum(sizes) - sizes
+            addr = buf.ctypes.data + 16 * self._offset
             where = np.asarray(self._where, dtype=np.int64)
             main = where[:, 1] == 0
             rec["matrix"][where[main, 0]] = addr[main]
@@ -171,16 +178,70 @@ class OpList:
         self._finish()
         return self._arr
 
-    def add_call(self, call: KernelCall, params=None, dmats=None):
+    @staticmethod
+    def _call_arrays(call: KernelCall, dmats=None):
+        """(op type, matrix, derivative matrices) of a kernel call as add_call
+        records them: a diagonal call with derivatives becomes a dense gate."""
         if call.diagonal:
             d = np.asarray(call.matrix, dtype=np.complex128)
             if dmats is not None:
                 dmats = [np.diag(np.asarray(x)) if np.ndim(x) == 1 else x for x in dmats]
-                self.add(L.OP_GATE, call.targets, np.diag(d), call.controls, call.cvals, params=params, dmats=dmats)
-            else:
-                self.add(L.OP_DIAG, call.targets, d, call.controls, call.cvals)
-        else:
-            self.add(L.OP_GATE, call.targets, call.matrix, call.controls, call.cvals, params=params, dmats=dmats)
+                return L.OP_GATE, np.diag(d), dmats
+            return L.OP_DIAG, d, None
+        return L.OP_GATE, call.matrix, dmats
+
+    def add_call(self, call: KernelCall, params=None, dmats=None):
+        typ, m, dm = self._call_arrays(call, dmats)
+        self.add(typ, call.targets, m, call.controls, call.cvals, params=params, dmats=dm)
+
+    def bind(self, theta) -> "OpList":
+        """Rebind the parameters of an op list lowered with a param_map: every
+        mapped angle (i, j) becomes theta[param_map[(i, j)]] (as
+        adjoint.with_params sets it) and the op's matrix and derivative
+        matrices are recomputed in place -- the same numbers a new lowering of
+        the rebound instructions gives, without lowering the circuit again.
+        Pauli rotations are recomputed together; other gates one by one."""
+        self._finish()
+        th = np.asarray(theta, dtype=np.float64)
+        buf = self._buf if self._mats else None
+        if self._rot:
+            if self._rot_arrays is None:
+                where = {w: int(o) for w, o in zip(self._where, self._offset)}
+                r = np.asarray(self._rot, dtype=np.int64)
+                self._rot_arrays = (r[:, 1], r[:, 2],
+                                    np.asarray([where[(int(i), 0)] for i in r[:, 0]], dtype=np.int64),
+                                    np.asarray([where[(int(i), 1)] for i in r[:, 0]], dtype=np.int64))
+            code, pidx, m_off, d_off = self._rot_arrays
+            t = th[pidx]
+            cols = np.arange(4)
+            for scale, ang, off in ((1.0, t, m_off), (0.5, t + math.pi, d_off)):
+                half = (ang / 2.0).tolist()
+                s = np.array([math.sin(x) for x in half])
+                c = np.array([math.cos(x) for x in half])
+                m = np.zeros((t.size, 4), dtype=np.complex128)
+                rx, ry, rz = code == 0, code == 1, code == 2
+                m[:, 0] = np.where(rz, c - 1j * s, c)
+                m[:, 3] = np.where(rz, c + 1j * s, c)
+                m[:, 1] = np.where(rx, -1j * s, np.where(ry, s, 0.0))
+                m[:, 2] = np.where(rx, -1j * s, np.where(ry, -s, 0.0))
+                buf[off[:, None] + cols] = scale * m if scale != 1.0 else m
+        if self._gen:
+            where = {w: int(o) for w, o in zip(self._where, self._offset)}
+            for op_i, ins, jp in self._gen:
+                g = ins.operator
+                ang = list(g.angles)
+                for j, p in jp.items():
+                    ang[j] = float(th[p])
+                g2 = g.with_angles(ang)
+                js = sorted(jp)
+                _, m, dms = self._call_arrays(kernel_call(g2, ins.target),
+                                              derivative_matrices(g2, ins.target, js))
+                for slot, a in enumerate([m] + list(dms)):
+                    a = np.asarray(a, dtype=np.complex128)
+                    flat = a.ravel(order="F") if a.ndim == 2 else a.ravel()
+                    o = where[(op_i, slot)]
+                    buf[o:o + flat.size] = flat
+        return self
 
     @property
     def ptr(self):
@@ -227,6 +288,7 @@ _CALL_CACHE: dict = {}
 # Pauli rotations at exponent 1: matrix raw(theta) and derivative
 # raw(theta + pi) / 2, the numbers kernel_call / derivative_matrices give
 _ROTATIONS = {G.Rx: G._rx, G.Ry: G._ry, G.Rz: G._rz}
+_ROT_CODE = {G.Rx: 0, G.Ry: 1, G.Rz: 2}
 
 
 def lower_instructions(instructions, kind: int, noise_mode: str = "reference", param_map=None,
@@ -246,6 +308,7 @@ def lower_instructions(instructions, kind: int, noise_mode: str = "reference", p
             params = dmats = None
             if param_map is not None and (i, 0) in param_map:
                 params, dmats = [param_map[(i, 0)]], [0.5 * raw(th + math.pi)]
+                ol._rot.append((ol.n, _ROT_CODE[tp], params[0]))
             ol.add(L.OP_GATE, [int(ins.target[0])], raw(th), params=params, dmats=dmats)
             continue
         if fast and tp in _FIXED_GATES:
@@ -265,6 +328,7 @@ def lower_instructions(instructions, kind: int, noise_mode: str = "reference", p
                 if idx:
                     params = [p for _, p in idx]
                     dmats = derivative_matrices(op, ins.target, [j for j, _ in idx])
+                    ol._gen.append((ol.n, ins, dict(idx)))
             ol.add_call(call, params, dmats)
         elif isinstance(op, Superoperator):
             if kind != L.BSIM_DM:

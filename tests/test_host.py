@@ -241,6 +241,52 @@ def test_fast_lowering_matches_general_path():
         assert a._buf.tobytes() == b._buf.tobytes()
 
 
+def test_bind_matches_lowering_of_rebound_instructions():
+    """OpList.bind(theta) on an op list lowered with a param_map gives the
+    op records and matrix values of lowering the rebound instructions
+    (adjoint.with_params) from scratch: Pauli rotations (recomputed
+    together), other parametrized gates (U, PhaseShift, controlled
+    rotations, XX, ZZ, CPhaseShift) and gates without parameters; binding
+    twice in a row and binding back to the original angles included."""
+    from bsim import adjoint as A
+
+    from circuits import one_qubit_runs
+
+    rng = np.random.default_rng(11)
+    cases = [W.hea(9, 3, seed=4), one_qubit_runs(8, 3, 5)]
+    ins, pmap, p = [], {}, 0
+    for q in range(6):
+        for g, k in ((G.U, 3), (G.PhaseShift, 1), (G.Rx, 1)):
+            ins.append(Instruction(g(*rng.uniform(-3, 3, k)), [q]))
+            for j in range(k):
+                pmap[(len(ins) - 1, j)] = p
+                p += 1
+        g = [G.XX, G.ZZ, G.CPhaseShift][q % 3](float(rng.uniform(-3, 3)))
+        ins.append(Instruction(g, [q, (q + 1) % 6]))
+        pmap[(len(ins) - 1, 0)] = p
+        p += 1
+        ins.append(Instruction(G.Control(G.Ry(float(rng.uniform(-3, 3))), (0,)), [(q + 2) % 6, q]))
+        pmap[(len(ins) - 1, 0)] = p
+        p += 1
+        ins.append(Instruction(G.CNot(), [q, (q + 3) % 6]))
+    th0 = np.zeros(p)
+    for (i, j), k in pmap.items():
+        th0[k] = ins[i].operator.angles[j]
+    cases.append((ins, pmap, th0))
+    for instrs, pm, th in cases:
+        ol = lower_instructions(list(instrs), L.BSIM_SV, param_map=pm)
+        ol.ptr  # the records and buffer exist before rebinding
+        for t in (th + rng.uniform(-1, 1, th.size), th - 0.25, th):
+            ol.bind(t)
+            ref = lower_instructions(A.with_params(list(instrs), pm, t), L.BSIM_SV, param_map=pm)
+            ref._finish()
+            assert ol.n == ref.n
+            for f in ("type", "n_targets", "n_controls", "targets", "controls", "control_values", "n_params",
+                      "param_index"):
+                np.testing.assert_array_equal(ol._rec[f], ref._rec[f], err_msg=f)
+            np.testing.assert_array_equal(ol._buf, ref._buf)
+
+
 def test_merge_width_setting():
     """bsim_set_merge_width: -1 reads, 0 and 3..6 set, anything else is refused."""
     import ctypes
