@@ -147,14 +147,249 @@ __global__ __launch_bounds__(AB) void k_adj_diag(double2* __restrict__ psi, doub
     if (P > 0) block_sum_adj<P>(acc, lds, partials + (size_t)blockIdx.x * 3);
 }
 
+// lam = H psi by gathers, one per word (accumulate: added to lam)
 __global__ __launch_bounds__(AB) void k_pauli_sum(const double2* __restrict__ psi, double2* __restrict__ lam,
-                                                  int64_t len, const PWord* __restrict__ words, int nw) {
+                                                  int64_t len, const PWord* __restrict__ words, int nw,
+                                                  int accumulate) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride) {
-        double2 acc = make_double2(0, 0);
+        double2 acc = accumulate ? lam[i] : make_double2(0, 0);
         for (int w = 0; w < nw; ++w) acc = cfma(pphase((uint64_t)i, words[w]), psi[(uint64_t)i ^ words[w].f], acc);
         lam[i] = acc;
     }
+}
+
+// lam = H psi as tile passes.  A word maps amplitude i ^ f to i with a phase
+// that depends on i: phase(i) = (-1)^(n0 ^ parity(i & m)) (cr + i ci) with
+// m = y ^ z, n0 = parity(y) and cr + i ci = c i^ny (pphase: the parity of
+// popcount(~i & y) + popcount(i & z)).  Words are grouped by f.  A pass
+// holds one tile of psi (PS_T local bits) in LDS and adds, for every group
+// whose f lies in the tile's bits, the group's phase sum times psi[i ^ f]
+// read from LDS; the first pass's tile is the low PS_T bits (contiguous),
+// later ones bits 0..2 and PS_T - 3 bits chosen for the groups left.
+// The first pass writes lam (and, when asked, the working copy of psi),
+// later ones add to it.  Groups that no tile covers are added by
+// k_pauli_sum.  Traffic per pass: psi read, lam written (and read after the
+// first): 2 or 3 x 16 B x 2^n, against one gather of psi per word.
+constexpr int PS_T = 11, PS_TB = 256, PS_R = (1 << PS_T) / PS_TB;  // 8 amplitudes per thread
+
+struct PSWord {
+    uint64_t m;
+    double cr, ci;
+    int32_t n0, pad_;
+};
+struct PSGroup {
+    uint32_t fl;  // flip mask in local index bits
+    int32_t w0, w1;
+    int32_t uniform;  // every word has m = 0: the phase sum is the same for all amplitudes
+};
+struct PSPass {
+    int32_t lbits[PS_T];  // vector bit of local bit b, ascending
+    uint64_t lmask;
+    int32_t g0, g1;
+};
+
+__global__ __launch_bounds__(PS_TB) void k_pauli_tiles(const double2* __restrict__ psi, double2* __restrict__ lam,
+                                                       double2* __restrict__ work, int vq, int first,
+                                                       const PSPass* __restrict__ pass,
+                                                       const PSGroup* __restrict__ groups,
+                                                       const PSWord* __restrict__ words,
+                                                       double* __restrict__ dot) {
+    extern __shared__ double2 ps_tile[];  // 2^PS_T amplitudes
+    const int t = threadIdx.x;
+    uint64_t lo = 0;  // local bits 0..7 of the thread's amplitudes
+    for (int b = 0; b < 8; ++b)
+        if ((t >> b) & 1) lo |= 1ull << pass->lbits[b];
+    uint64_t base = 0;
+    {
+        const uint64_t lm = pass->lmask, w = blockIdx.x;
+        for (int bit = 0, k = 0; bit < vq; ++bit)
+            if (!((lm >> bit) & 1)) base |= ((w >> k++) & 1) << bit;
+    }
+    const uint64_t gt = base | lo;
+    uint64_t sh[PS_R];  // the same for every thread: scalar
+#pragma unroll
+    for (int r = 0; r < PS_R; ++r) {
+        uint64_t h = 0;
+#pragma unroll
+        for (int b = 8; b < PS_T; ++b)
+            if ((r >> (b - 8)) & 1) h |= 1ull << pass->lbits[b];
+        sh[r] = h;
+    }
+    uint64_t g[PS_R];
+    double2 acc[PS_R];
+#pragma unroll
+    for (int r = 0; r < PS_R; ++r) {
+        g[r] = gt | sh[r];
+        const double2 v = psi[g[r]];
+        ps_tile[t + (r << 8)] = v;
+        if (work) work[g[r]] = v;
+        acc[r] = first ? make_double2(0.0, 0.0) : lam[g[r]];
+    }
+    __syncthreads();
+    for (int k = pass->g0; k < pass->g1; ++k) {
+        const PSGroup gr = groups[k];
+        if (gr.uniform) {
+            double sr = 0.0, si = 0.0;
+            for (int w = gr.w0; w < gr.w1; ++w) {
+                const double sg = words[w].n0 ? -1.0 : 1.0;
+                sr += sg * words[w].cr;
+                si += sg * words[w].ci;
+            }
+            const double2 sp = make_double2(sr, si);
+#pragma unroll
+            for (int r = 0; r < PS_R; ++r) acc[r] = cfma(sp, ps_tile[(t + (r << 8)) ^ gr.fl], acc[r]);
+        } else {
+            // parity(g_r & m) = parity(gt & m) ^ parity(sh[r] & m): a sign per
+            // thread and word times a scalar per word and row
+            double2 sp[PS_R];
+#pragma unroll
+            for (int r = 0; r < PS_R; ++r) sp[r] = make_double2(0.0, 0.0);
+            for (int w = gr.w0; w < gr.w1; ++w) {
+                const PSWord wd = words[w];
+                const double sg = ((__popcll(gt & wd.m) + wd.n0) & 1) ? -1.0 : 1.0;
+                if (wd.ci == 0.0) {
+#pragma unroll
+                    for (int r = 0; r < PS_R; ++r) {
+                        const double u = (__popcll(sh[r] & wd.m) & 1) ? -wd.cr : wd.cr;
+                        sp[r].x = fma(sg, u, sp[r].x);
+                    }
+                } else {
+#pragma unroll
+                    for (int r = 0; r < PS_R; ++r) {
+                        const bool hn = (__popcll(sh[r] & wd.m) & 1) != 0;
+                        sp[r].x = fma(sg, hn ? -wd.cr : wd.cr, sp[r].x);
+                        sp[r].y = fma(sg, hn ? -wd.ci : wd.ci, sp[r].y);
+                    }
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < PS_R; ++r) acc[r] = cfma(sp[r], ps_tile[(t + (r << 8)) ^ gr.fl], acc[r]);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < PS_R; ++r) lam[g[r]] = acc[r];
+    if (dot) {
+        // the last pass: <psi|lam> over the tile, dot[2 b .. 2 b + 1] (fixed order)
+        __shared__ double red[2][PS_TB / 64];
+        double dr = 0.0, di = 0.0;
+#pragma unroll
+        for (int r = 0; r < PS_R; ++r) {
+            const double2 x = ps_tile[t + (r << 8)];
+            dr += x.x * acc[r].x + x.y * acc[r].y;
+            di += x.x * acc[r].y - x.y * acc[r].x;
+        }
+        dr = wave_sum_adj(dr);
+        di = wave_sum_adj(di);
+        if ((t & 63) == 0) {
+            red[0][t >> 6] = dr;
+            red[1][t >> 6] = di;
+        }
+        __syncthreads();
+        if (t < 2) {
+            double v = 0.0;
+            for (int w = 0; w < PS_TB / 64; ++w) v += red[t][w];
+            dot[2 * (size_t)blockIdx.x + t] = v;
+        }
+    }
+}
+
+// out[i] = sum over b of part[2 b + i], i = 0, 1 (one block each, fixed order)
+__global__ __launch_bounds__(256) void k_ps_dot_finish(const double* __restrict__ part, int64_t nb,
+                                                       double* __restrict__ out) {
+    __shared__ double sh[256];
+    double v = 0.0;
+    for (int64_t b = threadIdx.x; b < nb; b += 256) v += part[2 * b + blockIdx.x];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+}
+
+// The passes of lam = H psi over a 2^vq vector (vq >= PS_T): groups of words
+// by flip mask, each group in the first pass whose tile holds its f; `rest`
+// gets the words of groups no tile holds (at most PS_T - 3 bits of f above
+// bit 2 fit one).
+struct PauliTilePlan {
+    std::vector<PSPass> passes;
+    std::vector<PSGroup> groups;
+    std::vector<PSWord> words;
+    std::vector<PWord> rest;
+};
+
+static void pauli_tile_plan(int vq, const std::vector<PWord>& in, PauliTilePlan& out) {
+    std::vector<uint64_t> fs;
+    std::vector<std::vector<int>> members;
+    for (int w = 0; w < (int)in.size(); ++w) {
+        size_t k = 0;
+        while (k < fs.size() && fs[k] != in[w].f) ++k;
+        if (k == fs.size()) {
+            fs.push_back(in[w].f);
+            members.emplace_back();
+        }
+        members[k].push_back(w);
+    }
+    std::vector<int> left(fs.size());
+    for (size_t k = 0; k < fs.size(); ++k) left[k] = (int)k;
+    uint64_t tm = (1ull << PS_T) - 1;  // the first tile: the low PS_T bits
+    const uint64_t all = vq >= 64 ? ~0ull : (1ull << vq) - 1;
+    while (!left.empty()) {
+        PSPass P;
+        memset(&P, 0, sizeof(P));
+        P.lmask = tm;
+        for (int bit = 0, b = 0; bit < vq; ++bit)
+            if ((tm >> bit) & 1) P.lbits[b++] = bit;
+        P.g0 = (int)out.groups.size();
+        std::vector<int> keep;
+        for (int k : left) {
+            if (fs[k] & ~tm) {
+                keep.push_back(k);
+                continue;
+            }
+            PSGroup G;
+            G.fl = 0;
+            for (int b = 0; b < PS_T; ++b)
+                if ((fs[k] >> P.lbits[b]) & 1) G.fl |= 1u << b;
+            G.w0 = (int)out.words.size();
+            G.uniform = 1;
+            for (int w : members[k]) {
+                const PWord& x = in[w];
+                PSWord y;
+                y.m = x.y ^ x.z;
+                y.n0 = __builtin_popcountll(x.y) & 1;
+                y.pad_ = 0;
+                const double c = x.c;
+                switch (x.ny & 3) {
+                    case 0: y.cr = c; y.ci = 0.0; break;
+                    case 1: y.cr = 0.0; y.ci = c; break;
+                    case 2: y.cr = -c; y.ci = 0.0; break;
+                    default: y.cr = 0.0; y.ci = -c; break;
+                }
+                G.uniform = G.uniform && y.m == 0;
+                out.words.push_back(y);
+            }
+            G.w1 = (int)out.words.size();
+            out.groups.push_back(G);
+        }
+        P.g1 = (int)out.groups.size();
+        if (P.g1 == P.g0) break;  // no group fits the tile
+        out.passes.push_back(P);
+        left.swap(keep);
+        // the next tile: bits 0..2 (coalesced runs) and the flip bits of the
+        // groups left, as many whole groups as fit, then the lowest free bits
+        tm = 7;
+        for (int k : left) {
+            const uint64_t need = fs[k] & ~7ull;
+            if (__builtin_popcountll(tm | need) <= PS_T) tm |= need;
+        }
+        for (int bit = 3; bit < vq && __builtin_popcountll(tm) < PS_T; ++bit) tm |= 1ull << bit;
+        tm &= all;
+    }
+    for (int k : left)
+        for (int w : members[k]) out.rest.push_back(in[w]);
 }
 
 // One block per parameter slot: partials[slot][block][3] -> out[slot][3].
@@ -174,6 +409,62 @@ __global__ __launch_bounds__(256) void k_adj_finish(const double* __restrict__ p
         __syncthreads();
     }
     if (threadIdx.x < 3) out[(size_t)blockIdx.x * 3 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+static std::atomic<bool> g_ps_shmem{ false };
+
+// lam = H psi on h's vector with the plan `pt`: its tables go to
+// dev + off_*, the gather words to dwords; `work` (optional) receives a
+// copy of psi.
+static int pauli_sum_passes(bsim_state* h, const PauliTilePlan& pt, char* dev, size_t off_pass, size_t off_grp,
+                            size_t off_wd, PWord* dwords, double2* lam, double2* work, double* dot2) {
+    const int64_t len = h->len;
+    // <psi|lam> from the last tile pass when no gather follows it
+    const bool fused_dot = !pt.passes.empty() && pt.rest.empty();
+    double* part = nullptr;
+    if (fused_dot) {
+        if (int rc = ensure_ws(h->red, sizeof(double) * 2 * ((size_t)1 << (h->vq - PS_T)))) return rc;
+        part = (double*)h->red.ptr;
+    }
+    if (!pt.passes.empty()) {
+        if (!g_ps_shmem) {
+            BSIM_HIP(hipFuncSetAttribute((const void*)k_pauli_tiles, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)(sizeof(double2) << PS_T)));
+            g_ps_shmem = true;
+        }
+        BSIM_HIP(hipMemcpyAsync(dev + off_pass, pt.passes.data(), sizeof(PSPass) * pt.passes.size(),
+                                hipMemcpyHostToDevice, h->stream));
+        BSIM_HIP(hipMemcpyAsync(dev + off_grp, pt.groups.data(), sizeof(PSGroup) * pt.groups.size(),
+                                hipMemcpyHostToDevice, h->stream));
+        BSIM_HIP(hipMemcpyAsync(dev + off_wd, pt.words.data(), sizeof(PSWord) * pt.words.size(),
+                                hipMemcpyHostToDevice, h->stream));
+        const unsigned grid = (unsigned)(1ull << (h->vq - PS_T));
+        for (size_t p = 0; p < pt.passes.size(); ++p) {
+            const bool last = p + 1 == pt.passes.size();
+            hipLaunchKernelGGL(k_pauli_tiles, dim3(grid), dim3(PS_TB), sizeof(double2) << PS_T, h->stream, h->d, lam,
+                               p == 0 ? work : nullptr, h->vq, p == 0 ? 1 : 0, (const PSPass*)(dev + off_pass) + p,
+                               (const PSGroup*)(dev + off_grp), (const PSWord*)(dev + off_wd),
+                               fused_dot && last ? part : nullptr);
+            BSIM_HIP(hipGetLastError());
+        }
+        if (fused_dot) {
+            hipLaunchKernelGGL(k_ps_dot_finish, dim3(2), dim3(256), 0, h->stream, (const double*)part, (int64_t)grid,
+                               dot2);
+            BSIM_HIP(hipGetLastError());
+            return BSIM_OK;
+        }
+    } else if (work) {
+        BSIM_HIP(hipMemcpyAsync(work, h->d, sizeof(double2) * (size_t)len, hipMemcpyDeviceToDevice, h->stream));
+    }
+    if (!pt.rest.empty() || pt.passes.empty()) {
+        BSIM_HIP(hipMemcpyAsync(dwords, pt.rest.data(), sizeof(PWord) * pt.rest.size(), hipMemcpyHostToDevice,
+                                h->stream));
+        const int64_t g = grid_for(len, AB, (int64_t)device_cu_count() * 32);
+        hipLaunchKernelGGL(k_pauli_sum, dim3((unsigned)g), dim3(AB), 0, h->stream, h->d, lam, len, dwords,
+                           (int)pt.rest.size(), pt.passes.empty() ? 0 : 1);
+        BSIM_HIP(hipGetLastError());
+    }
+    return dot_device(h->d, lam, len, h->red, h->stream, dot2);
 }
 
 static void conj_transpose(const std::vector<double2>& m, int D, double2* out) {
@@ -634,28 +925,36 @@ int bsim_adjoint_gradient(bsim_state* h, const bsim_op* ops, int n_ops, int n_wo
     int nslots = 0;
     for (const EOp& e : eops) nslots += e.nparams > 0 ? 1 : 0;
     // partials: [slot][block][3]; sized so that the <H> dot product (which
-    // uses the same workspace first, up to 4096 x 2 doubles) never regrows it
-    // after `partials` is taken
-    size_t red_doubles = std::max((size_t)(nslots + 1) * grid * 3, (size_t)4096 * 2) + 64;
+    // uses the same workspace first: up to 4096 x 2 doubles, or 2 per tile of
+    // the last lambda pass) never regrows it after `partials` is taken
+    const size_t dot_doubles = h->vq >= PS_T ? (size_t)2 << (h->vq - PS_T) : 0;
+    size_t red_doubles = std::max({ (size_t)(nslots + 1) * grid * 3, (size_t)4096 * 2, dot_doubles }) + 64;
     if (int rc = ensure_ws(h->red, sizeof(double) * red_doubles)) return rc;
     double* partials = (double*)h->red.ptr;
-    if (int rc = ensure_ws(h->res, sizeof(double) * (3 * (size_t)nslots + 8) + sizeof(PWord) * (size_t)n_words))
-        return rc;
+    // lambda = H psi: tile passes (vectors of at least PS_T bits), the groups
+    // no tile holds by gathers
+    PauliTilePlan pt;
+    if (h->vq >= PS_T) pauli_tile_plan(h->vq, words, pt);
+    else pt.rest = words;
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t res_bytes = al(sizeof(double) * (3 * (size_t)nslots + 8));
+    const size_t off_pass = res_bytes + al(sizeof(PWord) * (size_t)n_words);
+    const size_t off_grp = off_pass + al(sizeof(PSPass) * pt.passes.size());
+    const size_t off_wd = off_grp + al(sizeof(PSGroup) * pt.groups.size());
+    if (int rc = ensure_ws(h->res, off_wd + al(sizeof(PSWord) * pt.words.size()))) return rc;
     double* res = (double*)h->res.ptr;
-    PWord* dwords = (PWord*)(res + 3 * (size_t)nslots + 8);
+    PWord* dwords = (PWord*)((char*)h->res.ptr + res_bytes);
     // forward
     if (int rc = bsim_state_reset(h)) return rc;
     Plan plan;
     if (int rc = make_plan(h->vq, eops, tile_config(), plan)) return rc;
     if (int rc = prepare_tile_kernel()) return rc;
     if (int rc = run_plan(h, h->d, eops, plan)) return rc;
-    // lambda = H psi, <H>
-    BSIM_HIP(hipMemcpyAsync(dwords, words.data(), sizeof(PWord) * n_words, hipMemcpyHostToDevice, h->stream));
-    int64_t g = grid_for(len, AB, (int64_t)device_cu_count() * 32);
-    hipLaunchKernelGGL(k_pauli_sum, dim3((unsigned)g), dim3(AB), 0, h->stream, h->d, lam, len, dwords, n_words);
-    BSIM_HIP(hipGetLastError());
-    if (int rc = dot_device(h->d, lam, len, h->red, h->stream, res + 3 * (size_t)nslots)) return rc;
-    BSIM_HIP(hipMemcpyAsync(work, h->d, sizeof(double2) * (size_t)len, hipMemcpyDeviceToDevice, h->stream));
+    // lambda = H psi (the first tile pass also writes the working copy of
+    // psi, the last one <H>)
+    if (int rc = pauli_sum_passes(h, pt, (char*)h->res.ptr, off_pass, off_grp, off_wd, dwords, lam, work,
+                                  res + 3 * (size_t)nslots))
+        return rc;
     // backward
     int slot = nslots;
     // tile passes over the reversed list of U^dagger ops (both vectors in LDS,

@@ -170,3 +170,54 @@ def test_fused_adjoint_merged_runs(n, layers, seed):
             tm = th.copy(); tm[k] -= hstep
             fd[k] = (energy(tp) - energy(tm)) / (2 * hstep)
         np.testing.assert_allclose(g_f, fd, atol=1e-8)
+
+
+def _mixed_words(n, rng, gathered=True):
+    """Pauli words of every shape lambda = H psi plans for: Z Z and X terms
+    (diagonal group, single flips in the first tile and in later ones), Y X
+    across the low and high bits, two-flip words within the low bits, and
+    with `gathered` a word flipping every qubit (no tile holds it: the
+    gather kernel, after which <psi|lambda> is its own reduction) and random
+    words with random coefficients."""
+    words, coeffs = W.tfim(n)
+    words, coeffs = list(words), list(coeffs)
+    words.append("Y" + "I" * (n - 2) + "X")
+    coeffs.append(0.3)
+    words.append("I" * (n - 4) + "XZYI")
+    coeffs.append(-0.35)
+    if gathered:
+        words.append("X" * n)
+        coeffs.append(-0.2)
+        words.append("Y" * (n // 2) + "Z" * (n - n // 2))
+        coeffs.append(0.15)
+        for _ in range(6):
+            words.append("".join(rng.choice(list("IXYZ"), n)))
+            coeffs.append(float(rng.uniform(-1, 1)))
+    return words, coeffs
+
+
+@pytest.mark.parametrize("n,gathered", [(11, True), (12, True), (13, True), (16, True), (13, False), (16, False)])
+def test_lambda_tile_passes_vs_oracle(n, gathered):
+    """lambda = H psi as tile passes (words grouped by flip mask, each group
+    added in the first pass whose LDS tile of psi holds its flips; the rest
+    by gathers; <H> from the last tile pass when no gather follows): the
+    adjoint's <H> = Re <psi|lambda> against the oracle's sum of word
+    expectations, and the gradient (which reads all of lambda) against the
+    oracle's parameter shift, exact for RY and RZ, on every parameter
+    (n <= 13) or a sample of 12 (n = 16)."""
+    _say(f"lambda tile passes n={n} gathered={gathered}")
+    rng = np.random.default_rng(500 + n)
+    ins, pmap, th = W.hea(n, 2, seed=60 + n)
+    words, coeffs = _mixed_words(n, rng, gathered)
+    ev, grad = A.adjoint_gradient(StateVectorSimulator(n, 0), ins, pmap, words, coeffs, fused=True)
+
+    def energy(t):
+        psi = O.evolve_sv(n, A.with_params(ins, pmap, t))
+        return sum(c * O.sv_expect_pauli(psi, n, w) for w, c in zip(words, coeffs))
+
+    assert abs(ev - energy(th)) < TOL
+    sample = range(len(th)) if n <= 13 else rng.choice(len(th), 12, replace=False)
+    for p in sample:
+        tp = th.copy(); tp[p] += np.pi / 2
+        tm = th.copy(); tm[p] -= np.pi / 2
+        assert abs(grad[p] - 0.5 * (energy(tp) - energy(tm))) < TOL, p
