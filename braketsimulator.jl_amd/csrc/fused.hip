@@ -868,7 +868,7 @@ static void chain_pass(Plan& plan, const Pass& P) {
     plan.top_src.swap(src);
 }
 
-int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan& plan) {
+static int make_plan_ops(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan& plan) {
     plan = Plan();
     const size_t n = eops.size();
     int T = std::min(cfg.tile_bits, vq);
@@ -1262,6 +1262,73 @@ int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan&
         }
     }
     return BSIM_OK;
+}
+
+// CNOT(c, t) = H_t CZ(c, t) H_t.  With every other op free of controls,
+// the rewrite leaves an op list the lean streaming passes take: the CZ is a
+// sign op, and the H's join the one-qubit ops of t in register chains (a
+// controlled op keeps a pass on the full kernel).  Returns false (out
+// untouched) when some other op keeps a control or cannot run in a tile.
+static bool cnot_rewrite(const std::vector<EOp>& in, std::vector<EOp>& out) {
+    auto is_cnot = [](const EOp& e) {
+        const GateSpec& g = e.g;
+        return e.kind == E_DENSE && !g.diag && g.k == 1 && popc(g.cmask) == 1 && g.cval == g.cmask &&
+               e.nparams == 0 && e.aslot < 0 && g.m[0].x == 0.0 && g.m[0].y == 0.0 && g.m[1].x == 1.0 &&
+               g.m[1].y == 0.0 && g.m[2].x == 1.0 && g.m[2].y == 0.0 && g.m[3].x == 0.0 && g.m[3].y == 0.0;
+    };
+    size_t ncnot = 0;
+    for (const EOp& e : in) {
+        if (is_cnot(e)) {
+            ++ncnot;
+            continue;
+        }
+        uint64_t pm;
+        int pd;
+        if ((e.kind != E_DENSE && e.kind != E_DIAG) || !e.fusable()) return false;
+        if (e.g.cmask && !(e.kind == E_DIAG && sign_pair(e, pm, pd))) return false;
+    }
+    if (!ncnot) return false;
+    out.clear();
+    out.reserve(in.size() + 2 * ncnot);
+    const double h = 0.70710678118654752440;
+    for (const EOp& e : in) {
+        if (!is_cnot(e)) {
+            out.push_back(e);
+            continue;
+        }
+        EOp H;
+        H.kind = E_DENSE;
+        H.g.k = 1;
+        H.g.tb[0] = e.g.tb[0];
+        H.g.m = { make_double2(h, 0.0), make_double2(h, 0.0), make_double2(h, 0.0), make_double2(-h, 0.0) };
+        EOp Z;
+        Z.kind = E_DIAG;
+        Z.g.k = 1;
+        Z.g.tb[0] = e.g.tb[0];
+        Z.g.cmask = e.g.cmask;
+        Z.g.cval = e.g.cval;
+        Z.g.diag = true;
+        Z.g.m = { make_double2(1.0, 0.0), make_double2(-1.0, 0.0) };
+        out.push_back(H);
+        out.push_back(Z);
+        out.push_back(H);
+    }
+    return true;
+}
+
+int make_plan(int vq, const std::vector<EOp>& eops, const TileConfig& cfg, Plan& plan) {
+    std::vector<EOp> rw;
+    if (cfg.chains && cfg.tile_bits == 12 && cfg.reg_bits == 4 && vq >= 12 && cnot_rewrite(eops, rw)) {
+        if (int rc = make_plan_ops(vq, rw, cfg, plan)) return rc;
+        // single steps index the rewritten list: they carry their op
+        for (Plan::Step& st : plan.steps)
+            if (!st.tile && st.merged < 0) {
+                plan.merged.push_back(rw[st.op]);
+                st.merged = (int)plan.merged.size() - 1;
+            }
+        return BSIM_OK;
+    }
+    return make_plan_ops(vq, eops, cfg, plan);
 }
 
 }  // namespace bsim

@@ -551,3 +551,51 @@ def test_planner_chains_density_matrix(interp, mode):
     ins = dm_sparse_noise(7, 6, 7)
     assert _chain_stats(interp, 7, ins, L.BSIM_DM, mode)["chain"] > 0
     np.testing.assert_allclose(_run_dm(interp, 7, ins, mode), O.evolve_dm(7, ins, mode), atol=1e-12)
+
+
+def _cnot_circuit(n, layers, seed, cval0=False):
+    """Random one-qubit gates and CNOT ladders, controls on low and high
+    bits (with `cval0` one CNOT with control value 0, which keeps the
+    original op list)."""
+    rng = np.random.default_rng(seed)
+    ins = []
+    for layer in range(layers):
+        for q in range(n):
+            ins.append(Instruction(G.U(*rng.uniform(-3, 3, 3)), [q]))
+        for q in range(layer % 2, n - 1, 2):
+            ins.append(Instruction(G.CNot(), [q, q + 1] if (q + layer) % 3 else [q + 1, q]))
+        ins.append(Instruction(G.CNot(), [0, n - 1]))
+        ins.append(Instruction(G.CZ(), [1, n - 2]))
+    if cval0:
+        ins.append(Instruction(G.Control(G.X(), (0,)), [2, 5]))
+    return ins
+
+
+@pytest.mark.parametrize("n,layers,seed", [(12, 4, 0), (13, 5, 1), (15, 3, 2)])
+def test_planner_cnot_rewrite(interp, n, layers, seed):
+    """CNOT(c, t) planned as H_t CZ(c, t) H_t when every other op is free of
+    controls (12-bit, 4-register passes with chains): the plan holds no
+    controlled tile op and forms chains, and the interpreted plan matches
+    the oracle; one CNOT with control value 0 keeps the original list."""
+    ins = _cnot_circuit(n, layers, seed)
+    ol = lower_instructions(ins, L.BSIM_SV)
+    assert interp.plan_kind_count(n, L.BSIM_SV, ol.ptr, ol.n, 0, 1) == 0
+    assert _chain_stats(interp, n, ins)["chain"] > 0
+    assert np.max(np.abs(_run(interp, n, ins) - O.evolve_sv(n, ins))) < 1e-12
+    hea, _, _ = W.hea(n, layers, seed=seed)
+    ol = lower_instructions(hea, L.BSIM_SV)
+    assert interp.plan_kind_count(n, L.BSIM_SV, ol.ptr, ol.n, 0, 1) == 0
+    assert np.max(np.abs(_run(interp, n, hea) - O.evolve_sv(n, hea))) < 1e-12
+    ins = _cnot_circuit(n, layers, seed, cval0=True)
+    ol = lower_instructions(ins, L.BSIM_SV)
+    assert interp.plan_kind_count(n, L.BSIM_SV, ol.ptr, ol.n, 0, 1) > 0
+    assert np.max(np.abs(_run(interp, n, ins) - O.evolve_sv(n, ins))) < 1e-12
+
+
+def test_planner_cnot_rewrite_density_matrix(interp):
+    """The rewrite on a noise-free density matrix (both halves of every
+    gate) against the oracle."""
+    ins = _cnot_circuit(6, 4, 9)
+    ol = lower_instructions(ins, L.BSIM_DM)
+    assert interp.plan_kind_count(6, L.BSIM_DM, ol.ptr, ol.n, 0, 1) == 0
+    np.testing.assert_allclose(_run_dm(interp, 6, ins), O.evolve_dm(6, ins, "reference"), atol=1e-12)
