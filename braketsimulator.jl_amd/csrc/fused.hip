@@ -1090,6 +1090,48 @@ static int make_plan_ops(int vq, const std::vector<EOp>& eops, const TileConfig&
                             col[c].kind = 5;
                             for (int r = 0; r < 16; ++r) col[c].m[r] = e.g.m[J(r) + 16 * J(c)];
                         }
+                        // A physical superoperator with entries only where the row
+                        // and column registers differ by an element of D = {0, P1,
+                        // P2, P1 ^ P2} (P1, P2: a pairing of the four registers, e.g.
+                        // a qubit's row and column bit; Pauli channels have this
+                        // form) is four 4x4 blocks on the orbits o ^ D: b = 1 +
+                        // pairing, the head and 3 TOps hold the blocks (orbit k,
+                        // representative the k-th register with the lowest bits of
+                        // P1 and P2 clear; m[r + 4 c] couples o ^ X[r] to o ^ X[c],
+                        // X = {0, P1, P2, P1 ^ P2}).
+                        static const int kP1[3] = { 0x3, 0x5, 0x9 }, kP2[3] = { 0xC, 0xA, 0x6 };
+                        for (int pi = 0; pi < 3 && e.kind == E_DENSE; ++pi) {
+                            const int P1 = kP1[pi], P2 = kP2[pi], X[4] = { 0, P1, P2, P1 ^ P2 };
+                            bool form = true;
+                            for (int r = 0; r < 16 && form; ++r)
+                                for (int c = 0; c < 16 && form; ++c) {
+                                    const int d = r ^ c;
+                                    const bool in = d == X[0] || d == X[1] || d == X[2] || d == X[3];
+                                    form = in || (col[c].m[r].x == 0.0 && col[c].m[r].y == 0.0);
+                                }
+                            if (!form) continue;
+                            const int a1 = __builtin_ctz(P1), a2 = __builtin_ctz(P2);
+                            TOp blk[4];
+                            int k = 0;
+                            for (int o = 0; o < 16; ++o) {
+                                if ((o >> a1) & 1 || (o >> a2) & 1) continue;
+                                memset(&blk[k], 0, sizeof(TOp));
+                                blk[k].kind = 5;
+                                for (int r = 0; r < 4; ++r)
+                                    for (int c = 0; c < 4; ++c) blk[k].m[r + 4 * c] = col[o ^ X[c]].m[o ^ X[r]];
+                                ++k;
+                            }
+                            op.b = 1 + pi;
+                            op.nd = 3;
+                            for (int r = 0; r < 16; ++r) op.m[r] = blk[0].m[r];
+                            plan.tops.push_back(op);
+                            plan.top_src.push_back(i);
+                            for (int q = 1; q < 4; ++q) {
+                                plan.tops.push_back(blk[q]);
+                                plan.top_src.push_back(i);
+                            }
+                            return;
+                        }
                         for (int r = 0; r < 16; ++r) op.m[r] = col[0].m[r];
                         plan.tops.push_back(op);
                         plan.top_src.push_back(i);

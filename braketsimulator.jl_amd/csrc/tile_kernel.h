@@ -515,6 +515,47 @@ __device__ __forceinline__ void sw_kraus4(SubCube<4>& s, const TOp* __restrict__
     });
 }
 
+// A 16x16 superoperator of orbit form (kind 4, b = 1 + pairing): entries
+// only where the row and column registers differ by an element of
+// X = {0, P1, P2, P1 ^ P2}, so each orbit {o ^ X[r]} is updated by its own
+// 4x4 block (op + k for the k-th orbit), in registers.
+template <int P1, int P2>
+__device__ __forceinline__ void sw_orbit4(SubCube<4>& s, const TOp* __restrict__ op) {
+    constexpr int A1 = __builtin_ctz(P1), A2 = __builtin_ctz(P2);
+    constexpr int REST = 0xF & ~((1 << A1) | (1 << A2));
+    constexpr int B1 = __builtin_ctz(REST), B2 = __builtin_ctz(REST & (REST - 1));
+    sfor<0, 16>([&](auto J) {
+        constexpr int o = decltype(J)::value;
+        if constexpr (((o >> A1) & 1) == 0 && ((o >> A2) & 1) == 0) {
+            constexpr int k = ((o >> B1) & 1) | (((o >> B2) & 1) << 1);
+            constexpr int i0 = o, i1 = o ^ P1, i2 = o ^ P2, i3 = o ^ P1 ^ P2;
+            const TOp* __restrict__ blk = op + k;
+            const c2 x0 = getv<i0>(s.v), x1 = getv<i1>(s.v), x2 = getv<i2>(s.v), x3 = getv<i3>(s.v);
+            auto row = [&](int r) {
+                c2 acc = cmul(to_c2(blk->m[r]), x0);
+                acc = cfma(to_c2(blk->m[r + 4]), x1, acc);
+                acc = cfma(to_c2(blk->m[r + 8]), x2, acc);
+                return cfma(to_c2(blk->m[r + 12]), x3, acc);
+            };
+            const c2 y0 = row(0), y1 = row(1), y2 = row(2), y3 = row(3);
+            setv<i0>(s.v, y0);
+            setv<i1>(s.v, y1);
+            setv<i2>(s.v, y2);
+            setv<i3>(s.v, y3);
+        }
+    });
+}
+
+__device__ __forceinline__ void sw_kraus16(SubCube<4>& s, const TOp* __restrict__ op, const OpHead& hd,
+                                           double2* __restrict__ tile, int pl, int o0, int o1, int o2, int o3) {
+    switch (hd.b) {
+        case 0: sw_kraus4(s, op, tile, pl, o0, o1, o2, o3); break;
+        case 1: sw_orbit4<0x3, 0xC>(s, op); break;
+        case 2: sw_orbit4<0x5, 0xA>(s, op); break;
+        default: sw_orbit4<0x9, 0x6>(s, op); break;
+    }
+}
+
 // Apply one tile op; returns the number of TOp entries it occupies.  KRAUS:
 // the plan may hold reference Kraus ops (density-matrix plans only, R = 4;
 // other instantiations carry no code for them).
@@ -525,7 +566,7 @@ __device__ __forceinline__ int run_op(SubCube<R>& s, const TOp* __restrict__ op,
     if constexpr (KRAUS && R == 4) {
         if (hd.kind == 4) {
             if (hd.nd) {
-                sw_kraus4(s, op, tile, pl, o0, o1, o2, o3);
+                sw_kraus16(s, op, hd, tile, pl, o0, o1, o2, o3);
                 return 1 + hd.nd;
             }
             switch (hd.a * 4 + hd.b) {
@@ -937,7 +978,7 @@ __device__ __forceinline__ int run_op_lean_kraus(SubCube<4>& s, Acc& A, const TO
     }
     if (hd.kind == 4) {
         if (hd.nd) {
-            sw_kraus4(s, op, tile, pl, o0, o1, o2, o3);
+            sw_kraus16(s, op, hd, tile, pl, o0, o1, o2, o3);
             return 1 + hd.nd;
         }
         switch (hd.a * 4 + hd.b) {

@@ -30,6 +30,27 @@ static void apply_top(const Plan& p, int o, int R, const int* vb, uint64_t G, ui
             for (int k = 0; k < 4; ++k) v[id[k]] = y[k];
         }
     } else if (op.kind == 4) {
+        // the 16x16 in register order: columns in the 16 TOps, or (b = 1 +
+        // pairing) the orbit blocks: entry (o ^ X[r], o ^ X[c]) of orbit k
+        // in TOp k, m[r + 4 c], every other entry zero
+        std::vector<cd> F(256, cd(0.0, 0.0));
+        if (op.b == 0) {
+            for (int c = 0; c < 16; ++c)
+                for (int r = 0; r < 16; ++r) F[r + 16 * c] = cd(p.tops[o + c].m[r].x, p.tops[o + c].m[r].y);
+        } else {
+            const int P1[3] = { 0x3, 0x5, 0x9 }, P2[3] = { 0xC, 0xA, 0x6 };
+            const int q1 = P1[op.b - 1], q2 = P2[op.b - 1], X[4] = { 0, q1, q2, q1 ^ q2 };
+            const int a1 = __builtin_ctz(q1), a2 = __builtin_ctz(q2);
+            int k = 0;
+            for (int ob = 0; ob < 16; ++ob) {
+                if (((ob >> a1) & 1) || ((ob >> a2) & 1)) continue;
+                for (int r = 0; r < 4; ++r)
+                    for (int c = 0; c < 4; ++c)
+                        F[(ob ^ X[r]) + 16 * (ob ^ X[c])] =
+                            cd(p.tops[o + k].m[r + 4 * c].x, p.tops[o + k].m[r + 4 * c].y);
+                ++k;
+            }
+        }
         for (uint64_t i = 0; i < N; ++i) {
             if (i & G) continue;
             if (!pred(i)) continue;
@@ -42,7 +63,7 @@ static void apply_top(const Plan& p, int o, int R, const int* vb, uint64_t G, ui
             for (int k = 0; k < 16; ++k) x[k] = v[id[k]];
             for (int r = 0; r < 16; ++r) {
                 y[r] = 0;
-                for (int c = 0; c < 16; ++c) y[r] += cd(p.tops[o + c].m[r].x, p.tops[o + c].m[r].y) * x[c];
+                for (int c = 0; c < 16; ++c) y[r] += F[r + 16 * c] * x[c];
             }
             for (int k = 0; k < 16; ++k) v[id[k]] = y[k];
         }
@@ -548,5 +569,18 @@ extern "C" int plan_adj_stats(int nq, const bsim_op* ops, int n, int* out) {
         out[3] += (t.cval >> 32) ? 1 : 0;
         out[4] += (uint32_t)t.cval ? 1 : 0;
     }
+    return 0;
+}
+
+// 16x16 superoperator tile ops of the plan: out[0] in orbit form (kind 4,
+// b > 0), out[1] as 16 columns (kind 4, b = 0, nd = 15)
+extern "C" int plan_superop_forms(int nq, int kind, const bsim_op* ops, int n, int* out) {
+    std::vector<EOp> e;
+    if (int rc = lower_ops(nq, kind, ops, n, e, false)) return rc;
+    Plan p;
+    if (int rc = make_plan(kind == BSIM_SV ? nq : 2 * nq, e, tile_config(), p)) return rc;
+    out[0] = out[1] = 0;
+    for (const TOp& t : p.tops)
+        if (t.kind == 4 && t.nd) ++out[t.b ? 0 : 1];
     return 0;
 }

@@ -63,6 +63,9 @@ def interp():
     h.adjexec.restype = ctypes.c_int
     h.adjexec.argtypes = [ctypes.c_int, ctypes.POINTER(L.Op), ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                           ctypes.c_void_p, ctypes.c_void_p]
+    h.plan_superop_forms.restype = ctypes.c_int
+    h.plan_superop_forms.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(L.Op), ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_int)]
     h.plan_adj_stats.restype = ctypes.c_int
     h.plan_adj_stats.argtypes = [ctypes.c_int, ctypes.POINTER(L.Op), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     return h
@@ -599,3 +602,29 @@ def test_planner_cnot_rewrite_density_matrix(interp):
     ol = lower_instructions(ins, L.BSIM_DM)
     assert interp.plan_kind_count(6, L.BSIM_DM, ol.ptr, ol.n, 0, 1) == 0
     np.testing.assert_allclose(_run_dm(interp, 6, ins), O.evolve_dm(6, ins, "reference"), atol=1e-12)
+
+
+@pytest.mark.parametrize("n,depth,seed", [(6, 6, 0), (7, 5, 1)])
+def test_planner_superop_orbit_form(interp, n, depth, seed):
+    """Physical two-qubit channels whose 16x16 superoperator couples only
+    registers that differ by one of {0, P1, P2, P1 ^ P2} (depolarizing: a
+    qubit's row and column bit flip together) plan as four 4x4 orbit blocks;
+    a channel without that form (a random 2-qubit unitary as its one Kraus
+    operator: U (x) conj(U) is dense) keeps the 16 columns; the interpreted
+    plans match the oracle."""
+    from bsim import noises as Nz
+    from circuits import random_unitary
+
+    ins = W.brickwall(n, depth, seed=seed, noise=True)
+    ol = lower_instructions(ins, L.BSIM_DM, noise_mode="physical")
+    out = (ctypes.c_int * 2)()
+    assert interp.plan_superop_forms(n, L.BSIM_DM, ol.ptr, ol.n, out) == 0
+    assert out[0] > 0 and out[1] == 0, list(out)
+    np.testing.assert_allclose(_run_dm(interp, n, ins, "physical"), O.evolve_dm(n, ins, "physical"), atol=1e-12)
+    rng = np.random.default_rng(seed)
+    ks = [random_unitary(2, rng)]
+    ins2 = list(ins[: len(ins) // 2]) + [Instruction(Nz.Kraus(ks), [1, 3])] + list(ins[len(ins) // 2:])
+    ol = lower_instructions(ins2, L.BSIM_DM, noise_mode="physical")
+    assert interp.plan_superop_forms(n, L.BSIM_DM, ol.ptr, ol.n, out) == 0
+    assert out[0] > 0 and out[1] == 1, list(out)
+    np.testing.assert_allclose(_run_dm(interp, n, ins2, "physical"), O.evolve_dm(n, ins2, "physical"), atol=1e-12)
