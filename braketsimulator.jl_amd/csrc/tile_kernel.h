@@ -468,6 +468,19 @@ template <int A, int B>
 __device__ __forceinline__ void sw_kraus2(SubCube<4>& s, const TOp* __restrict__ op, const OpHead& hd) {
     const uint64_t q = kraus_q(s, op);
     const uint64_t s0 = op->sgm[0], s1 = op->sgm[1], s2 = op->sgm[2], s3 = op->sgm[3];
+    // Reference-semantics blocks hold the registers whose row and column
+    // agree on every other qubit, a small fraction: a wave none of whose
+    // groups is in a block leaves the sub-cubes as they are (the selects
+    // below would keep every value).
+    bool any = false;
+    sfor<0, 16>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        if constexpr ((j & ((1 << A) | (1 << B))) == 0) {
+            const uint64_t uj = ((j & 1) ? s0 : 0) ^ ((j & 2) ? s1 : 0) ^ ((j & 4) ? s2 : 0) ^ ((j & 8) ? s3 : 0);
+            any = any || q == uj;
+        }
+    });
+    if (!__any(any)) return;
     const c2 col0[4] = { hd.m0, hd.m1, hd.m2, hd.m3 };
     sfor<0, 16>([&](auto J) {
         constexpr int j = decltype(J)::value;
