@@ -1307,9 +1307,10 @@ static int make_plan_ops(int vq, const std::vector<EOp>& eops, const TileConfig&
 }
 
 // CNOT(c, t) = H_t CZ(c, t) H_t.  With every other op free of controls,
-// the rewrite leaves an op list the lean streaming passes take: the CZ is a
-// sign op, and the H's join the one-qubit ops of t in register chains (a
-// controlled op keeps a pass on the full kernel).  Returns false (out
+// the rewrite leaves an op list the lean streaming passes take (with
+// reference Kraus ops, the lean Kraus passes): the CZ is a sign op, and the
+// H's join the one-qubit ops of t in register chains (a controlled op keeps
+// a pass on the full kernel).  Returns false (out
 // untouched) when some other op keeps a control or cannot run in a tile.
 static bool cnot_rewrite(const std::vector<EOp>& in, std::vector<EOp>& out) {
     auto is_cnot = [](const EOp& e) {
@@ -1326,7 +1327,9 @@ static bool cnot_rewrite(const std::vector<EOp>& in, std::vector<EOp>& out) {
         }
         uint64_t pm;
         int pd;
-        if ((e.kind != E_DENSE && e.kind != E_DIAG) || !e.fusable()) return false;
+        if (!e.fusable()) return false;
+        if (e.kind == E_KRAUS_REF) continue;  // a tile op of the Kraus passes, no controls
+        if (e.kind != E_DENSE && e.kind != E_DIAG) return false;
         if (e.g.cmask && !(e.kind == E_DIAG && sign_pair(e, pm, pd))) return false;
     }
     if (!ncnot) return false;

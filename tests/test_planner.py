@@ -602,6 +602,19 @@ def test_planner_cnot_rewrite_density_matrix(interp):
     ol = lower_instructions(ins, L.BSIM_DM)
     assert interp.plan_kind_count(6, L.BSIM_DM, ol.ptr, ol.n, 0, 1) == 0
     np.testing.assert_allclose(_run_dm(interp, 6, ins), O.evolve_dm(6, ins, "reference"), atol=1e-12)
+    # with reference Kraus channels between the gates (a CNOT list the lean
+    # Kraus passes take)
+    from bsim import noises as Nz
+
+    noisy = []
+    for k, i in enumerate(ins):
+        noisy.append(i)
+        if k % 3 == 0:
+            noisy.append(Instruction(Nz.Depolarizing(0.05), [int(i.target[-1])]))
+    ol = lower_instructions(noisy, L.BSIM_DM)
+    assert interp.plan_kind_count(6, L.BSIM_DM, ol.ptr, ol.n, 0, 1) == 0
+    for mode in ("reference", "physical"):
+        np.testing.assert_allclose(_run_dm(interp, 6, noisy, mode), O.evolve_dm(6, noisy, mode), atol=1e-12)
 
 
 @pytest.mark.parametrize("n,depth,seed", [(6, 6, 0), (7, 5, 1)])
